@@ -1,0 +1,166 @@
+"""MPHF construction throughput of the MI355X ShockHash engine.
+
+Workload (one step = one whole construction): bipartite ShockHash2, leaf size 64, bucket size 2000,
+10,000,000 uniform random 128-bit fingerprints resident in HBM before the timed region. A step sorts the
+fingerprints, buckets them, plans every RecSplit tree and searches every split and leaf seed on the
+GPU (one persistent kernel with a device work queue); the per-node codes come back to the host. The
+same step in the plain form (one kernel launch per tree node) is timed on the first buckets, the CPU
+oracle is timed on a bounded sample, and the GPU codes are checked against the oracle on that sample.
+
+Prints one JSON line (see the keys below). One GPU only: --gpus N > 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_CUS, SIMDS_PER_CU, LANES_PER_SIMD, CLOCK_GHZ = 256, 4, 16, 2.4
+INT32_PEAK_TOPS = MI355X_CUS * SIMDS_PER_CU * LANES_PER_SIMD * CLOCK_GHZ / 1e3  # 39.3 T lane-ops/s
+OPS_PER_EVAL = 32  # int32 ops per 64-bit-mix hash evaluation (remix + range reduction + count/mask update)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_evals(nodes, codes):
+    """Hash evaluations a sequential search makes to reach the stored codes, from the outputs alone:
+    split node / ShockHash1 leaf m * (code + 1); ShockHash2 leaf m >= 10: m * (s* + 1) with
+    s* = max(unpairElegant(c_j)), (c_j, c_i) = unpairTriangular(code); m < 10: m * (c_j + 1)."""
+    m = nodes["m"].astype(np.float64)
+    kind = nodes["kind"]
+    codes = codes.astype(np.float64)
+    evals = np.where(kind != 3, m * (codes + 1), 0.0)
+    leaf = kind == 3
+    c = codes[leaf]
+    cj = np.floor(0.5 + np.sqrt(0.25 + 2.0 * c))
+    f = np.floor(np.sqrt(cj))
+    r = cj - f * f
+    a = np.where(r < f, r, f)
+    b = np.where(r < f, f, r - f)
+    s_star = np.maximum(a, b)
+    ml = m[leaf]
+    evals_leaf = np.where(ml >= 10, ml * (s_star + 1), ml * (cj + 1))
+    return float(evals.sum() + evals_leaf.sum()), float(evals_leaf.sum())
+
+
+def cpu_baseline(n_sample, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from shockhash_amd import workload
+    h = workload.fingerprints(n_sample, seed=7)
+    t0 = time.perf_counter()
+    ob = O.OracleBuild("sh2", 64, h, 2000, threads=threads)
+    dt = time.perf_counter() - t0
+    return h, ob, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--plain-buckets", type=int, default=100)
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the construction engine does not shard across GPUs "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    from shockhash_amd import workload
+    from shockhash_amd.engine import Engine
+
+    eng = Engine(0)
+    log(f"device {eng.arch} CUs {eng.cu_count} clock {eng.clock_khz / 1e6:.2f} GHz; tuning {eng.tuning}")
+    h = workload.fingerprints(args.n, seed=1)
+    d_h = eng.array(h.reshape(-1))  # fingerprints resident in HBM
+    log(f"{args.n} fingerprints resident on the device")
+
+    for i in range(args.warmup):
+        b = eng.build_dev(d_h, args.n, "sh2", 64, 2000, copy_keys=False)
+        log(f"warmup {i}: timings {np.round(b.timings_ms, 2)}")
+    step_s, kernel_ms = [], []
+    for i in range(args.steps):
+        eng.sync()
+        t0 = time.perf_counter()
+        b = eng.build_dev(d_h, args.n, "sh2", 64, 2000, copy_keys=False)
+        eng.sync()
+        step_s.append(time.perf_counter() - t0)
+        kernel_ms.append(b.timings_ms[3])
+        log(f"step {i}: {step_s[-1] * 1e3:.1f} ms (sort {b.timings_ms[0]:.1f}, tree {b.timings_ms[1]:.1f}, "
+            f"search {b.timings_ms[2]:.1f}, kernel {b.timings_ms[3]:.1f})")
+    ms_per_step = 1e3 * float(np.mean(step_s))
+    value = args.n / (ms_per_step / 1e3)
+    evals, evals_leaf = algorithmic_evals(b.nodes, b.codes)
+    kms = float(np.mean(kernel_ms))
+    achieved_tops = evals * OPS_PER_EVAL / (kms / 1e3) / 1e12
+
+    # plain form: one launch per node over the first buckets, same device input
+    t0 = time.perf_counter()
+    bp = eng.build_dev(d_h, args.n, "sh2", 64, 2000, mode="plain", plain_bucket_limit=args.plain_buckets,
+                       copy_keys=False)
+    plain_s = time.perf_counter() - t0
+    n_plain_keys = int(b.size_acc[args.plain_buckets])
+    nn = len(bp.codes)
+    plain_match = bool(np.array_equal(bp.codes, b.codes[:nn]))
+    # the plain run also sorts all keys; charge it only for the searched buckets' share of sort time
+    plain_search_s = (bp.timings_ms[1] + bp.timings_ms[2]) / 1e3 + bp.timings_ms[0] / 1e3 * n_plain_keys / args.n
+    plain_value = n_plain_keys / plain_search_s
+    log(f"plain form: {args.plain_buckets} buckets ({n_plain_keys} keys) in {plain_s:.2f} s, codes equal: {plain_match}")
+
+    cpu = None
+    bit_exact = plain_match
+    if not args.no_cpu_baseline:
+        hs, ob, dt = cpu_baseline(args.cpu_sample, args.cpu_threads)
+        bs = eng.build(hs, "sh2", 64, 2000)
+        bit_exact = bit_exact and bool(np.array_equal(bs.codes, ob.node_code))
+        cpu = {"value": args.cpu_sample / dt, "unit": "keys/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"{args.cpu_sample} fingerprints (same generator, seed 7), ShockHash2 leaf 64, bucket 2000, "
+                         f"oracle C++ restatement, {args.cpu_threads} threads, {dt:.1f} s"}
+        log(f"cpu oracle: {cpu['value']:.0f} keys/s; GPU codes equal the oracle's on the sample: {bit_exact}")
+
+    out = {
+        "metric": "MPHF build keys/s at 10M keys (bit-exact) vs plain GPU form; p99 build latency",
+        "value": value,
+        "unit": "keys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "ShockHash2 leaf 64, bucket 2000, 10M uniform 128-bit fingerprints (SplitMix64, seed 1), "
+                               "persistent device-queue build", "keys": args.n, "leaf_size": 64, "bucket_size": 2000,
+                   "variant": "ShockHash2"},
+        "plain_form": {"keys_per_s": plain_value, "buckets": args.plain_buckets, "keys": n_plain_keys,
+                       "codes_equal_persistent": plain_match},
+        "speedup_vs_plain": value / plain_value,
+        "bit_exact": bit_exact,
+        "hash_evals": evals,
+        "hash_evals_leaves": evals_leaf,
+        "search_kernel_ms": kms,
+        "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": INT32_PEAK_TOPS, "unit": "TOP/s",
+                     "frac": achieved_tops / INT32_PEAK_TOPS, "traffic": None,
+                     "ops_per_eval": OPS_PER_EVAL, "evals_per_s": evals / (kms / 1e3)},
+        "cpu_baseline": cpu,
+        "p99_build_latency_ms": None,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

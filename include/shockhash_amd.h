@@ -1,0 +1,136 @@
+/*
+ * libshockhash_amd.so — MI355X (gfx950) construction engine for ShockHash's brute-force seed search.
+ *
+ * C ABI: plain pointers and sizes, int status codes, no exceptions across the boundary.
+ * Device pointers ("d_" prefix) are HIP device allocations; `stream` is a hipStream_t or NULL for the
+ * engine's own stream. Calls on one engine must not overlap (one submitting thread per engine).
+ *
+ * Reference seams these entry points replace (paths relative to the reference repository):
+ *  - shgpu_sh1_leaf_search: the inline ShockHash1 leaf loop of ShockHash::recSplit
+ *    (include/ShockHash.h:404-434), SIMDShockHash::leafLevel (include/SIMDShockHash.hpp:507-542) and
+ *    BijectionsShockHash1::calculateBijection (benchmark/bijections/ShockHash1.h:13-35).
+ *  - shgpu_sh2_leaf_search: the search half of
+ *    size_t shockhash2construct(size_t m, std::vector<uint64_t>&, std::vector<std::pair<uint64_t,uint8_t>>&)
+ *    (include/shockhash/ShockHash2-precompiled.h:9, src/ShockHash2-precompiled.cpp:49-61), i.e.
+ *    BijectionsShockHash2::findSeed (include/shockhash/ShockHash2-internal.h:501-546).
+ *  - shgpu_split_search: the inline split-seed loops + partitions of ShockHash::recSplit
+ *    (include/ShockHash.h:451-548), ShockHash2::recSplit (include/ShockHash2.h:290-387) and
+ *    SIMDShockHash::higherLevel/aggrLevel (include/SIMDShockHash.hpp:201-365).
+ *  - shgpu_build_run: parallelPartition (include/shockhash/Sorter.hpp:15-36) plus every recSplit
+ *    seed search of hash_gen/compute_thread (include/ShockHash.h:557-688, include/ShockHash2.h:396-498,
+ *    include/SIMDShockHash.hpp:619-760); encoding stays with the caller.
+ */
+#ifndef SHOCKHASH_AMD_H
+#define SHOCKHASH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../shockhash_amd/csrc/shgpu_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+enum {
+    SHGPU_OK = 0,
+    SHGPU_ERR_ARG = 1,               /* invalid argument */
+    SHGPU_ERR_HIP = 2,               /* HIP runtime error (message in shgpu_last_error) */
+    SHGPU_ERR_NOMEM = 3,             /* device allocation failed */
+    SHGPU_ERR_SEARCH_LIMIT = 4,      /* a node exhausted its seed bound (duplicate keys?) */
+    SHGPU_ERR_BUCKET_TOO_LARGE = 5,  /* bucket above the encodable maximum (2999 keys) */
+    SHGPU_ERR_UNSUPPORTED = 6,       /* configuration the GPU engine does not implement */
+    SHGPU_ERR_NO_DEVICE = 7          /* no HIP device / extension unusable */
+};
+
+/* Construction variants (the three reference class templates). */
+enum { SHGPU_SHOCKHASH = 0, SHGPU_SIMD_SHOCKHASH = 1, SHGPU_SHOCKHASH2 = 2 };
+
+/* Scheduling forms of shgpu_build_run. */
+enum { SHGPU_MODE_PERSISTENT = 0, SHGPU_MODE_PLAIN = 1 };
+
+typedef struct shgpu_engine shgpu_engine;
+typedef struct shgpu_build shgpu_build;
+
+/* Schedule knobs (see shgpu_autotune). */
+typedef struct shgpu_tuning {
+    uint32_t sh1_seeds_per_lane;    /* 1, 2 or 4: ShockHash1 leaf window = 64 * this */
+    uint32_t split_seeds_per_lane;  /* 1, 2 or 4: split window for aggregation nodes */
+    uint32_t waves_per_cu;          /* persistent grid = CUs * this */
+    uint64_t max_trials;            /* seed bound per split node / ShockHash1 leaf */
+} shgpu_tuning;
+
+/* Last error message of the calling thread. */
+const char *shgpu_last_error(void);
+const char *shgpu_status_string(int status);
+
+int shgpu_engine_create(int device, shgpu_engine **out);
+void shgpu_engine_destroy(shgpu_engine *e);
+/* CU count, peak engine clock (kHz) and gfx arch name of the engine's device. */
+int shgpu_engine_info(shgpu_engine *e, int *cu_count, int *clock_khz, char *arch, size_t arch_len);
+int shgpu_get_tuning(shgpu_engine *e, shgpu_tuning *out);
+int shgpu_set_tuning(shgpu_engine *e, const shgpu_tuning *t);
+/* Synchronises the engine's stream. */
+int shgpu_sync(shgpu_engine *e);
+
+/* Device memory helpers (ordered on the engine's stream; the copies are synchronous). */
+int shgpu_device_alloc(shgpu_engine *e, size_t bytes, void **d_out);
+int shgpu_device_free(shgpu_engine *e, void *d);
+int shgpu_copy_to_device(shgpu_engine *e, void *d_dst, const void *h_src, size_t bytes);
+int shgpu_copy_to_host(shgpu_engine *e, void *h_dst, const void *d_src, size_t bytes);
+
+/* Batched ShockHash1 leaf search: d_seeds[i] = smallest seed >= start_seed of task i
+ * (SHGPU_NOT_FOUND if none below start_seed + max_trials). m <= 64. */
+int shgpu_sh1_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_leaf_task *d_tasks, uint32_t n_tasks,
+                          uint64_t *d_seeds, void *stream);
+
+/* Batched ShockHash2 leaf search: d_codes[i] = the findSeed code of task i (start_seed and
+ * max_trials are ignored). m <= 64. */
+int shgpu_sh2_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_leaf_task *d_tasks, uint32_t n_tasks,
+                          uint64_t *d_codes, void *stream);
+
+/* Batched split-seed search; each task's keys are partitioned in place by the found seed.
+ * d_seeds[i] = found seed (absolute). Tasks must not overlap. fanout <= 4, m <= 65535. */
+int shgpu_split_search(shgpu_engine *e, uint64_t *d_keys, const shgpu_split_task *d_tasks, uint32_t n_tasks,
+                       uint64_t *d_seeds, void *stream);
+
+/* Whole-construction seed search. d_hashes: n 128-bit fingerprints as (first, second) pairs. Sorts by
+ * first, buckets by remap128(first, nbuckets), builds the RecSplit tree of every bucket and searches
+ * every node's seed. Synchronous; the result owns copies of everything the encoder needs. */
+typedef struct shgpu_build_desc {
+    int variant;           /* SHGPU_SHOCKHASH / SHGPU_SIMD_SHOCKHASH / SHGPU_SHOCKHASH2 */
+    uint32_t leaf_size;    /* template LEAF_SIZE */
+    uint64_t bucket_size;  /* constructor bucket_size */
+    int mode;              /* SHGPU_MODE_PERSISTENT or SHGPU_MODE_PLAIN */
+    uint32_t plain_bucket_limit; /* plain mode: build only the first N buckets (0 = all) */
+    int copy_keys;         /* keep a host copy of the keys in final leaf order */
+} shgpu_build_desc;
+
+int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
+                    shgpu_build **out);
+/* Same with the fingerprints in host memory (staged to the device by the engine). */
+int shgpu_build_run_host(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *h_hashes, uint64_t n,
+                         shgpu_build **out);
+void shgpu_build_free(shgpu_build *b);
+uint64_t shgpu_build_nbuckets(const shgpu_build *b);
+uint64_t shgpu_build_nnodes(const shgpu_build *b);
+/* size_acc: nbuckets + 1 cumulative bucket sizes. */
+int shgpu_build_bucket_sizes(const shgpu_build *b, uint64_t *size_acc);
+/* nodes (may be NULL) and codes (code = found seed - start_seed), nnodes each, DFS pre-order. */
+int shgpu_build_nodes(const shgpu_build *b, shgpu_node *nodes, uint64_t *codes);
+/* Keys (fingerprint .second) in final leaf order, n of them (needs desc->copy_keys). */
+int shgpu_build_keys(const shgpu_build *b, uint64_t *keys);
+/* Host-only: the forest shgpu_build_run would search for these bucket sizes (DFS pre-order). Call with
+ * out = NULL to get the count. Needs no GPU. */
+int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc, uint64_t nbuckets, shgpu_node *out,
+                      uint64_t capacity, uint64_t *count);
+/* Milliseconds spent in the build's phases: [0] sort + bucketing, [1] tree setup, [2] seed search
+ * (host clock, launch to results on the host), [3] the persistent search kernel alone (HIP events on
+ * the engine's stream; 0 in plain mode). */
+int shgpu_build_timings(const shgpu_build *b, double *ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

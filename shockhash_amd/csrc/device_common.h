@@ -1,0 +1,53 @@
+// Device-side primitives shared by every ShockHash seed-search kernel (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "shgpu_types.h"
+
+namespace shgpu {
+
+// Stafford Mix13, the 64-bit finaliser the reference calls `remix`
+// (in-tree text: include/shockhash/SimdUtils.h:42-46).
+__device__ __forceinline__ uint64_t remix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// MurmurHash3 fmix32 (include/SIMDShockHash.hpp:76-83).
+__device__ __forceinline__ uint32_t remix32(uint32_t z) {
+    z ^= z >> 16; z *= 0x85ebca6bu; z ^= z >> 13; z *= 0xc2b2ae35u; z ^= z >> 16;
+    return z;
+}
+
+// floor(x * n / 2^64) for n < 2^32 (fastrange64 at ShockHash2-internal.h:84,423,442).
+__device__ __forceinline__ uint32_t fastrange64(uint64_t x, uint32_t n) {
+    return (uint32_t)__umul64hi(x, (uint64_t)n);
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Orders LDS traffic between the lanes of one wavefront (a wave executes in lock step, LDS
+// operations of one wave complete in issue order; this stops the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+}  // namespace shgpu

@@ -1,0 +1,771 @@
+// libshockhash_amd.so: kernels and C ABI of the MI355X ShockHash seed-search engine.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/shockhash_amd.h"
+#include "search_sh1.h"
+#include "search_sh2.h"
+#include "search_split.h"
+
+using namespace shgpu;
+
+namespace {
+
+// Per-depth start seeds, include/ShockHash.h:47-49.
+const uint64_t START_SEED[20] = {
+    0x106393c187cae21a, 0x6453cec3f7376937, 0x643e521ddbd2be98, 0x3740c6412f6572cb, 0x717d47562f1ce470,
+    0x4cd6eb4c63befb7c, 0x9bfd8c5e18c8da73, 0x082f20e10092a9a3, 0x2ada2ce68d21defc, 0xe33cb4f3e7c6466b,
+    0x3980be458c509c59, 0xc466fd9584828e8c, 0x45f0aabe1a61ede6, 0xf6e7b8b33ad9b98d, 0x4ef95e25f4b4983d,
+    0x81175195173b92d3, 0x4e50927d8dd15978, 0x1ea2099d1fafae7f, 0x425c8a06fbaaa815, 0xcd4216006c74052a};
+
+constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
+constexpr int BLOCK = 256;
+constexpr int WAVES_PER_BLOCK = BLOCK / 64;
+constexpr uint32_t MAX_ENCODABLE_BUCKET = 2999;  // memo tables cover sizes < MAX_BUCKET_SIZE = 3000
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess) return fail(SHGPU_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want) {
+        if (want <= bytes) return SHGPU_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            return fail(SHGPU_ERR_NOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+        }
+        bytes = want;
+        return SHGPU_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct shgpu_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cu_count = 0;
+    int clock_khz = 0;
+    char arch[64] = {0};
+    shgpu_tuning tuning{4, 4, 8, 1ull << 40};
+    DevBuf counter, sh2_scratch, staging;
+    // build buffers
+    DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl;
+};
+
+struct shgpu_build {
+    uint64_t n = 0, nbuckets = 0;
+    std::vector<uint64_t> size_acc;
+    std::vector<shgpu_node> nodes;
+    std::vector<uint64_t> codes;
+    std::vector<uint64_t> keys;  // filled when desc->copy_keys
+    double ms[4] = {0, 0, 0, 0};  // sort+bucketing, tree setup, seed search (host clock), search kernel (HIP events)
+};
+
+// ===================================================================================== kernels
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__device__ __forceinline__ uint32_t grab(uint32_t *counter) {
+    uint32_t t = 0;
+    if (lane_id() == 0) t = atomicAdd(counter, 1u);
+    return uniform(__shfl(t, 0));
+}
+
+template <int S>
+__global__ void __launch_bounds__(BLOCK) k_sh1_batch(const uint64_t *__restrict__ keys, const shgpu_leaf_task *__restrict__ tasks,
+                                                     uint32_t n, uint64_t *__restrict__ out, uint32_t *counter) {
+    __shared__ PfLds lds[WAVES_PER_BLOCK];
+    PfLds &L = lds[threadIdx.x / 64];
+    for (;;) {
+        const uint32_t t = grab(counter);
+        if (t >= n) break;
+        const shgpu_leaf_task tk = tasks[t];
+        const uint64_t x = sh1_leaf_search<S>(keys + uniform(tk.offset), uniform(tk.m), uniform64(tk.start_seed),
+                                              uniform64(tk.max_trials), L);
+        if (lane_id() == 0) out[t] = x;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict__ keys, const shgpu_leaf_task *__restrict__ tasks,
+                                                     uint32_t n, uint64_t *__restrict__ out, uint32_t *counter, uint8_t *scratch) {
+    __shared__ Sh2Lds lds[WAVES_PER_BLOCK];
+    const uint32_t w = threadIdx.x / 64;
+    Sh2Lds &L = lds[w];
+    uint8_t *ws = scratch + (size_t)(blockIdx.x * WAVES_PER_BLOCK + w) * SH2_SCRATCH_BYTES;
+    for (;;) {
+        const uint32_t t = grab(counter);
+        if (t >= n) break;
+        const shgpu_leaf_task tk = tasks[t];
+        const uint64_t code = sh2_leaf_search(keys + uniform(tk.offset), uniform(tk.m), ws, L);
+        if (lane_id() == 0) out[t] = code;
+    }
+}
+
+template <int S>
+__global__ void __launch_bounds__(BLOCK) k_split_batch(uint64_t *keys, uint64_t *tmp, const shgpu_split_task *__restrict__ tasks,
+                                                       uint32_t n, uint64_t *__restrict__ out, uint32_t *counter) {
+    for (;;) {
+        const uint32_t t = grab(counter);
+        if (t >= n) break;
+        const shgpu_split_task tk = tasks[t];
+        const uint32_t off = uniform(tk.offset), m = uniform(tk.m);
+        const SplitShapeDev sh = make_split_shape(m, uniform(tk.unit), uniform(tk.kind), uniform(tk.hash));
+        const uint64_t x = split_search<S>(keys + off, sh, uniform64(tk.start_seed), uniform64(tk.max_trials));
+        if (x != SHGPU_NOT_FOUND) split_partition(keys + off, tmp + off, sh, x);
+        if (lane_id() == 0) out[t] = x;
+    }
+}
+
+__global__ void k_unzip_hashes(const uint64_t *__restrict__ h, uint64_t n, uint64_t *first, uint64_t *second) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        first[i] = h[2 * i];
+        second[i] = h[2 * i + 1];
+    }
+}
+
+// size_acc[b] = number of keys in buckets < b, from firsts sorted ascending (remap128 is monotone).
+__global__ void k_bucket_bounds(const uint64_t *__restrict__ first, uint64_t n, uint64_t nb, uint64_t *size_acc) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = i < n ? __umul64hi(first[i], nb) : nb;
+        const uint64_t bp = i > 0 ? __umul64hi(first[i - 1], nb) : 0;
+        const uint64_t lo = i > 0 ? bp + 1 : 0;
+        for (uint64_t k = lo; k <= b && k <= nb; k++) size_acc[k] = i;
+    }
+}
+
+struct BuildArgs {
+    uint64_t *keys;
+    uint64_t *tmp;
+    const shgpu_node *nodes;
+    uint64_t *codes;
+    uint32_t *queue;
+    uint32_t *ctrl;  // [0] head, [1] tail, [2] error flag (a queue slot never filled)
+    uint32_t total;
+    uint8_t *scratch;
+    int split_hash;
+};
+
+// One node: seed search (+ partition and child publication for split nodes).
+template <int VARIANT, int S_LEAF, int S_SPLIT>
+__device__ void process_node(const BuildArgs &a, uint32_t nid, void *lds_raw, uint8_t *ws, bool publish) {
+    const shgpu_node nd = a.nodes[nid];
+    const uint32_t off = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
+    const uint64_t start = uniform64(nd.start_seed), lim = uniform64(nd.max_trials);
+    uint64_t code;
+    if (kind == SHGPU_NODE_LEAF) {
+        if (VARIANT == SHGPU_SHOCKHASH2) {
+            code = sh2_leaf_search(a.keys + off, m, ws, *(Sh2Lds *)lds_raw);
+        } else {
+            const uint64_t x = sh1_leaf_search<S_LEAF>(a.keys + off, m, start, lim, *(PfLds *)lds_raw);
+            code = x == SHGPU_NOT_FOUND ? x : x - start;
+        }
+    } else {
+        const SplitShapeDev sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
+        uint64_t x;
+        if (kind == SHGPU_NODE_AGGR_LOWER)  // the many-trial nodes (~4000 seeds at 4 x 64 keys)
+            x = split_search<S_SPLIT>(a.keys + off, sh, start, lim);
+        else
+            x = split_search<1>(a.keys + off, sh, start, lim);
+        code = x == SHGPU_NOT_FOUND ? x : x - start;
+        if (x != SHGPU_NOT_FOUND) split_partition(a.keys + off, a.tmp + off, sh, x);
+        if (publish) {
+            __threadfence();  // children's keys visible device-wide before they are queued
+            const uint32_t nc = uniform(nd.n_children);
+            if (lane_id() == 0 && nc > 0) {
+                uint32_t pos = atomicAdd(&a.ctrl[1], nc);
+                uint32_t child = nid + 1;
+                for (uint32_t c = 0; c < nc; c++) {
+                    __hip_atomic_store(&a.queue[pos + c], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    child += a.nodes[child].subtree;
+                }
+            }
+        }
+    }
+    if (lane_id() == 0) a.codes[nid] = code;
+}
+
+// Persistent construction kernel: every wave pulls nodes from the device queue until all nodes of
+// the forest have been taken. Split nodes push their children when their partition is written.
+template <int VARIANT, int S_LEAF, int S_SPLIT>
+__global__ void __launch_bounds__(BLOCK) k_build(BuildArgs a) {
+    using LdsT = typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, PfLds>::type;
+    __shared__ LdsT lds[WAVES_PER_BLOCK];
+    const uint32_t w = threadIdx.x / 64;
+    uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)(blockIdx.x * WAVES_PER_BLOCK + w) * SH2_SCRATCH_BYTES : nullptr;
+    for (;;) {
+        const uint32_t idx = grab(&a.ctrl[0]);
+        if (idx >= a.total) break;
+        uint32_t nid = 0;
+        if (lane_id() == 0) {
+            // Bounded wait (~2^27 sleeps, well over ten seconds): a slot that is never filled means an
+            // inconsistent forest; flag it and stop instead of spinning forever.
+            for (uint32_t spin = 0;; spin++) {
+                nid = __hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (nid != EMPTY_SLOT) break;
+                if (spin >= (1u << 27)) { atomicExch(&a.ctrl[2], 1u); break; }
+                __builtin_amdgcn_s_sleep(4);
+            }
+        }
+        nid = uniform(__shfl(nid, 0));
+        if (nid == EMPTY_SLOT) break;
+        __threadfence();
+        process_node<VARIANT, S_LEAF, S_SPLIT>(a, nid, &lds[w], ws, true);
+    }
+}
+
+// Plain form: one node per launch (host walks the forest in DFS order on one stream).
+template <int VARIANT, int S_LEAF, int S_SPLIT>
+__global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
+    using LdsT = typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, PfLds>::type;
+    __shared__ LdsT lds;
+    process_node<VARIANT, S_LEAF, S_SPLIT>(a, nid, &lds, a.scratch, false);
+}
+
+// ===================================================================================== host helpers
+
+namespace {
+
+struct Strategy {
+    int variant;
+    uint32_t leaf, lower, upper;
+    Strategy(int v, uint32_t L) : variant(v), leaf(L) {
+        if (v == SHGPU_SHOCKHASH2) { lower = 4 * L; upper = 3 * lower; }                    // ShockHash2.h:49-56
+        else { lower = L * (L > 24 ? 4 : 2); upper = lower * (L > 24 ? 3 : 2); }         // ShockHash.h:63-72
+    }
+};
+
+// Emits the subtree of a node of m keys at `offset` in DFS pre-order (recSplit order,
+// ShockHash.h:303-555 / SIMDShockHash.hpp:563-617 / ShockHash2.h:259-394).
+void emit_subtree(std::vector<shgpu_node> &out, const Strategy &st, uint32_t offset, uint32_t m, int level,
+                  uint64_t max_trials) {
+    const size_t me = out.size();
+    out.push_back(shgpu_node{});
+    shgpu_node nd{};
+    nd.offset = offset;
+    nd.m = (uint16_t)m;
+    nd.max_trials = max_trials;
+    if (m <= st.leaf) {
+        nd.kind = SHGPU_NODE_LEAF;
+        nd.start_seed = st.variant == SHGPU_SHOCKHASH2 ? 0
+                        : st.variant == SHGPU_SIMD_SHOCKHASH ? START_SEED[19] : START_SEED[level];
+        nd.subtree = 1;
+        out[me] = nd;
+        return;
+    }
+    uint32_t unit;
+    if (m > st.upper) {
+        nd.kind = SHGPU_NODE_SPLIT2;
+        unit = (uint16_t(m / 2 + st.upper - 1) / st.upper) * st.upper;
+        nd.start_seed = START_SEED[level];
+    } else if (m > st.lower) {
+        nd.kind = SHGPU_NODE_AGGR_UPPER;
+        unit = st.lower;
+        nd.start_seed = st.variant == SHGPU_SIMD_SHOCKHASH ? START_SEED[17] : START_SEED[level];
+    } else {
+        nd.kind = SHGPU_NODE_AGGR_LOWER;
+        unit = st.leaf;
+        nd.start_seed = st.variant == SHGPU_SIMD_SHOCKHASH ? START_SEED[18] : START_SEED[level];
+    }
+    nd.unit = unit;
+    uint32_t nchild = 0, i = 0;
+    if (nd.kind == SHGPU_NODE_SPLIT2) {  // ShockHash.h:479-480
+        emit_subtree(out, st, offset, unit, level + 1, max_trials);
+        nchild++;
+        i = unit;
+    } else {  // ShockHash.h:510-514,543-547
+        for (; i < m - unit; i += unit) {
+            emit_subtree(out, st, offset + i, unit, level + 1, max_trials);
+            nchild++;
+        }
+    }
+    if (m - i > 1) {  // the last part only when it has at least 2 keys
+        emit_subtree(out, st, offset + i, m - i, level + 1, max_trials);
+        nchild++;
+    }
+    nd.n_children = (uint8_t)nchild;
+    nd.subtree = (uint32_t)(out.size() - me);
+    out[me] = nd;
+}
+
+// Every non-root node must be published exactly once by its parent (children of node i are
+// i + 1, then each next sibling follows the previous sibling's subtree).
+int check_forest(const std::vector<shgpu_node> &nodes, uint32_t n_roots) {
+    std::vector<uint8_t> seen(nodes.size(), 0);
+    uint64_t pushed = 0;
+    for (size_t i = 0; i < nodes.size(); i++) {
+        uint64_t child = i + 1;
+        for (uint32_t c = 0; c < nodes[i].n_children; c++) {
+            if (child >= nodes.size() || seen[child]) return fail(SHGPU_ERR_ARG, "inconsistent forest");
+            seen[child] = 1;
+            pushed++;
+            child += nodes[child].subtree;
+        }
+    }
+    if (pushed + n_roots != nodes.size()) return fail(SHGPU_ERR_ARG, "inconsistent forest: node count");
+    return SHGPU_OK;
+}
+
+int check_variant(int variant, uint32_t leaf) {
+    if (variant == SHGPU_SHOCKHASH || variant == SHGPU_SIMD_SHOCKHASH) {
+        if (leaf < 1 || leaf > 60) return fail(SHGPU_ERR_UNSUPPORTED, "ShockHash leaf size must be in [1, 60]");
+        return SHGPU_OK;
+    }
+    if (variant == SHGPU_SHOCKHASH2) {
+        if (leaf < 1 || leaf > 64) return fail(SHGPU_ERR_UNSUPPORTED, "ShockHash2 GPU leaf search supports leaf sizes <= 64");
+        return SHGPU_OK;
+    }
+    return fail(SHGPU_ERR_ARG, "unknown variant");
+}
+
+int grid_for(const shgpu_engine *e) { return std::max(1, e->cu_count * (int)e->tuning.waves_per_cu / WAVES_PER_BLOCK); }
+
+hipStream_t pick(shgpu_engine *e, void *stream) { return stream ? (hipStream_t)stream : e->stream; }
+
+int reset_counter(shgpu_engine *e, hipStream_t s) {
+    int rc = e->counter.ensure(64);
+    if (rc) return rc;
+    k_fill_u32<<<1, 64, 0, s>>>(e->counter.as<uint32_t>(), 0, 16);
+    HIP_TRY(hipGetLastError());
+    return SHGPU_OK;
+}
+
+int ensure_sh2_scratch(shgpu_engine *e, size_t waves) { return e->sh2_scratch.ensure(waves * SH2_SCRATCH_BYTES); }
+
+template <int VARIANT, int SL, int SS>
+int launch_build(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
+    k_build<VARIANT, SL, SS><<<grid, BLOCK, 0, s>>>(a);
+    HIP_TRY(hipGetLastError());
+    return SHGPU_OK;
+}
+
+template <int VARIANT>
+int launch_build_v(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
+    const uint32_t sl = e->tuning.sh1_seeds_per_lane, ss = e->tuning.split_seeds_per_lane;
+#define SHGPU_CASE(A, B) if (sl == A && ss == B) return launch_build<VARIANT, A, B>(e, a, grid, s);
+    SHGPU_CASE(1, 1) SHGPU_CASE(1, 2) SHGPU_CASE(1, 4) SHGPU_CASE(2, 1) SHGPU_CASE(2, 2) SHGPU_CASE(2, 4)
+    SHGPU_CASE(4, 1) SHGPU_CASE(4, 2) SHGPU_CASE(4, 4)
+#undef SHGPU_CASE
+    return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
+}
+
+template <int VARIANT>
+int launch_node_v(shgpu_engine *e, const BuildArgs &a, uint32_t nid, hipStream_t s) {
+    k_node<VARIANT, 4, 4><<<1, 64, 0, s>>>(a, nid);
+    HIP_TRY(hipGetLastError());
+    return SHGPU_OK;
+}
+
+}  // namespace
+
+// ===================================================================================== C ABI
+
+extern "C" {
+
+const char *shgpu_last_error(void) { return g_err.c_str(); }
+
+const char *shgpu_status_string(int status) {
+    switch (status) {
+        case SHGPU_OK: return "ok";
+        case SHGPU_ERR_ARG: return "invalid argument";
+        case SHGPU_ERR_HIP: return "HIP runtime error";
+        case SHGPU_ERR_NOMEM: return "device out of memory";
+        case SHGPU_ERR_SEARCH_LIMIT: return "seed search bound exceeded (duplicate keys?)";
+        case SHGPU_ERR_BUCKET_TOO_LARGE: return "bucket larger than the encodable maximum";
+        case SHGPU_ERR_UNSUPPORTED: return "unsupported configuration";
+        case SHGPU_ERR_NO_DEVICE: return "no usable HIP device";
+        default: return "unknown status";
+    }
+}
+
+int shgpu_engine_create(int device, shgpu_engine **out) {
+    if (!out) return fail(SHGPU_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+        return fail(SHGPU_ERR_NO_DEVICE, "no HIP device " + std::to_string(device));
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SHGPU_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+    shgpu_engine *e = new shgpu_engine();
+    e->device = device;
+    e->cu_count = prop.multiProcessorCount;
+    e->clock_khz = prop.clockRate;
+    std::snprintf(e->arch, sizeof(e->arch), "%s", prop.gcnArchName);
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        delete e;
+        return fail(SHGPU_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
+    }
+    *out = e;
+    return SHGPU_OK;
+}
+
+void shgpu_engine_destroy(shgpu_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipStreamDestroy(e->stream);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    delete e;
+}
+
+int shgpu_engine_info(shgpu_engine *e, int *cu_count, int *clock_khz, char *arch, size_t arch_len) {
+    if (!e) return fail(SHGPU_ERR_ARG, "engine is NULL");
+    if (cu_count) *cu_count = e->cu_count;
+    if (clock_khz) *clock_khz = e->clock_khz;
+    if (arch && arch_len) std::snprintf(arch, arch_len, "%s", e->arch);
+    return SHGPU_OK;
+}
+
+int shgpu_get_tuning(shgpu_engine *e, shgpu_tuning *out) {
+    if (!e || !out) return fail(SHGPU_ERR_ARG, "NULL argument");
+    *out = e->tuning;
+    return SHGPU_OK;
+}
+
+int shgpu_set_tuning(shgpu_engine *e, const shgpu_tuning *t) {
+    if (!e || !t) return fail(SHGPU_ERR_ARG, "NULL argument");
+    auto okS = [](uint32_t s) { return s == 1 || s == 2 || s == 4; };
+    if (!okS(t->sh1_seeds_per_lane) || !okS(t->split_seeds_per_lane))
+        return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
+    if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(SHGPU_ERR_ARG, "waves_per_cu must be in [1, 32]");
+    if (t->max_trials == 0) return fail(SHGPU_ERR_ARG, "max_trials must be positive");
+    e->tuning = *t;
+    return SHGPU_OK;
+}
+
+int shgpu_sync(shgpu_engine *e) {
+    if (!e) return fail(SHGPU_ERR_ARG, "engine is NULL");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return SHGPU_OK;
+}
+
+int shgpu_sh1_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_leaf_task *d_tasks, uint32_t n,
+                          uint64_t *d_seeds, void *stream) {
+    if (!e || (n && (!d_keys || !d_tasks || !d_seeds))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (n == 0) return SHGPU_OK;
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = pick(e, stream);
+    int rc = reset_counter(e, s);
+    if (rc) return rc;
+    const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
+    uint32_t *ctr = e->counter.as<uint32_t>();
+    switch (e->tuning.sh1_seeds_per_lane) {
+        case 1: k_sh1_batch<1><<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_seeds, ctr); break;
+        case 2: k_sh1_batch<2><<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_seeds, ctr); break;
+        default: k_sh1_batch<4><<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_seeds, ctr); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return SHGPU_OK;
+}
+
+int shgpu_sh2_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_leaf_task *d_tasks, uint32_t n,
+                          uint64_t *d_codes, void *stream) {
+    if (!e || (n && (!d_keys || !d_tasks || !d_codes))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (n == 0) return SHGPU_OK;
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = pick(e, stream);
+    int rc = reset_counter(e, s);
+    if (rc) return rc;
+    const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
+    rc = ensure_sh2_scratch(e, (size_t)grid * WAVES_PER_BLOCK);
+    if (rc) return rc;
+    k_sh2_batch<<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_codes, e->counter.as<uint32_t>(), e->sh2_scratch.as<uint8_t>());
+    HIP_TRY(hipGetLastError());
+    return SHGPU_OK;
+}
+
+int shgpu_split_search(shgpu_engine *e, uint64_t *d_keys, const shgpu_split_task *d_tasks, uint32_t n, uint64_t *d_seeds,
+                       void *stream) {
+    if (!e || (n && (!d_keys || !d_tasks || !d_seeds))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (n == 0) return SHGPU_OK;
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = pick(e, stream);
+    // The partition needs a scratch copy as large as the furthest key a task touches; the caller's
+    // tasks are on the device, so size it from the largest key offset reachable (tmp grows to the
+    // key array extent the caller declares through the tasks; read them back to size it).
+    std::vector<shgpu_split_task> h(n);
+    HIP_TRY(hipMemcpyAsync(h.data(), d_tasks, n * sizeof(shgpu_split_task), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint64_t extent = 0;
+    for (const auto &t : h) {
+        if (t.m > 65535) return fail(SHGPU_ERR_ARG, "split task with more than 65535 keys");
+        const uint32_t fanout = t.kind == SHGPU_NODE_SPLIT2 ? 2 : (t.unit ? (t.m + t.unit - 1) / t.unit : 0);
+        if (t.unit == 0 || fanout < 2 || fanout > 4 || t.unit >= t.m) return fail(SHGPU_ERR_ARG, "split task fanout must be in [2, 4]");
+        extent = std::max<uint64_t>(extent, (uint64_t)t.offset + t.m);
+    }
+    int rc = e->tmp.ensure(extent * 8 + 64);
+    if (rc) return rc;
+    rc = reset_counter(e, s);
+    if (rc) return rc;
+    const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
+    uint32_t *ctr = e->counter.as<uint32_t>();
+    switch (e->tuning.split_seeds_per_lane) {
+        case 1: k_split_batch<1><<<grid, BLOCK, 0, s>>>(d_keys, e->tmp.as<uint64_t>(), d_tasks, n, d_seeds, ctr); break;
+        case 2: k_split_batch<2><<<grid, BLOCK, 0, s>>>(d_keys, e->tmp.as<uint64_t>(), d_tasks, n, d_seeds, ctr); break;
+        default: k_split_batch<4><<<grid, BLOCK, 0, s>>>(d_keys, e->tmp.as<uint64_t>(), d_tasks, n, d_seeds, ctr); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return SHGPU_OK;
+}
+
+int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *d_hashes, uint64_t n, shgpu_build **out) {
+    using clk = std::chrono::steady_clock;
+    if (!e || !d || !out || (n && !d_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    int rc = check_variant(d->variant, d->leaf_size);
+    if (rc) return rc;
+    if (d->bucket_size == 0) return fail(SHGPU_ERR_ARG, "bucket_size must be positive");
+    if (n >= (1ull << 32)) return fail(SHGPU_ERR_UNSUPPORTED, "more than 2^32 - 1 keys");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    auto t0 = clk::now();
+    shgpu_build *b = new shgpu_build();
+    b->n = n;
+    b->nbuckets = std::max<uint64_t>(1, (n + d->bucket_size - 1) / d->bucket_size);
+    const uint64_t nb = b->nbuckets;
+    struct Guard { shgpu_build *&b; bool ok = false; ~Guard() { if (!ok) { delete b; b = nullptr; } } } guard{b};
+
+    // 1. partition: sort by .first, emit .second in order (Sorter.hpp:15-36), bucket bounds.
+    if ((rc = e->firsts_in.ensure(n * 8 + 8)) || (rc = e->firsts_out.ensure(n * 8 + 8)) ||
+        (rc = e->seconds_in.ensure(n * 8 + 8)) || (rc = e->keys.ensure(n * 8 + 64)) || (rc = e->tmp.ensure(n * 8 + 64)) ||
+        (rc = e->size_acc.ensure((nb + 1) * 8)))
+        return rc;
+    size_t sort_bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, e->firsts_in.as<uint64_t>(), e->firsts_out.as<uint64_t>(),
+                                               e->seconds_in.as<uint64_t>(), e->keys.as<uint64_t>(), (int)n, 0, 64, s));
+    if ((rc = e->sort_tmp.ensure(sort_bytes + 16))) return rc;
+    if (n) {
+        k_unzip_hashes<<<1024, 256, 0, s>>>(d_hashes, n, e->firsts_in.as<uint64_t>(), e->seconds_in.as<uint64_t>());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(e->sort_tmp.p, sort_bytes, e->firsts_in.as<uint64_t>(),
+                                                   e->firsts_out.as<uint64_t>(), e->seconds_in.as<uint64_t>(),
+                                                   e->keys.as<uint64_t>(), (int)n, 0, 64, s));
+    }
+    k_bucket_bounds<<<1024, 256, 0, s>>>(e->firsts_out.as<uint64_t>(), n, nb, e->size_acc.as<uint64_t>());
+    HIP_TRY(hipGetLastError());
+    b->size_acc.resize(nb + 1);
+    HIP_TRY(hipMemcpyAsync(b->size_acc.data(), e->size_acc.p, (nb + 1) * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    auto t1 = clk::now();
+
+    // 2. forest of RecSplit trees, DFS pre-order per bucket, buckets in order.
+    const Strategy st(d->variant, d->leaf_size);
+    uint64_t limit_buckets = nb;
+    if (d->mode == SHGPU_MODE_PLAIN && d->plain_bucket_limit) limit_buckets = std::min<uint64_t>(nb, d->plain_bucket_limit);
+    std::vector<uint32_t> roots;
+    b->nodes.reserve(n / std::max<uint32_t>(1, d->leaf_size) * 3 / 2 + 16);
+    for (uint64_t k = 0; k < limit_buckets; k++) {
+        const uint64_t m = b->size_acc[k + 1] - b->size_acc[k];
+        if (m > MAX_ENCODABLE_BUCKET) return fail(SHGPU_ERR_BUCKET_TOO_LARGE, "bucket " + std::to_string(k) + " holds " + std::to_string(m) + " keys");
+        if (m > 1) {
+            roots.push_back((uint32_t)b->nodes.size());
+            emit_subtree(b->nodes, st, (uint32_t)b->size_acc[k], (uint32_t)m, 0, e->tuning.max_trials);
+        }
+    }
+    const uint32_t total = (uint32_t)b->nodes.size();
+    b->codes.assign(total, 0);
+    if ((rc = e->nodes.ensure((size_t)total * sizeof(shgpu_node) + 64)) || (rc = e->codes.ensure((size_t)total * 8 + 64)) ||
+        (rc = e->queue.ensure((size_t)total * 4 + 64)) || (rc = e->ctrl.ensure(64)))
+        return rc;
+    if (total) HIP_TRY(hipMemcpyAsync(e->nodes.p, b->nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
+    auto t2 = clk::now();
+
+    if ((rc = check_forest(b->nodes, (uint32_t)roots.size()))) return rc;
+
+    // 3. seed search.
+    uint32_t ctrl_out[4] = {0, 0, 0, 0};
+    bool timed_kernel = false;
+    BuildArgs a;
+    a.keys = e->keys.as<uint64_t>();
+    a.tmp = e->tmp.as<uint64_t>();
+    a.nodes = e->nodes.as<shgpu_node>();
+    a.codes = e->codes.as<uint64_t>();
+    a.queue = e->queue.as<uint32_t>();
+    a.ctrl = e->ctrl.as<uint32_t>();
+    a.total = total;
+    a.split_hash = d->variant == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
+    a.scratch = nullptr;
+    if (total) {
+        if (d->mode == SHGPU_MODE_PERSISTENT) {
+            const int grid = grid_for(e);
+            if (d->variant == SHGPU_SHOCKHASH2) {
+                if ((rc = ensure_sh2_scratch(e, (size_t)grid * WAVES_PER_BLOCK))) return rc;
+                a.scratch = e->sh2_scratch.as<uint8_t>();
+            }
+            k_fill_u32<<<256, 256, 0, s>>>(a.queue, EMPTY_SLOT, total);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(a.queue, roots.data(), roots.size() * 4, hipMemcpyHostToDevice, s));
+            const uint32_t ctrl_init[4] = {0, (uint32_t)roots.size(), 0, 0};
+            HIP_TRY(hipMemcpyAsync(a.ctrl, ctrl_init, sizeof(ctrl_init), hipMemcpyHostToDevice, s));
+            HIP_TRY(hipEventRecord(e->ev0, s));
+            if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, a, grid, s);
+            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_build_v<SHGPU_SIMD_SHOCKHASH>(e, a, grid, s);
+            else rc = launch_build_v<SHGPU_SHOCKHASH2>(e, a, grid, s);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(e->ev1, s));
+            timed_kernel = true;
+        } else {
+            if (d->variant == SHGPU_SHOCKHASH2) {
+                if ((rc = ensure_sh2_scratch(e, 1))) return rc;
+                a.scratch = e->sh2_scratch.as<uint8_t>();
+            }
+            for (uint32_t nid = 0; nid < total; nid++) {
+                if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, a, nid, s);
+                else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, a, nid, s);
+                else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, a, nid, s);
+                if (rc) return rc;
+            }
+        }
+        HIP_TRY(hipMemcpyAsync(b->codes.data(), a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(ctrl_out, a.ctrl, sizeof(ctrl_out), hipMemcpyDeviceToHost, s));
+    }
+    if (d->copy_keys && n) {
+        b->keys.resize(n);
+        HIP_TRY(hipMemcpyAsync(b->keys.data(), e->keys.p, n * 8, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    auto t3 = clk::now();
+    if (d->mode == SHGPU_MODE_PERSISTENT && total && ctrl_out[2])
+        return fail(SHGPU_ERR_HIP, "persistent build: a queue slot was never filled (inconsistent forest)");
+    for (uint32_t i = 0; i < total; i++)
+        if (b->codes[i] == SHGPU_NOT_FOUND)
+            return fail(SHGPU_ERR_SEARCH_LIMIT, "node " + std::to_string(i) + " exhausted its seed bound");
+    b->ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    b->ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    b->ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    if (timed_kernel) {
+        float kms = 0;
+        HIP_TRY(hipEventElapsedTime(&kms, e->ev0, e->ev1));
+        b->ms[3] = kms;
+    }
+    guard.ok = true;
+    *out = b;
+    return SHGPU_OK;
+}
+
+int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc, uint64_t nbuckets, shgpu_node *out,
+                      uint64_t capacity, uint64_t *count) {
+    if (!size_acc || !count) return fail(SHGPU_ERR_ARG, "NULL argument");
+    int rc = check_variant(variant, leaf_size);
+    if (rc) return rc;
+    const Strategy st(variant, leaf_size);
+    std::vector<shgpu_node> nodes;
+    uint32_t roots = 0;
+    for (uint64_t k = 0; k < nbuckets; k++) {
+        const uint64_t m = size_acc[k + 1] - size_acc[k];
+        if (m > MAX_ENCODABLE_BUCKET) return fail(SHGPU_ERR_BUCKET_TOO_LARGE, "bucket too large");
+        if (m > 1) { roots++; emit_subtree(nodes, st, (uint32_t)size_acc[k], (uint32_t)m, 0, 1ull << 40); }
+    }
+    if ((rc = check_forest(nodes, roots))) return rc;
+    *count = nodes.size();
+    if (out) {
+        if (capacity < nodes.size()) return fail(SHGPU_ERR_ARG, "capacity too small");
+        std::memcpy(out, nodes.data(), nodes.size() * sizeof(shgpu_node));
+    }
+    return SHGPU_OK;
+}
+
+int shgpu_build_run_host(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *h_hashes, uint64_t n,
+                         shgpu_build **out) {
+    if (!e || !desc || !out || (n && !h_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(e->device));
+    int rc = e->staging.ensure(n * 16 + 16);
+    if (rc) return rc;
+    if (n) HIP_TRY(hipMemcpyAsync(e->staging.p, h_hashes, n * 16, hipMemcpyHostToDevice, e->stream));
+    return shgpu_build_run(e, desc, e->staging.as<uint64_t>(), n, out);
+}
+
+int shgpu_device_alloc(shgpu_engine *e, size_t bytes, void **d_out) {
+    if (!e || !d_out) return fail(SHGPU_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(e->device));
+    if (hipMalloc(d_out, bytes ? bytes : 1) != hipSuccess) return fail(SHGPU_ERR_NOMEM, "hipMalloc failed");
+    return SHGPU_OK;
+}
+
+int shgpu_device_free(shgpu_engine *e, void *d) {
+    if (!e) return fail(SHGPU_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (d) HIP_TRY(hipFree(d));
+    return SHGPU_OK;
+}
+
+int shgpu_copy_to_device(shgpu_engine *e, void *d_dst, const void *h_src, size_t bytes) {
+    if (!e || (bytes && (!d_dst || !h_src))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(e->device));
+    if (bytes) HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return SHGPU_OK;
+}
+
+int shgpu_copy_to_host(shgpu_engine *e, void *h_dst, const void *d_src, size_t bytes) {
+    if (!e || (bytes && (!d_src || !h_dst))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(e->device));
+    if (bytes) HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return SHGPU_OK;
+}
+
+void shgpu_build_free(shgpu_build *b) { delete b; }
+uint64_t shgpu_build_nbuckets(const shgpu_build *b) { return b ? b->nbuckets : 0; }
+uint64_t shgpu_build_nnodes(const shgpu_build *b) { return b ? b->nodes.size() : 0; }
+
+int shgpu_build_bucket_sizes(const shgpu_build *b, uint64_t *size_acc) {
+    if (!b || !size_acc) return fail(SHGPU_ERR_ARG, "NULL argument");
+    std::memcpy(size_acc, b->size_acc.data(), b->size_acc.size() * 8);
+    return SHGPU_OK;
+}
+
+int shgpu_build_nodes(const shgpu_build *b, shgpu_node *nodes, uint64_t *codes) {
+    if (!b) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (nodes) std::memcpy(nodes, b->nodes.data(), b->nodes.size() * sizeof(shgpu_node));
+    if (codes) std::memcpy(codes, b->codes.data(), b->codes.size() * 8);
+    return SHGPU_OK;
+}
+
+int shgpu_build_keys(const shgpu_build *b, uint64_t *keys) {
+    if (!b || !keys) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (b->keys.size() != b->n) return fail(SHGPU_ERR_ARG, "build was run without copy_keys");
+    std::memcpy(keys, b->keys.data(), b->n * 8);
+    return SHGPU_OK;
+}
+
+int shgpu_build_timings(const shgpu_build *b, double *ms4) {
+    if (!b || !ms4) return fail(SHGPU_ERR_ARG, "NULL argument");
+    for (int i = 0; i < 4; i++) ms4[i] = b->ms[i];
+    return SHGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// ShockHash1 leaf seed search for one wavefront (gfx950 wave64).
+//
+// Replaces the leaf loop of ShockHash::recSplit (include/ShockHash.h:404-434), the non-rotation
+// leafLevel of SIMDShockHash (include/SIMDShockHash.hpp:507-542) and the isolated base case
+// BijectionsShockHash1::calculateBijection (benchmark/bijections/ShockHash1.h:13-35). All three
+// return the smallest seed x >= start whose candidate cells
+//   r = remix(key + x); c1 = fastrange32(lo32 r, m/2); c2 = fastrange32(hi32 r, (m+1)/2) + m/2
+// (TinyBinaryCuckooHashTable.h:138-149) cover all m cells and form a pseudoforest (the cuckoo
+// insertion of TinyBinaryCuckooHashTable::construct succeeds exactly then).
+//
+// Mapping: lanes over seeds (S seeds per lane, window of 64*S consecutive seeds), keys broadcast
+// from one VGPR pair with v_readlane. The coverage mask is split into two 32-bit halves (cells
+// [0, m/2) and [m/2, m)), one v_lshl_or each. Candidates that cover every cell go, in seed order,
+// through a wave-cooperative pseudoforest test (lanes over keys, min-label propagation in LDS).
+#pragma once
+#include "device_common.h"
+
+namespace shgpu {
+
+// LDS a wave needs for pseudoforest_wave(): 64 labels + 128 counters.
+struct PfLds {
+    uint32_t lab[64];
+    uint32_t cnt[128];
+};
+
+// Exact pseudoforest test of the multigraph with vertices [0, nv) and one edge (a, b) per active
+// lane (nv <= 64, at most 64 edges). Every component must have #edges <= #vertices. Components are
+// found by min-label propagation with pointer jumping; labels are always vertices of the same
+// component and only decrease, so the loop ends (at most nv*nv rounds, a handful in practice).
+__device__ inline bool pseudoforest_wave(uint32_t a, uint32_t b, bool act, uint32_t nv, PfLds &L) {
+    const uint32_t lane = lane_id();
+    if (lane < nv) { L.lab[lane] = lane; L.cnt[lane] = 0; L.cnt[64 + lane] = 0; }
+    wave_sync();
+    for (uint32_t it = 0; it < 64 * 64; it++) {
+        uint32_t la = 0, lb = 0;
+        if (act) { la = L.lab[a]; lb = L.lab[b]; }
+        const bool diff = act && la != lb;
+        if (ballot(diff) == 0) break;
+        if (diff) {
+            const uint32_t mn = la < lb ? la : lb;
+            atomicMin(&L.lab[a], mn);
+            atomicMin(&L.lab[b], mn);
+            atomicMin(&L.lab[la], mn);
+            atomicMin(&L.lab[lb], mn);
+        }
+        wave_sync();
+        if (lane < nv) {
+            const uint32_t l = L.lab[lane];
+            const uint32_t l2 = L.lab[l];
+            L.lab[lane] = l2;
+        }
+        wave_sync();
+    }
+    if (act) atomicAdd(&L.cnt[L.lab[a]], 1u);            // edges per component
+    if (lane < nv) atomicAdd(&L.cnt[64 + L.lab[lane]], 1u);  // vertices per component
+    wave_sync();
+    const bool bad = lane < nv && L.cnt[lane] > L.cnt[64 + lane];
+    const bool ok = ballot(bad) == 0;
+    wave_sync();
+    return ok;
+}
+
+__device__ __forceinline__ uint32_t mask_bits(uint32_t n) { return n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u); }
+
+// Evaluates one key for S seeds: OR the candidate cells into the two half masks.
+template <int S>
+__device__ __forceinline__ void sh1_eval(uint64_t key, const uint64_t (&seed)[S], uint32_t halfA, uint32_t halfB,
+                                         uint32_t (&mA)[S], uint32_t (&mB)[S]) {
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const uint64_t r = remix(key + seed[s]);
+        mA[s] |= 1u << __umulhi((uint32_t)r, halfA);
+        mB[s] |= 1u << __umulhi((uint32_t)(r >> 32), halfB);
+    }
+}
+
+// Wave-uniform arguments. `keys` points at the leaf's m (<= 64) keys. Returns the absolute seed,
+// or SHGPU_NOT_FOUND when no seed in [start, start + max_trials) qualifies. `evals` (per wave,
+// lane 0 meaningful) accumulates hash evaluations performed.
+template <int S>
+__device__ uint64_t sh1_leaf_search(const uint64_t *keys, uint32_t m, uint64_t start, uint64_t max_trials,
+                                    PfLds &L) {
+    const uint32_t lane = lane_id();
+    if (m < 2) return start;
+    const uint32_t halfA = m / 2, halfB = (m + 1) / 2;
+    const uint32_t fullA = mask_bits(halfA), fullB = mask_bits(halfB);
+    const uint64_t kv = lane < m ? keys[lane] : 0;
+    for (uint64_t base = 0; base < max_trials; base += 64 * S) {
+        uint64_t seed[S];
+        uint32_t mA[S], mB[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) { seed[s] = start + base + (uint64_t)(s * 64 + lane); mA[s] = 0; mB[s] = 0; }
+        if (m == 64) {
+#pragma unroll 16
+            for (uint32_t k = 0; k < 64; k++) {
+                const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
+                                     __builtin_amdgcn_readlane((uint32_t)kv, k);
+                sh1_eval<S>(key, seed, halfA, halfB, mA, mB);
+            }
+        } else {
+            for (uint32_t k = 0; k < m; k++) {
+                const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
+                                     __builtin_amdgcn_readlane((uint32_t)kv, k);
+                sh1_eval<S>(key, seed, halfA, halfB, mA, mB);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            uint64_t cand = ballot(mA[s] == fullA && mB[s] == fullB);
+            while (cand) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(cand);
+                cand &= cand - 1;
+                const uint64_t off = base + (uint64_t)(s * 64 + l);
+                if (off >= max_trials) return SHGPU_NOT_FOUND;
+                const uint64_t x = start + off;
+                uint32_t a = 0, b = 0;
+                if (lane < m) {
+                    const uint64_t r = remix(kv + x);
+                    a = __umulhi((uint32_t)r, halfA);
+                    b = __umulhi((uint32_t)(r >> 32), halfB) + halfA;
+                }
+                if (pseudoforest_wave(a, b, lane < m, m, L)) return x;
+            }
+        }
+    }
+    return SHGPU_NOT_FOUND;
+}
+
+}  // namespace shgpu

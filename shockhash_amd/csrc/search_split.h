@@ -1,0 +1,159 @@
+// RecSplit split-seed search and stable partition for one wavefront (gfx950 wave64).
+//
+// Replaces the split loops of ShockHash::recSplit (include/ShockHash.h:451-548), ShockHash2::recSplit
+// (include/ShockHash2.h:290-387) and SIMDShockHash::higherLevel/aggrLevel
+// (include/SIMDShockHash.hpp:201-365). A node of m keys with part size `unit` searches the smallest
+// seed x >= start such that, with h(key) = hash(key + x) mod-mapped to [0, m), every part but the
+// last receives exactly `unit` keys (kind 0: exactly `unit` keys with h < unit).
+//
+// Hash flavours:
+//   sux  : h = ((remix(key + x) & (2^48 - 1)) * m) >> 48             (sux::remap16 o remix)
+//   SIMD : h = ((fmix32(lo32(key + x) ^ hi32(key + x)) & 0xFFFF) * m) >> 16
+// Part counting is done with cumulative thresholds: part p < fanout-1 holds exactly `unit` keys for
+// all p  <=>  #{h < (p+1)*unit} == (p+1)*unit for all p. Each test h < t is evaluated without the
+// multiply as v < ceil(t * 2^w / m), where v is the w-bit value before scaling (w = 48 or 16).
+// SIMDShockHash counts with packed 8-bit lanes mod 2^32; for every fanout/size the splitting
+// strategies produce (fanout <= 4, part sizes < 256 in the 4-way case) the packed test accepts
+// exactly the seeds whose plain counts match, so plain counts give the same minimal seed.
+//
+// Mapping: lanes over seeds (S seeds per lane, window 64*S), keys staged 64 at a time in one VGPR
+// pair per lane and broadcast with v_readlane; counters live in registers.
+#pragma once
+#include "device_common.h"
+
+namespace shgpu {
+
+struct SplitShapeDev {
+    uint32_t m, unit, fanout, nt;  // nt = number of thresholds (1 for kind 0, fanout-1 otherwise)
+    uint32_t t[3];                 // target cumulative counts
+    uint64_t thr[3];               // thresholds in the pre-scaling domain
+    int simd;
+};
+
+__device__ __forceinline__ SplitShapeDev make_split_shape(uint32_t m, uint32_t unit, uint32_t kind, uint32_t hash) {
+    SplitShapeDev s;
+    s.m = m;
+    s.unit = unit;
+    s.simd = hash == SHGPU_SPLIT_SIMD;
+    s.fanout = kind == SHGPU_NODE_SPLIT2 ? 2u : (m + unit - 1) / unit;
+    s.nt = s.fanout - 1;
+    const uint32_t w = s.simd ? 16 : 48;
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const uint32_t tp = (uint32_t)(p + 1) * unit;
+        s.t[p] = tp;
+        // ceil(tp * 2^w / m); tp < m <= 3000 so tp * 2^48 < 2^60.
+        s.thr[p] = (((uint64_t)tp << w) + m - 1) / m;
+    }
+    return s;
+}
+
+// Value before the final scaling: 48-bit (sux) or 16-bit (SIMD) fraction of the hash.
+__device__ __forceinline__ uint64_t split_value(uint64_t z, int simd) {
+    if (simd) return remix32((uint32_t)(z >> 32) ^ (uint32_t)z) & 0xFFFFu;
+    return remix(z) & 0xFFFFFFFFFFFFull;
+}
+
+template <int S, int NT, int SIMD>
+__device__ __forceinline__ void split_eval(uint64_t key, const uint64_t (&seed)[S], const SplitShapeDev &sh,
+                                           uint32_t (&c)[S][NT]) {
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const uint64_t z = key + seed[s];
+        if (SIMD) {
+            const uint32_t v = remix32((uint32_t)(z >> 32) ^ (uint32_t)z) & 0xFFFFu;
+#pragma unroll
+            for (int p = 0; p < NT; p++) c[s][p] += v < (uint32_t)sh.thr[p] ? 1u : 0u;
+        } else {
+            const uint64_t v = remix(z) & 0xFFFFFFFFFFFFull;
+#pragma unroll
+            for (int p = 0; p < NT; p++) c[s][p] += v < sh.thr[p] ? 1u : 0u;
+        }
+    }
+}
+
+template <int S, int NT, int SIMD>
+__device__ uint64_t split_search_t(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start, uint64_t max_trials) {
+    const uint32_t lane = lane_id();
+    const uint32_t m = sh.m;
+    for (uint64_t base = 0; base < max_trials; base += 64 * S) {
+        uint64_t seed[S];
+        uint32_t c[S][NT];
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            seed[s] = start + base + (uint64_t)(s * 64 + lane);
+#pragma unroll
+            for (int p = 0; p < NT; p++) c[s][p] = 0;
+        }
+        for (uint32_t chunk = 0; chunk < m; chunk += 64) {
+            const uint64_t kv = chunk + lane < m ? keys[chunk + lane] : 0;
+            const uint32_t cnt = m - chunk < 64 ? m - chunk : 64;
+            if (cnt == 64) {
+#pragma unroll 8
+                for (uint32_t k = 0; k < 64; k++) {
+                    const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
+                                         __builtin_amdgcn_readlane((uint32_t)kv, k);
+                    split_eval<S, NT, SIMD>(key, seed, sh, c);
+                }
+            } else {
+                for (uint32_t k = 0; k < cnt; k++) {
+                    const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
+                                         __builtin_amdgcn_readlane((uint32_t)kv, k);
+                    split_eval<S, NT, SIMD>(key, seed, sh, c);
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            bool ok = true;
+#pragma unroll
+            for (int p = 0; p < NT; p++) ok = ok && c[s][p] == sh.t[p];
+            const uint64_t b = ballot(ok);
+            if (b) {
+                const uint64_t off = base + (uint64_t)(s * 64 + __builtin_ctzll(b));
+                return off < max_trials ? start + off : SHGPU_NOT_FOUND;
+            }
+        }
+    }
+    return SHGPU_NOT_FOUND;
+}
+
+template <int S>
+__device__ uint64_t split_search(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start, uint64_t max_trials) {
+    if (sh.simd) {
+        if (sh.nt == 1) return split_search_t<S, 1, 1>(keys, sh, start, max_trials);
+        if (sh.nt == 2) return split_search_t<S, 2, 1>(keys, sh, start, max_trials);
+        return split_search_t<S, 3, 1>(keys, sh, start, max_trials);
+    }
+    if (sh.nt == 1) return split_search_t<S, 1, 0>(keys, sh, start, max_trials);
+    if (sh.nt == 2) return split_search_t<S, 2, 0>(keys, sh, start, max_trials);
+    return split_search_t<S, 3, 0>(keys, sh, start, max_trials);
+}
+
+// Stable counting-sort partition of keys[0..m) by part for seed x into tmp, then copied back
+// (ShockHash.h:464-469,495-499,528-532; SIMDShockHash.hpp:234-252,337-354): part p starts at p*unit.
+__device__ inline void split_partition(uint64_t *keys, uint64_t *tmp, const SplitShapeDev &sh, uint64_t x) {
+    const uint32_t lane = lane_id();
+    const uint32_t m = sh.m;
+    uint32_t base[4] = {0, sh.unit, 2 * sh.unit, 3 * sh.unit};
+    for (uint32_t chunk = 0; chunk < m; chunk += 64) {
+        const bool act = chunk + lane < m;
+        const uint64_t key = act ? keys[chunk + lane] : 0;
+        const uint64_t v = split_value(key + x, sh.simd);
+        uint32_t part = 0;
+#pragma unroll
+        for (uint32_t p = 0; p < 3; p++) part += (p < sh.nt && v >= sh.thr[p]) ? 1u : 0u;
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t p = 0; p < 4; p++) {
+            const uint64_t b = ballot(act && part == p);
+            if (act && part == p) pos = base[p] + lanes_below(b);
+            base[p] += (uint32_t)__builtin_popcountll(b);
+        }
+        if (act) tmp[pos] = key;
+    }
+    __threadfence_block();
+    for (uint32_t i = lane; i < m; i += 64) keys[i] = tmp[i];
+}
+
+}  // namespace shgpu

@@ -1,0 +1,296 @@
+"""ctypes binding of libshockhash_amd.so (C ABI in include/shockhash_amd.h).
+
+Device buffers are allocated through the engine's own C ABI helpers (hipMalloc on the engine's
+device), so the Python side needs only ctypes and numpy. There is no CPU fallback: every entry point
+raises when the library or the GPU is unusable.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import ENGINE_LIB
+
+VARIANTS = {"sh1": 0, "simd": 1, "sh2": 2}
+MODES = {"persistent": 0, "plain": 1}
+NOT_FOUND = (1 << 64) - 1
+
+# Status codes (include/shockhash_amd.h)
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SEARCH_LIMIT, ERR_BUCKET_TOO_LARGE, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(8)
+
+
+class ShgpuError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"shgpu status {status}: {msg}")
+        self.status = status
+
+
+class LeafTask(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint32), ("m", ctypes.c_uint32), ("start_seed", ctypes.c_uint64),
+                ("max_trials", ctypes.c_uint64)]
+
+
+class SplitTask(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint32), ("m", ctypes.c_uint32), ("start_seed", ctypes.c_uint64),
+                ("max_trials", ctypes.c_uint64), ("unit", ctypes.c_uint32), ("kind", ctypes.c_uint8),
+                ("hash", ctypes.c_uint8), ("pad", ctypes.c_uint16)]
+
+
+class Node(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint32), ("m", ctypes.c_uint16), ("kind", ctypes.c_uint8),
+                ("n_children", ctypes.c_uint8), ("unit", ctypes.c_uint32), ("subtree", ctypes.c_uint32),
+                ("start_seed", ctypes.c_uint64), ("max_trials", ctypes.c_uint64)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("sh1_seeds_per_lane", ctypes.c_uint32), ("split_seeds_per_lane", ctypes.c_uint32),
+                ("waves_per_cu", ctypes.c_uint32), ("max_trials", ctypes.c_uint64)]
+
+
+class BuildDesc(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("leaf_size", ctypes.c_uint32), ("bucket_size", ctypes.c_uint64),
+                ("mode", ctypes.c_int), ("plain_bucket_limit", ctypes.c_uint32), ("copy_keys", ctypes.c_int)]
+
+
+LEAF_DTYPE = np.dtype([("offset", "<u4"), ("m", "<u4"), ("start_seed", "<u8"), ("max_trials", "<u8")])
+SPLIT_DTYPE = np.dtype([("offset", "<u4"), ("m", "<u4"), ("start_seed", "<u8"), ("max_trials", "<u8"),
+                        ("unit", "<u4"), ("kind", "u1"), ("hash", "u1"), ("pad", "<u2")])
+NODE_DTYPE = np.dtype([("offset", "<u4"), ("m", "<u2"), ("kind", "u1"), ("n_children", "u1"), ("unit", "<u4"),
+                       ("subtree", "<u4"), ("start_seed", "<u8"), ("max_trials", "<u8")])
+assert LEAF_DTYPE.itemsize == ctypes.sizeof(LeafTask) == 24
+assert SPLIT_DTYPE.itemsize == ctypes.sizeof(SplitTask) == 32
+assert NODE_DTYPE.itemsize == ctypes.sizeof(Node) == 32
+
+P = ctypes.c_void_p
+_SIGS = {
+    "shgpu_last_error": (ctypes.c_char_p, []),
+    "shgpu_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "shgpu_engine_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
+    "shgpu_engine_destroy": (None, [P]),
+    "shgpu_engine_info": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                         ctypes.c_char_p, ctypes.c_size_t]),
+    "shgpu_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
+    "shgpu_set_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
+    "shgpu_sync": (ctypes.c_int, [P]),
+    "shgpu_sh1_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
+    "shgpu_sh2_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
+    "shgpu_split_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
+    "shgpu_build_run": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
+    "shgpu_build_free": (None, [P]),
+    "shgpu_build_nbuckets": (ctypes.c_uint64, [P]),
+    "shgpu_build_nnodes": (ctypes.c_uint64, [P]),
+    "shgpu_build_bucket_sizes": (ctypes.c_int, [P, P]),
+    "shgpu_build_nodes": (ctypes.c_int, [P, P, P]),
+    "shgpu_build_keys": (ctypes.c_int, [P, P]),
+    "shgpu_build_timings": (ctypes.c_int, [P, P]),
+    "shgpu_device_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
+    "shgpu_device_free": (ctypes.c_int, [P, P]),
+    "shgpu_copy_to_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
+    "shgpu_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
+    "shgpu_build_run_host": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
+    "shgpu_plan_forest": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, P, ctypes.c_uint64, P, ctypes.c_uint64,
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libshockhash_amd.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ENGINE_LIB):
+            raise RuntimeError(f"{ENGINE_LIB} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+        lib = ctypes.CDLL(ENGINE_LIB)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(rc):
+    if rc != OK:
+        lib = load()
+        raise ShgpuError(rc, lib.shgpu_last_error().decode(errors="replace"))
+
+
+class DeviceArray:
+    """A device allocation of the engine holding a numpy-typed array."""
+
+    def __init__(self, engine, count, dtype):
+        self.engine = engine
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = P()
+        _check(load().shgpu_device_alloc(engine.h, max(self.nbytes, 1), ctypes.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, engine, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(engine, arr.size, arr.dtype)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes <= self.nbytes
+        _check(load().shgpu_copy_to_device(self.engine.h, self.ptr, arr.ctypes.data_as(P), arr.nbytes))
+
+    def download(self):
+        out = np.empty(self.count, dtype=self.dtype)
+        _check(load().shgpu_copy_to_host(self.engine.h, out.ctypes.data_as(P), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None) and self.engine.h:
+            load().shgpu_device_free(self.engine.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def plan_forest(variant, leaf_size, size_acc):
+    """Host-only tree planning (no GPU): DFS-ordered node records for the given bucket sizes."""
+    lib = load()
+    size_acc = np.ascontiguousarray(size_acc, dtype=np.uint64)
+    nb = len(size_acc) - 1
+    cnt = ctypes.c_uint64()
+    _check(lib.shgpu_plan_forest(VARIANTS[variant], leaf_size, size_acc.ctypes.data_as(P), nb, None, 0,
+                                 ctypes.byref(cnt)))
+    out = np.zeros(cnt.value, dtype=NODE_DTYPE)
+    _check(lib.shgpu_plan_forest(VARIANTS[variant], leaf_size, size_acc.ctypes.data_as(P), nb,
+                                 out.ctypes.data_as(P), cnt.value, ctypes.byref(cnt)))
+    return out
+
+
+class Build:
+    """Result of shgpu_build_run: bucket sizes, DFS-ordered nodes and codes, leaf-ordered keys."""
+
+    def __init__(self, handle):
+        self.h = handle
+        lib = load()
+        self.nbuckets = lib.shgpu_build_nbuckets(handle)
+        nn = lib.shgpu_build_nnodes(handle)
+        self.size_acc = np.zeros(self.nbuckets + 1, dtype=np.uint64)
+        _check(lib.shgpu_build_bucket_sizes(handle, self.size_acc.ctypes.data_as(P)))
+        self.nodes = np.zeros(nn, dtype=NODE_DTYPE)
+        self.codes = np.zeros(nn, dtype=np.uint64)
+        _check(lib.shgpu_build_nodes(handle, self.nodes.ctypes.data_as(P), self.codes.ctypes.data_as(P)))
+        t = np.zeros(4, dtype=np.float64)
+        _check(lib.shgpu_build_timings(handle, t.ctypes.data_as(P)))
+        self.timings_ms = t
+
+    def keys(self, n):
+        out = np.zeros(n, dtype=np.uint64)
+        _check(load().shgpu_build_keys(self.h, out.ctypes.data_as(P)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().shgpu_build_free(self.h)
+            self.h = None
+
+
+class Engine:
+    def __init__(self, device=0):
+        lib = load()
+        h = P()
+        _check(lib.shgpu_engine_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        cu, clk = ctypes.c_int(), ctypes.c_int()
+        arch = ctypes.create_string_buffer(64)
+        _check(lib.shgpu_engine_info(h, ctypes.byref(cu), ctypes.byref(clk), arch, 64))
+        self.cu_count, self.clock_khz, self.arch = cu.value, clk.value, arch.value.decode()
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().shgpu_engine_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def tuning(self):
+        t = Tuning()
+        _check(load().shgpu_get_tuning(self.h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning
+        cur.update(kw)
+        t = Tuning(**cur)
+        _check(load().shgpu_set_tuning(self.h, ctypes.byref(t)))
+
+    def sync(self):
+        _check(load().shgpu_sync(self.h))
+
+    # ---- batched searches on device arrays (DeviceArray)
+    def sh1_leaf_search_dev(self, d_keys, d_tasks, n, d_out):
+        _check(load().shgpu_sh1_leaf_search(self.h, d_keys.ptr, d_tasks.ptr, n, d_out.ptr, None))
+
+    def sh2_leaf_search_dev(self, d_keys, d_tasks, n, d_out):
+        _check(load().shgpu_sh2_leaf_search(self.h, d_keys.ptr, d_tasks.ptr, n, d_out.ptr, None))
+
+    def split_search_dev(self, d_keys, d_tasks, n, d_out):
+        _check(load().shgpu_split_search(self.h, d_keys.ptr, d_tasks.ptr, n, d_out.ptr, None))
+
+    def array(self, host_array):
+        return DeviceArray.from_host(self, host_array)
+
+    # ---- host-array conveniences (copy in, run, copy out)
+    def _stage(self, keys, tasks):
+        d_keys = DeviceArray.from_host(self, np.asarray(keys, dtype=np.uint64))
+        d_tasks = DeviceArray.from_host(self, np.asarray(tasks))
+        d_out = DeviceArray(self, len(tasks), np.uint64)
+        return d_keys, d_tasks, d_out
+
+    def sh1_leaf_search(self, keys, offsets, sizes, starts, max_trials=1 << 40):
+        tasks = np.zeros(len(sizes), dtype=LEAF_DTYPE)
+        tasks["offset"], tasks["m"], tasks["start_seed"], tasks["max_trials"] = offsets, sizes, starts, max_trials
+        d_keys, d_tasks, d_out = self._stage(keys, tasks)
+        self.sh1_leaf_search_dev(d_keys, d_tasks, len(tasks), d_out)
+        self.sync()
+        return d_out.download()
+
+    def sh2_leaf_search(self, keys, offsets, sizes):
+        tasks = np.zeros(len(sizes), dtype=LEAF_DTYPE)
+        tasks["offset"], tasks["m"] = offsets, sizes
+        d_keys, d_tasks, d_out = self._stage(keys, tasks)
+        self.sh2_leaf_search_dev(d_keys, d_tasks, len(tasks), d_out)
+        self.sync()
+        return d_out.download()
+
+    def split_search(self, keys, tasks):
+        """tasks: SPLIT_DTYPE array. Returns (seeds, permuted keys)."""
+        d_keys, d_tasks, d_out = self._stage(keys, tasks)
+        self.split_search_dev(d_keys, d_tasks, len(tasks), d_out)
+        self.sync()
+        return d_out.download(), d_keys.download()
+
+    def build_dev(self, d_hashes, n, variant, leaf_size, bucket_size, mode="persistent", plain_bucket_limit=0,
+                  copy_keys=True):
+        desc = BuildDesc(VARIANTS[variant], leaf_size, bucket_size, MODES[mode], plain_bucket_limit, int(copy_keys))
+        h = P()
+        _check(load().shgpu_build_run(self.h, ctypes.byref(desc), d_hashes.ptr, n, ctypes.byref(h)))
+        return Build(h)
+
+    def build(self, hashes, variant, leaf_size, bucket_size, **kw):
+        hashes = np.ascontiguousarray(hashes, dtype=np.uint64).reshape(-1, 2)
+        d = DeviceArray.from_host(self, hashes.reshape(-1))
+        try:
+            return self.build_dev(d, hashes.shape[0], variant, leaf_size, bucket_size, **kw)
+        finally:
+            d.free()

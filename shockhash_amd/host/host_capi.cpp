@@ -1,0 +1,165 @@
+// libshockhash_host.so: extern "C" handles on instantiations of the drop-in class templates
+// (ShockHash, SIMDShockHash, ShockHash2), used by the Python tests and the benchmark.
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../include/SIMDShockHash.hpp"
+#include "../include/ShockHash.h"
+#include "../include/ShockHash2.h"
+
+using namespace shockhash;
+
+namespace {
+
+struct Handle {
+    virtual ~Handle() = default;
+    virtual size_t query(const hash128_t &h) = 0;
+    virtual size_t query_string(const std::string &s) = 0;
+    virtual const std::vector<uint64_t> &words() const = 0;
+    virtual const std::vector<std::pair<uint64_t, uint8_t>> &ribbon() const = 0;
+    virtual const gpu::Forest &forest() const = 0;
+    virtual size_t bits() const = 0;
+    virtual double host_ms() const = 0;
+};
+
+template <class T>
+struct HandleT : Handle {
+    std::unique_ptr<T> obj;
+    explicit HandleT(T *o) : obj(o) {}
+    size_t query(const hash128_t &h) override { return (*obj)(h); }
+    size_t query_string(const std::string &s) override { return (*obj)(s); }
+    const std::vector<uint64_t> &words() const override { return obj->descriptorWords(); }
+    const std::vector<std::pair<uint64_t, uint8_t>> &ribbon() const override { return obj->ribbonInput(); }
+    const gpu::Forest &forest() const override { return obj->forest(); }
+    size_t bits() const override { return obj->getBits(); }
+    double host_ms() const override { return obj->hostEncodeMs(); }
+};
+
+template <class T>
+Handle *make_h(std::vector<hash128_t> &h, size_t bucket, size_t threads) {
+    return new HandleT<T>(new T(h, bucket, threads));
+}
+template <class T>
+Handle *make_s(const std::vector<std::string> &s, size_t bucket, size_t threads) {
+    return new HandleT<T>(new T(s, bucket, threads));
+}
+
+#define SHH_LEAVES_SH1(X) X(2) X(4) X(8) X(16) X(24) X(30)
+#define SHH_LEAVES_SIMD(X) X(8) X(16) X(24) X(58)
+#define SHH_LEAVES_SH2(X) X(4) X(8) X(16) X(24) X(32) X(64)
+
+template <class Keys, class MakeSH1, class MakeSIMD, class MakeSH2>
+Handle *dispatch(int variant, int leaf, Keys &keys, size_t bucket, size_t threads) {
+    if (variant == SHGPU_SHOCKHASH) {
+#define C(L) if (leaf == L) return MakeSH1::template make<ShockHash<L, false>>(keys, bucket, threads);
+        SHH_LEAVES_SH1(C)
+#undef C
+    } else if (variant == SHGPU_SIMD_SHOCKHASH) {
+#define C(L) if (leaf == L) return MakeSIMD::template make<SIMDShockHash<L, false>>(keys, bucket, threads);
+        SHH_LEAVES_SIMD(C)
+#undef C
+    } else if (variant == SHGPU_SHOCKHASH2) {
+#define C(L) if (leaf == L) return MakeSH2::template make<ShockHash2<L>>(keys, bucket, threads);
+        SHH_LEAVES_SH2(C)
+#undef C
+    }
+    throw gpu::Error(SHGPU_ERR_UNSUPPORTED, "leaf size " + std::to_string(leaf) + " not instantiated in libshockhash_host");
+}
+
+struct ByHash {
+    template <class T> static Handle *make(std::vector<hash128_t> &h, size_t b, size_t t) {
+        if constexpr (std::is_constructible_v<T, std::vector<hash128_t> &, size_t, size_t>) return make_h<T>(h, b, t);
+        else { T *o = new T(std::span<hash128_t>(h.data(), h.size()), b, t); return new HandleT<T>(o); }
+    }
+};
+struct ByString {
+    template <class T> static Handle *make(std::vector<std::string> &s, size_t b, size_t t) {
+        if constexpr (std::is_constructible_v<T, const std::vector<std::string> &, size_t, size_t>) return make_s<T>(s, b, t);
+        else { T *o = new T(std::span<const std::string>(s.data(), s.size()), b, t); return new HandleT<T>(o); }
+    }
+};
+
+void set_err(char *err, size_t len, const char *msg) {
+    if (err && len) std::snprintf(err, len, "%s", msg);
+}
+
+}  // namespace
+
+extern "C" {
+
+void *shh_build(int variant, int leaf, const uint64_t *hashes, size_t n, size_t bucket, int threads, char *err,
+                size_t errlen) {
+    try {
+        std::vector<hash128_t> h(n);
+        std::memcpy(h.data(), hashes, n * 16);
+        return dispatch<std::vector<hash128_t>, ByHash, ByHash, ByHash>(variant, leaf, h, bucket, (size_t)threads);
+    } catch (const std::exception &e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
+
+void *shh_build_strings(int variant, int leaf, const char *data, const uint64_t *offsets, size_t n, size_t bucket,
+                        int threads, char *err, size_t errlen) {
+    try {
+        std::vector<std::string> s(n);
+        for (size_t i = 0; i < n; i++) s[i].assign(data + offsets[i], offsets[i + 1] - offsets[i]);
+        return dispatch<std::vector<std::string>, ByString, ByString, ByString>(variant, leaf, s, bucket, (size_t)threads);
+    } catch (const std::exception &e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
+
+void shh_free(void *h) { delete (Handle *)h; }
+
+void shh_query(void *h, const uint64_t *hashes, size_t n, uint64_t *out) {
+    Handle *x = (Handle *)h;
+    for (size_t i = 0; i < n; i++) out[i] = x->query(hash128_t(hashes[2 * i], hashes[2 * i + 1]));
+}
+
+void shh_query_strings(void *h, const char *data, const uint64_t *offsets, size_t n, uint64_t *out) {
+    Handle *x = (Handle *)h;
+    for (size_t i = 0; i < n; i++) out[i] = x->query_string(std::string(data + offsets[i], offsets[i + 1] - offsets[i]));
+}
+
+size_t shh_nwords(void *h) { return ((Handle *)h)->words().size(); }
+void shh_words(void *h, uint64_t *out) {
+    const auto &w = ((Handle *)h)->words();
+    std::memcpy(out, w.data(), w.size() * 8);
+}
+size_t shh_ribbon_size(void *h) { return ((Handle *)h)->ribbon().size(); }
+void shh_ribbon(void *h, uint64_t *keys, uint8_t *bits) {
+    const auto &r = ((Handle *)h)->ribbon();
+    for (size_t i = 0; i < r.size(); i++) { keys[i] = r[i].first; bits[i] = r[i].second; }
+}
+size_t shh_nnodes(void *h) { return ((Handle *)h)->forest().codes.size(); }
+void shh_codes(void *h, uint64_t *out) {
+    const auto &c = ((Handle *)h)->forest().codes;
+    std::memcpy(out, c.data(), c.size() * 8);
+}
+size_t shh_bits(void *h) { return ((Handle *)h)->bits(); }
+void shh_timings(void *h, double *out4) {
+    const auto &f = ((Handle *)h)->forest();
+    out4[0] = f.ms[0]; out4[1] = f.ms[1]; out4[2] = f.ms[2];
+    out4[3] = ((Handle *)h)->host_ms();
+}
+
+// Compile-time Golomb-Rice memo of a class (for comparison with the oracle's run-time formula).
+int shh_memo(int variant, int leaf, uint64_t *out) {
+    if (variant == SHGPU_SHOCKHASH || variant == SHGPU_SIMD_SHOCKHASH) {
+#define C(L) if (leaf == L) { auto m = golomb_memo<L, false>(); std::memcpy(out, m.data(), m.size() * 8); return 0; }
+        SHH_LEAVES_SH1(C) SHH_LEAVES_SIMD(C)
+#undef C
+    } else {
+#define C(L) if (leaf == L) { auto m = golomb_memo<L, true>(); std::memcpy(out, m.data(), m.size() * 8); return 0; }
+        SHH_LEAVES_SH2(C)
+#undef C
+    }
+    return 1;
+}
+
+}  // extern "C"

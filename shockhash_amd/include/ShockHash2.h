@@ -1,0 +1,30 @@
+// shockhash::ShockHash2<LEAF_SIZE> (bipartite ShockHash) backed by the MI355X engine.
+// Same constructors, const operator() and descriptor layout as the reference class
+// (include/ShockHash2.h:117-532). The GPU engine's ShockHash2 leaf search covers leaf sizes up to 64.
+#pragma once
+#include <span>
+#include <string>
+#include <vector>
+
+#include "shockhash/GpuRecSplit.h"
+
+namespace shockhash {
+
+template <size_t LEAF_SIZE>
+class ShockHash2 : public detail::GpuRecSplit<SHGPU_SHOCKHASH2, LEAF_SIZE> {
+    static_assert(LEAF_SIZE <= MAX_LEAF_SIZE2);
+
+  public:
+    ShockHash2() {}
+    ShockHash2(std::span<const std::string> keys, const size_t bucket_size, size_t num_threads = 1) {
+        const std::vector<hash128_t> h = firstHashes(keys, num_threads);
+        this->construct(h.data(), h.size(), bucket_size, num_threads);
+    }
+    ShockHash2(std::span<hash128_t> keys, const size_t bucket_size, size_t num_threads = 1) {
+        this->construct(keys.data(), keys.size(), bucket_size, num_threads);
+    }
+    size_t operator()(const hash128_t &hash) const { return this->query(hash); }
+    size_t operator()(const std::string &key) const { return this->query(spooky(key.c_str(), key.size(), 0)); }
+};
+
+}  // namespace shockhash

@@ -1,0 +1,83 @@
+// Host-side handle of the MI355X seed-search engine (libshockhash_amd.so, C ABI in
+// include/shockhash_amd.h). One engine per process and device, created on first use.
+#pragma once
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/shockhash_amd.h"
+#include "Hashing.h"
+
+namespace shockhash {
+namespace gpu {
+
+class Error : public std::runtime_error {
+  public:
+    int status;
+    Error(int st, const std::string &what) : std::runtime_error(what), status(st) {}
+};
+
+inline void check(int rc, const char *what) {
+    if (rc != SHGPU_OK)
+        throw Error(rc, std::string(what) + ": " + shgpu_status_string(rc) + ": " + shgpu_last_error());
+}
+
+// Seed-search results of one construction, everything the host encoder needs.
+struct Forest {
+    uint64_t nbuckets = 0;
+    std::vector<uint64_t> size_acc;  // nbuckets + 1
+    std::vector<shgpu_node> nodes;   // DFS pre-order per bucket, buckets in order
+    std::vector<uint64_t> codes;     // per node: found seed - start seed (ShockHash2 leaves: pair code)
+    std::vector<uint64_t> keys;      // fingerprint .second in final leaf order
+    double ms[4] = {0, 0, 0, 0};     // sort + bucketing, tree setup, seed search, search kernel
+};
+
+class Engine {
+    shgpu_engine *e = nullptr;
+    std::mutex mtx;
+
+  public:
+    explicit Engine(int device = 0) { check(shgpu_engine_create(device, &e), "shgpu_engine_create"); }
+    ~Engine() { shgpu_engine_destroy(e); }
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    static Engine &instance() {
+        static Engine eng(0);
+        return eng;
+    }
+
+    shgpu_engine *handle() { return e; }
+
+    Forest build(int variant, uint32_t leaf_size, const hash128_t *hashes, uint64_t n, uint64_t bucket_size) {
+        static_assert(sizeof(hash128_t) == 16, "hash128_t must be two packed 64-bit words");
+        std::lock_guard<std::mutex> lock(mtx);
+        shgpu_build_desc d{};
+        d.variant = variant;
+        d.leaf_size = leaf_size;
+        d.bucket_size = bucket_size;
+        d.mode = SHGPU_MODE_PERSISTENT;
+        d.copy_keys = 1;
+        shgpu_build *b = nullptr;
+        check(shgpu_build_run_host(e, &d, reinterpret_cast<const uint64_t *>(hashes), n, &b), "shgpu_build_run");
+        Forest f;
+        f.nbuckets = shgpu_build_nbuckets(b);
+        f.size_acc.resize(f.nbuckets + 1);
+        const uint64_t nn = shgpu_build_nnodes(b);
+        f.nodes.resize(nn);
+        f.codes.resize(nn);
+        f.keys.resize(n);
+        int rc = shgpu_build_bucket_sizes(b, f.size_acc.data());
+        if (rc == SHGPU_OK) rc = shgpu_build_nodes(b, f.nodes.data(), f.codes.data());
+        if (rc == SHGPU_OK && n) rc = shgpu_build_keys(b, f.keys.data());
+        if (rc == SHGPU_OK) rc = shgpu_build_timings(b, f.ms);
+        shgpu_build_free(b);
+        check(rc, "shgpu_build results");
+        return f;
+    }
+};
+
+}  // namespace gpu
+}  // namespace shockhash

@@ -1,0 +1,136 @@
+// Common body of the three drop-in class templates: construction through the MI355X engine, host
+// encoding, and the RecSplit query walk.
+#pragma once
+#include <chrono>
+#include <cstdint>
+#include <iostream>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "RecSplitHost.h"
+
+namespace shockhash {
+namespace detail {
+
+template <int VARIANT, size_t LEAF_SIZE>
+class GpuRecSplit {
+  protected:
+    static constexpr bool BIPARTITE = VARIANT == SHGPU_SHOCKHASH2;
+    static constexpr size_t _leaf = LEAF_SIZE;
+    static constexpr size_t lower_aggr = Splitting<LEAF_SIZE, BIPARTITE>::lower_aggr;
+    static constexpr size_t upper_aggr = Splitting<LEAF_SIZE, BIPARTITE>::upper_aggr;
+    static constexpr std::array<uint64_t, MAX_BUCKET_SIZE> memo = golomb_memo<LEAF_SIZE, BIPARTITE>();
+    static int golomb_param(size_t m) { return (int)(memo[m] >> 27); }
+    static uint64_t skip_bits(size_t m) { return memo[m] & 0xFFFF; }
+    static uint64_t skip_nodes(size_t m) { return (memo[m] >> 16) & 0x7FF; }
+
+    size_t bucket_size = 0;
+    size_t nbuckets = 0;
+    size_t keys_count = 0;
+    RiceBitVector descriptors;
+    DoubleEF ef;
+    BitRetrieval ribbon;
+    std::vector<std::pair<uint64_t, uint8_t>> ribbon_input;  // kept for verification
+    gpu::Forest forest_meta;                                  // codes and nodes, kept for verification
+    double host_ms = 0;
+
+    void construct(const hash128_t *h, size_t n, size_t bsize, size_t num_threads) {
+        bucket_size = bsize;
+        keys_count = n;
+        gpu::Forest f = gpu::Engine::instance().build(VARIANT, (uint32_t)LEAF_SIZE, h, n, bsize);
+        const auto t0 = std::chrono::steady_clock::now();
+        nbuckets = f.nbuckets;
+        std::vector<uint64_t> words, pos_acc;
+        encodeForest(f, memo, BIPARTITE ? LeafKind::ShockHash2 : LeafKind::ShockHash1, num_threads, words, pos_acc,
+                     ribbon_input);
+        descriptors = RiceBitVector(std::move(words));
+        ef = DoubleEF(f.size_acc, pos_acc);
+        ribbon = BitRetrieval(ribbon_input);
+        f.keys.clear();
+        f.keys.shrink_to_fit();
+        forest_meta = std::move(f);
+        host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    uint64_t split_hash(uint64_t x, size_t m) const {
+        if (VARIANT == SHGPU_SIMD_SHOCKHASH) return remixAndRemap(x, (uint32_t)m);
+        return remap16(remix(x), m);
+    }
+
+  public:
+    size_t query(const hash128_t &hash) const {
+        const size_t bucket = remap128(hash.first, nbuckets);
+        uint64_t cum_keys, cum_keys_next, bit_pos;
+        ef.get(bucket, cum_keys, cum_keys_next, bit_pos);
+        size_t m = cum_keys_next - cum_keys;
+        if (m <= 1) return cum_keys;
+        auto reader = descriptors.reader();
+        reader.readReset(bit_pos, skip_bits(m));
+        int level = 0;
+        while (m > upper_aggr) {  // fanout 2
+            const uint64_t d = reader.readNext(golomb_param(m));
+            const uint64_t hmod = split_hash(hash.second + d + start_seed[level], m);
+            const uint32_t split = ((uint16_t(m / 2 + upper_aggr - 1) / upper_aggr)) * upper_aggr;
+            if (hmod < split) {
+                m = split;
+            } else {
+                reader.skipSubtree(skip_nodes(split), skip_bits(split));
+                m -= split;
+                cum_keys += split;
+            }
+            level++;
+        }
+        if (m > lower_aggr) {
+            const uint64_t d = reader.readNext(golomb_param(m));
+            const uint64_t s = VARIANT == SHGPU_SIMD_SHOCKHASH ? start_seed[NUM_START_SEEDS - 3] : start_seed[level];
+            const uint64_t hmod = split_hash(hash.second + d + s, m);
+            const uint64_t part = uint16_t(hmod) / lower_aggr;
+            m = std::min<size_t>(lower_aggr, m - part * lower_aggr);
+            cum_keys += lower_aggr * part;
+            if (part) reader.skipSubtree(skip_nodes(lower_aggr) * part, skip_bits(lower_aggr) * part);
+            if (VARIANT != SHGPU_SIMD_SHOCKHASH) level++;
+        }
+        if (m > _leaf) {
+            const uint64_t d = reader.readNext(golomb_param(m));
+            const uint64_t s = VARIANT == SHGPU_SIMD_SHOCKHASH ? start_seed[NUM_START_SEEDS - 2] : start_seed[level];
+            const uint64_t hmod = split_hash(hash.second + d + s, m);
+            const uint64_t part = uint16_t(hmod) / _leaf;
+            m = std::min<size_t>(_leaf, m - part * _leaf);
+            cum_keys += _leaf * part;
+            if (part) reader.skipSubtree(part, skip_bits(_leaf) * part);
+            if (VARIANT != SHGPU_SIMD_SHOCKHASH) level++;
+        }
+        const uint64_t b = reader.readNext(golomb_param(m));
+        const uint64_t retrieved = ribbon.retrieve(hash.second);
+        if (BIPARTITE) return cum_keys + sh2Cell((uint32_t)m, b, hash.second, retrieved);
+        if (m <= 1) return cum_keys;
+        const uint64_t seed = b + (VARIANT == SHGPU_SIMD_SHOCKHASH ? start_seed[NUM_START_SEEDS - 1] : start_seed[level]);
+        const CandidateCells c = sh1CandidateCells(hash.second, seed, (uint32_t)m);
+        return cum_keys + (retrieved == 0 ? c.cell1 : c.cell2);
+    }
+
+    size_t size() const { return keys_count; }
+    size_t getBits() const {
+        return ef.bitCountCumKeys() + ef.bitCountPosition() + descriptors.getBits() + 8 * ribbon.sizeBytes() +
+               8 * sizeof(*this);
+    }
+    void printBits() const {
+        std::cout << "EF 1:   " << (double)ef.bitCountCumKeys() / keys_count << std::endl;
+        std::cout << "EF 2:   " << (double)ef.bitCountPosition() / keys_count << std::endl;
+        std::cout << "trees:  " << (double)descriptors.getBits() / keys_count << std::endl;
+        std::cout << "ribbon: " << (double)(8 * ribbon.sizeBytes()) / keys_count << std::endl;
+    }
+
+    // Verification accessors (not in the reference interface).
+    const std::vector<uint64_t> &descriptorWords() const { return descriptors.words(); }
+    const std::vector<std::pair<uint64_t, uint8_t>> &ribbonInput() const { return ribbon_input; }
+    const gpu::Forest &forest() const { return forest_meta; }
+    size_t bucketCount() const { return nbuckets; }
+    double hostEncodeMs() const { return host_ms; }
+};
+
+}  // namespace detail
+}  // namespace shockhash

@@ -1,0 +1,113 @@
+"""GPU engine vs the CPU oracle, bit-exact, through the C ABI (libshockhash_amd.so)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from shockhash_amd import workload
+from shockhash_amd.engine import SPLIT_DTYPE
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def test_sh1_tiny_leaf_first(engine):
+    """One small leaf first: a minimal end-to-end call of the kernel."""
+    log("sh1 tiny leaf")
+    keys = workload.isolated_leaves(1, 8, seed=10)
+    want = O.sh1_leaf_batch(keys, start=0, threads=1)
+    got = engine.sh1_leaf_search(keys.reshape(-1), [0], [8], np.zeros(1, np.uint64), max_trials=1 << 20)
+    log(f"  got {got} want {want}")
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("S", [1, 2, 4])
+def test_sh1_isolated_leaves_n40(engine, S):
+    """Isolated base case (bijections/ShockHash1.h semantics: seed from 0), n = 40."""
+    log(f"sh1 isolated n=40 S={S}")
+    engine.set_tuning(sh1_seeds_per_lane=S)
+    keys = workload.isolated_leaves(48, 40, seed=11)
+    want = O.sh1_leaf_batch(keys, start=0, threads=8)
+    log("  oracle done")
+    got = engine.sh1_leaf_search(keys.reshape(-1), np.arange(48) * 40, np.full(48, 40), np.zeros(48, np.uint64),
+                                 max_trials=1 << 26)
+    log("  gpu done")
+    engine.set_tuning(sh1_seeds_per_lane=4)
+    assert np.array_equal(got, want)
+
+
+def test_sh1_leaves_all_sizes(engine):
+    """Leaves of every size 1..60 with the per-level start seeds used inside the trees."""
+    log("sh1 all sizes")
+    rng = np.random.default_rng(5)
+    sizes = np.array([m for m in range(1, 61) for _ in range(2)], dtype=np.uint32)
+    sizes = sizes[sizes <= 34]  # larger leaves are covered by the n=40 batch and the builds
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint32)
+    keys = workload.fingerprints(int(sizes.sum()), seed=12)[:, 1]
+    starts = np.array([O.lib().orc_start_seed(int(rng.integers(0, 20))) for _ in sizes], dtype=np.uint64)
+    got = engine.sh1_leaf_search(keys, offsets, sizes, starts, max_trials=1 << 26)
+    want = np.array([O.sh1_leaf(keys[o:o + m], int(s)) if m >= 2 else int(s)
+                     for o, m, s in zip(offsets, sizes, starts)], dtype=np.uint64)
+    assert np.array_equal(got, want)
+
+
+def test_sh2_leaves_all_sizes(engine):
+    """ShockHash2 leaves of every size 2..64 (Basic finder below 10, QuadSplit from 10)."""
+    log("sh2 all sizes")
+    sizes = np.array(list(range(2, 65)) * 2 + [64] * 32, dtype=np.uint32)
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint32)
+    keys = workload.fingerprints(int(sizes.sum()), seed=13)[:, 1]
+    want = O.sh2_leaf_batch(keys, offsets, sizes, threads=8)
+    got = engine.sh2_leaf_search(keys, offsets, sizes)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, f"mismatch at sizes {sizes[bad][:10]} got {got[bad][:5]} want {want[bad][:5]}"
+
+
+@pytest.mark.parametrize("variant,leaf", [(0, 64), (1, 58), (0, 30)])
+def test_split_search(engine, variant, leaf):
+    """Split nodes of every kind: seeds and stable partitions equal the oracle's."""
+    log(f"split variant={variant} leaf={leaf}")
+    rng = np.random.default_rng(7 + variant)
+    # (m, level) covering fanout-2, upper and lower aggregation nodes of the leaf size.
+    sizes = [2000, 1536, 768, 700, 464, 256, 230, 200, 130, 100, 70]
+    tasks, chunks, expect_codes, expect_keys = [], [], [], []
+    off = 0
+    for m in sizes:
+        level = int(rng.integers(0, 5))
+        keys = workload.fingerprints(m, seed=100 + m)[:, 1]
+        st_variant = 2 if (variant == 0 and leaf == 64) else variant
+        # oracle split: variant 0/2 -> sux flavour (ShockHash, ShockHash2), 1 -> SIMD flavour
+        code, perm, kind, fanout, unit, start = O.split(st_variant, leaf, level, keys)
+        if fanout > 4 or m <= leaf:
+            continue
+        t = np.zeros(1, dtype=SPLIT_DTYPE)
+        t["offset"], t["m"], t["start_seed"], t["max_trials"] = off, m, start, 1 << 40
+        t["unit"], t["kind"], t["hash"] = unit, kind, 1 if variant == 1 else 0
+        tasks.append(t)
+        chunks.append(keys)
+        expect_codes.append(code)
+        expect_keys.append(perm)
+        off += m
+    tasks = np.concatenate(tasks)
+    tasks["max_trials"] = 1 << 26
+    seeds, perm = engine.split_search(np.concatenate(chunks), tasks)
+    assert np.array_equal(seeds - tasks["start_seed"], np.array(expect_codes, dtype=np.uint64))
+    assert np.array_equal(perm, np.concatenate(expect_keys))
+
+
+@pytest.mark.parametrize("variant,leaf,n", [("sh2", 64, 30000), ("sh1", 30, 20000), ("simd", 16, 20000),
+                                            ("sh2", 16, 20000), ("sh1", 8, 5000)])
+@pytest.mark.parametrize("mode", ["persistent", "plain"])
+def test_build_matches_oracle(engine, variant, leaf, n, mode):
+    """Whole construction: bucket sizes, every node code (DFS order) and the final key order."""
+    log(f"build {variant} L={leaf} n={n} mode={mode}")
+    engine.set_tuning(max_trials=1 << 26)
+    h = workload.fingerprints(n, seed=21)
+    ob = O.OracleBuild(variant, leaf, h, 2000, threads=8)
+    b = engine.build(h, variant, leaf, 2000, mode=mode)
+    assert np.array_equal(b.size_acc, ob.size_acc)
+    assert np.array_equal(b.nodes["m"], ob.node_m)
+    assert np.array_equal(b.codes, ob.node_code)
+    log(f"  ok, timings {b.timings_ms}")

@@ -1,0 +1,107 @@
+"""Pins the oracle's primitives with published vectors and closed forms (no GPU)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")))
+M64 = (1 << 64) - 1
+
+
+def test_remix_is_splitmix64_finaliser():
+    L = O.lib()
+    s = 1234567
+    for want in GOLD["splitmix64"]["seed_1234567"]:
+        s = (s + 0x9E3779B97F4A7C15) & M64
+        assert L.orc_remix(s) == int(want)
+    assert L.orc_remix(0x9E3779B97F4A7C15) == int(GOLD["splitmix64"]["seed_0_first"], 16)
+
+
+def murmur3_32(data: bytes, seed: int, fmix) -> int:
+    """MurmurHash3_x86_32 body; its finaliser is the oracle function under test."""
+    c1, c2 = 0xCC9E2D51, 0x1B873593
+    h = seed & 0xFFFFFFFF
+    n = len(data) // 4
+    rotl = lambda x, r: ((x << r) | (x >> (32 - r))) & 0xFFFFFFFF
+    for i in range(n):
+        k = int.from_bytes(data[4 * i:4 * i + 4], "little")
+        k = (k * c1) & 0xFFFFFFFF
+        k = rotl(k, 15)
+        k = (k * c2) & 0xFFFFFFFF
+        h ^= k
+        h = rotl(h, 13)
+        h = (h * 5 + 0xE6546B64) & 0xFFFFFFFF
+    tail = data[4 * n:]
+    k = 0
+    if len(tail) >= 3:
+        k ^= tail[2] << 16
+    if len(tail) >= 2:
+        k ^= tail[1] << 8
+    if len(tail) >= 1:
+        k ^= tail[0]
+        k = (k * c1) & 0xFFFFFFFF
+        k = rotl(k, 15)
+        k = (k * c2) & 0xFFFFFFFF
+        h ^= k
+    h ^= len(data)
+    return fmix(h)
+
+
+def test_remix32_is_murmur3_fmix32():
+    L = O.lib()
+    for c in GOLD["murmur3_x86_32"]["cases"]:
+        assert murmur3_32(bytes.fromhex(c["hex"]), c["seed"], L.orc_remix32) == int(c["out"], 16), c
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_range_reductions_closed_forms(seed):
+    L = O.lib()
+    rng = np.random.default_rng(seed)
+    xs = [int(v) for v in rng.integers(0, 2**63, size=200, dtype=np.uint64)] + [0, M64, 1 << 63, (1 << 48) - 1]
+    ns = [1, 2, 3, 7, 32, 64, 1000, 2999, 3000]
+    for x in xs:
+        x2 = (x * 2 + 1) & M64
+        for n in ns:
+            assert L.orc_fastrange64(x2, n) == (x2 * n) >> 64
+            assert L.orc_remap128(x2, n) == (x2 * n) >> 64
+            assert L.orc_remap16(x2, n) == ((x2 & ((1 << 48) - 1)) * n) >> 48 < max(n, 1)
+            lo = x2 & 0xFFFFFFFF
+            assert L.orc_fastrange32(lo, n) == (lo * n) >> 32
+            r32 = L.orc_remix32((x2 >> 32) ^ lo)
+            assert L.orc_remix_and_remap(x2, n) == ((r32 & 0xFFFF) * n) >> 16
+
+
+def test_pairing_tables_and_round_trips():
+    L = O.lib()
+    for x, y, z in GOLD["pairing_tables"]["triangular"]:
+        assert L.orc_pair_triangular(x, y) == z
+    for x, y, z in GOLD["pairing_tables"]["elegant"]:
+        assert L.orc_pair_elegant(x, y) == z
+    import ctypes
+    a, b = ctypes.c_uint64(), ctypes.c_uint64()
+    for x in range(1, 300):
+        for y in range(0, x):
+            z = L.orc_pair_triangular(x, y)
+            L.orc_unpair_triangular(z, ctypes.byref(a), ctypes.byref(b))
+            assert (a.value, b.value) == (x, y)
+    for x in range(0, 200):
+        for y in range(0, 200):
+            z = L.orc_pair_elegant(x, y)
+            L.orc_unpair_elegant(z, ctypes.byref(a), ctypes.byref(b))
+            assert (a.value, b.value) == (x, y)
+    # triangular codes are lexicographic in (x, y): the first success in (j, i) order is the minimum
+    codes = [L.orc_pair_triangular(x, y) for x in range(1, 60) for y in range(x)]
+    assert codes == sorted(codes) and len(set(codes)) == len(codes)
+    # large values stay exact where the reference's float checks would pass
+    assert L.orc_float_accurate_triangular(1 << 24, 12345) and L.orc_float_accurate_elegant(1 << 20, 77)
+
+
+def test_spooky_readme_kat_end_to_end():
+    """README known answer through the whole oracle (SpookyV2 + bucketing + ShockHash1 leaf + orientation)."""
+    kat = GOLD["readme_kat"]
+    h = np.array([O.spooky128(k.encode(), 0) for k in kat["keys"]], dtype=np.uint64)
+    b = O.OracleBuild("sh1", kat["leaf_size"], h, kat["bucket_size"])
+    assert list(b.query(h)) == kat["values"]
