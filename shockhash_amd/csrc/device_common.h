@@ -37,6 +37,21 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Lane l's 64-bit value (l wave-uniform). The builtin returns int: cast through uint32_t so the
+// low word is not sign-extended into the high word.
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Takes the next index of a device counter for the whole wave without a divergent branch: lane 0
+// adds 1, the other lanes add 0, and every lane uses lane 0's old value.
+__device__ __forceinline__ uint32_t wave_fetch_add(uint32_t *counter, uint32_t add) {
+    const uint32_t old = atomicAdd(counter, __lane_id() == 0 ? add : 0u);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)old);
+}
+
 // Orders LDS traffic between the lanes of one wavefront (a wave executes in lock step, LDS
 // operations of one wave complete in issue order; this stops the compiler from moving them).
 __device__ __forceinline__ void wave_sync() {

@@ -92,11 +92,8 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
-__device__ __forceinline__ uint32_t grab(uint32_t *counter) {
-    uint32_t t = 0;
-    if (lane_id() == 0) t = atomicAdd(counter, 1u);
-    return uniform(__shfl(t, 0));
-}
+// Next task index for the whole wave (no divergent branch, see wave_fetch_add).
+__device__ __forceinline__ uint32_t grab(uint32_t *counter) { return wave_fetch_add(counter, 1u); }
 
 template <int S>
 __global__ void __launch_bounds__(BLOCK) k_sh1_batch(const uint64_t *__restrict__ keys, const shgpu_leaf_task *__restrict__ tasks,
@@ -109,7 +106,7 @@ __global__ void __launch_bounds__(BLOCK) k_sh1_batch(const uint64_t *__restrict_
         const shgpu_leaf_task tk = tasks[t];
         const uint64_t x = sh1_leaf_search<S>(keys + uniform(tk.offset), uniform(tk.m), uniform64(tk.start_seed),
                                               uniform64(tk.max_trials), L);
-        if (lane_id() == 0) out[t] = x;
+        out[t] = x;  // wave-uniform value, stored by every lane
     }
 }
 
@@ -124,7 +121,7 @@ __global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict_
         if (t >= n) break;
         const shgpu_leaf_task tk = tasks[t];
         const uint64_t code = sh2_leaf_search(keys + uniform(tk.offset), uniform(tk.m), ws, L);
-        if (lane_id() == 0) out[t] = code;
+        out[t] = code;
     }
 }
 
@@ -139,7 +136,7 @@ __global__ void __launch_bounds__(BLOCK) k_split_batch(uint64_t *keys, uint64_t 
         const SplitShapeDev sh = make_split_shape(m, uniform(tk.unit), uniform(tk.kind), uniform(tk.hash));
         const uint64_t x = split_search<S>(keys + off, sh, uniform64(tk.start_seed), uniform64(tk.max_trials));
         if (x != SHGPU_NOT_FOUND) split_partition(keys + off, tmp + off, sh, x);
-        if (lane_id() == 0) out[t] = x;
+        out[t] = x;
     }
 }
 
@@ -174,7 +171,7 @@ struct BuildArgs {
 
 // One node: seed search (+ partition and child publication for split nodes).
 template <int VARIANT, int S_LEAF, int S_SPLIT>
-__device__ void process_node(const BuildArgs &a, uint32_t nid, void *lds_raw, uint8_t *ws, bool publish) {
+__device__ __forceinline__ void process_node(const BuildArgs &a, uint32_t nid, void *lds_raw, uint8_t *ws, bool publish) {
     const shgpu_node nd = a.nodes[nid];
     const uint32_t off = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
     const uint64_t start = uniform64(nd.start_seed), lim = uniform64(nd.max_trials);
@@ -198,17 +195,18 @@ __device__ void process_node(const BuildArgs &a, uint32_t nid, void *lds_raw, ui
         if (publish) {
             __threadfence();  // children's keys visible device-wide before they are queued
             const uint32_t nc = uniform(nd.n_children);
-            if (lane_id() == 0 && nc > 0) {
-                uint32_t pos = atomicAdd(&a.ctrl[1], nc);
+            if (nc > 0) {
+                // every lane computes the same child ids and stores the same slots (no divergence)
+                const uint32_t pos = wave_fetch_add(&a.ctrl[1], nc);
                 uint32_t child = nid + 1;
                 for (uint32_t c = 0; c < nc; c++) {
                     __hip_atomic_store(&a.queue[pos + c], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    child += a.nodes[child].subtree;
+                    child = uniform(child + a.nodes[child].subtree);
                 }
             }
         }
     }
-    if (lane_id() == 0) a.codes[nid] = code;
+    a.codes[nid] = code;
 }
 
 // Persistent construction kernel: every wave pulls nodes from the device queue until all nodes of
@@ -222,18 +220,16 @@ __global__ void __launch_bounds__(BLOCK) k_build(BuildArgs a) {
     for (;;) {
         const uint32_t idx = grab(&a.ctrl[0]);
         if (idx >= a.total) break;
-        uint32_t nid = 0;
-        if (lane_id() == 0) {
-            // Bounded wait (~2^27 sleeps, well over ten seconds): a slot that is never filled means an
-            // inconsistent forest; flag it and stop instead of spinning forever.
-            for (uint32_t spin = 0;; spin++) {
-                nid = __hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                if (nid != EMPTY_SLOT) break;
-                if (spin >= (1u << 27)) { atomicExch(&a.ctrl[2], 1u); break; }
-                __builtin_amdgcn_s_sleep(4);
-            }
+        // Bounded wait (~2^27 sleeps, well over ten seconds): a slot that is never filled means an
+        // inconsistent forest; flag it and stop instead of spinning forever. Every lane loads the
+        // slot; the decision uses lane 0's value so the loop stays wave-uniform.
+        uint32_t nid = EMPTY_SLOT;
+        for (uint32_t spin = 0;; spin++) {
+            nid = uniform(__hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+            if (nid != EMPTY_SLOT) break;
+            if (spin >= (1u << 27)) { atomicExch(&a.ctrl[2], 1u); break; }
+            __builtin_amdgcn_s_sleep(4);
         }
-        nid = uniform(__shfl(nid, 0));
         if (nid == EMPTY_SLOT) break;
         __threadfence();
         process_node<VARIANT, S_LEAF, S_SPLIT>(a, nid, &lds[w], ws, true);

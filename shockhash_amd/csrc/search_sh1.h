@@ -27,7 +27,7 @@ struct PfLds {
 // lane (nv <= 64, at most 64 edges). Every component must have #edges <= #vertices. Components are
 // found by min-label propagation with pointer jumping; labels are always vertices of the same
 // component and only decrease, so the loop ends (at most nv*nv rounds, a handful in practice).
-__device__ inline bool pseudoforest_wave(uint32_t a, uint32_t b, bool act, uint32_t nv, PfLds &L) {
+__device__ __forceinline__ bool pseudoforest_wave(uint32_t a, uint32_t b, bool act, uint32_t nv, PfLds &L) {
     const uint32_t lane = lane_id();
     if (lane < nv) { L.lab[lane] = lane; L.cnt[lane] = 0; L.cnt[64 + lane] = 0; }
     wave_sync();
@@ -78,7 +78,7 @@ __device__ __forceinline__ void sh1_eval(uint64_t key, const uint64_t (&seed)[S]
 // or SHGPU_NOT_FOUND when no seed in [start, start + max_trials) qualifies. `evals` (per wave,
 // lane 0 meaningful) accumulates hash evaluations performed.
 template <int S>
-__device__ uint64_t sh1_leaf_search(const uint64_t *keys, uint32_t m, uint64_t start, uint64_t max_trials,
+__device__ __forceinline__ uint64_t sh1_leaf_search(const uint64_t *keys, uint32_t m, uint64_t start, uint64_t max_trials,
                                     PfLds &L) {
     const uint32_t lane = lane_id();
     if (m < 2) return start;
@@ -93,14 +93,12 @@ __device__ uint64_t sh1_leaf_search(const uint64_t *keys, uint32_t m, uint64_t s
         if (m == 64) {
 #pragma unroll 16
             for (uint32_t k = 0; k < 64; k++) {
-                const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
-                                     __builtin_amdgcn_readlane((uint32_t)kv, k);
+                const uint64_t key = readlane64(kv, k);
                 sh1_eval<S>(key, seed, halfA, halfB, mA, mB);
             }
         } else {
             for (uint32_t k = 0; k < m; k++) {
-                const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
-                                     __builtin_amdgcn_readlane((uint32_t)kv, k);
+                const uint64_t key = readlane64(kv, k);
                 sh1_eval<S>(key, seed, halfA, halfB, mA, mB);
             }
         }

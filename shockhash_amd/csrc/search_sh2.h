@@ -67,20 +67,14 @@ __device__ __forceinline__ uint32_t uf_find(const uint8_t *par, uint32_t x, uint
     }
 }
 
-// Tests the 64 queued pairs (one per lane). Returns the lane mask of pseudoforest pairs.
-__device__ inline uint64_t sh2_uf_batch(const Sh2Scratch &g, Sh2Lds &L, uint32_t m) {
+// Tests the 64 queued pairs (one per lane). Returns the lane mask of pseudoforest pairs. Each lane
+// streams its two candidates' cell bytes four keys at a time from the scratch area.
+__device__ __forceinline__ uint64_t sh2_uf_batch(const Sh2Scratch &g, Sh2Lds &L, uint32_t m) {
     const uint32_t lane = lane_id();
     const uint32_t pr = L.queue[lane];
     const uint32_t j = pr >> 16, i = pr & 0xFFFF;
-    const uint4 *pj = (const uint4 *)(g.hashes + (size_t)j * 64);
-    const uint4 *pi = (const uint4 *)(g.hashes + (size_t)i * 64);
-    uint32_t hj[16], hi[16];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const uint4 a = pj[q], b = pi[q];
-        hj[4 * q] = a.x; hj[4 * q + 1] = a.y; hj[4 * q + 2] = a.z; hj[4 * q + 3] = a.w;
-        hi[4 * q] = b.x; hi[4 * q + 1] = b.y; hi[4 * q + 2] = b.z; hi[4 * q + 3] = b.w;
-    }
+    const uint32_t *pj = (const uint32_t *)(g.hashes + (size_t)j * 64);
+    const uint32_t *pi = (const uint32_t *)(g.hashes + (size_t)i * 64);
     uint8_t *par = (uint8_t *)L.par;
 #pragma unroll
     for (uint32_t q = 0; q < 16; q++) {
@@ -91,12 +85,16 @@ __device__ inline uint64_t sh2_uf_batch(const Sh2Scratch &g, Sh2Lds &L, uint32_t
     const uint32_t shift = m / 2;
     uint64_t tree = ~0ull;
     bool alive = true;
+    uint32_t wj = pj[0], wi = pi[0];
+    const uint32_t nq = (m + 3) / 4;
+    for (uint32_t q = 0; q < nq; q++) {
+        const uint32_t cj = wj, ci = wi;
+        if (q + 1 < nq) { wj = pj[q + 1]; wi = pi[q + 1]; }  // prefetch the next four keys
 #pragma unroll
-    for (uint32_t k = 0; k < 64; k++) {
-        if (k < m) {
-            if (alive) {
-                const uint32_t u = ((hj[k >> 2] >> ((k & 3) * 8)) & 0xFF) + shift;
-                const uint32_t v = (hi[k >> 2] >> ((k & 3) * 8)) & 0xFF;
+        for (uint32_t r = 0; r < 4; r++) {
+            if (4 * q + r < m && alive) {
+                const uint32_t u = ((cj >> (8 * r)) & 0xFF) + shift;
+                const uint32_t v = (ci >> (8 * r)) & 0xFF;
                 const uint32_t ru = uf_find(par, u, lane);
                 const uint32_t rv = uf_find(par, v, lane);
                 par[uf_addr(ru, lane)] = (uint8_t)rv;
@@ -104,8 +102,8 @@ __device__ inline uint64_t sh2_uf_batch(const Sh2Scratch &g, Sh2Lds &L, uint32_t
                 if (!(tu && tv && ru != rv)) tree &= ~(1ull << rv);
                 alive = tu || tv;
             }
-            if ((k & 7) == 7 && ballot(alive) == 0) break;
         }
+        if ((q & 1) && ballot(alive) == 0) break;
     }
     const uint64_t res = ballot(alive);
     wave_sync();
@@ -113,7 +111,7 @@ __device__ inline uint64_t sh2_uf_batch(const Sh2Scratch &g, Sh2Lds &L, uint32_t
 }
 
 // Appends candidate (A, B) with the given code: cells of every key and its isolated-key mask.
-__device__ inline void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L, uint32_t c, uint64_t kv, uint32_t m,
+__device__ __forceinline__ void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L, uint32_t c, uint64_t kv, uint32_t m,
                                          uint32_t H, uint32_t A, uint32_t B, uint32_t code) {
     const uint32_t lane = lane_id();
     const bool act = lane < m;
@@ -126,11 +124,12 @@ __device__ inline void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L, uint32_
     const uint64_t iso = ballot(act && L.hist[h] == 1);
     wave_sync();
     g.hashes[(size_t)c * 64 + lane] = act ? (uint8_t)h : 0;
-    if (lane == 0) { g.code[c] = code; g.iso[c] = iso; }
+    g.code[c] = code;  // every lane stores the same wave-uniform values
+    g.iso[c] = iso;
 }
 
 // Returns the ShockHash2 leaf code, or SHGPU_NOT_FOUND when a scratch bound is exceeded.
-__device__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32_t m, uint8_t *scratch, Sh2Lds &L) {
+__device__ __forceinline__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32_t m, uint8_t *scratch, Sh2Lds &L) {
     const uint32_t lane = lane_id();
     if (m < 2) return 0;
     const Sh2Scratch g(scratch);
@@ -147,8 +146,7 @@ __device__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32_t m, uint8_t *s
                 const uint64_t s = n_hs + lane;
                 uint32_t ma = 0, mb = 0;
                 for (uint32_t k = 0; k < m; k++) {
-                    const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
-                                         __builtin_amdgcn_readlane((uint32_t)kv, k);
+                    const uint64_t key = readlane64(kv, k);
                     const uint32_t bit = 1u << fastrange64(remix(key + s), H);
                     if ((key & 1) == 0) ma |= bit; else mb |= bit;
                 }

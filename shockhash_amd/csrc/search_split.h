@@ -73,7 +73,7 @@ __device__ __forceinline__ void split_eval(uint64_t key, const uint64_t (&seed)[
 }
 
 template <int S, int NT, int SIMD>
-__device__ uint64_t split_search_t(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start, uint64_t max_trials) {
+__device__ __forceinline__ uint64_t split_search_t(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start, uint64_t max_trials) {
     const uint32_t lane = lane_id();
     const uint32_t m = sh.m;
     for (uint64_t base = 0; base < max_trials; base += 64 * S) {
@@ -91,14 +91,12 @@ __device__ uint64_t split_search_t(const uint64_t *keys, const SplitShapeDev &sh
             if (cnt == 64) {
 #pragma unroll 8
                 for (uint32_t k = 0; k < 64; k++) {
-                    const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
-                                         __builtin_amdgcn_readlane((uint32_t)kv, k);
+                    const uint64_t key = readlane64(kv, k);
                     split_eval<S, NT, SIMD>(key, seed, sh, c);
                 }
             } else {
                 for (uint32_t k = 0; k < cnt; k++) {
-                    const uint64_t key = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kv >> 32), k) << 32) |
-                                         __builtin_amdgcn_readlane((uint32_t)kv, k);
+                    const uint64_t key = readlane64(kv, k);
                     split_eval<S, NT, SIMD>(key, seed, sh, c);
                 }
             }
@@ -119,7 +117,7 @@ __device__ uint64_t split_search_t(const uint64_t *keys, const SplitShapeDev &sh
 }
 
 template <int S>
-__device__ uint64_t split_search(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start, uint64_t max_trials) {
+__device__ __forceinline__ uint64_t split_search(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start, uint64_t max_trials) {
     if (sh.simd) {
         if (sh.nt == 1) return split_search_t<S, 1, 1>(keys, sh, start, max_trials);
         if (sh.nt == 2) return split_search_t<S, 2, 1>(keys, sh, start, max_trials);
@@ -132,7 +130,7 @@ __device__ uint64_t split_search(const uint64_t *keys, const SplitShapeDev &sh, 
 
 // Stable counting-sort partition of keys[0..m) by part for seed x into tmp, then copied back
 // (ShockHash.h:464-469,495-499,528-532; SIMDShockHash.hpp:234-252,337-354): part p starts at p*unit.
-__device__ inline void split_partition(uint64_t *keys, uint64_t *tmp, const SplitShapeDev &sh, uint64_t x) {
+__device__ __forceinline__ void split_partition(uint64_t *keys, uint64_t *tmp, const SplitShapeDev &sh, uint64_t x) {
     const uint32_t lane = lane_id();
     const uint32_t m = sh.m;
     uint32_t base[4] = {0, sh.unit, 2 * sh.unit, 3 * sh.unit};
