@@ -23,6 +23,9 @@ sys.path.insert(0, ROOT)
 MI355X_CUS, SIMDS_PER_CU, LANES_PER_SIMD, CLOCK_GHZ = 256, 4, 16, 2.4
 INT32_PEAK_TOPS = MI355X_CUS * SIMDS_PER_CU * LANES_PER_SIMD * CLOCK_GHZ / 1e3  # 39.3 T lane-ops/s
 OPS_PER_EVAL = 32  # int32 ops per 64-bit-mix hash evaluation (remix + range reduction + count/mask update)
+# HBM bytes per k_build launch on this workload: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of
+# this script, scaled by the same counters on the 256 MiB copy probe (profiles/k_build_shockhash2_L64_10M.md).
+MEASURED_TRAFFIC_BYTES = 2.838e9
 
 
 def log(msg):
@@ -50,6 +53,12 @@ def algorithmic_evals(nodes, codes):
     return float(evals.sum() + evals_leaf.sum()), float(evals_leaf.sum())
 
 
+def O_build(h, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    return O.OracleBuild("sh2", 64, h, 2000, threads=threads)
+
+
 def cpu_baseline(n_sample, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -68,9 +77,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--plain-buckets", type=int, default=100)
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="CPU oracle sample size (0: the whole workload)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=256,
+                    help="size of the plain copy kernel run once as the HBM-counter calibration reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the construction engine does not shard across GPUs "
@@ -85,6 +96,16 @@ def main():
     h = workload.fingerprints(args.n, seed=1)
     d_h = eng.array(h.reshape(-1))  # fingerprints resident in HBM
     log(f"{args.n} fingerprints resident on the device")
+    if args.copy_probe_mib:
+        # known-size reference for FETCH_SIZE / WRITE_SIZE calibration under rocprofv3 --pmc
+        from shockhash_amd.engine import DeviceArray
+        nb = args.copy_probe_mib << 20
+        src, dst = DeviceArray(eng, nb // 8, np.uint64), DeviceArray(eng, nb // 8, np.uint64)
+        t0 = time.perf_counter()
+        eng.device_copy(dst, src, nb)
+        log(f"copy probe: {nb >> 20} MiB in {1e3 * (time.perf_counter() - t0):.2f} ms (k_copy_u64)")
+        src.free()
+        dst.free()
 
     for i in range(args.warmup):
         b = eng.build_dev(d_h, args.n, "sh2", 64, 2000, copy_keys=False)
@@ -121,13 +142,24 @@ def main():
     cpu = None
     bit_exact = plain_match
     if not args.no_cpu_baseline:
-        hs, ob, dt = cpu_baseline(args.cpu_sample, args.cpu_threads)
-        bs = eng.build(hs, "sh2", 64, 2000)
-        bit_exact = bit_exact and bool(np.array_equal(bs.codes, ob.node_code))
-        cpu = {"value": args.cpu_sample / dt, "unit": "keys/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"{args.cpu_sample} fingerprints (same generator, seed 7), ShockHash2 leaf 64, bucket 2000, "
-                         f"oracle C++ restatement, {args.cpu_threads} threads, {dt:.1f} s"}
-        log(f"cpu oracle: {cpu['value']:.0f} keys/s; GPU codes equal the oracle's on the sample: {bit_exact}")
+        n_cpu = args.cpu_sample or args.n
+        if n_cpu == args.n:
+            hs = h
+            t0 = time.perf_counter()
+            ob = O_build(hs, args.cpu_threads)
+            dt = time.perf_counter() - t0
+            bs = b  # the timed GPU build of the same keys
+            sample = f"the whole workload ({args.n} keys, seed 1)"
+        else:
+            hs, ob, dt = cpu_baseline(n_cpu, args.cpu_threads)
+            bs = eng.build(hs, "sh2", 64, 2000)
+            sample = f"{n_cpu} fingerprints (same generator, seed 7)"
+        bit_exact = bit_exact and bool(np.array_equal(bs.codes, ob.node_code)) and \
+            bool(np.array_equal(bs.size_acc, ob.size_acc))
+        cpu = {"value": n_cpu / dt, "unit": "keys/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"{sample}, ShockHash2 leaf 64, bucket 2000, oracle C++ restatement, "
+                         f"{args.cpu_threads} threads, {dt:.1f} s"}
+        log(f"cpu oracle: {cpu['value']:.0f} keys/s; GPU codes equal the oracle's: {bit_exact}")
 
     out = {
         "metric": "MPHF build keys/s at 10M keys (bit-exact) vs plain GPU form; p99 build latency",
@@ -153,7 +185,7 @@ def main():
         "hash_evals_leaves": evals_leaf,
         "search_kernel_ms": kms,
         "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": INT32_PEAK_TOPS, "unit": "TOP/s",
-                     "frac": achieved_tops / INT32_PEAK_TOPS, "traffic": None,
+                     "frac": achieved_tops / INT32_PEAK_TOPS, "traffic": MEASURED_TRAFFIC_BYTES if args.n == 10_000_000 else None,
                      "ops_per_eval": OPS_PER_EVAL, "evals_per_s": evals / (kms / 1e3)},
         "cpu_baseline": cpu,
         "p99_build_latency_ms": None,

@@ -79,6 +79,9 @@ int shgpu_device_alloc(shgpu_engine *e, size_t bytes, void **d_out);
 int shgpu_device_free(shgpu_engine *e, void *d);
 int shgpu_copy_to_device(shgpu_engine *e, void *d_dst, const void *h_src, size_t bytes);
 int shgpu_copy_to_host(shgpu_engine *e, void *h_dst, const void *d_src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (k_copy_u64); bytes must be a multiple of 8. Used as
+ * the known-size reference when HBM traffic counters are calibrated. Synchronous. */
+int shgpu_device_copy(shgpu_engine *e, void *d_dst, const void *d_src, size_t bytes);
 
 /* Batched ShockHash1 leaf search: d_seeds[i] = smallest seed >= start_seed of task i
  * (SHGPU_NOT_FOUND if none below start_seed + max_trials). m <= 64. */

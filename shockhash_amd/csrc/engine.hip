@@ -140,6 +140,12 @@ __global__ void __launch_bounds__(BLOCK) k_split_batch(uint64_t *keys, uint64_t 
     }
 }
 
+// Plain streaming copy (bandwidth / counter calibration probe).
+__global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_unzip_hashes(const uint64_t *__restrict__ h, uint64_t n, uint64_t *first, uint64_t *second) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         first[i] = h[2 * i];
@@ -722,6 +728,17 @@ int shgpu_copy_to_device(shgpu_engine *e, void *d_dst, const void *h_src, size_t
     if (!e || (bytes && (!d_dst || !h_src))) return fail(SHGPU_ERR_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(e->device));
     if (bytes) HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return SHGPU_OK;
+}
+
+int shgpu_device_copy(shgpu_engine *e, void *d_dst, const void *d_src, size_t bytes) {
+    if (!e || (bytes && (!d_dst || !d_src)) || bytes % 8) return fail(SHGPU_ERR_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(e->device));
+    if (bytes) {
+        k_copy_u64<<<e->cu_count * 8, 256, 0, e->stream>>>((const uint64_t *)d_src, (uint64_t *)d_dst, bytes / 8);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipStreamSynchronize(e->stream));
     return SHGPU_OK;
 }

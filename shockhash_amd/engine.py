@@ -87,6 +87,7 @@ _SIGS = {
     "shgpu_device_free": (ctypes.c_int, [P, P]),
     "shgpu_copy_to_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
     "shgpu_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
+    "shgpu_device_copy": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
     "shgpu_build_run_host": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
     "shgpu_plan_forest": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, P, ctypes.c_uint64, P, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_uint64)]),
@@ -246,6 +247,9 @@ class Engine:
 
     def split_search_dev(self, d_keys, d_tasks, n, d_out):
         _check(load().shgpu_split_search(self.h, d_keys.ptr, d_tasks.ptr, n, d_out.ptr, None))
+
+    def device_copy(self, d_dst, d_src, nbytes):
+        _check(load().shgpu_device_copy(self.h, d_dst.ptr, d_src.ptr, nbytes))
 
     def array(self, host_array):
         return DeviceArray.from_host(self, host_array)
