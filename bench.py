@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="CPU oracle sample size (0: the whole workload)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--split-seeds-per-lane", type=int, default=None)
+    ap.add_argument("--leaf-seeds-per-lane", type=int, default=None)
+    ap.add_argument("--waves-per-cu", type=int, default=None)
     ap.add_argument("--copy-probe-mib", type=int, default=256,
                     help="size of the plain copy kernel run once as the HBM-counter calibration reference")
     args = ap.parse_args()
@@ -92,7 +95,13 @@ def main():
     from shockhash_amd.engine import Engine
 
     eng = Engine(0)
-    log(f"device {eng.arch} CUs {eng.cu_count} clock {eng.clock_khz / 1e6:.2f} GHz; tuning {eng.tuning}")
+    tune = {k: v for k, v in (("split_seeds_per_lane", args.split_seeds_per_lane),
+                                ("sh1_seeds_per_lane", args.leaf_seeds_per_lane),
+                                ("waves_per_cu", args.waves_per_cu)) if v is not None}
+    if tune:
+        eng.set_variant_tuning("sh2", **tune)
+    log(f"device {eng.arch} CUs {eng.cu_count} clock {eng.clock_khz / 1e6:.2f} GHz; "
+        f"ShockHash2 schedule {eng.variant_tuning('sh2')}")
     h = workload.fingerprints(args.n, seed=1)
     d_h = eng.array(h.reshape(-1))  # fingerprints resident in HBM
     log(f"{args.n} fingerprints resident on the device")

@@ -69,8 +69,18 @@ int shgpu_engine_create(int device, shgpu_engine **out);
 void shgpu_engine_destroy(shgpu_engine *e);
 /* CU count, peak engine clock (kHz) and gfx arch name of the engine's device. */
 int shgpu_engine_info(shgpu_engine *e, int *cu_count, int *clock_khz, char *arch, size_t arch_len);
+/* Schedule of the batched per-node entry points; shgpu_set_tuning also sets every variant's build
+ * schedule. */
 int shgpu_get_tuning(shgpu_engine *e, shgpu_tuning *out);
 int shgpu_set_tuning(shgpu_engine *e, const shgpu_tuning *t);
+/* Build schedule of one variant (SHGPU_SHOCKHASH / SHGPU_SIMD_SHOCKHASH / SHGPU_SHOCKHASH2). */
+int shgpu_get_variant_tuning(shgpu_engine *e, int variant, shgpu_tuning *out);
+int shgpu_set_variant_tuning(shgpu_engine *e, int variant, const shgpu_tuning *t);
+/* Autotuner: builds the given sample with every candidate schedule (seeds per lane for leaf and
+ * split windows, waves per CU), keeps the fastest for that variant and returns it. Results are
+ * identical for every schedule; only the time differs. d_hashes: device fingerprints. */
+int shgpu_autotune(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
+                   shgpu_tuning *best, double *best_ms);
 /* Synchronises the engine's stream. */
 int shgpu_sync(shgpu_engine *e);
 

@@ -71,10 +71,13 @@ struct shgpu_engine {
     int cu_count = 0;
     int clock_khz = 0;
     char arch[64] = {0};
-    shgpu_tuning tuning{4, 4, 8, 1ull << 40};
+    // Schedule per variant (ShockHash, SIMDShockHash, ShockHash2); the defaults are the measured best on
+    // MI355X for leaf 30 / 58 / 64 builds (see shgpu_autotune).
+    shgpu_tuning tuning_v[3] = {{4, 2, 24, 1ull << 40}, {4, 2, 24, 1ull << 40}, {4, 2, 24, 1ull << 40}};
+    shgpu_tuning tuning{4, 2, 24, 1ull << 40};  // batched per-node entry points
     DevBuf counter, sh2_scratch, staging;
     // build buffers
-    DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl;
+    DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl, states, help;
 };
 
 struct shgpu_build {
@@ -163,42 +166,60 @@ __global__ void k_bucket_bounds(const uint64_t *__restrict__ first, uint64_t n, 
     }
 }
 
+// Per-node search state of a build (windows of seeds shared by every wave that works on the node).
+struct NodeState {
+    unsigned long long gd;    // high 32 bits: windows handed out; low 32 bits: windows completed
+    unsigned long long best;  // smallest qualifying seed offset found so far (SHGPU_NOT_FOUND: none)
+    uint32_t finalized;       // number of waves that claimed finalisation (the first one does it)
+    uint32_t pad[3];
+};
+
 struct BuildArgs {
     uint64_t *keys;
     uint64_t *tmp;
     const shgpu_node *nodes;
     uint64_t *codes;
     uint32_t *queue;
-    uint32_t *ctrl;  // [0] head, [1] tail, [2] error flag (a queue slot never filled)
+    uint32_t *ctrl;  // [0] queue head, [1] queue tail, [2] error flag, [3] nodes finished
+    NodeState *state;
+    uint32_t *help;  // per wave slot: node whose windows the slot's wave searches, or EMPTY_SLOT
     uint32_t total;
+    uint32_t nslots;
     uint8_t *scratch;
     int split_hash;
 };
 
-// One node: seed search (+ partition and child publication for split nodes).
-template <int VARIANT, int S_LEAF, int S_SPLIT>
-__device__ __forceinline__ void process_node(const BuildArgs &a, uint32_t nid, void *lds_raw, uint8_t *ws, bool publish) {
+__device__ __forceinline__ unsigned long long wave_fetch_add64(unsigned long long *p, unsigned long long add) {
+    const unsigned long long old = atomicAdd(p, __lane_id() == 0 ? add : 0ull);
+    return uniform64(old);
+}
+
+// A relaxed load of the best seed may be served by the local XCD's L2 and lag behind: it is only a
+// hint (skip windows that cannot win). Read-modify-write atomics resolve at the coherence point, so
+// the value that decides a node's code is read with one (atomicOr 0).
+__device__ __forceinline__ uint64_t hint_best(const NodeState *st) {
+    return uniform64(__hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ uint64_t read_best(NodeState *st) {
+    return uniform64(atomicOr(&st->best, 0ull));
+}
+
+template <int SL, int SS>
+__device__ __forceinline__ uint32_t window_seeds(uint32_t kind) {
+    return kind == SHGPU_NODE_LEAF ? 64u * SL : kind == SHGPU_NODE_AGGR_LOWER ? 64u * SS : 64u;
+}
+
+// Stores the node's code; for a split node also partitions its keys by the found seed and (in the
+// persistent form) publishes the children. An exhausted search still publishes the children so that
+// the forest drains; the host then reports the error.
+template <int VARIANT>
+__device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, uint64_t off, bool persistent) {
     const shgpu_node nd = a.nodes[nid];
-    const uint32_t off = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
-    const uint64_t start = uniform64(nd.start_seed), lim = uniform64(nd.max_trials);
-    uint64_t code;
-    if (kind == SHGPU_NODE_LEAF) {
-        if (VARIANT == SHGPU_SHOCKHASH2) {
-            code = sh2_leaf_search(a.keys + off, m, ws, *(Sh2Lds *)lds_raw);
-        } else {
-            const uint64_t x = sh1_leaf_search<S_LEAF>(a.keys + off, m, start, lim, *(PfLds *)lds_raw);
-            code = x == SHGPU_NOT_FOUND ? x : x - start;
-        }
-    } else {
+    const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
+    if (kind != SHGPU_NODE_LEAF) {
         const SplitShapeDev sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
-        uint64_t x;
-        if (kind == SHGPU_NODE_AGGR_LOWER)  // the many-trial nodes (~4000 seeds at 4 x 64 keys)
-            x = split_search<S_SPLIT>(a.keys + off, sh, start, lim);
-        else
-            x = split_search<1>(a.keys + off, sh, start, lim);
-        code = x == SHGPU_NOT_FOUND ? x : x - start;
-        if (x != SHGPU_NOT_FOUND) split_partition(a.keys + off, a.tmp + off, sh, x);
-        if (publish) {
+        if (off != SHGPU_NOT_FOUND) split_partition(a.keys + o, a.tmp + o, sh, uniform64(nd.start_seed) + off);
+        if (persistent) {
             __threadfence();  // children's keys visible device-wide before they are queued
             const uint32_t nc = uniform(nd.n_children);
             if (nc > 0) {
@@ -212,42 +233,164 @@ __device__ __forceinline__ void process_node(const BuildArgs &a, uint32_t nid, v
             }
         }
     }
-    a.codes[nid] = code;
+    a.codes[nid] = off;  // read by the host after the kernel ends
+    if (persistent) wave_fetch_add(&a.ctrl[3], 1u);
 }
 
-// Persistent construction kernel: every wave pulls nodes from the device queue until all nodes of
-// the forest have been taken. Split nodes push their children when their partition is written.
-template <int VARIANT, int S_LEAF, int S_SPLIT>
+// Searches windows of a split node or ShockHash1 leaf until its smallest qualifying seed is known.
+// Windows are handed out in increasing order; a window above the best seed found so far is skipped.
+// The wave that completes the last outstanding window once the best seed is known finalises the
+// node, so every window below the winner has been searched: the result is the sequential minimum.
+template <int VARIANT, int SL, int SS>
+__device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent) {
+    const shgpu_node nd = a.nodes[nid];
+    const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
+    const uint64_t start = uniform64(nd.start_seed), lim = uniform64(nd.max_trials);
+    NodeState *st = &a.state[nid];
+    const uint64_t W = window_seeds<SL, SS>(kind);
+    SplitShapeDev sh;
+    if (kind != SHGPU_NODE_LEAF) sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
+    for (;;) {
+        if (hint_best(st) != SHGPU_NOT_FOUND) return;
+        const uint64_t w = wave_fetch_add64(&st->gd, 1ull << 32) >> 32;
+        const uint64_t woff = w * W;
+        const bool exhausted = woff >= lim;
+        uint64_t r = SHGPU_NOT_FOUND;
+        if (!exhausted && hint_best(st) > woff) {
+            if (kind == SHGPU_NODE_LEAF) {
+                if constexpr (VARIANT != SHGPU_SHOCKHASH2)  // ShockHash2 leaves never come here
+                    r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND) : sh1_window<SL>(a.keys + o, m, start, woff, L);
+            } else if (kind == SHGPU_NODE_AGGR_LOWER)
+                r = split_window<SS>(a.keys + o, sh, start, woff);
+            else
+                r = split_window<1>(a.keys + o, sh, start, woff);
+            if (r != SHGPU_NOT_FOUND && r >= lim) r = SHGPU_NOT_FOUND;
+            if (r != SHGPU_NOT_FOUND) {
+                // same value in every lane; waiting for the returned value orders this atomic before the
+                // completion count below (both resolve at the coherence point)
+                const unsigned long long prev = atomicMin(&st->best, (unsigned long long)r);
+                asm volatile("" ::"v"(prev));
+            }
+        }
+        const unsigned long long gd = wave_fetch_add64(&st->gd, 1ull) + 1ull;
+        if ((gd >> 32) == (gd & 0xFFFFFFFFull)) {
+            const uint64_t b = read_best(st);
+            if (b != SHGPU_NOT_FOUND || exhausted) {
+                if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, b, persistent);
+            }
+        }
+        if (r != SHGPU_NOT_FOUND || exhausted) return;
+    }
+}
+
+// Joins one node another wave is searching in windows, if any. Returns true when it did some work.
+template <int VARIANT, int SL, int SS>
+__device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLds &L) {
+    const uint32_t lane = lane_id();
+    const uint32_t first = (slot * 97u) % a.nslots;
+    for (uint32_t base = 0; base < a.nslots; base += 64) {
+        uint32_t i = first + base + lane;
+        if (i >= a.nslots) i -= a.nslots;
+        const uint32_t v = base + lane < a.nslots ? __hip_atomic_load(&a.help[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                                  : EMPTY_SLOT;
+        // a registered node whose best seed is already known needs no help
+        const bool open = v != EMPTY_SLOT &&
+                          __hip_atomic_load(&a.state[v].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == SHGPU_NOT_FOUND;
+        const uint64_t hit = ballot(open);
+        if (hit) {
+            const uint32_t nid = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));
+            __threadfence();
+            work_windows<VARIANT, SL, SS>(a, nid, L, true);
+            return true;
+        }
+    }
+    return false;
+}
+
+template <int VARIANT>
+struct BuildLds {
+    PfLds pf;
+    typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, char>::type sh2;
+};
+
+// One node taken from the queue: ShockHash2 leaves are searched by this wave alone; splits and
+// ShockHash1 leaves are searched in windows that idle waves may join (help registry).
+template <int VARIANT, int SL, int SS>
+__device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint32_t slot, BuildLds<VARIANT> &L,
+                                         uint8_t *ws) {
+    const shgpu_node nd = a.nodes[nid];
+    const uint32_t kind = uniform(nd.kind);
+    if (VARIANT == SHGPU_SHOCKHASH2 && kind == SHGPU_NODE_LEAF) {
+        if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
+            const uint64_t code = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2);
+            a.codes[nid] = code;
+            wave_fetch_add(&a.ctrl[3], 1u);
+        }
+        return;
+    }
+    __hip_atomic_store(&a.help[slot], nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    work_windows<VARIANT, SL, SS>(a, nid, L.pf, true);
+    __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr uint32_t SPIN_LIMIT = 1u << 27;  // sleeps before a wait is declared stuck (well over ten seconds)
+
+// Persistent construction kernel. Every wave takes queue slots in order; while a slot is not yet
+// published (its parent is still being split) the wave helps windowed searches. When the queue is
+// exhausted, waves keep helping until every node is finished. Split nodes publish their children
+// when their partition is written.
+template <int VARIANT, int SL, int SS>
 __global__ void __launch_bounds__(BLOCK) k_build(BuildArgs a) {
-    using LdsT = typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, PfLds>::type;
-    __shared__ LdsT lds[WAVES_PER_BLOCK];
+    __shared__ BuildLds<VARIANT> lds[WAVES_PER_BLOCK];
     const uint32_t w = threadIdx.x / 64;
-    uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)(blockIdx.x * WAVES_PER_BLOCK + w) * SH2_SCRATCH_BYTES : nullptr;
+    const uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
+    BuildLds<VARIANT> &L = lds[w];
+    uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)slot * SH2_SCRATCH_BYTES : nullptr;
     for (;;) {
         const uint32_t idx = grab(&a.ctrl[0]);
         if (idx >= a.total) break;
-        // Bounded wait (~2^27 sleeps, well over ten seconds): a slot that is never filled means an
-        // inconsistent forest; flag it and stop instead of spinning forever. Every lane loads the
-        // slot; the decision uses lane 0's value so the loop stays wave-uniform.
         uint32_t nid = EMPTY_SLOT;
         for (uint32_t spin = 0;; spin++) {
             nid = uniform(__hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
             if (nid != EMPTY_SLOT) break;
-            if (spin >= (1u << 27)) { atomicExch(&a.ctrl[2], 1u); break; }
+            if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
+            if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
             __builtin_amdgcn_s_sleep(4);
         }
-        if (nid == EMPTY_SLOT) break;
+        if (nid == EMPTY_SLOT) return;
         __threadfence();
-        process_node<VARIANT, S_LEAF, S_SPLIT>(a, nid, &lds[w], ws, true);
+        run_node<VARIANT, SL, SS>(a, nid, slot, L, ws);
+    }
+    for (uint32_t spin = 0;; spin++) {
+        if (uniform(__hip_atomic_load(&a.ctrl[3], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= a.total) break;
+        if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
+        if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
+        __builtin_amdgcn_s_sleep(8);
     }
 }
 
-// Plain form: one node per launch (host walks the forest in DFS order on one stream).
-template <int VARIANT, int S_LEAF, int S_SPLIT>
+// Plain form: one launch per node (the host walks the forest in DFS order on one stream). Every
+// wave of the launch joins the node's windowed search; a ShockHash2 leaf is searched by wave 0.
+template <int VARIANT, int SL, int SS>
 __global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
-    using LdsT = typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, PfLds>::type;
-    __shared__ LdsT lds;
-    process_node<VARIANT, S_LEAF, S_SPLIT>(a, nid, &lds, a.scratch, false);
+    __shared__ BuildLds<VARIANT> L;
+    const shgpu_node nd = a.nodes[nid];
+    if (VARIANT == SHGPU_SHOCKHASH2 && uniform(nd.kind) == SHGPU_NODE_LEAF) {
+        if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
+            if (blockIdx.x == 0) a.codes[nid] = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2);
+        }
+        return;
+    }
+    work_windows<VARIANT, SL, SS>(a, nid, L.pf, false);
+}
+
+__global__ void k_init_states(NodeState *st, uint32_t n, uint32_t *help, uint32_t nslots) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        st[i].gd = 0;
+        st[i].best = SHGPU_NOT_FOUND;
+        st[i].finalized = 0;
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) help[i] = EMPTY_SLOT;
 }
 
 // ===================================================================================== host helpers
@@ -346,7 +489,10 @@ int check_variant(int variant, uint32_t leaf) {
     return fail(SHGPU_ERR_ARG, "unknown variant");
 }
 
-int grid_for(const shgpu_engine *e) { return std::max(1, e->cu_count * (int)e->tuning.waves_per_cu / WAVES_PER_BLOCK); }
+int grid_for(const shgpu_engine *e, const shgpu_tuning &t) {
+    return std::max(1, e->cu_count * (int)t.waves_per_cu / WAVES_PER_BLOCK);
+}
+int grid_for(const shgpu_engine *e) { return grid_for(e, e->tuning); }
 
 hipStream_t pick(shgpu_engine *e, void *stream) { return stream ? (hipStream_t)stream : e->stream; }
 
@@ -369,7 +515,7 @@ int launch_build(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
 
 template <int VARIANT>
 int launch_build_v(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
-    const uint32_t sl = e->tuning.sh1_seeds_per_lane, ss = e->tuning.split_seeds_per_lane;
+    const uint32_t sl = e->tuning_v[VARIANT].sh1_seeds_per_lane, ss = e->tuning_v[VARIANT].split_seeds_per_lane;
 #define SHGPU_CASE(A, B) if (sl == A && ss == B) return launch_build<VARIANT, A, B>(e, a, grid, s);
     SHGPU_CASE(1, 1) SHGPU_CASE(1, 2) SHGPU_CASE(1, 4) SHGPU_CASE(2, 1) SHGPU_CASE(2, 2) SHGPU_CASE(2, 4)
     SHGPU_CASE(4, 1) SHGPU_CASE(4, 2) SHGPU_CASE(4, 4)
@@ -378,8 +524,8 @@ int launch_build_v(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s)
 }
 
 template <int VARIANT>
-int launch_node_v(shgpu_engine *e, const BuildArgs &a, uint32_t nid, hipStream_t s) {
-    k_node<VARIANT, 4, 4><<<1, 64, 0, s>>>(a, nid);
+int launch_node_v(shgpu_engine *e, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
+    k_node<VARIANT, 4, 4><<<waves, 64, 0, s>>>(a, nid);
     HIP_TRY(hipGetLastError());
     return SHGPU_OK;
 }
@@ -455,14 +601,35 @@ int shgpu_get_tuning(shgpu_engine *e, shgpu_tuning *out) {
     return SHGPU_OK;
 }
 
-int shgpu_set_tuning(shgpu_engine *e, const shgpu_tuning *t) {
-    if (!e || !t) return fail(SHGPU_ERR_ARG, "NULL argument");
+static int check_tuning(const shgpu_tuning *t) {
     auto okS = [](uint32_t s) { return s == 1 || s == 2 || s == 4; };
     if (!okS(t->sh1_seeds_per_lane) || !okS(t->split_seeds_per_lane))
         return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(SHGPU_ERR_ARG, "waves_per_cu must be in [1, 32]");
     if (t->max_trials == 0) return fail(SHGPU_ERR_ARG, "max_trials must be positive");
+    return SHGPU_OK;
+}
+
+int shgpu_set_tuning(shgpu_engine *e, const shgpu_tuning *t) {
+    if (!e || !t) return fail(SHGPU_ERR_ARG, "NULL argument");
+    int rc = check_tuning(t);
+    if (rc) return rc;
     e->tuning = *t;
+    for (auto &v : e->tuning_v) v = *t;
+    return SHGPU_OK;
+}
+
+int shgpu_get_variant_tuning(shgpu_engine *e, int variant, shgpu_tuning *out) {
+    if (!e || !out || variant < 0 || variant > 2) return fail(SHGPU_ERR_ARG, "bad argument");
+    *out = e->tuning_v[variant];
+    return SHGPU_OK;
+}
+
+int shgpu_set_variant_tuning(shgpu_engine *e, int variant, const shgpu_tuning *t) {
+    if (!e || !t || variant < 0 || variant > 2) return fail(SHGPU_ERR_ARG, "bad argument");
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    e->tuning_v[variant] = *t;
     return SHGPU_OK;
 }
 
@@ -593,13 +760,16 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
         if (m > MAX_ENCODABLE_BUCKET) return fail(SHGPU_ERR_BUCKET_TOO_LARGE, "bucket " + std::to_string(k) + " holds " + std::to_string(m) + " keys");
         if (m > 1) {
             roots.push_back((uint32_t)b->nodes.size());
-            emit_subtree(b->nodes, st, (uint32_t)b->size_acc[k], (uint32_t)m, 0, e->tuning.max_trials);
+            emit_subtree(b->nodes, st, (uint32_t)b->size_acc[k], (uint32_t)m, 0, e->tuning_v[d->variant].max_trials);
         }
     }
     const uint32_t total = (uint32_t)b->nodes.size();
     b->codes.assign(total, 0);
+    const int grid = grid_for(e, e->tuning_v[d->variant]);
+    const uint32_t nslots = (uint32_t)grid * WAVES_PER_BLOCK;
     if ((rc = e->nodes.ensure((size_t)total * sizeof(shgpu_node) + 64)) || (rc = e->codes.ensure((size_t)total * 8 + 64)) ||
-        (rc = e->queue.ensure((size_t)total * 4 + 64)) || (rc = e->ctrl.ensure(64)))
+        (rc = e->queue.ensure((size_t)total * 4 + 64)) || (rc = e->ctrl.ensure(64)) ||
+        (rc = e->states.ensure((size_t)total * sizeof(NodeState) + 64)) || (rc = e->help.ensure((size_t)nslots * 4 + 64)))
         return rc;
     if (total) HIP_TRY(hipMemcpyAsync(e->nodes.p, b->nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
     auto t2 = clk::now();
@@ -619,9 +789,13 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
     a.total = total;
     a.split_hash = d->variant == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
     a.scratch = nullptr;
+    a.state = e->states.as<NodeState>();
+    a.help = e->help.as<uint32_t>();
+    a.nslots = nslots;
     if (total) {
+        k_init_states<<<256, 256, 0, s>>>(a.state, total, a.help, nslots);
+        HIP_TRY(hipGetLastError());
         if (d->mode == SHGPU_MODE_PERSISTENT) {
-            const int grid = grid_for(e);
             if (d->variant == SHGPU_SHOCKHASH2) {
                 if ((rc = ensure_sh2_scratch(e, (size_t)grid * WAVES_PER_BLOCK))) return rc;
                 a.scratch = e->sh2_scratch.as<uint8_t>();
@@ -629,7 +803,7 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
             k_fill_u32<<<256, 256, 0, s>>>(a.queue, EMPTY_SLOT, total);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(a.queue, roots.data(), roots.size() * 4, hipMemcpyHostToDevice, s));
-            const uint32_t ctrl_init[4] = {0, (uint32_t)roots.size(), 0, 0};
+            const uint32_t ctrl_init[4] = {0, (uint32_t)roots.size(), 0, 0};  // head, tail, error, finished
             HIP_TRY(hipMemcpyAsync(a.ctrl, ctrl_init, sizeof(ctrl_init), hipMemcpyHostToDevice, s));
             HIP_TRY(hipEventRecord(e->ev0, s));
             if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, a, grid, s);
@@ -643,10 +817,13 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
                 if ((rc = ensure_sh2_scratch(e, 1))) return rc;
                 a.scratch = e->sh2_scratch.as<uint8_t>();
             }
+            const uint32_t waves = (uint32_t)e->cu_count;  // one wave per CU joins every node's windows
             for (uint32_t nid = 0; nid < total; nid++) {
-                if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, a, nid, s);
-                else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, a, nid, s);
-                else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, a, nid, s);
+                const bool sh2_leaf = d->variant == SHGPU_SHOCKHASH2 && b->nodes[nid].kind == SHGPU_NODE_LEAF;
+                const uint32_t wv = sh2_leaf ? 1u : waves;
+                if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, a, nid, wv, s);
+                else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, a, nid, wv, s);
+                else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, a, nid, wv, s);
                 if (rc) return rc;
             }
         }
@@ -696,6 +873,43 @@ int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc,
         if (capacity < nodes.size()) return fail(SHGPU_ERR_ARG, "capacity too small");
         std::memcpy(out, nodes.data(), nodes.size() * sizeof(shgpu_node));
     }
+    return SHGPU_OK;
+}
+
+int shgpu_autotune(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
+                   shgpu_tuning *best, double *best_ms) {
+    if (!e || !desc || (n && !d_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
+    int rc = check_variant(desc->variant, desc->leaf_size);
+    if (rc) return rc;
+    const shgpu_tuning base = e->tuning_v[desc->variant];
+    shgpu_build_desc d = *desc;
+    d.mode = SHGPU_MODE_PERSISTENT;
+    d.copy_keys = 0;
+    const uint32_t leafS[3] = {1, 2, 4}, splitS[3] = {1, 2, 4}, waves[3] = {8, 16, 24};
+    const int nleaf = desc->variant == SHGPU_SHOCKHASH2 ? 1 : 3;  // ShockHash2 leaves are not windowed
+    shgpu_tuning winner = base;
+    double winner_ms = 1e30;
+    for (int li = 0; li < nleaf; li++)
+        for (uint32_t ss : splitS)
+            for (uint32_t wv : waves) {
+                shgpu_tuning t = base;
+                t.sh1_seeds_per_lane = desc->variant == SHGPU_SHOCKHASH2 ? base.sh1_seeds_per_lane : leafS[li];
+                t.split_seeds_per_lane = ss;
+                t.waves_per_cu = wv;
+                e->tuning_v[desc->variant] = t;
+                double ms = 1e30;
+                for (int rep = 0; rep < 2; rep++) {
+                    shgpu_build *b = nullptr;
+                    rc = shgpu_build_run(e, &d, d_hashes, n, &b);
+                    if (rc) { e->tuning_v[desc->variant] = base; return rc; }
+                    ms = std::min(ms, b->ms[3]);
+                    shgpu_build_free(b);
+                }
+                if (ms < winner_ms) { winner_ms = ms; winner = t; }
+            }
+    e->tuning_v[desc->variant] = winner;
+    if (best) *best = winner;
+    if (best_ms) *best_ms = winner_ms;
     return SHGPU_OK;
 }
 

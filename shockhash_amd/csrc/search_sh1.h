@@ -74,52 +74,54 @@ __device__ __forceinline__ void sh1_eval(uint64_t key, const uint64_t (&seed)[S]
     }
 }
 
-// Wave-uniform arguments. `keys` points at the leaf's m (<= 64) keys. Returns the absolute seed,
-// or SHGPU_NOT_FOUND when no seed in [start, start + max_trials) qualifies. `evals` (per wave,
-// lane 0 meaningful) accumulates hash evaluations performed.
+// One window of a leaf search: seeds start + woff + [0, 64*S), m >= 2 keys (m <= 64) at `keys`.
+// Returns the offset (seed - start) of the smallest qualifying seed in the window or SHGPU_NOT_FOUND.
 template <int S>
-__device__ __forceinline__ uint64_t sh1_leaf_search(const uint64_t *keys, uint32_t m, uint64_t start, uint64_t max_trials,
-                                    PfLds &L) {
+__device__ __forceinline__ uint64_t sh1_window(const uint64_t *keys, uint32_t m, uint64_t start, uint64_t woff,
+                                               PfLds &L) {
     const uint32_t lane = lane_id();
-    if (m < 2) return start;
     const uint32_t halfA = m / 2, halfB = (m + 1) / 2;
     const uint32_t fullA = mask_bits(halfA), fullB = mask_bits(halfB);
     const uint64_t kv = lane < m ? keys[lane] : 0;
-    for (uint64_t base = 0; base < max_trials; base += 64 * S) {
-        uint64_t seed[S];
-        uint32_t mA[S], mB[S];
+    uint64_t seed[S];
+    uint32_t mA[S], mB[S];
 #pragma unroll
-        for (int s = 0; s < S; s++) { seed[s] = start + base + (uint64_t)(s * 64 + lane); mA[s] = 0; mB[s] = 0; }
-        if (m == 64) {
+    for (int s = 0; s < S; s++) { seed[s] = start + woff + (uint64_t)(s * 64 + lane); mA[s] = 0; mB[s] = 0; }
+    if (m == 64) {
 #pragma unroll 16
-            for (uint32_t k = 0; k < 64; k++) {
-                const uint64_t key = readlane64(kv, k);
-                sh1_eval<S>(key, seed, halfA, halfB, mA, mB);
-            }
-        } else {
-            for (uint32_t k = 0; k < m; k++) {
-                const uint64_t key = readlane64(kv, k);
-                sh1_eval<S>(key, seed, halfA, halfB, mA, mB);
-            }
-        }
+        for (uint32_t k = 0; k < 64; k++) sh1_eval<S>(readlane64(kv, k), seed, halfA, halfB, mA, mB);
+    } else {
+        for (uint32_t k = 0; k < m; k++) sh1_eval<S>(readlane64(kv, k), seed, halfA, halfB, mA, mB);
+    }
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            uint64_t cand = ballot(mA[s] == fullA && mB[s] == fullB);
-            while (cand) {
-                const uint32_t l = (uint32_t)__builtin_ctzll(cand);
-                cand &= cand - 1;
-                const uint64_t off = base + (uint64_t)(s * 64 + l);
-                if (off >= max_trials) return SHGPU_NOT_FOUND;
-                const uint64_t x = start + off;
-                uint32_t a = 0, b = 0;
-                if (lane < m) {
-                    const uint64_t r = remix(kv + x);
-                    a = __umulhi((uint32_t)r, halfA);
-                    b = __umulhi((uint32_t)(r >> 32), halfB) + halfA;
-                }
-                if (pseudoforest_wave(a, b, lane < m, m, L)) return x;
+    for (int s = 0; s < S; s++) {
+        uint64_t cand = ballot(mA[s] == fullA && mB[s] == fullB);
+        while (cand) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(cand);
+            cand &= cand - 1;
+            const uint64_t off = woff + (uint64_t)(s * 64 + l);
+            const uint64_t x = start + off;
+            uint32_t a = 0, b = 0;
+            if (lane < m) {
+                const uint64_t r = remix(kv + x);
+                a = __umulhi((uint32_t)r, halfA);
+                b = __umulhi((uint32_t)(r >> 32), halfB) + halfA;
             }
+            if (pseudoforest_wave(a, b, lane < m, m, L)) return off;
         }
+    }
+    return SHGPU_NOT_FOUND;
+}
+
+// Sequential leaf search by one wave. Returns the absolute seed, or SHGPU_NOT_FOUND when no seed in
+// [start, start + max_trials) qualifies.
+template <int S>
+__device__ __forceinline__ uint64_t sh1_leaf_search(const uint64_t *keys, uint32_t m, uint64_t start,
+                                                    uint64_t max_trials, PfLds &L) {
+    if (m < 2) return start;
+    for (uint64_t woff = 0; woff < max_trials; woff += 64 * S) {
+        const uint64_t off = sh1_window<S>(keys, m, start, woff, L);
+        if (off != SHGPU_NOT_FOUND) return off < max_trials ? start + off : SHGPU_NOT_FOUND;
     }
     return SHGPU_NOT_FOUND;
 }

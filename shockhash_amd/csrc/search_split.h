@@ -72,60 +72,64 @@ __device__ __forceinline__ void split_eval(uint64_t key, const uint64_t (&seed)[
     }
 }
 
+// One window: seeds start + woff + [0, 64*S). Returns the offset (seed - start) of the first seed whose
+// parts have the target sizes, or SHGPU_NOT_FOUND. Seeds are tried in (slot, lane) order.
 template <int S, int NT, int SIMD>
-__device__ __forceinline__ uint64_t split_search_t(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start, uint64_t max_trials) {
+__device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
+                                                   uint64_t woff) {
     const uint32_t lane = lane_id();
     const uint32_t m = sh.m;
-    for (uint64_t base = 0; base < max_trials; base += 64 * S) {
-        uint64_t seed[S];
-        uint32_t c[S][NT];
+    uint64_t seed[S];
+    uint32_t c[S][NT];
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            seed[s] = start + base + (uint64_t)(s * 64 + lane);
+    for (int s = 0; s < S; s++) {
+        seed[s] = start + woff + (uint64_t)(s * 64 + lane);
 #pragma unroll
-            for (int p = 0; p < NT; p++) c[s][p] = 0;
-        }
-        for (uint32_t chunk = 0; chunk < m; chunk += 64) {
-            const uint64_t kv = chunk + lane < m ? keys[chunk + lane] : 0;
-            const uint32_t cnt = m - chunk < 64 ? m - chunk : 64;
-            if (cnt == 64) {
+        for (int p = 0; p < NT; p++) c[s][p] = 0;
+    }
+    for (uint32_t chunk = 0; chunk < m; chunk += 64) {
+        const uint64_t kv = chunk + lane < m ? keys[chunk + lane] : 0;
+        const uint32_t cnt = m - chunk < 64 ? m - chunk : 64;
+        if (cnt == 64) {
 #pragma unroll 8
-                for (uint32_t k = 0; k < 64; k++) {
-                    const uint64_t key = readlane64(kv, k);
-                    split_eval<S, NT, SIMD>(key, seed, sh, c);
-                }
-            } else {
-                for (uint32_t k = 0; k < cnt; k++) {
-                    const uint64_t key = readlane64(kv, k);
-                    split_eval<S, NT, SIMD>(key, seed, sh, c);
-                }
-            }
+            for (uint32_t k = 0; k < 64; k++) split_eval<S, NT, SIMD>(readlane64(kv, k), seed, sh, c);
+        } else {
+            for (uint32_t k = 0; k < cnt; k++) split_eval<S, NT, SIMD>(readlane64(kv, k), seed, sh, c);
         }
+    }
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            bool ok = true;
+    for (int s = 0; s < S; s++) {
+        bool ok = true;
 #pragma unroll
-            for (int p = 0; p < NT; p++) ok = ok && c[s][p] == sh.t[p];
-            const uint64_t b = ballot(ok);
-            if (b) {
-                const uint64_t off = base + (uint64_t)(s * 64 + __builtin_ctzll(b));
-                return off < max_trials ? start + off : SHGPU_NOT_FOUND;
-            }
-        }
+        for (int p = 0; p < NT; p++) ok = ok && c[s][p] == sh.t[p];
+        const uint64_t b = ballot(ok);
+        if (b) return woff + (uint64_t)(s * 64 + __builtin_ctzll(b));
     }
     return SHGPU_NOT_FOUND;
 }
 
 template <int S>
-__device__ __forceinline__ uint64_t split_search(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start, uint64_t max_trials) {
+__device__ __forceinline__ uint64_t split_window(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
+                                                 uint64_t woff) {
     if (sh.simd) {
-        if (sh.nt == 1) return split_search_t<S, 1, 1>(keys, sh, start, max_trials);
-        if (sh.nt == 2) return split_search_t<S, 2, 1>(keys, sh, start, max_trials);
-        return split_search_t<S, 3, 1>(keys, sh, start, max_trials);
+        if (sh.nt == 1) return split_window_t<S, 1, 1>(keys, sh, start, woff);
+        if (sh.nt == 2) return split_window_t<S, 2, 1>(keys, sh, start, woff);
+        return split_window_t<S, 3, 1>(keys, sh, start, woff);
     }
-    if (sh.nt == 1) return split_search_t<S, 1, 0>(keys, sh, start, max_trials);
-    if (sh.nt == 2) return split_search_t<S, 2, 0>(keys, sh, start, max_trials);
-    return split_search_t<S, 3, 0>(keys, sh, start, max_trials);
+    if (sh.nt == 1) return split_window_t<S, 1, 0>(keys, sh, start, woff);
+    if (sh.nt == 2) return split_window_t<S, 2, 0>(keys, sh, start, woff);
+    return split_window_t<S, 3, 0>(keys, sh, start, woff);
+}
+
+// Sequential search by one wave: the absolute seed, or SHGPU_NOT_FOUND below start + max_trials.
+template <int S>
+__device__ __forceinline__ uint64_t split_search(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
+                                                 uint64_t max_trials) {
+    for (uint64_t woff = 0; woff < max_trials; woff += 64 * S) {
+        const uint64_t off = split_window<S>(keys, sh, start, woff);
+        if (off != SHGPU_NOT_FOUND) return off < max_trials ? start + off : SHGPU_NOT_FOUND;
+    }
+    return SHGPU_NOT_FOUND;
 }
 
 // Stable counting-sort partition of keys[0..m) by part for seed x into tmp, then copied back

@@ -72,6 +72,10 @@ _SIGS = {
     "shgpu_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
     "shgpu_set_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
     "shgpu_sync": (ctypes.c_int, [P]),
+    "shgpu_get_variant_tuning": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(Tuning)]),
+    "shgpu_set_variant_tuning": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(Tuning)]),
+    "shgpu_autotune": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(Tuning),
+                                      ctypes.POINTER(ctypes.c_double)]),
     "shgpu_sh1_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_sh2_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_split_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
@@ -234,6 +238,25 @@ class Engine:
         cur.update(kw)
         t = Tuning(**cur)
         _check(load().shgpu_set_tuning(self.h, ctypes.byref(t)))
+
+    def variant_tuning(self, variant):
+        t = Tuning()
+        _check(load().shgpu_get_variant_tuning(self.h, VARIANTS[variant], ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+    def set_variant_tuning(self, variant, **kw):
+        cur = self.variant_tuning(variant)
+        cur.update(kw)
+        t = Tuning(**cur)
+        _check(load().shgpu_set_variant_tuning(self.h, VARIANTS[variant], ctypes.byref(t)))
+
+    def autotune(self, d_hashes, n, variant, leaf_size, bucket_size):
+        """Times every candidate schedule on the sample and keeps the fastest for the variant."""
+        desc = BuildDesc(VARIANTS[variant], leaf_size, bucket_size, 0, 0, 0)
+        t = Tuning()
+        ms = ctypes.c_double()
+        _check(load().shgpu_autotune(self.h, ctypes.byref(desc), d_hashes.ptr, n, ctypes.byref(t), ctypes.byref(ms)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}, ms.value
 
     def sync(self):
         _check(load().shgpu_sync(self.h))
