@@ -76,11 +76,6 @@ int shgpu_set_tuning(shgpu_engine *e, const shgpu_tuning *t);
 /* Build schedule of one variant (SHGPU_SHOCKHASH / SHGPU_SIMD_SHOCKHASH / SHGPU_SHOCKHASH2). */
 int shgpu_get_variant_tuning(shgpu_engine *e, int variant, shgpu_tuning *out);
 int shgpu_set_variant_tuning(shgpu_engine *e, int variant, const shgpu_tuning *t);
-/* Autotuner: builds the given sample with every candidate schedule (seeds per lane for leaf and
- * split windows, waves per CU), keeps the fastest for that variant and returns it. Results are
- * identical for every schedule; only the time differs. d_hashes: device fingerprints. */
-int shgpu_autotune(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
-                   shgpu_tuning *best, double *best_ms);
 /* Synchronises the engine's stream. */
 int shgpu_sync(shgpu_engine *e);
 
@@ -122,6 +117,11 @@ typedef struct shgpu_build_desc {
 
 int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
                     shgpu_build **out);
+/* Autotuner: builds the given sample with every candidate schedule (seeds per lane for leaf and
+ * split windows, waves per CU), keeps the fastest for that variant and returns it. Results are
+ * identical for every schedule; only the time differs. d_hashes: device fingerprints. */
+int shgpu_autotune(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
+                   shgpu_tuning *best, double *best_ms);
 /* Same with the fingerprints in host memory (staged to the device by the engine). */
 int shgpu_build_run_host(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *h_hashes, uint64_t n,
                          shgpu_build **out);
