@@ -41,7 +41,8 @@ enum {
     SHGPU_ERR_SEARCH_LIMIT = 4,      /* a node exhausted its seed bound (duplicate keys?) */
     SHGPU_ERR_BUCKET_TOO_LARGE = 5,  /* bucket above the encodable maximum (2999 keys) */
     SHGPU_ERR_UNSUPPORTED = 6,       /* configuration the GPU engine does not implement */
-    SHGPU_ERR_NO_DEVICE = 7          /* no HIP device / extension unusable */
+    SHGPU_ERR_NO_DEVICE = 7,         /* no HIP device / extension unusable */
+    SHGPU_ERR_ADMISSION = 8          /* request larger than the serving HBM budget / server stopped */
 };
 
 /* Construction variants (the three reference class templates). */
@@ -142,6 +143,42 @@ int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc,
  * (host clock, launch to results on the host), [3] the persistent search kernel alone (HIP events on
  * the engine's stream; 0 in plain mode). */
 int shgpu_build_timings(const shgpu_build *b, double *ms4);
+
+/* ---- Serving: continuous batching of construction requests.
+ * A server owns `slots` concurrent builds (one stream, buffer set and host worker each; each slot's
+ * persistent kernel uses 1/slots of the device). Requests are admitted in arrival order as soon as a
+ * slot is free and their device footprint fits the HBM budget, so work of requests that arrive while
+ * others run overlaps on the device. Results equal a lone shgpu_build_run of the same keys. */
+typedef struct shgpu_server shgpu_server;
+typedef struct shgpu_server_config {
+    int variant;
+    uint32_t leaf_size;
+    uint64_t bucket_size;
+    uint32_t slots;             /* concurrent builds, 1..16 */
+    uint64_t hbm_budget_bytes;  /* device bytes the server may hold for running requests */
+    int copy_keys;              /* results keep the leaf-ordered keys */
+} shgpu_server_config;
+
+int shgpu_server_create(shgpu_engine *e, const shgpu_server_config *cfg, shgpu_server **out);
+/* Waits for running requests; queued ones are dropped. Results not taken are freed. */
+void shgpu_server_destroy(shgpu_server *s);
+/* h_hashes (n fingerprints, host memory) must stay valid until the request is taken.
+ * SHGPU_ERR_ADMISSION if the request alone exceeds the budget. */
+int shgpu_server_submit(shgpu_server *s, const uint64_t *h_hashes, uint64_t n, uint64_t *id);
+/* The same for fingerprints already resident in device memory (read only, valid until taken). */
+int shgpu_server_submit_device(shgpu_server *s, const uint64_t *d_hashes, uint64_t n, uint64_t *id);
+/* Blocks until the request finished (timeout_s < 0: no limit); status 2 = done, < 0 = -(error code). */
+int shgpu_server_wait(shgpu_server *s, uint64_t id, double timeout_s, int *status);
+/* Removes a finished request and hands over its result; latency = submit to completion, queue = submit
+ * to admission. */
+int shgpu_server_take(shgpu_server *s, uint64_t id, shgpu_build **out, double *latency_ms, double *queue_ms);
+/* Device bytes the server reserves for a request of n keys while it runs. Besides these, a ShockHash2
+ * server holds a fixed per-slot leaf scratch for its lifetime; it counts against the budget from
+ * creation (shgpu_server_create fails with SHGPU_ERR_ADMISSION when it alone exceeds the budget). */
+int shgpu_server_request_bytes(const shgpu_server *s, uint64_t n, uint64_t *bytes);
+/* reserved_bytes = fixed scratch + reservations of running requests. */
+int shgpu_server_info(shgpu_server *s, uint64_t *queued, uint64_t *running, uint64_t *reserved_bytes,
+                      uint64_t *peak_reserved_bytes);
 
 #ifdef __cplusplus
 }

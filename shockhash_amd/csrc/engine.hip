@@ -4,6 +4,12 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -64,10 +70,42 @@ struct DevBuf {
 
 }  // namespace
 
+// Device buffers of one construction in flight (the engine's own, or one serving slot's).
+struct BuildBuffers {
+    DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl, states, help,
+        sh2_scratch, staging;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t bytes() const {
+        const DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
+                               &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging};
+        size_t t = 0;
+        for (const DevBuf *d : all) t += d->bytes;
+        return t;
+    }
+    void release() {
+        DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
+                         &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging};
+        for (DevBuf *d : all) {
+            if (d->p) (void)hipFree(d->p);
+            d->p = nullptr;
+            d->bytes = 0;
+        }
+    }
+    int create_events() {
+        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
+            return fail(SHGPU_ERR_HIP, "hipEventCreate failed");
+        return SHGPU_OK;
+    }
+    void destroy_events() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        ev0 = ev1 = nullptr;
+    }
+};
+
 struct shgpu_engine {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cu_count = 0;
     int clock_khz = 0;
     char arch[64] = {0};
@@ -75,9 +113,8 @@ struct shgpu_engine {
     // MI355X for leaf 30 / 58 / 64 builds (see shgpu_autotune).
     shgpu_tuning tuning_v[3] = {{4, 2, 24, 1ull << 40}, {4, 2, 24, 1ull << 40}, {4, 2, 24, 1ull << 40}};
     shgpu_tuning tuning{4, 2, 24, 1ull << 40};  // batched per-node entry points
-    DevBuf counter, sh2_scratch, staging;
-    // build buffers
-    DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl, states, help;
+    DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
+    BuildBuffers main;                          // shgpu_build_run
 };
 
 struct shgpu_build {
@@ -548,6 +585,7 @@ const char *shgpu_status_string(int status) {
         case SHGPU_ERR_BUCKET_TOO_LARGE: return "bucket larger than the encodable maximum";
         case SHGPU_ERR_UNSUPPORTED: return "unsupported configuration";
         case SHGPU_ERR_NO_DEVICE: return "no usable HIP device";
+        case SHGPU_ERR_ADMISSION: return "request exceeds the serving budget or the server stopped";
         default: return "unknown status";
     }
 }
@@ -568,8 +606,7 @@ int shgpu_engine_create(int device, shgpu_engine **out) {
     e->cu_count = prop.multiProcessorCount;
     e->clock_khz = prop.clockRate;
     std::snprintf(e->arch, sizeof(e->arch), "%s", prop.gcnArchName);
-    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess || e->main.create_events()) {
         delete e;
         return fail(SHGPU_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
@@ -582,8 +619,7 @@ void shgpu_engine_destroy(shgpu_engine *e) {
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->stream);
     (void)hipStreamDestroy(e->stream);
-    if (e->ev0) (void)hipEventDestroy(e->ev0);
-    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    e->main.destroy_events();
     delete e;
 }
 
@@ -709,7 +745,10 @@ int shgpu_split_search(shgpu_engine *e, uint64_t *d_keys, const shgpu_split_task
     return SHGPU_OK;
 }
 
-int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *d_hashes, uint64_t n, shgpu_build **out) {
+// The construction pipeline on a buffer set and stream; the persistent grid uses 1/grid_div of the
+// variant's schedule (serving slots share the device).
+static int run_build(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t grid_div, const shgpu_build_desc *d,
+                     const uint64_t *d_hashes, uint64_t n, shgpu_build **out) {
     using clk = std::chrono::steady_clock;
     if (!e || !d || !out || (n && !d_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
     *out = nullptr;
@@ -718,7 +757,6 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
     if (d->bucket_size == 0) return fail(SHGPU_ERR_ARG, "bucket_size must be positive");
     if (n >= (1ull << 32)) return fail(SHGPU_ERR_UNSUPPORTED, "more than 2^32 - 1 keys");
     HIP_TRY(hipSetDevice(e->device));
-    hipStream_t s = e->stream;
     auto t0 = clk::now();
     shgpu_build *b = new shgpu_build();
     b->n = n;
@@ -727,25 +765,25 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
     struct Guard { shgpu_build *&b; bool ok = false; ~Guard() { if (!ok) { delete b; b = nullptr; } } } guard{b};
 
     // 1. partition: sort by .first, emit .second in order (Sorter.hpp:15-36), bucket bounds.
-    if ((rc = e->firsts_in.ensure(n * 8 + 8)) || (rc = e->firsts_out.ensure(n * 8 + 8)) ||
-        (rc = e->seconds_in.ensure(n * 8 + 8)) || (rc = e->keys.ensure(n * 8 + 64)) || (rc = e->tmp.ensure(n * 8 + 64)) ||
-        (rc = e->size_acc.ensure((nb + 1) * 8)))
+    if ((rc = B.firsts_in.ensure(n * 8 + 8)) || (rc = B.firsts_out.ensure(n * 8 + 8)) ||
+        (rc = B.seconds_in.ensure(n * 8 + 8)) || (rc = B.keys.ensure(n * 8 + 64)) || (rc = B.tmp.ensure(n * 8 + 64)) ||
+        (rc = B.size_acc.ensure((nb + 1) * 8)))
         return rc;
     size_t sort_bytes = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, e->firsts_in.as<uint64_t>(), e->firsts_out.as<uint64_t>(),
-                                               e->seconds_in.as<uint64_t>(), e->keys.as<uint64_t>(), (int)n, 0, 64, s));
-    if ((rc = e->sort_tmp.ensure(sort_bytes + 16))) return rc;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, B.firsts_in.as<uint64_t>(), B.firsts_out.as<uint64_t>(),
+                                               B.seconds_in.as<uint64_t>(), B.keys.as<uint64_t>(), (int)n, 0, 64, s));
+    if ((rc = B.sort_tmp.ensure(sort_bytes + 16))) return rc;
     if (n) {
-        k_unzip_hashes<<<1024, 256, 0, s>>>(d_hashes, n, e->firsts_in.as<uint64_t>(), e->seconds_in.as<uint64_t>());
+        k_unzip_hashes<<<1024, 256, 0, s>>>(d_hashes, n, B.firsts_in.as<uint64_t>(), B.seconds_in.as<uint64_t>());
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(e->sort_tmp.p, sort_bytes, e->firsts_in.as<uint64_t>(),
-                                                   e->firsts_out.as<uint64_t>(), e->seconds_in.as<uint64_t>(),
-                                                   e->keys.as<uint64_t>(), (int)n, 0, 64, s));
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(B.sort_tmp.p, sort_bytes, B.firsts_in.as<uint64_t>(),
+                                                   B.firsts_out.as<uint64_t>(), B.seconds_in.as<uint64_t>(),
+                                                   B.keys.as<uint64_t>(), (int)n, 0, 64, s));
     }
-    k_bucket_bounds<<<1024, 256, 0, s>>>(e->firsts_out.as<uint64_t>(), n, nb, e->size_acc.as<uint64_t>());
+    k_bucket_bounds<<<1024, 256, 0, s>>>(B.firsts_out.as<uint64_t>(), n, nb, B.size_acc.as<uint64_t>());
     HIP_TRY(hipGetLastError());
     b->size_acc.resize(nb + 1);
-    HIP_TRY(hipMemcpyAsync(b->size_acc.data(), e->size_acc.p, (nb + 1) * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(b->size_acc.data(), B.size_acc.p, (nb + 1) * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     auto t1 = clk::now();
 
@@ -765,13 +803,13 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
     }
     const uint32_t total = (uint32_t)b->nodes.size();
     b->codes.assign(total, 0);
-    const int grid = grid_for(e, e->tuning_v[d->variant]);
+    const int grid = std::max(1, grid_for(e, e->tuning_v[d->variant]) / (int)std::max(1u, grid_div));
     const uint32_t nslots = (uint32_t)grid * WAVES_PER_BLOCK;
-    if ((rc = e->nodes.ensure((size_t)total * sizeof(shgpu_node) + 64)) || (rc = e->codes.ensure((size_t)total * 8 + 64)) ||
-        (rc = e->queue.ensure((size_t)total * 4 + 64)) || (rc = e->ctrl.ensure(64)) ||
-        (rc = e->states.ensure((size_t)total * sizeof(NodeState) + 64)) || (rc = e->help.ensure((size_t)nslots * 4 + 64)))
+    if ((rc = B.nodes.ensure((size_t)total * sizeof(shgpu_node) + 64)) || (rc = B.codes.ensure((size_t)total * 8 + 64)) ||
+        (rc = B.queue.ensure((size_t)total * 4 + 64)) || (rc = B.ctrl.ensure(64)) ||
+        (rc = B.states.ensure((size_t)total * sizeof(NodeState) + 64)) || (rc = B.help.ensure((size_t)nslots * 4 + 64)))
         return rc;
-    if (total) HIP_TRY(hipMemcpyAsync(e->nodes.p, b->nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
+    if (total) HIP_TRY(hipMemcpyAsync(B.nodes.p, b->nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
     auto t2 = clk::now();
 
     if ((rc = check_forest(b->nodes, (uint32_t)roots.size()))) return rc;
@@ -780,42 +818,42 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
     uint32_t ctrl_out[4] = {0, 0, 0, 0};
     bool timed_kernel = false;
     BuildArgs a;
-    a.keys = e->keys.as<uint64_t>();
-    a.tmp = e->tmp.as<uint64_t>();
-    a.nodes = e->nodes.as<shgpu_node>();
-    a.codes = e->codes.as<uint64_t>();
-    a.queue = e->queue.as<uint32_t>();
-    a.ctrl = e->ctrl.as<uint32_t>();
+    a.keys = B.keys.as<uint64_t>();
+    a.tmp = B.tmp.as<uint64_t>();
+    a.nodes = B.nodes.as<shgpu_node>();
+    a.codes = B.codes.as<uint64_t>();
+    a.queue = B.queue.as<uint32_t>();
+    a.ctrl = B.ctrl.as<uint32_t>();
     a.total = total;
     a.split_hash = d->variant == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
     a.scratch = nullptr;
-    a.state = e->states.as<NodeState>();
-    a.help = e->help.as<uint32_t>();
+    a.state = B.states.as<NodeState>();
+    a.help = B.help.as<uint32_t>();
     a.nslots = nslots;
     if (total) {
         k_init_states<<<256, 256, 0, s>>>(a.state, total, a.help, nslots);
         HIP_TRY(hipGetLastError());
         if (d->mode == SHGPU_MODE_PERSISTENT) {
             if (d->variant == SHGPU_SHOCKHASH2) {
-                if ((rc = ensure_sh2_scratch(e, (size_t)grid * WAVES_PER_BLOCK))) return rc;
-                a.scratch = e->sh2_scratch.as<uint8_t>();
+                if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * SH2_SCRATCH_BYTES))) return rc;
+                a.scratch = B.sh2_scratch.as<uint8_t>();
             }
             k_fill_u32<<<256, 256, 0, s>>>(a.queue, EMPTY_SLOT, total);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(a.queue, roots.data(), roots.size() * 4, hipMemcpyHostToDevice, s));
             const uint32_t ctrl_init[4] = {0, (uint32_t)roots.size(), 0, 0};  // head, tail, error, finished
             HIP_TRY(hipMemcpyAsync(a.ctrl, ctrl_init, sizeof(ctrl_init), hipMemcpyHostToDevice, s));
-            HIP_TRY(hipEventRecord(e->ev0, s));
+            HIP_TRY(hipEventRecord(B.ev0, s));
             if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, a, grid, s);
             else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_build_v<SHGPU_SIMD_SHOCKHASH>(e, a, grid, s);
             else rc = launch_build_v<SHGPU_SHOCKHASH2>(e, a, grid, s);
             if (rc) return rc;
-            HIP_TRY(hipEventRecord(e->ev1, s));
+            HIP_TRY(hipEventRecord(B.ev1, s));
             timed_kernel = true;
         } else {
             if (d->variant == SHGPU_SHOCKHASH2) {
-                if ((rc = ensure_sh2_scratch(e, 1))) return rc;
-                a.scratch = e->sh2_scratch.as<uint8_t>();
+                if ((rc = B.sh2_scratch.ensure(SH2_SCRATCH_BYTES))) return rc;
+                a.scratch = B.sh2_scratch.as<uint8_t>();
             }
             const uint32_t waves = (uint32_t)e->cu_count;  // one wave per CU joins every node's windows
             for (uint32_t nid = 0; nid < total; nid++) {
@@ -832,7 +870,7 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
     }
     if (d->copy_keys && n) {
         b->keys.resize(n);
-        HIP_TRY(hipMemcpyAsync(b->keys.data(), e->keys.p, n * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(b->keys.data(), B.keys.p, n * 8, hipMemcpyDeviceToHost, s));
     }
     HIP_TRY(hipStreamSynchronize(s));
     auto t3 = clk::now();
@@ -846,12 +884,17 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
     b->ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
     if (timed_kernel) {
         float kms = 0;
-        HIP_TRY(hipEventElapsedTime(&kms, e->ev0, e->ev1));
+        HIP_TRY(hipEventElapsedTime(&kms, B.ev0, B.ev1));
         b->ms[3] = kms;
     }
     guard.ok = true;
     *out = b;
     return SHGPU_OK;
+}
+
+int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *d_hashes, uint64_t n, shgpu_build **out) {
+    if (!e) return fail(SHGPU_ERR_ARG, "NULL argument");
+    return run_build(e, e->main, e->stream, 1, d, d_hashes, n, out);
 }
 
 int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc, uint64_t nbuckets, shgpu_node *out,
@@ -992,6 +1035,269 @@ int shgpu_build_keys(const shgpu_build *b, uint64_t *keys) {
 int shgpu_build_timings(const shgpu_build *b, double *ms4) {
     if (!b || !ms4) return fail(SHGPU_ERR_ARG, "NULL argument");
     for (int i = 0; i < 4; i++) ms4[i] = b->ms[i];
+    return SHGPU_OK;
+}
+
+}  // extern "C"
+
+// ===================================================================================== serving
+
+namespace {
+struct Request {
+    uint64_t id = 0;
+    const uint64_t *hashes = nullptr;  // caller-owned fingerprints (valid until taken)
+    bool on_device = false;            // hashes is a device pointer (already resident in HBM)
+    uint64_t n = 0;
+    size_t need = 0;                   // device bytes reserved while running
+    std::chrono::steady_clock::time_point t_submit, t_start, t_done;
+    int status = 0;                    // 0 queued, 1 running, 2 done, < 0: -(error code)
+    std::string error;
+    shgpu_build *result = nullptr;
+};
+}  // namespace
+
+struct shgpu_server {
+    shgpu_engine *e = nullptr;
+    shgpu_server_config cfg{};
+    struct Slot {
+        BuildBuffers B;
+        size_t held = 0;  // B.bytes() after the slot's last request (read and written under mtx)
+        hipStream_t stream = nullptr;
+        std::thread worker;
+    };
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mtx;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> waiting;
+    std::unordered_map<uint64_t, Request *> requests;
+    uint64_t next_id = 1;
+    size_t reserved = 0, peak_reserved = 0;
+    uint64_t running = 0;
+    bool stop = false;
+
+    // Device bytes a build of n keys holds while it runs: sort buffers and keys (5 x 8 B), staged
+    // fingerprints (16 B), radix-sort scratch (about 24 B), forest records (node, code, queue slot,
+    // search state) and a fixed margin.
+    size_t need_for(uint64_t n) const {
+        const uint64_t nodes = n / std::max<uint32_t>(1, cfg.leaf_size) * 2 + 64;
+        return n * (5 * 8 + 16 + 24) + nodes * (sizeof(shgpu_node) + 8 + 4 + sizeof(NodeState)) + (8u << 20);
+    }
+    // Bytes every slot holds regardless of the request: the ShockHash2 per-wave leaf scratch.
+    size_t fixed_bytes() const {
+        return cfg.variant == SHGPU_SHOCKHASH2 ? cfg.slots * slot_waves() * SH2_SCRATCH_BYTES : 0;
+    }
+    size_t slot_waves() const {
+        const int grid = std::max(1, grid_for(e, e->tuning_v[cfg.variant]) / (int)cfg.slots);
+        return (size_t)grid * WAVES_PER_BLOCK;
+    }
+    size_t allocated() const {
+        size_t t = 0;
+        for (auto &sl : slots) t += sl->held;
+        return t;
+    }
+
+    void work(Slot &slot) {
+        (void)hipSetDevice(e->device);
+        for (;;) {
+            Request *r = nullptr;
+            {
+                std::unique_lock<std::mutex> lock(mtx);
+                cv_work.wait(lock, [&] {
+                    return stop || (!waiting.empty() && reserved + waiting.front()->need <= cfg.hbm_budget_bytes);
+                });
+                if (stop && waiting.empty()) return;
+                if (waiting.empty()) continue;
+                r = waiting.front();
+                waiting.pop_front();
+                reserved += r->need;
+                peak_reserved = std::max(peak_reserved, reserved);
+                running++;
+                r->status = 1;
+                r->t_start = std::chrono::steady_clock::now();
+            }
+            int rc = SHGPU_OK;
+            const uint64_t *d_hashes = r->hashes;
+            if (!r->on_device) {
+                rc = slot.B.staging.ensure(r->n * 16 + 16);
+                if (rc == SHGPU_OK && r->n &&
+                    hipMemcpyAsync(slot.B.staging.p, r->hashes, r->n * 16, hipMemcpyHostToDevice, slot.stream) != hipSuccess)
+                    rc = fail(SHGPU_ERR_HIP, "hipMemcpyAsync of request fingerprints failed");
+                d_hashes = slot.B.staging.as<uint64_t>();
+            }
+            shgpu_build *b = nullptr;
+            if (rc == SHGPU_OK) {
+                shgpu_build_desc d{};
+                d.variant = cfg.variant;
+                d.leaf_size = cfg.leaf_size;
+                d.bucket_size = cfg.bucket_size;
+                d.mode = SHGPU_MODE_PERSISTENT;
+                d.copy_keys = cfg.copy_keys;
+                rc = run_build(e, slot.B, slot.stream, cfg.slots, &d, d_hashes, r->n, &b);
+            }
+            {
+                std::lock_guard<std::mutex> lock(mtx);
+                r->t_done = std::chrono::steady_clock::now();
+                r->result = b;
+                r->status = rc == SHGPU_OK ? 2 : -rc;
+                if (rc != SHGPU_OK) r->error = g_err;
+                reserved -= r->need;
+                running--;
+                // keep the device footprint within the budget: a slot whose buffers grew beyond what the
+                // budget allows gives them back
+                slot.held = slot.B.bytes();
+                if (allocated() > cfg.hbm_budget_bytes) {
+                    slot.B.release();
+                    slot.held = 0;
+                }
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int shgpu_server_create(shgpu_engine *e, const shgpu_server_config *cfg, shgpu_server **out) {
+    if (!e || !cfg || !out) return fail(SHGPU_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    int rc = check_variant(cfg->variant, cfg->leaf_size);
+    if (rc) return rc;
+    if (cfg->slots < 1 || cfg->slots > 16) return fail(SHGPU_ERR_ARG, "slots must be in [1, 16]");
+    if (cfg->bucket_size == 0) return fail(SHGPU_ERR_ARG, "bucket_size must be positive");
+    HIP_TRY(hipSetDevice(e->device));
+    auto *sv = new shgpu_server();
+    sv->e = e;
+    sv->cfg = *cfg;
+    sv->reserved = sv->peak_reserved = sv->fixed_bytes();
+    if (sv->reserved + sv->need_for(0) > cfg->hbm_budget_bytes) {
+        const std::string msg = "HBM budget below the slots' fixed scratch (" + std::to_string(sv->reserved) + " bytes)";
+        delete sv;
+        return fail(SHGPU_ERR_ADMISSION, msg);
+    }
+    for (uint32_t i = 0; i < cfg->slots; i++) {
+        auto sl = std::make_unique<shgpu_server::Slot>();
+        if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess || sl->B.create_events()) {
+            sv->stop = true;
+            for (auto &x : sv->slots) x->worker.join();
+            delete sv;
+            return fail(SHGPU_ERR_HIP, "stream creation failed");
+        }
+        sv->slots.push_back(std::move(sl));
+    }
+    for (auto &sl : sv->slots) {
+        shgpu_server::Slot *p = sl.get();
+        sl->worker = std::thread([sv, p] { sv->work(*p); });
+    }
+    *out = sv;
+    return SHGPU_OK;
+}
+
+void shgpu_server_destroy(shgpu_server *sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lock(sv->mtx);
+        sv->stop = true;
+        for (Request *r : sv->waiting) { r->status = -SHGPU_ERR_ADMISSION; r->error = "server stopped"; }
+        sv->waiting.clear();
+    }
+    sv->cv_work.notify_all();
+    for (auto &sl : sv->slots) sl->worker.join();
+    (void)hipSetDevice(sv->e->device);
+    for (auto &sl : sv->slots) {
+        (void)hipStreamSynchronize(sl->stream);
+        (void)hipStreamDestroy(sl->stream);
+        sl->B.destroy_events();
+        sl->B.release();
+    }
+    for (auto &kv : sv->requests) { shgpu_build_free(kv.second->result); delete kv.second; }
+    delete sv;
+}
+
+static int server_submit(shgpu_server *sv, const uint64_t *hashes, bool on_device, uint64_t n, uint64_t *id) {
+    if (!sv || !id || (n && !hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
+    auto *r = new Request();
+    r->hashes = hashes;
+    r->on_device = on_device;
+    r->n = n;
+    r->need = sv->need_for(n) - (on_device ? n * 16 : 0);
+    if (sv->fixed_bytes() + r->need > sv->cfg.hbm_budget_bytes) {
+        delete r;
+        return fail(SHGPU_ERR_ADMISSION, "request needs more device memory than the server's budget");
+    }
+    if (n >= (1ull << 32)) {
+        delete r;
+        return fail(SHGPU_ERR_UNSUPPORTED, "more than 2^32 - 1 keys");
+    }
+    {
+        std::lock_guard<std::mutex> lock(sv->mtx);
+        if (sv->stop) {
+            delete r;
+            return fail(SHGPU_ERR_ADMISSION, "server stopped");
+        }
+        r->id = sv->next_id++;
+        r->t_submit = std::chrono::steady_clock::now();
+        sv->requests[r->id] = r;
+        sv->waiting.push_back(r);
+        *id = r->id;
+    }
+    sv->cv_work.notify_all();
+    return SHGPU_OK;
+}
+
+int shgpu_server_submit(shgpu_server *sv, const uint64_t *h_hashes, uint64_t n, uint64_t *id) {
+    return server_submit(sv, h_hashes, false, n, id);
+}
+
+int shgpu_server_submit_device(shgpu_server *sv, const uint64_t *d_hashes, uint64_t n, uint64_t *id) {
+    return server_submit(sv, d_hashes, true, n, id);
+}
+
+int shgpu_server_wait(shgpu_server *sv, uint64_t id, double timeout_s, int *status) {
+    if (!sv) return fail(SHGPU_ERR_ARG, "NULL argument");
+    std::unique_lock<std::mutex> lock(sv->mtx);
+    auto it = sv->requests.find(id);
+    if (it == sv->requests.end()) return fail(SHGPU_ERR_ARG, "unknown request id");
+    Request *r = it->second;
+    auto done = [&] { return r->status >= 2 || r->status < 0; };
+    if (timeout_s < 0) sv->cv_done.wait(lock, done);
+    else sv->cv_done.wait_for(lock, std::chrono::duration<double>(timeout_s), done);
+    if (status) *status = r->status;
+    return SHGPU_OK;
+}
+
+int shgpu_server_take(shgpu_server *sv, uint64_t id, shgpu_build **out, double *latency_ms, double *queue_ms) {
+    if (!sv || !out) return fail(SHGPU_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lock(sv->mtx);
+    auto it = sv->requests.find(id);
+    if (it == sv->requests.end()) return fail(SHGPU_ERR_ARG, "unknown request id");
+    Request *r = it->second;
+    if (r->status >= 0 && r->status < 2) return fail(SHGPU_ERR_ARG, "request not finished");
+    const int st = r->status;
+    std::string err = r->error;
+    *out = r->result;
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (queue_ms) *queue_ms = std::chrono::duration<double, std::milli>(r->t_start - r->t_submit).count();
+    sv->requests.erase(it);
+    delete r;
+    if (st < 0) return fail(-st, err);
+    return SHGPU_OK;
+}
+
+int shgpu_server_request_bytes(const shgpu_server *sv, uint64_t n, uint64_t *bytes) {
+    if (!sv || !bytes) return fail(SHGPU_ERR_ARG, "NULL argument");
+    *bytes = sv->need_for(n);
+    return SHGPU_OK;
+}
+
+int shgpu_server_info(shgpu_server *sv, uint64_t *queued, uint64_t *running, uint64_t *reserved_bytes,
+                      uint64_t *peak_reserved_bytes) {
+    if (!sv) return fail(SHGPU_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lock(sv->mtx);
+    if (queued) *queued = sv->waiting.size();
+    if (running) *running = sv->running;
+    if (reserved_bytes) *reserved_bytes = sv->reserved;
+    if (peak_reserved_bytes) *peak_reserved_bytes = sv->peak_reserved;
     return SHGPU_OK;
 }
 

@@ -16,7 +16,7 @@ MODES = {"persistent": 0, "plain": 1}
 NOT_FOUND = (1 << 64) - 1
 
 # Status codes (include/shockhash_amd.h)
-OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SEARCH_LIMIT, ERR_BUCKET_TOO_LARGE, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(8)
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SEARCH_LIMIT, ERR_BUCKET_TOO_LARGE, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_ADMISSION = range(9)
 
 
 class ShgpuError(RuntimeError):
@@ -57,6 +57,11 @@ SPLIT_DTYPE = np.dtype([("offset", "<u4"), ("m", "<u4"), ("start_seed", "<u8"), 
                         ("unit", "<u4"), ("kind", "u1"), ("hash", "u1"), ("pad", "<u2")])
 NODE_DTYPE = np.dtype([("offset", "<u4"), ("m", "<u2"), ("kind", "u1"), ("n_children", "u1"), ("unit", "<u4"),
                        ("subtree", "<u4"), ("start_seed", "<u8"), ("max_trials", "<u8")])
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("leaf_size", ctypes.c_uint32), ("bucket_size", ctypes.c_uint64),
+                ("slots", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64), ("copy_keys", ctypes.c_int)]
+
+
 assert LEAF_DTYPE.itemsize == ctypes.sizeof(LeafTask) == 24
 assert SPLIT_DTYPE.itemsize == ctypes.sizeof(SplitTask) == 32
 assert NODE_DTYPE.itemsize == ctypes.sizeof(Node) == 32
@@ -95,6 +100,16 @@ _SIGS = {
     "shgpu_build_run_host": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
     "shgpu_plan_forest": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, P, ctypes.c_uint64, P, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "shgpu_server_create": (ctypes.c_int, [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
+    "shgpu_server_destroy": (None, [P]),
+    "shgpu_server_submit": (ctypes.c_int, [P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "shgpu_server_submit_device": (ctypes.c_int, [P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "shgpu_server_wait": (ctypes.c_int, [P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
+    "shgpu_server_take": (ctypes.c_int, [P, ctypes.c_uint64, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]),
+    "shgpu_server_request_bytes": (ctypes.c_int, [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "shgpu_server_info": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 _lib = None
@@ -148,6 +163,13 @@ class DeviceArray:
         arr = np.ascontiguousarray(arr, dtype=self.dtype)
         assert arr.nbytes <= self.nbytes
         _check(load().shgpu_copy_to_device(self.engine.h, self.ptr, arr.ctypes.data_as(P), arr.nbytes))
+
+    def upload_at(self, offset, arr):
+        """Copies arr into the allocation starting at element `offset`."""
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert (offset + arr.size) * self.dtype.itemsize <= self.nbytes
+        dst = P(self.ptr.value + offset * self.dtype.itemsize)
+        _check(load().shgpu_copy_to_device(self.engine.h, dst, arr.ctypes.data_as(P), arr.nbytes))
 
     def download(self):
         out = np.empty(self.count, dtype=self.dtype)
@@ -308,10 +330,13 @@ class Engine:
         return d_out.download(), d_keys.download()
 
     def build_dev(self, d_hashes, n, variant, leaf_size, bucket_size, mode="persistent", plain_bucket_limit=0,
-                  copy_keys=True):
+                  copy_keys=True, offset=0):
+        """Builds over n fingerprints at `offset` (in fingerprints) of a device array of uint64 pairs."""
+        assert n == 0 or (offset + n) * 16 <= d_hashes.nbytes
         desc = BuildDesc(VARIANTS[variant], leaf_size, bucket_size, MODES[mode], plain_bucket_limit, int(copy_keys))
         h = P()
-        _check(load().shgpu_build_run(self.h, ctypes.byref(desc), d_hashes.ptr, n, ctypes.byref(h)))
+        ptr = P((d_hashes.ptr.value or 0) + offset * 16)
+        _check(load().shgpu_build_run(self.h, ctypes.byref(desc), ptr, n, ctypes.byref(h)))
         return Build(h)
 
     def build(self, hashes, variant, leaf_size, bucket_size, **kw):
@@ -321,3 +346,69 @@ class Engine:
             return self.build_dev(d, hashes.shape[0], variant, leaf_size, bucket_size, **kw)
         finally:
             d.free()
+
+
+class Server:
+    """Continuous-batching construction server (shgpu_server_*): concurrent builds in `slots` streams,
+    admitted in arrival order within an HBM budget. Each result equals Engine.build of the same keys."""
+
+    def __init__(self, engine, variant, leaf_size, bucket_size, slots=4, hbm_budget_bytes=8 << 30, copy_keys=False):
+        self.engine = engine
+        self._pending = {}  # id -> host fingerprints (kept alive until taken)
+        cfg = ServerConfig(VARIANTS[variant], leaf_size, bucket_size, slots, hbm_budget_bytes, int(copy_keys))
+        h = P()
+        _check(load().shgpu_server_create(engine.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, hashes):
+        hashes = np.ascontiguousarray(hashes, dtype=np.uint64).reshape(-1, 2)
+        rid = ctypes.c_uint64()
+        _check(load().shgpu_server_submit(self.h, hashes.ctypes.data_as(P), hashes.shape[0], ctypes.byref(rid)))
+        self._pending[rid.value] = hashes
+        return rid.value
+
+    def submit_device(self, d_hashes, n, offset=0):
+        """Fingerprints n at `offset` (in fingerprints) of a DeviceArray of uint64 pairs."""
+        assert (offset + n) * 16 <= d_hashes.nbytes
+        rid = ctypes.c_uint64()
+        ptr = P((d_hashes.ptr.value or 0) + offset * 16)
+        _check(load().shgpu_server_submit_device(self.h, ptr, n, ctypes.byref(rid)))
+        self._pending[rid.value] = d_hashes
+        return rid.value
+
+    def wait(self, rid, timeout_s=-1.0):
+        st = ctypes.c_int()
+        _check(load().shgpu_server_wait(self.h, rid, timeout_s, ctypes.byref(st)))
+        return st.value
+
+    def take(self, rid):
+        """Waits for request `rid` and returns (Build, latency_ms, queue_ms)."""
+        self.wait(rid)
+        h, lat, q = P(), ctypes.c_double(), ctypes.c_double()
+        try:
+            _check(load().shgpu_server_take(self.h, rid, ctypes.byref(h), ctypes.byref(lat), ctypes.byref(q)))
+        finally:
+            self._pending.pop(rid, None)
+        return Build(h), lat.value, q.value
+
+    def request_bytes(self, n):
+        v = ctypes.c_uint64()
+        _check(load().shgpu_server_request_bytes(self.h, n, ctypes.byref(v)))
+        return v.value
+
+    def info(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        _check(load().shgpu_server_info(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("queued", "running", "reserved_bytes", "peak_reserved_bytes"), (x.value for x in v)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().shgpu_server_destroy(self.h)
+            self.h = None
+        self._pending.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

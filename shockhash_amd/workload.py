@@ -29,18 +29,13 @@ def splitmix64(seed, count, offset=0):
 
 
 def fingerprints(n, seed=1):
-    """n uniform 128-bit fingerprints as an (n, 2) uint64 array of (first, second); distinct .second."""
-    out = splitmix64(seed, 2 * n).reshape(n, 2)
-    # 64-bit collisions among 1e7 values are ~3e-6 likely; redraw any duplicate .second so that no
-    # leaf can hold two equal keys (the reference would search forever on them).
-    sec = out[:, 1]
-    if len(np.unique(sec)) != n:
-        _, first_idx = np.unique(sec, return_index=True)
-        dup = np.setdiff1d(np.arange(n), first_idx)
-        extra = splitmix64(seed ^ 0x5DEECE66D, 2 * len(dup) + 64)
-        out[dup, 1] = extra[: len(dup)]
-        assert len(np.unique(out[:, 1])) == n
-    return out
+    """n uniform 128-bit fingerprints as an (n, 2) uint64 array of (first, second).
+
+    The .second values are distinct by construction: they are outputs 2, 4, ..., 2n of one SplitMix64
+    stream, whose states seed + k * GAMMA are distinct for k < 2^64 (GAMMA is odd) and Mix13 is a
+    bijection (xor-shifts and odd multipliers are invertible mod 2^64). No leaf can hold two equal keys,
+    on which the seed searches would run to their bounds."""
+    return splitmix64(seed, 2 * n).reshape(n, 2)
 
 
 def isolated_leaves(n_leaves, m, seed=2):

@@ -39,7 +39,7 @@ def test_host_library_exports():
 
 def test_status_strings():
     lib = E.load()
-    for st in range(8):
+    for st in range(9):
         assert lib.shgpu_status_string(st)
 
 
