@@ -89,7 +89,8 @@ def serving_mix(eng, args):
         d.upload_at(2 * a, workload.splitmix64(2, 2 * (b - a), offset=2 * a))
     log(f"serving mix: {args.serve_requests} requests, {total} keys resident, rate {args.serve_rate}/s, "
         f"{args.serve_slots} slots, budget {args.serve_budget_gib} GiB")
-    srv = Server(eng, "sh2", 64, 2000, slots=args.serve_slots, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)))
+    srv = Server(eng, "sh2", 64, 2000, slots=args.serve_slots, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)),
+                 max_batch_keys=args.serve_batch_keys)
     # warm the slots (first-use buffer allocation)
     for rid in [srv.submit_device(d, int(sizes[i]), int(offs[i])) for i in range(args.serve_slots)]:
         srv.take(rid)
@@ -115,6 +116,7 @@ def serving_mix(eng, args):
     srv.close()
     d.free()
     out = {"requests": args.serve_requests, "rate_per_s": args.serve_rate, "keys": total, "slots": args.serve_slots,
+           "batch_keys": args.serve_batch_keys,
            "hbm_budget_bytes": int(args.serve_budget_gib * (1 << 30)), "peak_reserved_bytes": info["peak_reserved_bytes"],
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "max_ms": float(lat.max()), "queue_p99_ms": float(np.percentile(queue, 99)),
@@ -140,8 +142,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2048)
     ap.add_argument("--serve-rate", type=float, default=800.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--serve-batch-keys", type=int, default=1 << 22, help="keys a slot builds together at most")
     ap.add_argument("--copy-probe-mib", type=int, default=256,
                     help="size of the plain copy kernel run once as the HBM-counter calibration reference")
     args = ap.parse_args()

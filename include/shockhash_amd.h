@@ -147,8 +147,10 @@ int shgpu_build_timings(const shgpu_build *b, double *ms4);
 /* ---- Serving: continuous batching of construction requests.
  * A server owns `slots` concurrent builds (one stream, buffer set and host worker each; each slot's
  * persistent kernel uses 1/slots of the device). Requests are admitted in arrival order as soon as a
- * slot is free and their device footprint fits the HBM budget, so work of requests that arrive while
- * others run overlaps on the device. Results equal a lone shgpu_build_run of the same keys. */
+ * slot is free and their device footprint fits the HBM budget: a free slot takes every waiting request
+ * that fits (up to the batch limits) and runs their trees in one persistent kernel, so work of requests
+ * that arrive while others run overlaps on the device. Results equal a lone shgpu_build_run of the same
+ * keys. */
 typedef struct shgpu_server shgpu_server;
 typedef struct shgpu_server_config {
     int variant;
@@ -157,6 +159,8 @@ typedef struct shgpu_server_config {
     uint32_t slots;             /* concurrent builds, 1..16 */
     uint64_t hbm_budget_bytes;  /* device bytes the server may hold for running requests */
     int copy_keys;              /* results keep the leaf-ordered keys */
+    uint64_t max_batch_keys;    /* a slot builds the waiting requests together up to this many keys (0: 8M) */
+    uint32_t max_batch_requests;/* ... and up to this many requests (0: 256) */
 } shgpu_server_config;
 
 int shgpu_server_create(shgpu_engine *e, const shgpu_server_config *cfg, shgpu_server **out);

@@ -745,74 +745,125 @@ int shgpu_split_search(shgpu_engine *e, uint64_t *d_keys, const shgpu_split_task
     return SHGPU_OK;
 }
 
-// The construction pipeline on a buffer set and stream; the persistent grid uses 1/grid_div of the
-// variant's schedule (serving slots share the device).
-static int run_build(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t grid_div, const shgpu_build_desc *d,
-                     const uint64_t *d_hashes, uint64_t n, shgpu_build **out) {
+struct BuildInput {
+    const uint64_t *d_hashes;  // n fingerprints {first, second} in device memory
+    uint64_t n;
+};
+
+// The construction pipeline for a batch of independent key sets on one buffer set and stream: each
+// set is sorted and bucketed on its own, their forests share one node array, queue and persistent
+// kernel (the trees are independent, so every set's codes equal those of a build of that set alone).
+// The persistent grid uses 1/grid_div of the variant's schedule (serving slots share the device).
+// status[i] receives the outcome of set i; the return value is the first failure (SHGPU_OK if none).
+static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t grid_div, const shgpu_build_desc *d,
+                     const BuildInput *in, size_t count, shgpu_build **out, int *status) {
     using clk = std::chrono::steady_clock;
-    if (!e || !d || !out || (n && !d_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
-    *out = nullptr;
-    int rc = check_variant(d->variant, d->leaf_size);
-    if (rc) return rc;
-    if (d->bucket_size == 0) return fail(SHGPU_ERR_ARG, "bucket_size must be positive");
-    if (n >= (1ull << 32)) return fail(SHGPU_ERR_UNSUPPORTED, "more than 2^32 - 1 keys");
-    HIP_TRY(hipSetDevice(e->device));
+    if (!e || !d || !out || !in || !status || count == 0) return fail(SHGPU_ERR_ARG, "NULL argument");
+    int rc = SHGPU_OK;
+    auto fail_all = [&](int code) {
+        for (size_t r = 0; r < count; r++) status[r] = code;
+        return code;
+    };
+    for (size_t r = 0; r < count; r++) out[r] = nullptr;
+    fail_all(SHGPU_OK);
+    if ((rc = check_variant(d->variant, d->leaf_size))) return fail_all(rc);
+    if (d->bucket_size == 0) return fail_all(fail(SHGPU_ERR_ARG, "bucket_size must be positive"));
+    uint64_t N = 0, NB = 0;
+    std::vector<uint64_t> base(count), nbk(count), bbase(count);
+    for (size_t r = 0; r < count; r++) {
+        if (in[r].n && !in[r].d_hashes) return fail_all(fail(SHGPU_ERR_ARG, "NULL argument"));
+        base[r] = N;
+        bbase[r] = NB;
+        nbk[r] = std::max<uint64_t>(1, (in[r].n + d->bucket_size - 1) / d->bucket_size);
+        N += in[r].n;
+        NB += nbk[r] + 1;
+    }
+    if (N >= (1ull << 32)) return fail_all(fail(SHGPU_ERR_UNSUPPORTED, "more than 2^32 - 1 keys"));
+    if (hipSetDevice(e->device) != hipSuccess) return fail_all(fail(SHGPU_ERR_HIP, "hipSetDevice failed"));
+#define BATCH_HIP(call)                                                                                        \
+    do {                                                                                                       \
+        hipError_t err_ = (call);                                                                              \
+        if (err_ != hipSuccess) return fail_all(fail(SHGPU_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_))); \
+    } while (0)
     auto t0 = clk::now();
-    shgpu_build *b = new shgpu_build();
-    b->n = n;
-    b->nbuckets = std::max<uint64_t>(1, (n + d->bucket_size - 1) / d->bucket_size);
-    const uint64_t nb = b->nbuckets;
-    struct Guard { shgpu_build *&b; bool ok = false; ~Guard() { if (!ok) { delete b; b = nullptr; } } } guard{b};
+    std::vector<std::unique_ptr<shgpu_build>> bs(count);
+    for (size_t r = 0; r < count; r++) {
+        bs[r].reset(new shgpu_build());
+        bs[r]->n = in[r].n;
+        bs[r]->nbuckets = nbk[r];
+    }
 
     // 1. partition: sort by .first, emit .second in order (Sorter.hpp:15-36), bucket bounds.
-    if ((rc = B.firsts_in.ensure(n * 8 + 8)) || (rc = B.firsts_out.ensure(n * 8 + 8)) ||
-        (rc = B.seconds_in.ensure(n * 8 + 8)) || (rc = B.keys.ensure(n * 8 + 64)) || (rc = B.tmp.ensure(n * 8 + 64)) ||
-        (rc = B.size_acc.ensure((nb + 1) * 8)))
-        return rc;
+    if ((rc = B.firsts_in.ensure(N * 8 + 8)) || (rc = B.firsts_out.ensure(N * 8 + 8)) ||
+        (rc = B.seconds_in.ensure(N * 8 + 8)) || (rc = B.keys.ensure(N * 8 + 64)) || (rc = B.tmp.ensure(N * 8 + 64)) ||
+        (rc = B.size_acc.ensure(NB * 8)))
+        return fail_all(rc);
+    uint64_t nmax = 0;
+    for (size_t r = 0; r < count; r++) nmax = std::max(nmax, in[r].n);
     size_t sort_bytes = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, B.firsts_in.as<uint64_t>(), B.firsts_out.as<uint64_t>(),
-                                               B.seconds_in.as<uint64_t>(), B.keys.as<uint64_t>(), (int)n, 0, 64, s));
-    if ((rc = B.sort_tmp.ensure(sort_bytes + 16))) return rc;
-    if (n) {
-        k_unzip_hashes<<<1024, 256, 0, s>>>(d_hashes, n, B.firsts_in.as<uint64_t>(), B.seconds_in.as<uint64_t>());
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(B.sort_tmp.p, sort_bytes, B.firsts_in.as<uint64_t>(),
-                                                   B.firsts_out.as<uint64_t>(), B.seconds_in.as<uint64_t>(),
-                                                   B.keys.as<uint64_t>(), (int)n, 0, 64, s));
+    BATCH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, B.firsts_in.as<uint64_t>(), B.firsts_out.as<uint64_t>(),
+                                                 B.seconds_in.as<uint64_t>(), B.keys.as<uint64_t>(), (int)nmax, 0, 64, s));
+    if ((rc = B.sort_tmp.ensure(sort_bytes + 16))) return fail_all(rc);
+    for (size_t r = 0; r < count; r++) {
+        const uint64_t n = in[r].n, o = base[r];
+        if (n) {
+            k_unzip_hashes<<<1024, 256, 0, s>>>(in[r].d_hashes, n, B.firsts_in.as<uint64_t>() + o, B.seconds_in.as<uint64_t>() + o);
+            BATCH_HIP(hipGetLastError());
+            size_t bytes = sort_bytes;
+            BATCH_HIP(hipcub::DeviceRadixSort::SortPairs(B.sort_tmp.p, bytes, B.firsts_in.as<uint64_t>() + o,
+                                                         B.firsts_out.as<uint64_t>() + o, B.seconds_in.as<uint64_t>() + o,
+                                                         B.keys.as<uint64_t>() + o, (int)n, 0, 64, s));
+        }
+        k_bucket_bounds<<<1024, 256, 0, s>>>(B.firsts_out.as<uint64_t>() + o, n, nbk[r], B.size_acc.as<uint64_t>() + bbase[r]);
+        BATCH_HIP(hipGetLastError());
     }
-    k_bucket_bounds<<<1024, 256, 0, s>>>(B.firsts_out.as<uint64_t>(), n, nb, B.size_acc.as<uint64_t>());
-    HIP_TRY(hipGetLastError());
-    b->size_acc.resize(nb + 1);
-    HIP_TRY(hipMemcpyAsync(b->size_acc.data(), B.size_acc.p, (nb + 1) * 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<uint64_t> acc_all(NB);
+    BATCH_HIP(hipMemcpyAsync(acc_all.data(), B.size_acc.p, NB * 8, hipMemcpyDeviceToHost, s));
+    BATCH_HIP(hipStreamSynchronize(s));
+    for (size_t r = 0; r < count; r++)
+        bs[r]->size_acc.assign(acc_all.begin() + bbase[r], acc_all.begin() + bbase[r] + nbk[r] + 1);
     auto t1 = clk::now();
 
-    // 2. forest of RecSplit trees, DFS pre-order per bucket, buckets in order.
+    // 2. forest of RecSplit trees, DFS pre-order per bucket, buckets in order, sets in order.
     const Strategy st(d->variant, d->leaf_size);
-    uint64_t limit_buckets = nb;
-    if (d->mode == SHGPU_MODE_PLAIN && d->plain_bucket_limit) limit_buckets = std::min<uint64_t>(nb, d->plain_bucket_limit);
+    std::vector<shgpu_node> nodes;
     std::vector<uint32_t> roots;
-    b->nodes.reserve(n / std::max<uint32_t>(1, d->leaf_size) * 3 / 2 + 16);
-    for (uint64_t k = 0; k < limit_buckets; k++) {
-        const uint64_t m = b->size_acc[k + 1] - b->size_acc[k];
-        if (m > MAX_ENCODABLE_BUCKET) return fail(SHGPU_ERR_BUCKET_TOO_LARGE, "bucket " + std::to_string(k) + " holds " + std::to_string(m) + " keys");
-        if (m > 1) {
-            roots.push_back((uint32_t)b->nodes.size());
-            emit_subtree(b->nodes, st, (uint32_t)b->size_acc[k], (uint32_t)m, 0, e->tuning_v[d->variant].max_trials);
+    std::vector<uint64_t> node_base(count + 1, 0);
+    nodes.reserve(N / std::max<uint32_t>(1, d->leaf_size) * 3 / 2 + 16 * count);
+    for (size_t r = 0; r < count; r++) {
+        node_base[r] = nodes.size();
+        const auto &acc = bs[r]->size_acc;
+        uint64_t limit_buckets = nbk[r];
+        if (d->mode == SHGPU_MODE_PLAIN && d->plain_bucket_limit) limit_buckets = std::min<uint64_t>(nbk[r], d->plain_bucket_limit);
+        for (uint64_t k = 0; k < limit_buckets; k++) {
+            const uint64_t m = acc[k + 1] - acc[k];
+            if (m > MAX_ENCODABLE_BUCKET) {
+                status[r] = fail(SHGPU_ERR_BUCKET_TOO_LARGE, "bucket " + std::to_string(k) + " holds " + std::to_string(m) + " keys");
+                break;
+            }
+        }
+        if (status[r] != SHGPU_OK) continue;  // this set is not searched
+        for (uint64_t k = 0; k < limit_buckets; k++) {
+            const uint64_t m = acc[k + 1] - acc[k];
+            if (m > 1) {
+                roots.push_back((uint32_t)nodes.size());
+                emit_subtree(nodes, st, (uint32_t)(base[r] + acc[k]), (uint32_t)m, 0, e->tuning_v[d->variant].max_trials);
+            }
         }
     }
-    const uint32_t total = (uint32_t)b->nodes.size();
-    b->codes.assign(total, 0);
+    node_base[count] = nodes.size();
+    const uint32_t total = (uint32_t)nodes.size();
+    std::vector<uint64_t> codes(total, 0);
     const int grid = std::max(1, grid_for(e, e->tuning_v[d->variant]) / (int)std::max(1u, grid_div));
     const uint32_t nslots = (uint32_t)grid * WAVES_PER_BLOCK;
     if ((rc = B.nodes.ensure((size_t)total * sizeof(shgpu_node) + 64)) || (rc = B.codes.ensure((size_t)total * 8 + 64)) ||
         (rc = B.queue.ensure((size_t)total * 4 + 64)) || (rc = B.ctrl.ensure(64)) ||
         (rc = B.states.ensure((size_t)total * sizeof(NodeState) + 64)) || (rc = B.help.ensure((size_t)nslots * 4 + 64)))
-        return rc;
-    if (total) HIP_TRY(hipMemcpyAsync(B.nodes.p, b->nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
+        return fail_all(rc);
+    if (total) BATCH_HIP(hipMemcpyAsync(B.nodes.p, nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
     auto t2 = clk::now();
 
-    if ((rc = check_forest(b->nodes, (uint32_t)roots.size()))) return rc;
+    if ((rc = check_forest(nodes, (uint32_t)roots.size()))) return fail_all(rc);
 
     // 3. seed search.
     uint32_t ctrl_out[4] = {0, 0, 0, 0};
@@ -832,64 +883,87 @@ static int run_build(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     a.nslots = nslots;
     if (total) {
         k_init_states<<<256, 256, 0, s>>>(a.state, total, a.help, nslots);
-        HIP_TRY(hipGetLastError());
+        BATCH_HIP(hipGetLastError());
         if (d->mode == SHGPU_MODE_PERSISTENT) {
             if (d->variant == SHGPU_SHOCKHASH2) {
-                if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * SH2_SCRATCH_BYTES))) return rc;
+                if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * SH2_SCRATCH_BYTES))) return fail_all(rc);
                 a.scratch = B.sh2_scratch.as<uint8_t>();
             }
             k_fill_u32<<<256, 256, 0, s>>>(a.queue, EMPTY_SLOT, total);
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemcpyAsync(a.queue, roots.data(), roots.size() * 4, hipMemcpyHostToDevice, s));
+            BATCH_HIP(hipGetLastError());
+            BATCH_HIP(hipMemcpyAsync(a.queue, roots.data(), roots.size() * 4, hipMemcpyHostToDevice, s));
             const uint32_t ctrl_init[4] = {0, (uint32_t)roots.size(), 0, 0};  // head, tail, error, finished
-            HIP_TRY(hipMemcpyAsync(a.ctrl, ctrl_init, sizeof(ctrl_init), hipMemcpyHostToDevice, s));
-            HIP_TRY(hipEventRecord(B.ev0, s));
+            BATCH_HIP(hipMemcpyAsync(a.ctrl, ctrl_init, sizeof(ctrl_init), hipMemcpyHostToDevice, s));
+            BATCH_HIP(hipEventRecord(B.ev0, s));
             if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, a, grid, s);
             else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_build_v<SHGPU_SIMD_SHOCKHASH>(e, a, grid, s);
             else rc = launch_build_v<SHGPU_SHOCKHASH2>(e, a, grid, s);
-            if (rc) return rc;
-            HIP_TRY(hipEventRecord(B.ev1, s));
+            if (rc) return fail_all(rc);
+            BATCH_HIP(hipEventRecord(B.ev1, s));
             timed_kernel = true;
         } else {
             if (d->variant == SHGPU_SHOCKHASH2) {
-                if ((rc = B.sh2_scratch.ensure(SH2_SCRATCH_BYTES))) return rc;
+                if ((rc = B.sh2_scratch.ensure(SH2_SCRATCH_BYTES))) return fail_all(rc);
                 a.scratch = B.sh2_scratch.as<uint8_t>();
             }
             const uint32_t waves = (uint32_t)e->cu_count;  // one wave per CU joins every node's windows
             for (uint32_t nid = 0; nid < total; nid++) {
-                const bool sh2_leaf = d->variant == SHGPU_SHOCKHASH2 && b->nodes[nid].kind == SHGPU_NODE_LEAF;
+                const bool sh2_leaf = d->variant == SHGPU_SHOCKHASH2 && nodes[nid].kind == SHGPU_NODE_LEAF;
                 const uint32_t wv = sh2_leaf ? 1u : waves;
                 if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, a, nid, wv, s);
                 else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, a, nid, wv, s);
                 else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, a, nid, wv, s);
-                if (rc) return rc;
+                if (rc) return fail_all(rc);
             }
         }
-        HIP_TRY(hipMemcpyAsync(b->codes.data(), a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(ctrl_out, a.ctrl, sizeof(ctrl_out), hipMemcpyDeviceToHost, s));
+        BATCH_HIP(hipMemcpyAsync(codes.data(), a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
+        BATCH_HIP(hipMemcpyAsync(ctrl_out, a.ctrl, sizeof(ctrl_out), hipMemcpyDeviceToHost, s));
     }
-    if (d->copy_keys && n) {
-        b->keys.resize(n);
-        HIP_TRY(hipMemcpyAsync(b->keys.data(), B.keys.p, n * 8, hipMemcpyDeviceToHost, s));
+    std::vector<uint64_t> keys_all;
+    if (d->copy_keys && N) {
+        keys_all.resize(N);
+        BATCH_HIP(hipMemcpyAsync(keys_all.data(), B.keys.p, N * 8, hipMemcpyDeviceToHost, s));
     }
-    HIP_TRY(hipStreamSynchronize(s));
+    BATCH_HIP(hipStreamSynchronize(s));
+#undef BATCH_HIP
     auto t3 = clk::now();
     if (d->mode == SHGPU_MODE_PERSISTENT && total && ctrl_out[2])
-        return fail(SHGPU_ERR_HIP, "persistent build: a queue slot was never filled (inconsistent forest)");
-    for (uint32_t i = 0; i < total; i++)
-        if (b->codes[i] == SHGPU_NOT_FOUND)
-            return fail(SHGPU_ERR_SEARCH_LIMIT, "node " + std::to_string(i) + " exhausted its seed bound");
-    b->ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    b->ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
-    b->ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
-    if (timed_kernel) {
-        float kms = 0;
-        HIP_TRY(hipEventElapsedTime(&kms, B.ev0, B.ev1));
-        b->ms[3] = kms;
+        return fail_all(fail(SHGPU_ERR_HIP, "persistent build: a queue slot was never filled (inconsistent forest)"));
+    float kms = 0;
+    if (timed_kernel && hipEventElapsedTime(&kms, B.ev0, B.ev1) != hipSuccess) return fail_all(fail(SHGPU_ERR_HIP, "hipEventElapsedTime failed"));
+    int first_rc = SHGPU_OK;
+    for (size_t r = 0; r < count; r++) {
+        shgpu_build &b = *bs[r];
+        if (status[r] == SHGPU_OK) {
+            b.nodes.assign(nodes.begin() + node_base[r], nodes.begin() + node_base[r + 1]);
+            b.codes.assign(codes.begin() + node_base[r], codes.begin() + node_base[r + 1]);
+            for (auto &nd : b.nodes) nd.offset -= (uint32_t)base[r];  // offsets within this set's keys
+            for (size_t i = 0; i < b.codes.size(); i++)
+                if (b.codes[i] == SHGPU_NOT_FOUND) {
+                    status[r] = fail(SHGPU_ERR_SEARCH_LIMIT, "node " + std::to_string(i) + " exhausted its seed bound");
+                    break;
+                }
+        }
+        if (status[r] != SHGPU_OK) {
+            if (first_rc == SHGPU_OK) first_rc = status[r];
+            continue;
+        }
+        if (!keys_all.empty()) b.keys.assign(keys_all.begin() + base[r], keys_all.begin() + base[r] + b.n);
+        b.ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        b.ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        b.ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+        b.ms[3] = timed_kernel ? kms : 0.0;
+        out[r] = bs[r].release();
     }
-    guard.ok = true;
-    *out = b;
-    return SHGPU_OK;
+    return first_rc;
+}
+
+static int run_build(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t grid_div, const shgpu_build_desc *d,
+                     const uint64_t *d_hashes, uint64_t n, shgpu_build **out) {
+    if (!out) return fail(SHGPU_ERR_ARG, "NULL argument");
+    const BuildInput in{d_hashes, n};
+    int st = SHGPU_OK;
+    return run_batch(e, B, s, grid_div, d, &in, 1, out, &st);
 }
 
 int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *d_hashes, uint64_t n, shgpu_build **out) {
@@ -1096,10 +1170,19 @@ struct shgpu_server {
         return t;
     }
 
+    uint64_t batch_keys() const { return cfg.max_batch_keys ? cfg.max_batch_keys : (8ull << 20); }
+    uint32_t batch_requests() const { return cfg.max_batch_requests ? cfg.max_batch_requests : 256u; }
+
+    // A free slot takes every waiting request, in arrival order, that fits the budget and the batch
+    // limits, and builds them together in one persistent kernel.
     void work(Slot &slot) {
         (void)hipSetDevice(e->device);
+        std::vector<Request *> batch;
+        std::vector<BuildInput> in;
+        std::vector<shgpu_build *> res;
+        std::vector<int> st;
         for (;;) {
-            Request *r = nullptr;
+            batch.clear();
             {
                 std::unique_lock<std::mutex> lock(mtx);
                 cv_work.wait(lock, [&] {
@@ -1107,24 +1190,43 @@ struct shgpu_server {
                 });
                 if (stop && waiting.empty()) return;
                 if (waiting.empty()) continue;
-                r = waiting.front();
-                waiting.pop_front();
-                reserved += r->need;
+                uint64_t keys = 0;
+                const auto now = std::chrono::steady_clock::now();
+                while (!waiting.empty() && batch.size() < batch_requests()) {
+                    Request *r = waiting.front();
+                    if (reserved + r->need > cfg.hbm_budget_bytes) break;
+                    if (!batch.empty() && keys + r->n > batch_keys()) break;
+                    waiting.pop_front();
+                    reserved += r->need;
+                    keys += r->n;
+                    r->status = 1;
+                    r->t_start = now;
+                    batch.push_back(r);
+                }
                 peak_reserved = std::max(peak_reserved, reserved);
-                running++;
-                r->status = 1;
-                r->t_start = std::chrono::steady_clock::now();
+                running += batch.size();
             }
             int rc = SHGPU_OK;
-            const uint64_t *d_hashes = r->hashes;
-            if (!r->on_device) {
-                rc = slot.B.staging.ensure(r->n * 16 + 16);
-                if (rc == SHGPU_OK && r->n &&
-                    hipMemcpyAsync(slot.B.staging.p, r->hashes, r->n * 16, hipMemcpyHostToDevice, slot.stream) != hipSuccess)
-                    rc = fail(SHGPU_ERR_HIP, "hipMemcpyAsync of request fingerprints failed");
-                d_hashes = slot.B.staging.as<uint64_t>();
+            std::string err;
+            // stage the requests that came from host memory
+            uint64_t staged = 0;
+            for (Request *r : batch)
+                if (!r->on_device) staged += r->n;
+            if (staged) rc = slot.B.staging.ensure(staged * 16 + 16);
+            in.clear();
+            staged = 0;
+            for (Request *r : batch) {
+                BuildInput x{r->hashes, r->n};
+                if (!r->on_device && rc == SHGPU_OK) {
+                    x.d_hashes = slot.B.staging.as<uint64_t>() + 2 * staged;
+                    if (r->n && hipMemcpyAsync((void *)x.d_hashes, r->hashes, r->n * 16, hipMemcpyHostToDevice, slot.stream) != hipSuccess)
+                        rc = fail(SHGPU_ERR_HIP, "hipMemcpyAsync of request fingerprints failed");
+                    staged += r->n;
+                }
+                in.push_back(x);
             }
-            shgpu_build *b = nullptr;
+            res.assign(batch.size(), nullptr);
+            st.assign(batch.size(), rc);
             if (rc == SHGPU_OK) {
                 shgpu_build_desc d{};
                 d.variant = cfg.variant;
@@ -1132,16 +1234,21 @@ struct shgpu_server {
                 d.bucket_size = cfg.bucket_size;
                 d.mode = SHGPU_MODE_PERSISTENT;
                 d.copy_keys = cfg.copy_keys;
-                rc = run_build(e, slot.B, slot.stream, cfg.slots, &d, d_hashes, r->n, &b);
+                rc = run_batch(e, slot.B, slot.stream, cfg.slots, &d, in.data(), in.size(), res.data(), st.data());
             }
+            if (rc != SHGPU_OK) err = g_err;
             {
                 std::lock_guard<std::mutex> lock(mtx);
-                r->t_done = std::chrono::steady_clock::now();
-                r->result = b;
-                r->status = rc == SHGPU_OK ? 2 : -rc;
-                if (rc != SHGPU_OK) r->error = g_err;
-                reserved -= r->need;
-                running--;
+                const auto now = std::chrono::steady_clock::now();
+                for (size_t i = 0; i < batch.size(); i++) {
+                    Request *r = batch[i];
+                    r->t_done = now;
+                    r->result = res[i];
+                    r->status = st[i] == SHGPU_OK ? 2 : -st[i];
+                    if (st[i] != SHGPU_OK) r->error = err.empty() ? shgpu_status_string(st[i]) : err;
+                    reserved -= r->need;
+                }
+                running -= batch.size();
                 // keep the device footprint within the budget: a slot whose buffers grew beyond what the
                 // budget allows gives them back
                 slot.held = slot.B.bytes();

@@ -59,7 +59,8 @@ NODE_DTYPE = np.dtype([("offset", "<u4"), ("m", "<u2"), ("kind", "u1"), ("n_chil
                        ("subtree", "<u4"), ("start_seed", "<u8"), ("max_trials", "<u8")])
 class ServerConfig(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("leaf_size", ctypes.c_uint32), ("bucket_size", ctypes.c_uint64),
-                ("slots", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64), ("copy_keys", ctypes.c_int)]
+                ("slots", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64), ("copy_keys", ctypes.c_int),
+                ("max_batch_keys", ctypes.c_uint64), ("max_batch_requests", ctypes.c_uint32)]
 
 
 assert LEAF_DTYPE.itemsize == ctypes.sizeof(LeafTask) == 24
@@ -352,10 +353,12 @@ class Server:
     """Continuous-batching construction server (shgpu_server_*): concurrent builds in `slots` streams,
     admitted in arrival order within an HBM budget. Each result equals Engine.build of the same keys."""
 
-    def __init__(self, engine, variant, leaf_size, bucket_size, slots=4, hbm_budget_bytes=8 << 30, copy_keys=False):
+    def __init__(self, engine, variant, leaf_size, bucket_size, slots=4, hbm_budget_bytes=8 << 30, copy_keys=False,
+                 max_batch_keys=0, max_batch_requests=0):
         self.engine = engine
         self._pending = {}  # id -> host fingerprints (kept alive until taken)
-        cfg = ServerConfig(VARIANTS[variant], leaf_size, bucket_size, slots, hbm_budget_bytes, int(copy_keys))
+        cfg = ServerConfig(VARIANTS[variant], leaf_size, bucket_size, slots, hbm_budget_bytes, int(copy_keys),
+                           max_batch_keys, max_batch_requests)
         h = P()
         _check(load().shgpu_server_create(engine.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

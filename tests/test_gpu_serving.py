@@ -25,8 +25,8 @@ def same(a, b):
         np.array_equal(a.nodes, b.nodes)
 
 
-@pytest.mark.parametrize("variant,leaf,bucket,slots", [("sh2", 64, 2000, 4), ("sh2", 16, 500, 3), ("sh1", 30, 1000, 2),
-                                                       ("simd", 24, 1000, 4)])
+@pytest.mark.parametrize("variant,leaf,bucket,slots", [("sh2", 64, 2000, 4), ("sh2", 64, 2000, 1), ("sh2", 16, 500, 3),
+                                                       ("sh1", 30, 1000, 2), ("simd", 24, 1000, 4)])
 def test_results_identical_alone_and_in_the_mix(engine, variant, leaf, bucket, slots):
     log(f"serving {variant} L{leaf} b{bucket} slots {slots}")
     mix = request_mix(12, 50, 120_000, seed=leaf)
@@ -55,6 +55,22 @@ def test_results_identical_alone_and_in_the_mix(engine, variant, leaf, bucket, s
             ob = O.OracleBuild(variant, leaf, h, bucket, threads=8)
             assert np.array_equal(b.codes, ob.node_code) and np.array_equal(b.size_acc, ob.size_acc)
     log("  identical alone, in the mix and to the oracle")
+
+
+def test_requests_batched_behind_a_long_one(engine):
+    """One slot: while a large request runs, the small ones queue up and are then built together in one
+    kernel; each equals its build alone."""
+    log("serving batch")
+    big = workload.fingerprints(1_000_000, seed=21)
+    small = request_mix(16, 100, 30_000, seed=22)
+    srv = Server(engine, "sh2", 64, 2000, slots=1, max_batch_keys=1 << 20, max_batch_requests=8)
+    try:
+        ids = [srv.submit(big)] + [srv.submit(h) for h in small]
+        got = [srv.take(i)[0] for i in ids]
+    finally:
+        srv.close()
+    for h, b in zip([big] + small, got):
+        assert same(b, engine.build(h, "sh2", 64, 2000))
 
 
 def test_budget_serialises_and_rejects(engine):
