@@ -25,7 +25,7 @@ INT32_PEAK_TOPS = MI355X_CUS * SIMDS_PER_CU * LANES_PER_SIMD * CLOCK_GHZ / 1e3  
 OPS_PER_EVAL = 32  # int32 ops per 64-bit-mix hash evaluation (remix + range reduction + count/mask update)
 # HBM bytes per k_build launch on this workload: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of
 # this script, scaled by the same counters on the 256 MiB copy probe (profiles/k_build_shockhash2_L64_10M.md).
-MEASURED_TRAFFIC_BYTES = 2.838e9
+MEASURED_TRAFFIC_BYTES = 3.890e9
 
 
 def log(msg):
