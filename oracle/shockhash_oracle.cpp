@@ -812,9 +812,14 @@ uint64_t orc_sh2_query_cell(uint32_t m, uint64_t code, uint64_t key, uint64_t re
 uint64_t orc_sh2_hash(uint32_t m, uint64_t key, uint64_t seed) { return sh2_hash(m, key, seed); }
 
 // Split node search + partition (keys permuted in place). variant 0 sux flavour, 1 SIMD flavour.
-// Returns the code (x - start); shape fields returned for inspection.
+// Returns the code (x - start); shape fields returned for inspection. A node of at most `leaf` keys
+// is a leaf, not a split: returns UINT64_MAX with kind = -1 instead of searching for an impossible split.
 uint64_t orc_split(int variant, int leaf, int level, uint64_t *keys, uint32_t m,
                    int *kind, uint32_t *fanout, uint32_t *unit, uint64_t *start) {
+    if (m <= (uint32_t)leaf) {
+        if (kind) *kind = -1;
+        return UINT64_MAX;
+    }
     Strategy st((Variant)variant, (uint32_t)leaf);
     SplitShape s = split_shape(st, m, level);
     std::vector<uint64_t> temp(m + 64);

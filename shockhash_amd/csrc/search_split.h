@@ -16,6 +16,17 @@
 // strategies produce (fanout <= 4, part sizes < 256 in the 4-way case) the packed test accepts
 // exactly the seeds whose plain counts match, so plain counts give the same minimal seed.
 //
+// Exact power-of-two fanouts. When m = F * unit with F = 2 or 4 (the full lower-aggregation nodes,
+// fanout-2 nodes that halve exactly, upper aggregation of small leaves), the part of a key is
+// floor(v * F / 2^w): the top one or two bits of the w-bit value. For the sux flavour those are bits
+// 46..47 of remix(z), which the final xor-shift (z ^ z >> 31) leaves untouched, so they are bits 46..47
+// of the last 64-bit product; only the low 16 bits of that product's high word are needed, which
+// 24-bit multiplies give exactly. Counting then needs no compares: F = 2 counts the top bit; F = 4
+// adds 1 << (8 * part) into packed 8-bit fields, and the packed sum equals unit * 0x01010101 (mod 2^32)
+// exactly when every part holds unit keys, because each part's deviation from unit lies in
+// [-unit, 3 * unit] and 3 * unit < 256 (the low field forces d0 = 0, then d1 = 0, then d2 = 0, and the
+// counts sum to m).
+//
 // Mapping: lanes over seeds (S seeds per lane, window 64*S), keys staged 64 at a time in one VGPR
 // pair per lane and broadcast with v_readlane; counters live in registers.
 #pragma once
@@ -23,11 +34,17 @@
 
 namespace shgpu {
 
+// Counting modes of a split window.
+constexpr int SPLIT_THRESHOLDS = 0;  // cumulative threshold compares (any shape)
+constexpr int SPLIT_POW2_F2 = 1;     // m = 2 * unit: count the top bit
+constexpr int SPLIT_POW2_F4 = 2;     // m = 4 * unit, unit <= 85: packed 8-bit counts of the top two bits
+
 struct SplitShapeDev {
     uint32_t m, unit, fanout, nt;  // nt = number of thresholds (1 for kind 0, fanout-1 otherwise)
     uint32_t t[3];                 // target cumulative counts
     uint64_t thr[3];               // thresholds in the pre-scaling domain
     int simd;
+    int mode;                      // SPLIT_THRESHOLDS / SPLIT_POW2_F2 / SPLIT_POW2_F4
 };
 
 __device__ __forceinline__ SplitShapeDev make_split_shape(uint32_t m, uint32_t unit, uint32_t kind, uint32_t hash) {
@@ -45,7 +62,21 @@ __device__ __forceinline__ SplitShapeDev make_split_shape(uint32_t m, uint32_t u
         // ceil(tp * 2^w / m); tp < m <= 3000 so tp * 2^48 < 2^60.
         s.thr[p] = (((uint64_t)tp << w) + m - 1) / m;
     }
+    s.mode = SPLIT_THRESHOLDS;
+    if (s.fanout == 2 && m == 2 * unit) s.mode = SPLIT_POW2_F2;
+    if (s.fanout == 4 && m == 4 * unit && 3 * unit < 256) s.mode = SPLIT_POW2_F4;
     return s;
+}
+
+// Mix13 of z up to the low 16 bits of the high word of the last product (bits 46..47 of remix(z) are
+// bits 14..15 of the result).
+__device__ __forceinline__ uint32_t remix_top16(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z ^= z >> 27;
+    const uint32_t lo = (uint32_t)z, hi = (uint32_t)(z >> 32);
+    // high word of lo64(z * C) = mulhi(lo, C_lo) + lo * C_hi + hi * C_lo (mod 2^32); mod 2^16 the
+    // cross terms need only the low 16 bits of their operands
+    return __umulhi(lo, 0x133111ebu) + __umul24(lo, 0x94d049bbu) + __umul24(hi, 0x133111ebu);
 }
 
 // Value before the final scaling: 48-bit (sux) or 16-bit (SIMD) fraction of the hash.
@@ -54,13 +85,18 @@ __device__ __forceinline__ uint64_t split_value(uint64_t z, int simd) {
     return remix(z) & 0xFFFFFFFFFFFFull;
 }
 
-template <int S, int NT, int SIMD>
+template <int S, int NT, int SIMD, int MODE>
 __device__ __forceinline__ void split_eval(uint64_t key, const uint64_t (&seed)[S], const SplitShapeDev &sh,
                                            uint32_t (&c)[S][NT]) {
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const uint64_t z = key + seed[s];
-        if (SIMD) {
+        if (MODE != SPLIT_THRESHOLDS) {
+            // the top bits of the w-bit value sit at bits 14..15 of `top`
+            const uint32_t top = SIMD ? remix32((uint32_t)(z >> 32) ^ (uint32_t)z) : remix_top16(z);
+            if (MODE == SPLIT_POW2_F2) c[s][0] += (top >> 15) & 1u;
+            else c[s][0] += 1u << ((top >> 11) & 0x18u);
+        } else if (SIMD) {
             const uint32_t v = remix32((uint32_t)(z >> 32) ^ (uint32_t)z) & 0xFFFFu;
 #pragma unroll
             for (int p = 0; p < NT; p++) c[s][p] += v < (uint32_t)sh.thr[p] ? 1u : 0u;
@@ -74,7 +110,7 @@ __device__ __forceinline__ void split_eval(uint64_t key, const uint64_t (&seed)[
 
 // One window: seeds start + woff + [0, 64*S). Returns the offset (seed - start) of the first seed whose
 // parts have the target sizes, or SHGPU_NOT_FOUND. Seeds are tried in (slot, lane) order.
-template <int S, int NT, int SIMD>
+template <int S, int NT, int SIMD, int MODE>
 __device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
                                                    uint64_t woff) {
     const uint32_t lane = lane_id();
@@ -92,16 +128,19 @@ __device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const S
         const uint32_t cnt = m - chunk < 64 ? m - chunk : 64;
         if (cnt == 64) {
 #pragma unroll 8
-            for (uint32_t k = 0; k < 64; k++) split_eval<S, NT, SIMD>(readlane64(kv, k), seed, sh, c);
+            for (uint32_t k = 0; k < 64; k++) split_eval<S, NT, SIMD, MODE>(readlane64(kv, k), seed, sh, c);
         } else {
-            for (uint32_t k = 0; k < cnt; k++) split_eval<S, NT, SIMD>(readlane64(kv, k), seed, sh, c);
+            for (uint32_t k = 0; k < cnt; k++) split_eval<S, NT, SIMD, MODE>(readlane64(kv, k), seed, sh, c);
         }
     }
 #pragma unroll
     for (int s = 0; s < S; s++) {
         bool ok = true;
+        if (MODE == SPLIT_POW2_F2) ok = c[s][0] == sh.unit;  // keys in the upper half
+        else if (MODE == SPLIT_POW2_F4) ok = c[s][0] == sh.unit * 0x01010101u;
+        else
 #pragma unroll
-        for (int p = 0; p < NT; p++) ok = ok && c[s][p] == sh.t[p];
+            for (int p = 0; p < NT; p++) ok = ok && c[s][p] == sh.t[p];
         const uint64_t b = ballot(ok);
         if (b) return woff + (uint64_t)(s * 64 + __builtin_ctzll(b));
     }
@@ -112,13 +151,17 @@ template <int S>
 __device__ __forceinline__ uint64_t split_window(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
                                                  uint64_t woff) {
     if (sh.simd) {
-        if (sh.nt == 1) return split_window_t<S, 1, 1>(keys, sh, start, woff);
-        if (sh.nt == 2) return split_window_t<S, 2, 1>(keys, sh, start, woff);
-        return split_window_t<S, 3, 1>(keys, sh, start, woff);
+        if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 1, SPLIT_POW2_F2>(keys, sh, start, woff);
+        if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 1, SPLIT_POW2_F4>(keys, sh, start, woff);
+        if (sh.nt == 1) return split_window_t<S, 1, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff);
+        if (sh.nt == 2) return split_window_t<S, 2, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff);
+        return split_window_t<S, 3, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff);
     }
-    if (sh.nt == 1) return split_window_t<S, 1, 0>(keys, sh, start, woff);
-    if (sh.nt == 2) return split_window_t<S, 2, 0>(keys, sh, start, woff);
-    return split_window_t<S, 3, 0>(keys, sh, start, woff);
+    if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 0, SPLIT_POW2_F2>(keys, sh, start, woff);
+    if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 0, SPLIT_POW2_F4>(keys, sh, start, woff);
+    if (sh.nt == 1) return split_window_t<S, 1, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff);
+    if (sh.nt == 2) return split_window_t<S, 2, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff);
+    return split_window_t<S, 3, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff);
 }
 
 // Sequential search by one wave: the absolute seed, or SHGPU_NOT_FOUND below start + max_trials.

@@ -65,17 +65,22 @@ def test_sh2_leaves_all_sizes(engine):
     assert len(bad) == 0, f"mismatch at sizes {sizes[bad][:10]} got {got[bad][:5]} want {want[bad][:5]}"
 
 
-@pytest.mark.parametrize("variant,leaf", [(0, 64), (1, 58), (0, 30)])
+@pytest.mark.parametrize("variant,leaf", [(0, 64), (1, 58), (0, 30), (0, 16), (1, 16), (1, 24)])
 def test_split_search(engine, variant, leaf):
-    """Split nodes of every kind: seeds and stable partitions equal the oracle's."""
+    """Split nodes of every kind: seeds and stable partitions equal the oracle's. The sizes include
+    exact power-of-two fanouts (m = 2 * unit, m = 4 * unit), which the kernel counts from the top
+    bits of the hash, and near misses, which it counts with thresholds."""
     log(f"split variant={variant} leaf={leaf}")
     rng = np.random.default_rng(7 + variant)
     # (m, level) covering fanout-2, upper and lower aggregation nodes of the leaf size.
-    sizes = [2000, 1536, 768, 700, 464, 256, 230, 200, 130, 100, 70]
+    sizes = [2000, 1536, 768, 700, 464, 512, 256, 232, 230, 200, 130, 128, 120, 116, 100, 96, 70, 64, 60, 48,
+             33, 32]
     tasks, chunks, expect_codes, expect_keys = [], [], [], []
     off = 0
     for m in sizes:
         level = int(rng.integers(0, 5))
+        if m <= leaf:
+            continue  # a leaf, not a split node
         keys = workload.fingerprints(m, seed=100 + m)[:, 1]
         st_variant = 2 if (variant == 0 and leaf == 64) else variant
         # oracle split: variant 0/2 -> sux flavour (ShockHash, ShockHash2), 1 -> SIMD flavour

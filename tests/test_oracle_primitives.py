@@ -105,3 +105,40 @@ def test_spooky_readme_kat_end_to_end():
     h = np.array([O.spooky128(k.encode(), 0) for k in kat["keys"]], dtype=np.uint64)
     b = O.OracleBuild("sh1", kat["leaf_size"], h, kat["bucket_size"])
     assert list(b.query(h)) == kat["values"]
+
+
+def test_power_of_two_split_parts_from_top_bits():
+    """The split kernel's shortcut for m = F * unit, F in {2, 4}: the part of a key is bits 46..47 of
+    remix(z) (sux flavour), and those equal bits 14..15 of mulhi(lo, C_lo) + lo * C_hi + hi * C_lo
+    (mod 2^16) where (hi, lo) is Mix13's state before the last multiply. Checked against the oracle's
+    remix and against floor(h / unit) with h = remap16(remix(z), m)."""
+    rng = np.random.default_rng(3)
+    z = rng.integers(0, 2**64 - 1, size=20000, dtype=np.uint64)
+    M = 0xFFFFFFFFFFFFFFFF
+    for zz in z[:4000]:
+        x = int(zz)
+        x = ((x ^ (x >> 30)) * 0xbf58476d1ce4e5b9) & M
+        x ^= x >> 27
+        lo, hi = x & 0xFFFFFFFF, x >> 32
+        top = ((lo * 0x133111eb) >> 32) + (lo & 0xFFFFFF) * (0x94d049bb & 0xFFFFFF) + \
+            (hi & 0xFFFFFF) * (0x133111eb & 0xFFFFFF)
+        r = O.lib().orc_remix(int(zz))
+        assert (top >> 14) & 3 == (r >> 46) & 3
+        for m, unit in ((256, 64), (120, 30), (1536, 768), (128, 64)):
+            h = ((r & (2**48 - 1)) * m) >> 48
+            F = m // unit
+            assert h // unit == ((r >> 46) & 3) >> (2 - F.bit_length() + 1)
+
+
+@pytest.mark.parametrize("unit", [8, 30, 58, 64, 85])
+def test_packed_four_way_counts_are_exact(unit):
+    """F = 4: sum of 1 << (8 * part) equals unit * 0x01010101 (mod 2^32) exactly when every part holds
+    unit keys, over every count vector (c0, c1, c2, c3) with c0 + c1 + c2 + c3 = 4 * unit."""
+    m = 4 * unit
+    c0, c1, c2 = np.meshgrid(np.arange(m + 1), np.arange(m + 1), np.arange(m + 1), indexing="ij")
+    c3 = m - c0 - c1 - c2
+    ok = c3 >= 0
+    c0, c1, c2, c3 = c0[ok], c1[ok], c2[ok], c3[ok]
+    packed = (c0 + (c1 << 8) + (c2 << 16) + (c3 << 24)) & 0xFFFFFFFF
+    hit = packed == unit * 0x01010101
+    assert np.array_equal(hit, (c0 == unit) & (c1 == unit) & (c2 == unit))
