@@ -20,9 +20,6 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MI355X_CUS, SIMDS_PER_CU, LANES_PER_SIMD, CLOCK_GHZ = 256, 4, 16, 2.4
-INT32_PEAK_TOPS = MI355X_CUS * SIMDS_PER_CU * LANES_PER_SIMD * CLOCK_GHZ / 1e3  # 39.3 T lane-ops/s
-OPS_PER_EVAL = 32  # int32 ops per 64-bit-mix hash evaluation (remix + range reduction + count/mask update)
 # HBM bytes per k_build launch on this workload: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of
 # this script, scaled by the same counters on the 256 MiB copy probe (profiles/k_build_shockhash2_L64_10M.md).
 MEASURED_TRAFFIC_BYTES = 3.890e9
@@ -30,27 +27,6 @@ MEASURED_TRAFFIC_BYTES = 3.890e9
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def algorithmic_evals(nodes, codes):
-    """Hash evaluations a sequential search makes to reach the stored codes, from the outputs alone:
-    split node / ShockHash1 leaf m * (code + 1); ShockHash2 leaf m >= 10: m * (s* + 1) with
-    s* = max(unpairElegant(c_j)), (c_j, c_i) = unpairTriangular(code); m < 10: m * (c_j + 1)."""
-    m = nodes["m"].astype(np.float64)
-    kind = nodes["kind"]
-    codes = codes.astype(np.float64)
-    evals = np.where(kind != 3, m * (codes + 1), 0.0)
-    leaf = kind == 3
-    c = codes[leaf]
-    cj = np.floor(0.5 + np.sqrt(0.25 + 2.0 * c))
-    f = np.floor(np.sqrt(cj))
-    r = cj - f * f
-    a = np.where(r < f, r, f)
-    b = np.where(r < f, f, r - f)
-    s_star = np.maximum(a, b)
-    ml = m[leaf]
-    evals_leaf = np.where(ml >= 10, ml * (s_star + 1), ml * (cj + 1))
-    return float(evals.sum() + evals_leaf.sum()), float(evals_leaf.sum())
 
 
 def O_build(h, threads):
@@ -155,6 +131,7 @@ def main():
 
     from shockhash_amd import workload
     from shockhash_amd.engine import Engine
+    from shockhash_amd.roofline import roofline
 
     eng = Engine(0)
     tune = {k: v for k, v in (("split_seeds_per_lane", args.split_seeds_per_lane),
@@ -193,9 +170,8 @@ def main():
             f"search {b.timings_ms[2]:.1f}, kernel {b.timings_ms[3]:.1f})")
     ms_per_step = 1e3 * float(np.mean(step_s))
     value = args.n / (ms_per_step / 1e3)
-    evals, evals_leaf = algorithmic_evals(b.nodes, b.codes)
     kms = float(np.mean(kernel_ms))
-    achieved_tops = evals * OPS_PER_EVAL / (kms / 1e3) / 1e12
+    roof = roofline(b.nodes, b.codes, "sh2", kms, MEASURED_TRAFFIC_BYTES if args.n == 10_000_000 else None)
 
     # plain form: one launch per node over the first buckets, same device input
     t0 = time.perf_counter()
@@ -256,12 +232,8 @@ def main():
                        "codes_equal_persistent": plain_match},
         "speedup_vs_plain": value / plain_value,
         "bit_exact": bit_exact,
-        "hash_evals": evals,
-        "hash_evals_leaves": evals_leaf,
         "search_kernel_ms": kms,
-        "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": INT32_PEAK_TOPS, "unit": "TOP/s",
-                     "frac": achieved_tops / INT32_PEAK_TOPS, "traffic": MEASURED_TRAFFIC_BYTES if args.n == 10_000_000 else None,
-                     "ops_per_eval": OPS_PER_EVAL, "evals_per_s": evals / (kms / 1e3)},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "p99_build_latency_ms": serving["p99_ms"] if serving else None,
         "serving": serving,

@@ -59,8 +59,13 @@ typedef struct shgpu_tuning {
     uint32_t sh1_seeds_per_lane;    /* 1, 2 or 4: ShockHash1 leaf window = 64 * this */
     uint32_t split_seeds_per_lane;  /* 1, 2 or 4: split window for aggregation nodes */
     uint32_t waves_per_cu;          /* persistent grid = CUs * this */
+    uint32_t flags;                 /* SHGPU_TUNE_* bits, 0 by default */
     uint64_t max_trials;            /* seed bound per split node / ShockHash1 leaf */
 } shgpu_tuning;
+
+/* shgpu_tuning.flags: search ShockHash2 leaves with the scratch-memory path only (by default leaves run on
+ * the register path and fall back to it when they outgrow the registers; both give the same codes). */
+#define SHGPU_TUNE_SH2_GENERAL_ONLY 1u
 
 /* Last error message of the calling thread. */
 const char *shgpu_last_error(void);

@@ -60,6 +60,20 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// 24-bit multiply-add d = (a & 0xFFFFFF) * (b & 0xFFFFFF) + c (mod 2^32), one full-rate VALU op. Exact mod 2^24
+// for any a, b, which is all the callers use. The compiler splits mul24 + add into a 32-bit multiply and an
+// add3, so the instruction is written out.
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t d;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+
+// Keys a wave stages in LDS and reads back as wave-wide broadcasts (every lane reads the same address:
+// one LDS instruction, no bank conflicts, no VALU work), so the hash loops spend their VALU issue on
+// hashing only.
+constexpr uint32_t STAGE_KEYS = 256;
+
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));

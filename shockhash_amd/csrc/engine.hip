@@ -1,6 +1,5 @@
 // libshockhash_amd.so: kernels and C ABI of the MI355X ShockHash seed-search engine.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -19,6 +18,7 @@
 #include "search_sh1.h"
 #include "search_sh2.h"
 #include "search_split.h"
+#include "sort.h"
 
 using namespace shgpu;
 
@@ -30,6 +30,8 @@ const uint64_t START_SEED[20] = {
     0x4cd6eb4c63befb7c, 0x9bfd8c5e18c8da73, 0x082f20e10092a9a3, 0x2ada2ce68d21defc, 0xe33cb4f3e7c6466b,
     0x3980be458c509c59, 0xc466fd9584828e8c, 0x45f0aabe1a61ede6, 0xf6e7b8b33ad9b98d, 0x4ef95e25f4b4983d,
     0x81175195173b92d3, 0x4e50927d8dd15978, 0x1ea2099d1fafae7f, 0x425c8a06fbaaa815, 0xcd4216006c74052a};
+
+static_assert(SHGPU_TUNE_SH2_GENERAL_ONLY == SH2_FLAG_GENERAL_ONLY, "tuning flag of the ShockHash2 leaf search");
 
 constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
 constexpr int BLOCK = 256;
@@ -111,8 +113,8 @@ struct shgpu_engine {
     char arch[64] = {0};
     // Schedule per variant (ShockHash, SIMDShockHash, ShockHash2); the defaults are the measured best on
     // MI355X for leaf 30 / 58 / 64 builds (see shgpu_autotune).
-    shgpu_tuning tuning_v[3] = {{4, 2, 24, 1ull << 40}, {4, 2, 24, 1ull << 40}, {4, 2, 24, 1ull << 40}};
-    shgpu_tuning tuning{4, 2, 24, 1ull << 40};  // batched per-node entry points
+    shgpu_tuning tuning_v[3] = {{4, 2, 24, 0, 1ull << 40}, {4, 2, 24, 0, 1ull << 40}, {4, 2, 24, 0, 1ull << 40}};
+    shgpu_tuning tuning{4, 2, 24, 0, 1ull << 40};  // batched per-node entry points
     DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
     BuildBuffers main;                          // shgpu_build_run
 };
@@ -151,7 +153,8 @@ __global__ void __launch_bounds__(BLOCK) k_sh1_batch(const uint64_t *__restrict_
 }
 
 __global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict__ keys, const shgpu_leaf_task *__restrict__ tasks,
-                                                     uint32_t n, uint64_t *__restrict__ out, uint32_t *counter, uint8_t *scratch) {
+                                                     uint32_t n, uint64_t *__restrict__ out, uint32_t *counter, uint8_t *scratch,
+                                                     uint32_t flags) {
     __shared__ Sh2Lds lds[WAVES_PER_BLOCK];
     const uint32_t w = threadIdx.x / 64;
     Sh2Lds &L = lds[w];
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict_
         const uint32_t t = grab(counter);
         if (t >= n) break;
         const shgpu_leaf_task tk = tasks[t];
-        const uint64_t code = sh2_leaf_search(keys + uniform(tk.offset), uniform(tk.m), ws, L);
+        const uint64_t code = sh2_leaf_search(keys + uniform(tk.offset), uniform(tk.m), ws, L, flags);
         out[t] = code;
     }
 }
@@ -168,13 +171,15 @@ __global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict_
 template <int S>
 __global__ void __launch_bounds__(BLOCK) k_split_batch(uint64_t *keys, uint64_t *tmp, const shgpu_split_task *__restrict__ tasks,
                                                        uint32_t n, uint64_t *__restrict__ out, uint32_t *counter) {
+    __shared__ PfLds lds[WAVES_PER_BLOCK];
+    uint64_t *stage = lds[threadIdx.x / 64].stage;
     for (;;) {
         const uint32_t t = grab(counter);
         if (t >= n) break;
         const shgpu_split_task tk = tasks[t];
         const uint32_t off = uniform(tk.offset), m = uniform(tk.m);
         const SplitShapeDev sh = make_split_shape(m, uniform(tk.unit), uniform(tk.kind), uniform(tk.hash));
-        const uint64_t x = split_search<S>(keys + off, sh, uniform64(tk.start_seed), uniform64(tk.max_trials));
+        const uint64_t x = split_search<S>(keys + off, sh, uniform64(tk.start_seed), uniform64(tk.max_trials), stage);
         if (x != SHGPU_NOT_FOUND) split_partition(keys + off, tmp + off, sh, x);
         out[t] = x;
     }
@@ -224,6 +229,7 @@ struct BuildArgs {
     uint32_t nslots;
     uint8_t *scratch;
     int split_hash;
+    uint32_t flags;  // shgpu_tuning.flags of the variant
 };
 
 __device__ __forceinline__ unsigned long long wave_fetch_add64(unsigned long long *p, unsigned long long add) {
@@ -285,6 +291,7 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
     const uint64_t start = uniform64(nd.start_seed), lim = uniform64(nd.max_trials);
     NodeState *st = &a.state[nid];
     const uint64_t W = window_seeds<SL, SS>(kind);
+    constexpr int SPLIT_HASH = VARIANT == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
     SplitShapeDev sh;
     if (kind != SHGPU_NODE_LEAF) sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
     for (;;) {
@@ -298,9 +305,9 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
                 if constexpr (VARIANT != SHGPU_SHOCKHASH2)  // ShockHash2 leaves never come here
                     r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND) : sh1_window<SL>(a.keys + o, m, start, woff, L);
             } else if (kind == SHGPU_NODE_AGGR_LOWER)
-                r = split_window<SS>(a.keys + o, sh, start, woff);
+                r = split_window<SS, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage);
             else
-                r = split_window<1>(a.keys + o, sh, start, woff);
+                r = split_window<1, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage);
             if (r != SHGPU_NOT_FOUND && r >= lim) r = SHGPU_NOT_FOUND;
             if (r != SHGPU_NOT_FOUND) {
                 // same value in every lane; waiting for the returned value orders this atomic before the
@@ -344,8 +351,9 @@ __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLd
     return false;
 }
 
+// A wave runs either a windowed search (pf) or a ShockHash2 leaf (sh2), never both at once.
 template <int VARIANT>
-struct BuildLds {
+union BuildLds {
     PfLds pf;
     typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, char>::type sh2;
 };
@@ -359,7 +367,7 @@ __device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint3
     const uint32_t kind = uniform(nd.kind);
     if (VARIANT == SHGPU_SHOCKHASH2 && kind == SHGPU_NODE_LEAF) {
         if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            const uint64_t code = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2);
+            const uint64_t code = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.flags);
             a.codes[nid] = code;
             wave_fetch_add(&a.ctrl[3], 1u);
         }
@@ -377,7 +385,7 @@ constexpr uint32_t SPIN_LIMIT = 1u << 27;  // sleeps before a wait is declared s
 // exhausted, waves keep helping until every node is finished. Split nodes publish their children
 // when their partition is written.
 template <int VARIANT, int SL, int SS>
-__global__ void __launch_bounds__(BLOCK) k_build(BuildArgs a) {
+__global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
     __shared__ BuildLds<VARIANT> lds[WAVES_PER_BLOCK];
     const uint32_t w = threadIdx.x / 64;
     const uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
@@ -414,7 +422,8 @@ __global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
     const shgpu_node nd = a.nodes[nid];
     if (VARIANT == SHGPU_SHOCKHASH2 && uniform(nd.kind) == SHGPU_NODE_LEAF) {
         if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            if (blockIdx.x == 0) a.codes[nid] = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2);
+            if (blockIdx.x == 0)
+                a.codes[nid] = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2, a.flags);
         }
         return;
     }
@@ -553,7 +562,11 @@ int launch_build(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
 template <int VARIANT>
 int launch_build_v(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
     const uint32_t sl = e->tuning_v[VARIANT].sh1_seeds_per_lane, ss = e->tuning_v[VARIANT].split_seeds_per_lane;
-#define SHGPU_CASE(A, B) if (sl == A && ss == B) return launch_build<VARIANT, A, B>(e, a, grid, s);
+    // ShockHash2 leaves do not use the ShockHash1 window: one SL instantiation serves every setting
+#define SHGPU_CASE(A, B)                                                         \
+    if constexpr (VARIANT != SHGPU_SHOCKHASH2 || A == 1) {                       \
+        if ((VARIANT == SHGPU_SHOCKHASH2 || sl == A) && ss == B) return launch_build<VARIANT, A, B>(e, a, grid, s); \
+    }
     SHGPU_CASE(1, 1) SHGPU_CASE(1, 2) SHGPU_CASE(1, 4) SHGPU_CASE(2, 1) SHGPU_CASE(2, 2) SHGPU_CASE(2, 4)
     SHGPU_CASE(4, 1) SHGPU_CASE(4, 2) SHGPU_CASE(4, 4)
 #undef SHGPU_CASE
@@ -562,7 +575,7 @@ int launch_build_v(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s)
 
 template <int VARIANT>
 int launch_node_v(shgpu_engine *e, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
-    k_node<VARIANT, 4, 4><<<waves, 64, 0, s>>>(a, nid);
+    k_node<VARIANT, VARIANT == SHGPU_SHOCKHASH2 ? 1 : 4, 4><<<waves, 64, 0, s>>>(a, nid);
     HIP_TRY(hipGetLastError());
     return SHGPU_OK;
 }
@@ -643,6 +656,7 @@ static int check_tuning(const shgpu_tuning *t) {
         return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(SHGPU_ERR_ARG, "waves_per_cu must be in [1, 32]");
     if (t->max_trials == 0) return fail(SHGPU_ERR_ARG, "max_trials must be positive");
+    if (t->flags & ~SHGPU_TUNE_SH2_GENERAL_ONLY) return fail(SHGPU_ERR_ARG, "unknown tuning flags");
     return SHGPU_OK;
 }
 
@@ -706,7 +720,8 @@ int shgpu_sh2_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_l
     const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
     rc = ensure_sh2_scratch(e, (size_t)grid * WAVES_PER_BLOCK);
     if (rc) return rc;
-    k_sh2_batch<<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_codes, e->counter.as<uint32_t>(), e->sh2_scratch.as<uint8_t>());
+    k_sh2_batch<<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_codes, e->counter.as<uint32_t>(), e->sh2_scratch.as<uint8_t>(),
+                                       e->tuning.flags);
     HIP_TRY(hipGetLastError());
     return SHGPU_OK;
 }
@@ -801,8 +816,8 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     uint64_t nmax = 0;
     for (size_t r = 0; r < count; r++) nmax = std::max(nmax, in[r].n);
     size_t sort_bytes = 0;
-    BATCH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, B.firsts_in.as<uint64_t>(), B.firsts_out.as<uint64_t>(),
-                                                 B.seconds_in.as<uint64_t>(), B.keys.as<uint64_t>(), (int)nmax, 0, 64, s));
+    BATCH_HIP(shgpu_sort_pairs_u64(nullptr, sort_bytes, B.firsts_in.as<uint64_t>(), B.firsts_out.as<uint64_t>(),
+                                   B.seconds_in.as<uint64_t>(), B.keys.as<uint64_t>(), (int)nmax, s));
     if ((rc = B.sort_tmp.ensure(sort_bytes + 16))) return fail_all(rc);
     for (size_t r = 0; r < count; r++) {
         const uint64_t n = in[r].n, o = base[r];
@@ -810,9 +825,8 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             k_unzip_hashes<<<1024, 256, 0, s>>>(in[r].d_hashes, n, B.firsts_in.as<uint64_t>() + o, B.seconds_in.as<uint64_t>() + o);
             BATCH_HIP(hipGetLastError());
             size_t bytes = sort_bytes;
-            BATCH_HIP(hipcub::DeviceRadixSort::SortPairs(B.sort_tmp.p, bytes, B.firsts_in.as<uint64_t>() + o,
-                                                         B.firsts_out.as<uint64_t>() + o, B.seconds_in.as<uint64_t>() + o,
-                                                         B.keys.as<uint64_t>() + o, (int)n, 0, 64, s));
+            BATCH_HIP(shgpu_sort_pairs_u64(B.sort_tmp.p, bytes, B.firsts_in.as<uint64_t>() + o, B.firsts_out.as<uint64_t>() + o,
+                                           B.seconds_in.as<uint64_t>() + o, B.keys.as<uint64_t>() + o, (int)n, s));
         }
         k_bucket_bounds<<<1024, 256, 0, s>>>(B.firsts_out.as<uint64_t>() + o, n, nbk[r], B.size_acc.as<uint64_t>() + bbase[r]);
         BATCH_HIP(hipGetLastError());
@@ -877,6 +891,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     a.ctrl = B.ctrl.as<uint32_t>();
     a.total = total;
     a.split_hash = d->variant == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
+    a.flags = e->tuning_v[d->variant].flags;
     a.scratch = nullptr;
     a.state = B.states.as<NodeState>();
     a.help = B.help.as<uint32_t>();

@@ -8,8 +8,8 @@
 // (TinyBinaryCuckooHashTable.h:138-149) cover all m cells and form a pseudoforest (the cuckoo
 // insertion of TinyBinaryCuckooHashTable::construct succeeds exactly then).
 //
-// Mapping: lanes over seeds (S seeds per lane, window of 64*S consecutive seeds), keys broadcast
-// from one VGPR pair with v_readlane. The coverage mask is split into two 32-bit halves (cells
+// Mapping: lanes over seeds (S seeds per lane, window of 64*S consecutive seeds), keys staged in LDS
+// and read back as wave-wide broadcasts. The coverage mask is split into two 32-bit halves (cells
 // [0, m/2) and [m/2, m)), one v_lshl_or each. Candidates that cover every cell go, in seed order,
 // through a wave-cooperative pseudoforest test (lanes over keys, min-label propagation in LDS).
 #pragma once
@@ -17,8 +17,10 @@
 
 namespace shgpu {
 
-// LDS a wave needs for pseudoforest_wave(): 64 labels + 128 counters.
+// LDS of one wave for the windowed searches: the key stage (split and leaf windows) and the labels and
+// counters of pseudoforest_wave().
 struct PfLds {
+    __align__(16) uint64_t stage[STAGE_KEYS];
     uint32_t lab[64];
     uint32_t cnt[128];
 };
@@ -87,11 +89,28 @@ __device__ __forceinline__ uint64_t sh1_window(const uint64_t *keys, uint32_t m,
     uint32_t mA[S], mB[S];
 #pragma unroll
     for (int s = 0; s < S; s++) { seed[s] = start + woff + (uint64_t)(s * 64 + lane); mA[s] = 0; mB[s] = 0; }
+    // keys staged in LDS, read back as broadcasts (two per read)
+    wave_sync();
+    if (lane < m) L.stage[lane] = kv;
+    wave_sync();
+    const ulonglong2 *st2 = (const ulonglong2 *)L.stage;
     if (m == 64) {
-#pragma unroll 16
-        for (uint32_t k = 0; k < 64; k++) sh1_eval<S>(readlane64(kv, k), seed, halfA, halfB, mA, mB);
+#pragma unroll
+        for (uint32_t k = 0; k < 32; k++) {
+            const ulonglong2 kk = st2[k];
+            sh1_eval<S>(kk.x, seed, halfA, halfB, mA, mB);
+            sh1_eval<S>(kk.y, seed, halfA, halfB, mA, mB);
+        }
     } else {
-        for (uint32_t k = 0; k < m; k++) sh1_eval<S>(readlane64(kv, k), seed, halfA, halfB, mA, mB);
+        uint32_t k = 0;
+        for (; k + 4 <= m; k += 4) {
+            const ulonglong2 k0 = st2[k / 2], k1 = st2[k / 2 + 1];
+            sh1_eval<S>(k0.x, seed, halfA, halfB, mA, mB);
+            sh1_eval<S>(k0.y, seed, halfA, halfB, mA, mB);
+            sh1_eval<S>(k1.x, seed, halfA, halfB, mA, mB);
+            sh1_eval<S>(k1.y, seed, halfA, halfB, mA, mB);
+        }
+        for (; k < m; k++) sh1_eval<S>(L.stage[k], seed, halfA, halfB, mA, mB);
     }
 #pragma unroll
     for (int s = 0; s < S; s++) {

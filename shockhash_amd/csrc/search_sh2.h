@@ -16,13 +16,20 @@
 //    succeeds when the graph with edges (hash_j(key) + m/2, hash_i(key)) is a pseudoforest. The first
 //    success returns pairTriangular(code_j, code_i) (:519-541), the smallest successful code.
 //
-// Mapping: half-seed masks are computed 64 at a time (lanes over half-seeds). Candidates are
-// extracted with lanes over the earlier half-seeds (a linear scan of the mask table, as the
-// reference's CandidateList does). Each candidate's 64 cell indices and isolated-key mask go to a
-// per-wave global scratch area. Pairs passing the isolated filter are queued in LDS in (j, i) order and
-// tested 64 at a time, one pair per lane, with a per-lane union-find (tree flags, UnionFind.h:58-67
-// semantics) whose parent bytes are interleaved in LDS. Any exact pseudoforest predicate gives the
-// reference's result: CuckooUnionFind, UnionFind and the textbook component count agree.
+// Two implementations of that search, with identical results:
+//  * the register path (sh2_leaf_search_regs): half-seed masks of the first SH2_HS_BLOCKS * 64
+//    half-seeds and the isolated-key masks of the first SH2_CAND_BLOCKS * 64 candidates live
+//    in VGPRs (lane l of block b holds entry 64 b + l), so candidate extraction and the isolated-vertex
+//    filter are ballots over registers. Keys are staged in LDS split by parity and read back as
+//    broadcasts. A leaf that needs more half-seeds or candidates (at m = 64 the median is ~100
+//    half-seeds and ~40 candidates) reports SH2_OVERFLOW and is searched again by
+//  * the general path (sh2_leaf_search_general), whose tables live in the wave's global scratch area
+//    and are bounded only by SH2_MAX_HALF_SEEDS / SH2_MAX_CANDIDATES.
+// Both write each candidate's cell indices (64 bytes) to scratch and queue the pairs that pass the
+// isolated filter in LDS in (j, i) order; 64 queued pairs are tested at a time, one pair per lane,
+// with a per-lane union-find (tree flags, UnionFind.h:58-67 semantics; path halving, the smaller root
+// becomes the parent) whose parent bytes are interleaved in LDS. Any exact pseudoforest predicate gives
+// the reference's result: CuckooUnionFind, UnionFind and the textbook component count agree.
 #pragma once
 #include "device_common.h"
 
@@ -32,11 +39,16 @@ constexpr uint32_t SH2_MAX_HALF_SEEDS = 1024;  // half-seed bound (reference exp
 constexpr uint32_t SH2_MAX_CANDIDATES = 4096;
 constexpr size_t SH2_SCRATCH_BYTES =
     SH2_MAX_HALF_SEEDS * 4 * 2 + SH2_MAX_CANDIDATES * (4 + 8 + 64);
+constexpr uint32_t SH2_HS_BLOCKS = 4;    // register path: half-seeds [0, 256)
+constexpr uint32_t SH2_CAND_BLOCKS = 2;  // register path: candidates [0, 128)
+constexpr uint64_t SH2_OVERFLOW = SHGPU_NOT_FOUND - 1;
 
 struct Sh2Lds {
     uint32_t par[16 * 64];  // union-find parents: byte x of lane l at dword (x >> 2) * 64 + l
     uint32_t queue[128];    // pending pairs j << 16 | i
     uint32_t hist[64];
+    __align__(16) uint64_t keysA[64];  // register path: even keys, then
+    __align__(16) uint64_t keysB[64];  // odd keys, in leaf order
 };
 
 struct Sh2Scratch {
@@ -59,22 +71,25 @@ __device__ __forceinline__ void mem_sync() {
 
 __device__ __forceinline__ uint32_t uf_addr(uint32_t x, uint32_t lane) { return ((x >> 2) * 64 + lane) * 4 + (x & 3); }
 
-__device__ __forceinline__ uint32_t uf_find(const uint8_t *par, uint32_t x, uint32_t lane) {
+// Root of x with path halving (every visited node is re-pointed to its grandparent).
+__device__ __forceinline__ uint32_t uf_find(uint8_t *par, uint32_t x, uint32_t lane) {
     for (;;) {
         const uint32_t p = par[uf_addr(x, lane)];
         if (p == x) return x;
-        x = p;
+        const uint32_t gp = par[uf_addr(p, lane)];
+        par[uf_addr(x, lane)] = (uint8_t)gp;
+        x = gp;
     }
 }
 
 // Tests the 64 queued pairs (one per lane). Returns the lane mask of pseudoforest pairs. Each lane
 // streams its two candidates' cell bytes four keys at a time from the scratch area.
-__device__ __forceinline__ uint64_t sh2_uf_batch(const Sh2Scratch &g, Sh2Lds &L, uint32_t m) {
+__device__ __forceinline__ uint64_t sh2_uf_batch(const uint8_t *hashes, Sh2Lds &L, uint32_t m) {
     const uint32_t lane = lane_id();
     const uint32_t pr = L.queue[lane];
     const uint32_t j = pr >> 16, i = pr & 0xFFFF;
-    const uint32_t *pj = (const uint32_t *)(g.hashes + (size_t)j * 64);
-    const uint32_t *pi = (const uint32_t *)(g.hashes + (size_t)i * 64);
+    const uint32_t *pj = (const uint32_t *)(hashes + (size_t)j * 64);
+    const uint32_t *pi = (const uint32_t *)(hashes + (size_t)i * 64);
     uint8_t *par = (uint8_t *)L.par;
 #pragma unroll
     for (uint32_t q = 0; q < 16; q++) {
@@ -97,9 +112,10 @@ __device__ __forceinline__ uint64_t sh2_uf_batch(const Sh2Scratch &g, Sh2Lds &L,
                 const uint32_t v = (ci >> (8 * r)) & 0xFF;
                 const uint32_t ru = uf_find(par, u, lane);
                 const uint32_t rv = uf_find(par, v, lane);
-                par[uf_addr(ru, lane)] = (uint8_t)rv;
+                const uint32_t root = ru < rv ? ru : rv, child = ru < rv ? rv : ru;
+                par[uf_addr(child, lane)] = (uint8_t)root;
                 const bool tu = (tree >> ru) & 1, tv = (tree >> rv) & 1;
-                if (!(tu && tv && ru != rv)) tree &= ~(1ull << rv);
+                if (!(tu && tv && ru != rv)) tree &= ~(1ull << root);
                 alive = tu || tv;
             }
         }
@@ -110,9 +126,10 @@ __device__ __forceinline__ uint64_t sh2_uf_batch(const Sh2Scratch &g, Sh2Lds &L,
     return res;
 }
 
-// Appends candidate (A, B) with the given code: cells of every key and its isolated-key mask.
-__device__ __forceinline__ void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L, uint32_t c, uint64_t kv, uint32_t m,
-                                         uint32_t H, uint32_t A, uint32_t B, uint32_t code) {
+// Cell of this lane's key (lane < m) in candidate (A, B) and the candidate's isolated-key mask; the cells
+// go to row c of the scratch cell table.
+__device__ __forceinline__ uint64_t sh2_candidate_cells(uint8_t *hashes, Sh2Lds &L, uint32_t c, uint64_t kv, uint32_t m,
+                                                        uint32_t H, uint32_t A, uint32_t B) {
     const uint32_t lane = lane_id();
     const bool act = lane < m;
     uint32_t h = 0;
@@ -123,13 +140,23 @@ __device__ __forceinline__ void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L
     wave_sync();
     const uint64_t iso = ballot(act && L.hist[h] == 1);
     wave_sync();
-    g.hashes[(size_t)c * 64 + lane] = act ? (uint8_t)h : 0;
+    hashes[(size_t)c * 64 + lane] = act ? (uint8_t)h : 0;
+    return iso;
+}
+
+// ------------------------------------------------------------------ general path (scratch tables)
+
+// Appends candidate (A, B) with the given code: cells of every key and its isolated-key mask.
+__device__ __forceinline__ void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L, uint32_t c, uint64_t kv, uint32_t m,
+                                                  uint32_t H, uint32_t A, uint32_t B, uint32_t code) {
+    const uint64_t iso = sh2_candidate_cells(g.hashes, L, c, kv, m, H, A, B);
     g.code[c] = code;  // every lane stores the same wave-uniform values
     g.iso[c] = iso;
 }
 
 // Returns the ShockHash2 leaf code, or SHGPU_NOT_FOUND when a scratch bound is exceeded.
-__device__ __forceinline__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32_t m, uint8_t *scratch, Sh2Lds &L) {
+__device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys, uint32_t m, uint8_t *scratch,
+                                                            Sh2Lds &L) {
     const uint32_t lane = lane_id();
     if (m < 2) return 0;
     const Sh2Scratch g(scratch);
@@ -202,7 +229,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32
             wave_sync();
         }
         if (qlen >= 64) {
-            const uint64_t ok = sh2_uf_batch(g, L, m);
+            const uint64_t ok = sh2_uf_batch(g.hashes, L, m);
             if (ok) {
                 const uint32_t l = (uint32_t)__builtin_ctzll(ok);
                 const uint32_t pr = L.queue[l];
@@ -217,6 +244,167 @@ __device__ __forceinline__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32
             wave_sync();
         }
     }
+}
+
+// ------------------------------------------------------------------ register path
+
+// Four (or two) per-lane registers addressed by a wave-uniform block number through compare-and-select.
+// Separate members rather than an array: the compiler turns a dynamically indexed array into scratch
+// memory, while these stay in VGPRs.
+template <class T>
+struct Regs4 {
+    T r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    __device__ __forceinline__ T at(uint32_t b) const { return b == 0 ? r0 : b == 1 ? r1 : b == 2 ? r2 : r3; }
+    __device__ __forceinline__ void set_if(bool c, uint32_t b, T v) {
+        r0 = c && b == 0 ? v : r0;
+        r1 = c && b == 1 ? v : r1;
+        r2 = c && b == 2 ? v : r2;
+        r3 = c && b == 3 ? v : r3;
+    }
+};
+template <class T>
+struct Regs2 {
+    T r0 = 0, r1 = 0;
+    __device__ __forceinline__ T at(uint32_t b) const { return b == 0 ? r0 : r1; }
+    __device__ __forceinline__ void set_if(bool c, uint32_t b, T v) {
+        r0 = c && b == 0 ? v : r0;
+        r1 = c && b == 1 ? v : r1;
+    }
+};
+static_assert(SH2_HS_BLOCKS == 4 && SH2_CAND_BLOCKS == 2, "register tables are Regs4 / Regs2");
+
+// OR of 1 << fastrange64(remix(key + s), H) over n staged keys (broadcast LDS reads, two keys per read).
+__device__ __forceinline__ uint32_t sh2_half_mask(const uint64_t *stage, uint32_t n, uint64_t s, uint32_t H) {
+    const ulonglong2 *st2 = (const ulonglong2 *)stage;
+    uint32_t mask = 0;
+    uint32_t k = 0;
+    for (; k + 4 <= n; k += 4) {
+        const ulonglong2 k0 = st2[k / 2], k1 = st2[k / 2 + 1];
+        mask |= 1u << fastrange64(remix(k0.x + s), H);
+        mask |= 1u << fastrange64(remix(k0.y + s), H);
+        mask |= 1u << fastrange64(remix(k1.x + s), H);
+        mask |= 1u << fastrange64(remix(k1.y + s), H);
+    }
+    for (; k < n; k++) mask |= 1u << fastrange64(remix(stage[k] + s), H);
+    return mask;
+}
+
+// Returns the ShockHash2 leaf code, or SH2_OVERFLOW when the leaf needs more half-seeds or candidates
+// than the registers hold.
+__device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, uint32_t m, uint8_t *scratch,
+                                                         Sh2Lds &L) {
+    const uint32_t lane = lane_id();
+    if (m < 2) return 0;
+    const Sh2Scratch g(scratch);
+    const uint32_t H = (m + 1) / 2;
+    const uint32_t FULL = H >= 32 ? 0xFFFFFFFFu : ((1u << H) - 1u);
+    const bool basic = m < 10;
+    const bool act = lane < m;
+    const uint64_t kv = act ? keys[lane] : 0;
+    // stage even keys in keysA and odd keys in keysB (the masks are ORs, the order within a set is free)
+    const bool even = act && (kv & 1) == 0;
+    const uint64_t be = ballot(even), bo = ballot(act && !even);
+    const uint32_t nA = (uint32_t)__builtin_popcountll(be), nB = (uint32_t)__builtin_popcountll(bo);
+    wave_sync();
+    if (even) L.keysA[lanes_below(be)] = kv;
+    else if (act) L.keysB[lanes_below(bo)] = kv;
+    wave_sync();
+
+    Regs4<uint32_t> hsA, hsB;
+    Regs2<uint64_t> isoR;
+    uint32_t n_blocks = 0, s_next = 0, n_cand = 0, jq = 1, qlen = 0;
+
+    // Appends candidate (A, B) with the given code; false when the register table is full. The code goes to
+    // the scratch table (read once, for the winner).
+    auto add = [&](uint32_t A, uint32_t B, uint32_t code) -> bool {
+        if (n_cand >= SH2_CAND_BLOCKS * 64) return false;
+        const uint64_t iso = sh2_candidate_cells(g.hashes, L, n_cand, kv, m, H, A, B);
+        isoR.set_if(lane == (n_cand & 63), n_cand >> 6, iso);
+        g.code[n_cand] = code;  // same value in every lane
+        n_cand++;
+        return true;
+    };
+
+    for (;;) {
+        // Make candidate jq available.
+        while (n_cand <= jq) {
+            if (s_next == 64 * n_blocks) {
+                if (n_blocks == SH2_HS_BLOCKS) return SH2_OVERFLOW;
+                const uint64_t s = 64 * n_blocks + lane;
+                const uint32_t ma = sh2_half_mask(L.keysA, nA, s, H), mb = sh2_half_mask(L.keysB, nB, s, H);
+                hsA.set_if(true, n_blocks, ma);
+                hsB.set_if(true, n_blocks, mb);
+                n_blocks++;
+            }
+            const uint32_t s = s_next++;
+            const uint32_t sb = s >> 6, sl = s & 63;
+            const uint32_t ms_a = (uint32_t)__builtin_amdgcn_readlane((int)hsA.at(sb), (int)sl);
+            const uint32_t ms_b = (uint32_t)__builtin_amdgcn_readlane((int)hsB.at(sb), (int)sl);
+            if (basic) {
+                if ((ms_a | ms_b) == FULL && !add(s, s, s)) return SH2_OVERFLOW;
+                continue;
+            }
+            // (a, s), a < s: code s*s + a
+            for (uint32_t b = 0; b <= sb; b++) {
+                const uint32_t a = 64 * b + lane;
+                uint64_t hit = ballot(a < s && (hsA.at(b) | ms_b) == FULL);
+                while (hit) {
+                    const uint32_t l = (uint32_t)__builtin_ctzll(hit);
+                    hit &= hit - 1;
+                    if (!add(64 * b + l, s, s * s + 64 * b + l)) return SH2_OVERFLOW;
+                }
+            }
+            // (s, b), b <= s: code s*s + s + b
+            for (uint32_t b = 0; b <= sb; b++) {
+                const uint32_t bb = 64 * b + lane;
+                uint64_t hit = ballot(bb <= s && (ms_a | hsB.at(b)) == FULL);
+                while (hit) {
+                    const uint32_t l = (uint32_t)__builtin_ctzll(hit);
+                    hit &= hit - 1;
+                    if (!add(s, 64 * b + l, s * s + s + 64 * b + l)) return SH2_OVERFLOW;
+                }
+            }
+        }
+        // Queue the pairs (jq, i), i < jq, that pass the isolated-vertex filter, 64 i at a time, and test
+        // whenever 64 are queued.
+        const uint64_t iso_j = readlane64(isoR.at(jq >> 6), jq & 63);
+        for (uint32_t b = 0; 64 * b < jq; b++) {
+            const uint32_t i = 64 * b + lane;
+            const bool pass = i < jq && (isoR.at(b) & iso_j) == 0;
+            const uint64_t bits = ballot(pass);
+            if (pass) L.queue[qlen + lanes_below(bits)] = (jq << 16) | i;
+            qlen += (uint32_t)__builtin_popcountll(bits);
+            wave_sync();
+            if (qlen >= 64) {
+                const uint64_t ok = sh2_uf_batch(g.hashes, L, m);
+                if (ok) {
+                    const uint32_t pr = L.queue[__builtin_ctzll(ok)];
+                    const uint64_t cj = g.code[pr >> 16], ci = g.code[pr & 0xFFFF];
+                    return cj * (cj - 1) / 2 + ci;  // pairTriangular (PairingFunction.h:47-50)
+                }
+                const uint32_t rest = qlen - 64;
+                const uint32_t moved = lane < rest ? L.queue[64 + lane] : 0;
+                wave_sync();
+                if (lane < rest) L.queue[lane] = moved;
+                qlen = rest;
+                wave_sync();
+            }
+        }
+        jq++;
+    }
+}
+
+// Tuning flag: search every ShockHash2 leaf with the general path only (tests compare both paths).
+constexpr uint32_t SH2_FLAG_GENERAL_ONLY = 1;
+
+// The ShockHash2 leaf code, or SHGPU_NOT_FOUND when the general path's bounds are exceeded.
+__device__ __forceinline__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32_t m, uint8_t *scratch, Sh2Lds &L,
+                                                    uint32_t flags) {
+    if (!(flags & SH2_FLAG_GENERAL_ONLY)) {
+        const uint64_t r = sh2_leaf_search_regs(keys, m, scratch, L);
+        if (r != SH2_OVERFLOW) return r;
+    }
+    return sh2_leaf_search_general(keys, m, scratch, L);
 }
 
 }  // namespace shgpu

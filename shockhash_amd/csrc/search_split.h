@@ -27,8 +27,8 @@
 // [-unit, 3 * unit] and 3 * unit < 256 (the low field forces d0 = 0, then d1 = 0, then d2 = 0, and the
 // counts sum to m).
 //
-// Mapping: lanes over seeds (S seeds per lane, window 64*S), keys staged 64 at a time in one VGPR
-// pair per lane and broadcast with v_readlane; counters live in registers.
+// Mapping: lanes over seeds (S seeds per lane, window 64*S), keys staged in LDS and read back as
+// wave-wide broadcasts; counters live in registers.
 #pragma once
 #include "device_common.h"
 
@@ -75,8 +75,8 @@ __device__ __forceinline__ uint32_t remix_top16(uint64_t z) {
     z ^= z >> 27;
     const uint32_t lo = (uint32_t)z, hi = (uint32_t)(z >> 32);
     // high word of lo64(z * C) = mulhi(lo, C_lo) + lo * C_hi + hi * C_lo (mod 2^32); mod 2^16 the
-    // cross terms need only the low 16 bits of their operands
-    return __umulhi(lo, 0x133111ebu) + __umul24(lo, 0x94d049bbu) + __umul24(hi, 0x133111ebu);
+    // cross terms need only the low 16 bits of their operands, so two 24-bit multiply-adds give them
+    return mad24(hi, 0x133111ebu, mad24(lo, 0x94d049bbu, __umulhi(lo, 0x133111ebu)));
 }
 
 // Value before the final scaling: 48-bit (sux) or 16-bit (SIMD) fraction of the hash.
@@ -109,10 +109,12 @@ __device__ __forceinline__ void split_eval(uint64_t key, const uint64_t (&seed)[
 }
 
 // One window: seeds start + woff + [0, 64*S). Returns the offset (seed - start) of the first seed whose
-// parts have the target sizes, or SHGPU_NOT_FOUND. Seeds are tried in (slot, lane) order.
+// parts have the target sizes, or SHGPU_NOT_FOUND. Seeds are tried in (slot, lane) order. The keys are
+// staged STAGE_KEYS at a time in the wave's LDS area `stage` and read back as broadcasts, two per LDS
+// read.
 template <int S, int NT, int SIMD, int MODE>
 __device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
-                                                   uint64_t woff) {
+                                                   uint64_t woff, uint64_t *stage) {
     const uint32_t lane = lane_id();
     const uint32_t m = sh.m;
     uint64_t seed[S];
@@ -123,15 +125,22 @@ __device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const S
 #pragma unroll
         for (int p = 0; p < NT; p++) c[s][p] = 0;
     }
-    for (uint32_t chunk = 0; chunk < m; chunk += 64) {
-        const uint64_t kv = chunk + lane < m ? keys[chunk + lane] : 0;
-        const uint32_t cnt = m - chunk < 64 ? m - chunk : 64;
-        if (cnt == 64) {
-#pragma unroll 8
-            for (uint32_t k = 0; k < 64; k++) split_eval<S, NT, SIMD, MODE>(readlane64(kv, k), seed, sh, c);
-        } else {
-            for (uint32_t k = 0; k < cnt; k++) split_eval<S, NT, SIMD, MODE>(readlane64(kv, k), seed, sh, c);
+    const ulonglong2 *st2 = (const ulonglong2 *)stage;
+    for (uint32_t base = 0; base < m; base += STAGE_KEYS) {
+        const uint32_t cnt = m - base < STAGE_KEYS ? m - base : STAGE_KEYS;
+        wave_sync();  // the previous block's reads are done
+        for (uint32_t i = lane; i < cnt; i += 64) stage[i] = keys[base + i];
+        wave_sync();
+        uint32_t k = 0;
+        for (; k + 8 <= cnt; k += 8) {
+#pragma unroll
+            for (uint32_t q = 0; q < 4; q++) {
+                const ulonglong2 kk = st2[k / 2 + q];
+                split_eval<S, NT, SIMD, MODE>(kk.x, seed, sh, c);
+                split_eval<S, NT, SIMD, MODE>(kk.y, seed, sh, c);
+            }
         }
+        for (; k < cnt; k++) split_eval<S, NT, SIMD, MODE>(stage[k], seed, sh, c);
     }
 #pragma unroll
     for (int s = 0; s < S; s++) {
@@ -147,29 +156,31 @@ __device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const S
     return SHGPU_NOT_FOUND;
 }
 
-template <int S>
+// HASH: SHGPU_SPLIT_SUX or SHGPU_SPLIT_SIMD when the flavour is known at compile time (only that
+// flavour's loops are generated), -1 to follow sh.simd.
+template <int S, int HASH = -1>
 __device__ __forceinline__ uint64_t split_window(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
-                                                 uint64_t woff) {
-    if (sh.simd) {
-        if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 1, SPLIT_POW2_F2>(keys, sh, start, woff);
-        if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 1, SPLIT_POW2_F4>(keys, sh, start, woff);
-        if (sh.nt == 1) return split_window_t<S, 1, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff);
-        if (sh.nt == 2) return split_window_t<S, 2, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff);
-        return split_window_t<S, 3, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff);
+                                                 uint64_t woff, uint64_t *stage) {
+    if (HASH == SHGPU_SPLIT_SIMD || (HASH < 0 && sh.simd)) {
+        if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 1, SPLIT_POW2_F2>(keys, sh, start, woff, stage);
+        if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 1, SPLIT_POW2_F4>(keys, sh, start, woff, stage);
+        if (sh.nt == 1) return split_window_t<S, 1, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
+        if (sh.nt == 2) return split_window_t<S, 2, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
+        return split_window_t<S, 3, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
     }
-    if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 0, SPLIT_POW2_F2>(keys, sh, start, woff);
-    if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 0, SPLIT_POW2_F4>(keys, sh, start, woff);
-    if (sh.nt == 1) return split_window_t<S, 1, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff);
-    if (sh.nt == 2) return split_window_t<S, 2, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff);
-    return split_window_t<S, 3, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff);
+    if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 0, SPLIT_POW2_F2>(keys, sh, start, woff, stage);
+    if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 0, SPLIT_POW2_F4>(keys, sh, start, woff, stage);
+    if (sh.nt == 1) return split_window_t<S, 1, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
+    if (sh.nt == 2) return split_window_t<S, 2, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
+    return split_window_t<S, 3, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
 }
 
 // Sequential search by one wave: the absolute seed, or SHGPU_NOT_FOUND below start + max_trials.
 template <int S>
 __device__ __forceinline__ uint64_t split_search(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
-                                                 uint64_t max_trials) {
+                                                 uint64_t max_trials, uint64_t *stage) {
     for (uint64_t woff = 0; woff < max_trials; woff += 64 * S) {
-        const uint64_t off = split_window<S>(keys, sh, start, woff);
+        const uint64_t off = split_window<S>(keys, sh, start, woff, stage);
         if (off != SHGPU_NOT_FOUND) return off < max_trials ? start + off : SHGPU_NOT_FOUND;
     }
     return SHGPU_NOT_FOUND;

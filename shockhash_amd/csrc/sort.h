@@ -1,0 +1,8 @@
+// Stable LSD radix sort of n (key, value) u64 pairs by all 64 key bits on `stream`. With tmp == nullptr
+// only tmp_bytes is set (the scratch size for n pairs).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+hipError_t shgpu_sort_pairs_u64(void *tmp, size_t &tmp_bytes, const uint64_t *keys_in, uint64_t *keys_out,
+                                const uint64_t *values_in, uint64_t *values_out, int n, hipStream_t stream);

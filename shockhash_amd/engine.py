@@ -44,7 +44,7 @@ class Node(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("sh1_seeds_per_lane", ctypes.c_uint32), ("split_seeds_per_lane", ctypes.c_uint32),
-                ("waves_per_cu", ctypes.c_uint32), ("max_trials", ctypes.c_uint64)]
+                ("waves_per_cu", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("max_trials", ctypes.c_uint64)]
 
 
 class BuildDesc(ctypes.Structure):

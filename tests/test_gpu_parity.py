@@ -53,16 +53,42 @@ def test_sh1_leaves_all_sizes(engine):
     assert np.array_equal(got, want)
 
 
-def test_sh2_leaves_all_sizes(engine):
-    """ShockHash2 leaves of every size 2..64 (Basic finder below 10, QuadSplit from 10)."""
-    log("sh2 all sizes")
+@pytest.mark.parametrize("flags", [0, 1])
+def test_sh2_leaves_all_sizes(engine, flags):
+    """ShockHash2 leaves of every size 2..64 (Basic finder below 10, QuadSplit from 10), on the register
+    path (flags 0) and on the scratch-memory path alone (flags 1, SHGPU_TUNE_SH2_GENERAL_ONLY)."""
+    log(f"sh2 all sizes, flags {flags}")
     sizes = np.array(list(range(2, 65)) * 2 + [64] * 32, dtype=np.uint32)
     offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint32)
     keys = workload.fingerprints(int(sizes.sum()), seed=13)[:, 1]
     want = O.sh2_leaf_batch(keys, offsets, sizes, threads=8)
-    got = engine.sh2_leaf_search(keys, offsets, sizes)
+    engine.set_tuning(flags=flags)
+    try:
+        got = engine.sh2_leaf_search(keys, offsets, sizes)
+    finally:
+        engine.set_tuning(flags=0)
     bad = np.nonzero(got != want)[0]
     assert len(bad) == 0, f"mismatch at sizes {sizes[bad][:10]} got {got[bad][:5]} want {want[bad][:5]}"
+
+
+def test_sh2_leaves_m64_many(engine):
+    """2000 leaves of 64 keys: both paths agree with each other on every leaf and with the oracle on a
+    sample (the register path hands the rare leaves that outgrow its tables to the scratch path)."""
+    log("sh2 2000 leaves m=64")
+    n = 2000
+    sizes = np.full(n, 64, dtype=np.uint32)
+    offsets = (np.arange(n) * 64).astype(np.uint32)
+    keys = workload.fingerprints(n * 64, seed=14)[:, 1]
+    fast = engine.sh2_leaf_search(keys, offsets, sizes)
+    engine.set_tuning(flags=1)
+    try:
+        general = engine.sh2_leaf_search(keys, offsets, sizes)
+    finally:
+        engine.set_tuning(flags=0)
+    assert np.array_equal(fast, general)
+    pick = np.arange(0, n, 10)
+    want = O.sh2_leaf_batch(keys, offsets[pick], sizes[pick], threads=8)
+    assert np.array_equal(fast[pick], want)
 
 
 @pytest.mark.parametrize("variant,leaf", [(0, 64), (1, 58), (0, 30), (0, 16), (1, 16), (1, 24)])
