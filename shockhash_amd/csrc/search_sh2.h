@@ -27,8 +27,8 @@
 //    and are bounded only by SH2_MAX_HALF_SEEDS / SH2_MAX_CANDIDATES.
 // Both write each candidate's cell indices (64 bytes) to scratch and queue the pairs that pass the
 // isolated filter in LDS in (j, i) order; 64 queued pairs are tested at a time, one pair per lane,
-// with a per-lane union-find (tree flags, UnionFind.h:58-67 semantics; path halving, the smaller root
-// becomes the parent) whose parent bytes are interleaved in LDS. Any exact pseudoforest predicate gives
+// with a per-lane union-find (at most one cycle per component, UnionFind.h:58-67 semantics; the smaller
+// root becomes the parent, a root's byte carries its cycle flag) whose bytes are interleaved in LDS. Any exact pseudoforest predicate gives
 // the reference's result: CuckooUnionFind, UnionFind and the textbook component count agree.
 #pragma once
 #include "device_common.h"
@@ -44,7 +44,7 @@ constexpr uint32_t SH2_CAND_BLOCKS = 2;  // register path: candidates [0, 128)
 constexpr uint64_t SH2_OVERFLOW = SHGPU_NOT_FOUND - 1;
 
 struct Sh2Lds {
-    uint32_t par[16 * 64];  // union-find parents: byte x of lane l at dword (x >> 2) * 64 + l
+    uint32_t par[16 * 64];  // union-find bytes: vertex x of lane l at byte x * 64 + l
     uint32_t queue[128];    // pending pairs j << 16 | i
     uint32_t hist[64];
     __align__(16) uint64_t keysA[64];  // register path: even keys, then
@@ -69,36 +69,37 @@ __device__ __forceinline__ void mem_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ uint32_t uf_addr(uint32_t x, uint32_t lane) { return ((x >> 2) * 64 + lane) * 4 + (x & 3); }
+// Per-lane union-find over vertices [0, 64) in LDS, one byte per vertex: byte x of lane l at x * 64 + l
+// (one shift-add per address). A byte below 0x80 is the parent of a non-root vertex; a root holds 0x80,
+// plus 0x40 once its component contains a cycle.
+constexpr uint32_t UF_ROOT = 0x80, UF_CYCLIC = 0x40;
 
-// Root of x with path halving (every visited node is re-pointed to its grandparent).
-__device__ __forceinline__ uint32_t uf_find(uint8_t *par, uint32_t x, uint32_t lane) {
-    for (;;) {
-        const uint32_t p = par[uf_addr(x, lane)];
-        if (p == x) return x;
-        const uint32_t gp = par[uf_addr(p, lane)];
-        par[uf_addr(x, lane)] = (uint8_t)gp;
-        x = gp;
+// Root of x; *root_byte receives the root's byte (its cycle flag).
+__device__ __forceinline__ uint32_t uf_find(const uint8_t *lane_par, uint32_t x, uint32_t &root_byte) {
+    uint32_t p = lane_par[x << 6];
+    while (p < UF_ROOT) {
+        x = p;
+        p = lane_par[x << 6];
     }
+    root_byte = p;
+    return x;
 }
 
 // Tests the 64 queued pairs (one per lane). Returns the lane mask of pseudoforest pairs. Each lane
-// streams its two candidates' cell bytes four keys at a time from the scratch area.
+// streams its two candidates' cell bytes four keys at a time from the scratch area. An edge inside a
+// component with a cycle, or between two components with cycles, kills the lane (UnionFind.h:58-67:
+// a pseudoforest component holds at most one cycle); otherwise the smaller root becomes the parent.
 __device__ __forceinline__ uint64_t sh2_uf_batch(const uint8_t *hashes, Sh2Lds &L, uint32_t m) {
     const uint32_t lane = lane_id();
     const uint32_t pr = L.queue[lane];
     const uint32_t j = pr >> 16, i = pr & 0xFFFF;
     const uint32_t *pj = (const uint32_t *)(hashes + (size_t)j * 64);
     const uint32_t *pi = (const uint32_t *)(hashes + (size_t)i * 64);
-    uint8_t *par = (uint8_t *)L.par;
 #pragma unroll
-    for (uint32_t q = 0; q < 16; q++) {
-        const uint32_t x = 4 * q;
-        L.par[q * 64 + lane] = x | ((x + 1) << 8) | ((x + 2) << 16) | ((x + 3) << 24);
-    }
+    for (uint32_t q = 0; q < 16; q++) L.par[q * 64 + lane] = UF_ROOT * 0x01010101u;
     wave_sync();
+    uint8_t *lp = (uint8_t *)L.par + lane;
     const uint32_t shift = m / 2;
-    uint64_t tree = ~0ull;
     bool alive = true;
     uint32_t wj = pj[0], wi = pi[0];
     const uint32_t nq = (m + 3) / 4;
@@ -110,13 +111,14 @@ __device__ __forceinline__ uint64_t sh2_uf_batch(const uint8_t *hashes, Sh2Lds &
             if (4 * q + r < m && alive) {
                 const uint32_t u = ((cj >> (8 * r)) & 0xFF) + shift;
                 const uint32_t v = (ci >> (8 * r)) & 0xFF;
-                const uint32_t ru = uf_find(par, u, lane);
-                const uint32_t rv = uf_find(par, v, lane);
+                uint32_t bu, bv;
+                const uint32_t ru = uf_find(lp, u, bu);
+                const uint32_t rv = uf_find(lp, v, bv);
+                const bool same = ru == rv, cu = (bu & UF_CYCLIC) != 0, cv = (bv & UF_CYCLIC) != 0;
                 const uint32_t root = ru < rv ? ru : rv, child = ru < rv ? rv : ru;
-                par[uf_addr(child, lane)] = (uint8_t)root;
-                const bool tu = (tree >> ru) & 1, tv = (tree >> rv) & 1;
-                if (!(tu && tv && ru != rv)) tree &= ~(1ull << root);
-                alive = tu || tv;
+                lp[child << 6] = (uint8_t)(same ? UF_ROOT | UF_CYCLIC : root);
+                lp[root << 6] = (uint8_t)(same || cu || cv ? UF_ROOT | UF_CYCLIC : UF_ROOT);
+                alive = same ? !cu : !(cu && cv);
             }
         }
         if ((q & 1) && ballot(alive) == 0) break;
@@ -344,25 +346,38 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
                 if ((ms_a | ms_b) == FULL && !add(s, s, s)) return SH2_OVERFLOW;
                 continue;
             }
-            // (a, s), a < s: code s*s + a
-            for (uint32_t b = 0; b <= sb; b++) {
-                const uint32_t a = 64 * b + lane;
-                uint64_t hit = ballot(a < s && (hsA.at(b) | ms_b) == FULL);
-                while (hit) {
-                    const uint32_t l = (uint32_t)__builtin_ctzll(hit);
-                    hit &= hit - 1;
-                    if (!add(64 * b + l, s, s * s + 64 * b + l)) return SH2_OVERFLOW;
-                }
+            // Hits of (a, s), a < s (code s*s + a) and of (s, b), b <= s (code s*s + s + b), one ballot per
+            // register block; then the candidates in code order (block, then lane, side (a, s) first).
+            uint64_t hA0 = 0, hA1 = 0, hA2 = 0, hA3 = 0, hB0 = 0, hB1 = 0, hB2 = 0, hB3 = 0;
+            hA0 = ballot(lane < s && (hsA.r0 | ms_b) == FULL);
+            hB0 = ballot(lane <= s && (ms_a | hsB.r0) == FULL);
+            if (sb >= 1) {
+                hA1 = ballot(64 + lane < s && (hsA.r1 | ms_b) == FULL);
+                hB1 = ballot(64 + lane <= s && (ms_a | hsB.r1) == FULL);
             }
-            // (s, b), b <= s: code s*s + s + b
-            for (uint32_t b = 0; b <= sb; b++) {
-                const uint32_t bb = 64 * b + lane;
-                uint64_t hit = ballot(bb <= s && (ms_a | hsB.at(b)) == FULL);
-                while (hit) {
-                    const uint32_t l = (uint32_t)__builtin_ctzll(hit);
-                    hit &= hit - 1;
-                    if (!add(s, 64 * b + l, s * s + s + 64 * b + l)) return SH2_OVERFLOW;
-                }
+            if (sb >= 2) {
+                hA2 = ballot(128 + lane < s && (hsA.r2 | ms_b) == FULL);
+                hB2 = ballot(128 + lane <= s && (ms_a | hsB.r2) == FULL);
+            }
+            if (sb >= 3) {
+                hA3 = ballot(192 + lane < s && (hsA.r3 | ms_b) == FULL);
+                hB3 = ballot(192 + lane <= s && (ms_a | hsB.r3) == FULL);
+            }
+            for (;;) {
+                uint32_t x, side;
+#define SH2_TAKE(H, BASE, SIDE) { x = BASE + (uint32_t)__builtin_ctzll(H); H &= H - 1; side = SIDE; }
+                if (hA0) SH2_TAKE(hA0, 0, 0)
+                else if (hA1) SH2_TAKE(hA1, 64, 0)
+                else if (hA2) SH2_TAKE(hA2, 128, 0)
+                else if (hA3) SH2_TAKE(hA3, 192, 0)
+                else if (hB0) SH2_TAKE(hB0, 0, 1)
+                else if (hB1) SH2_TAKE(hB1, 64, 1)
+                else if (hB2) SH2_TAKE(hB2, 128, 1)
+                else if (hB3) SH2_TAKE(hB3, 192, 1)
+                else break;
+#undef SH2_TAKE
+                const bool ok = side == 0 ? add(x, s, s * s + x) : add(s, x, s * s + s + x);
+                if (!ok) return SH2_OVERFLOW;
             }
         }
         // Queue the pairs (jq, i), i < jq, that pass the isolated-vertex filter, 64 i at a time, and test
