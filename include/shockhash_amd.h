@@ -66,6 +66,9 @@ typedef struct shgpu_tuning {
 /* shgpu_tuning.flags: search ShockHash2 leaves with the scratch-memory path only (by default leaves run on
  * the register path and fall back to it when they outgrow the registers; both give the same codes). */
 #define SHGPU_TUNE_SH2_GENERAL_ONLY 1u
+/* shgpu_tuning.flags: the persistent build takes every split node before any leaf (two-part queue)
+ * instead of one FIFO over all nodes. */
+#define SHGPU_TUNE_SPLITS_FIRST 2u
 
 /* Last error message of the calling thread. */
 const char *shgpu_last_error(void);

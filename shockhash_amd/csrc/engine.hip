@@ -222,7 +222,7 @@ struct BuildArgs {
     const shgpu_node *nodes;
     uint64_t *codes;
     uint32_t *queue;
-    uint32_t *ctrl;  // [0] queue head, [1] queue tail, [2] error flag, [3] nodes finished
+    uint32_t *ctrl;  // [0] queue head, [1] split tail, [2] error flag, [3] nodes finished, [5] leaf tail
     NodeState *state;
     uint32_t *help;  // per wave slot: node whose windows the slot's wave searches, or EMPTY_SLOT
     uint32_t total;
@@ -265,14 +265,15 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
         if (persistent) {
             __threadfence();  // children's keys visible device-wide before they are queued
             const uint32_t nc = uniform(nd.n_children);
-            if (nc > 0) {
-                // every lane computes the same child ids and stores the same slots (no divergence)
-                const uint32_t pos = wave_fetch_add(&a.ctrl[1], nc);
-                uint32_t child = nid + 1;
-                for (uint32_t c = 0; c < nc; c++) {
-                    __hip_atomic_store(&a.queue[pos + c], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    child = uniform(child + a.nodes[child].subtree);
-                }
+            // every lane computes the same child ids and stores the same slots (no divergence); split
+            // children go to the split part of the queue, leaves to the leaf part
+            uint32_t child = nid + 1;
+            for (uint32_t c = 0; c < nc; c++) {
+                const shgpu_node cn = a.nodes[child];
+                const bool to_leaf_part = (a.flags & SHGPU_TUNE_SPLITS_FIRST) && uniform(cn.kind) == SHGPU_NODE_LEAF;
+                const uint32_t pos = wave_fetch_add(&a.ctrl[to_leaf_part ? 5 : 1], 1u);
+                __hip_atomic_store(&a.queue[pos], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                child = uniform(child + cn.subtree);
             }
         }
     }
@@ -282,8 +283,9 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
 
 // Searches windows of a split node or ShockHash1 leaf until its smallest qualifying seed is known.
 // Windows are handed out in increasing order; a window above the best seed found so far is skipped.
-// The wave that completes the last outstanding window once the best seed is known finalises the
-// node, so every window below the winner has been searched: the result is the sequential minimum.
+// A node is finalised by a wave whose completion leaves no window outstanding, with a best seed that
+// lies inside the windows completed by then: every window below the winner has been searched, so the
+// result is the sequential minimum.
 template <int VARIANT, int SL, int SS>
 __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent) {
     const shgpu_node nd = a.nodes[nid];
@@ -317,9 +319,15 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
             }
         }
         const unsigned long long gd = wave_fetch_add64(&st->gd, 1ull) + 1ull;
-        if ((gd >> 32) == (gd & 0xFFFFFFFFull)) {
+        const uint64_t handed = gd >> 32;
+        if (handed == (gd & 0xFFFFFFFFull)) {
+            // Windows [0, handed) are all complete at the moment of this completion. The best seed is read
+            // afterwards, so it may already come from a later window whose predecessors are still being
+            // searched: only a seed inside the complete windows is final. If none is, the wave that
+            // completes the last outstanding window sees the counts equal again and decides then.
             const uint64_t b = read_best(st);
-            if (b != SHGPU_NOT_FOUND || exhausted) {
+            const uint64_t covered = handed * W;
+            if (b < covered || (b == SHGPU_NOT_FOUND && covered >= lim)) {
                 if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, b, persistent);
             }
         }
@@ -380,10 +388,12 @@ __device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint3
 
 constexpr uint32_t SPIN_LIMIT = 1u << 27;  // sleeps before a wait is declared stuck (well over ten seconds)
 
-// Persistent construction kernel. Every wave takes queue slots in order; while a slot is not yet
-// published (its parent is still being split) the wave helps windowed searches. When the queue is
-// exhausted, waves keep helping until every node is finished. Split nodes publish their children
-// when their partition is written.
+// Persistent construction kernel. Every wave takes queue slots in order; a finished split node
+// publishes its children. With SHGPU_TUNE_SPLITS_FIRST the queue has a split part (slots [0, n_split))
+// and a leaf part (slots [n_split, total)) and children go to the part of their kind, so all split
+// nodes are taken before any leaf; otherwise it is one FIFO. While a slot is not yet published (its
+// parent is still being split) the wave helps windowed searches. When the queue is exhausted, waves
+// keep helping until every node is finished.
 template <int VARIANT, int SL, int SS>
 __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
     __shared__ BuildLds<VARIANT> lds[WAVES_PER_BLOCK];
@@ -656,7 +666,7 @@ static int check_tuning(const shgpu_tuning *t) {
         return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(SHGPU_ERR_ARG, "waves_per_cu must be in [1, 32]");
     if (t->max_trials == 0) return fail(SHGPU_ERR_ARG, "max_trials must be positive");
-    if (t->flags & ~SHGPU_TUNE_SH2_GENERAL_ONLY) return fail(SHGPU_ERR_ARG, "unknown tuning flags");
+    if (t->flags & ~(SHGPU_TUNE_SH2_GENERAL_ONLY | SHGPU_TUNE_SPLITS_FIRST)) return fail(SHGPU_ERR_ARG, "unknown tuning flags");
     return SHGPU_OK;
 }
 
@@ -904,10 +914,22 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
                 if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * SH2_SCRATCH_BYTES))) return fail_all(rc);
                 a.scratch = B.sh2_scratch.as<uint8_t>();
             }
+            // queue: one FIFO part, or (SHGPU_TUNE_SPLITS_FIRST) split nodes in [0, n_split) and leaves in
+            // [n_split, total); roots first in each part
+            const bool splits_first = (a.flags & SHGPU_TUNE_SPLITS_FIRST) != 0;
+            uint32_t n_split = 0;
+            for (const auto &nd : nodes) n_split += !splits_first || nd.kind != SHGPU_NODE_LEAF;
+            std::vector<uint32_t> split_roots, leaf_roots;
+            for (uint32_t r : roots)
+                (splits_first && nodes[r].kind == SHGPU_NODE_LEAF ? leaf_roots : split_roots).push_back(r);
             k_fill_u32<<<256, 256, 0, s>>>(a.queue, EMPTY_SLOT, total);
             BATCH_HIP(hipGetLastError());
-            BATCH_HIP(hipMemcpyAsync(a.queue, roots.data(), roots.size() * 4, hipMemcpyHostToDevice, s));
-            const uint32_t ctrl_init[4] = {0, (uint32_t)roots.size(), 0, 0};  // head, tail, error, finished
+            if (!split_roots.empty())
+                BATCH_HIP(hipMemcpyAsync(a.queue, split_roots.data(), split_roots.size() * 4, hipMemcpyHostToDevice, s));
+            if (!leaf_roots.empty())
+                BATCH_HIP(hipMemcpyAsync(a.queue + n_split, leaf_roots.data(), leaf_roots.size() * 4, hipMemcpyHostToDevice, s));
+            // head, split tail, error, finished, unused, leaf tail
+            const uint32_t ctrl_init[6] = {0, (uint32_t)split_roots.size(), 0, 0, 0, n_split + (uint32_t)leaf_roots.size()};
             BATCH_HIP(hipMemcpyAsync(a.ctrl, ctrl_init, sizeof(ctrl_init), hipMemcpyHostToDevice, s));
             BATCH_HIP(hipEventRecord(B.ev0, s));
             if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, a, grid, s);

@@ -4,7 +4,7 @@ import pytest
 
 import oracle_lib as O
 from shockhash_amd import workload
-from shockhash_amd.engine import SPLIT_DTYPE
+from shockhash_amd.engine import SPLIT_DTYPE, DeviceArray
 
 pytestmark = pytest.mark.gpu
 
@@ -142,3 +142,28 @@ def test_build_matches_oracle(engine, variant, leaf, n, mode):
     assert np.array_equal(b.nodes["m"], ob.node_m)
     assert np.array_equal(b.codes, ob.node_code)
     log(f"  ok, timings {b.timings_ms}")
+
+
+@pytest.mark.parametrize("flags", [0, 2])
+@pytest.mark.parametrize("variant,leaf,bucket,n,reps", [("simd", 24, 1000, 45047, 100), ("sh1", 30, 1000, 40000, 50)])
+def test_cooperative_windows_repeated(engine, variant, leaf, bucket, n, reps, flags):
+    """Many waves join the windowed searches of the last open nodes (leaves of ~500-2000 seeds are searched
+    by up to a dozen waves at once). Every one of `reps` builds must return the oracle's codes: a node may be
+    finalised only with a seed inside the windows completed when the window counts meet, never with a
+    seed a later window found while an earlier one was still being searched."""
+    log(f"cooperative windows {variant} L={leaf} b={bucket} n={n} x{reps}, flags {flags}")
+    engine.set_tuning(max_trials=1 << 26, flags=flags)
+    h = workload.fingerprints(n, seed=45)
+    ob = O.OracleBuild(variant, leaf, h, bucket, threads=8)
+    d = DeviceArray.from_host(engine, h.reshape(-1))
+    bad = []
+    for rep in range(reps):
+        b = engine.build_dev(d, n, variant, leaf, bucket, copy_keys=False)
+        diff = np.nonzero(b.codes != ob.node_code)[0]
+        if len(diff):
+            i = diff[0]
+            bad.append(f"rep {rep}: {len(diff)} nodes differ, node {i} kind {b.nodes['kind'][i]} m {b.nodes['m'][i]} "
+                       f"got {b.codes[i]} want {ob.node_code[i]}")
+    d.free()
+    engine.set_tuning(flags=0)
+    assert not bad, "; ".join(bad[:5])

@@ -25,6 +25,21 @@ def same(a, b):
         np.array_equal(a.nodes, b.nodes)
 
 
+def describe(a, b, oracle=None):
+    """The first differences between two builds (and which of them the oracle agrees with)."""
+    out = []
+    if not np.array_equal(a.size_acc, b.size_acc):
+        out.append("bucket sizes differ")
+    if len(a.nodes) != len(b.nodes) or not np.array_equal(a.nodes, b.nodes):
+        out.append(f"forests differ ({len(a.nodes)} vs {len(b.nodes)} nodes)")
+    elif not np.array_equal(a.codes, b.codes):
+        for i in np.nonzero(a.codes != b.codes)[0][:6]:
+            nd = a.nodes[i]
+            o = f", oracle {oracle.node_code[i]}" if oracle is not None else ""
+            out.append(f"node {i} (kind {nd['kind']}, m {nd['m']}, unit {nd['unit']}): {a.codes[i]} vs {b.codes[i]}{o}")
+    return "; ".join(out)
+
+
 @pytest.mark.parametrize("variant,leaf,bucket,slots", [("sh2", 64, 2000, 4), ("sh2", 64, 2000, 1), ("sh2", 16, 500, 3),
                                                        ("sh1", 30, 1000, 2), ("simd", 24, 1000, 4)])
 def test_results_identical_alone_and_in_the_mix(engine, variant, leaf, bucket, slots):
@@ -46,7 +61,9 @@ def test_results_identical_alone_and_in_the_mix(engine, variant, leaf, bucket, s
     assert info["peak_reserved_bytes"] <= 4 << 30
     for h, (b, lat, q) in zip(mix, got):
         alone = engine.build(h, variant, leaf, bucket)
-        assert same(b, alone), len(h)
+        if not same(b, alone):
+            ob = O.OracleBuild(variant, leaf, h, bucket, threads=8)
+            pytest.fail(f"{len(h)} keys, in the mix vs alone: {describe(b, alone, ob)}")
         assert np.array_equal(b.keys(len(h)), alone.keys(len(h)))
         assert lat >= q >= 0
     # the smallest requests also against the oracle
