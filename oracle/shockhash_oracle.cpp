@@ -761,6 +761,44 @@ void orc_spooky128(const void *data, size_t len, uint64_t seed, uint64_t *first,
     *first = h1; *second = h0;   // sux spooky() returns {h1, h0}
 }
 
+// Fingerprints of many byte strings (string i = data[offs[i], offs[i+1])): out[2i] = first, out[2i+1] =
+// second, as the string constructors compute them (ShockHash.h:165,175; spooky(k, len, 0)).
+void orc_spooky128_batch(const uint8_t *data, const uint64_t *offs, size_t n, uint64_t seed, uint64_t *out, int threads) {
+    if (threads < 1) threads = 1;
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++)
+        th.emplace_back([=]() {
+            for (size_t i = t; i < n; i += threads) {
+                uint64_t h0 = seed, h1 = seed;
+                spooky_short(data + offs[i], offs[i + 1] - offs[i], h0, h1);
+                out[2 * i] = h1;
+                out[2 * i + 1] = h0;
+            }
+        });
+    for (auto &x : th) x.join();
+}
+
+// Whether seeds[l] satisfies the ShockHash1 leaf predicate for leaf l (every cell covered and the cuckoo
+// insertion in leaf order succeeds, ShockHash.h:411-433): ok[l] = 1 or 0.
+void orc_sh1_valid_batch(const uint64_t *keys, size_t n_leaves, uint32_t m, const uint64_t *seeds, uint8_t *ok, int threads) {
+    if (threads < 1) threads = 1;
+    const uint64_t all = (m == 64) ? ~0ull : ((1ull << m) - 1);
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++)
+        th.emplace_back([=]() {
+            for (size_t l = t; l < n_leaves; l += threads) {
+                const uint64_t *k = keys + l * m;
+                uint64_t mask = 0;
+                for (uint32_t i = 0; i < m; i++) {
+                    Cells c = sh1_cells(k[i], seeds[l], m);
+                    mask |= (1ull << c.c1) | (1ull << c.c2);
+                }
+                ok[l] = mask == all && sh1_construct(k, m, seeds[l], nullptr);
+            }
+        });
+    for (auto &x : th) x.join();
+}
+
 // Sequential edge insertion; returns the number of edges accepted before the first failure
 // (== n on success). pred: 0 CuckooUnionFind, 1 UnionFind, 2 TinyBinaryCuckooHashTable::insert.
 size_t orc_pseudoforest_prefix(int pred, const uint32_t *c1, const uint32_t *c2, size_t n, size_t n_cells) {

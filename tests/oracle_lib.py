@@ -27,6 +27,8 @@ _SIGS = {
     "orc_float_accurate_triangular": (ctypes.c_int, [U64, U64]),
     "orc_float_accurate_elegant": (ctypes.c_int, [U64, U64]),
     "orc_spooky128": (None, [P, ctypes.c_size_t, U64, P, P]),
+    "orc_spooky128_batch": (None, [P, P, ctypes.c_size_t, U64, P, ctypes.c_int]),
+    "orc_sh1_valid_batch": (None, [P, ctypes.c_size_t, U32, P, P, ctypes.c_int]),
     "orc_pseudoforest_prefix": (ctypes.c_size_t, [ctypes.c_int, P, P, ctypes.c_size_t, ctypes.c_size_t]),
     "orc_sh1_cells": (None, [U64, U64, U32, P, P]),
     "orc_sh1_leaf": (U64, [P, U32, U64, P]),
@@ -139,6 +141,28 @@ def spooky128(data: bytes, seed=0):
     buf = ctypes.create_string_buffer(data, len(data))
     lib().orc_spooky128(buf, len(data), seed, ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+def spooky128_batch(strings, seed=0, threads=8):
+    """(n, 2) fingerprints (first, second) of byte strings."""
+    data = np.frombuffer(b"".join(strings), dtype=np.uint8)
+    offs = np.zeros(len(strings) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(x) for x in strings])
+    out = np.zeros((len(strings), 2), dtype=np.uint64)
+    if len(data) == 0:
+        data = np.zeros(1, dtype=np.uint8)
+    lib().orc_spooky128_batch(ptr(data), ptr(offs), len(strings), seed, ptr(out), threads)
+    return out
+
+
+def sh1_valid_batch(keys2d, seeds, threads=8):
+    """Bool per leaf: seeds[l] satisfies the ShockHash1 leaf predicate for keys2d[l]."""
+    keys2d = u64(keys2d)
+    seeds = u64(seeds)
+    n, m = keys2d.shape
+    ok = np.zeros(n, dtype=np.uint8)
+    lib().orc_sh1_valid_batch(ptr(keys2d), n, m, ptr(seeds), ptr(ok), threads)
+    return ok.astype(bool)
 
 
 class OracleBuild:

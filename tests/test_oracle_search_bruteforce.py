@@ -165,3 +165,21 @@ def test_simd_packed_counters_equal_plain_counts(split, fanout):
         accept_packed = packed == np.uint64(found & 0xFFFFFFFF)
         accept_plain = np.all(cs[:, : fanout - 1] == split, axis=1)
         assert np.array_equal(accept_packed, accept_plain)
+
+
+def test_spooky_batch_equals_single():
+    strings = [b"abc", b"", b"x" * 31, b"y" * 32, b"z" * 191] + workload.random_strings(200, seed=9)
+    got = O.spooky128_batch(strings, threads=3)
+    want = np.array([O.spooky128(x, 0) for x in strings], dtype=np.uint64)
+    assert np.array_equal(got, want)
+
+
+def test_sh1_valid_batch():
+    """The predicate check accepts the oracle's minimal seeds and rejects every smaller seed (brute force)."""
+    keys = workload.isolated_leaves(40, 12, seed=31)
+    seeds = O.sh1_leaf_batch(keys, start=0, threads=4)
+    assert O.sh1_valid_batch(keys, seeds).all()
+    for s in range(0, 64):
+        trial = np.where(seeds > s, np.uint64(s), seeds + np.uint64(1 << 40))
+        ok = O.sh1_valid_batch(keys, trial)
+        assert not ok[seeds > s].any()
