@@ -209,8 +209,14 @@ __global__ void k_bucket_bounds(const uint64_t *__restrict__ first, uint64_t n, 
 }
 
 // Per-node search state of a build (windows of seeds shared by every wave that works on the node).
+// gd: bit 63 set once a qualifying seed is known (GD_FOUND), bits 32..62 windows handed out, bits 0..31
+// windows completed. One read-modify-write on it both completes a window and hands out the next.
+constexpr unsigned long long GD_FOUND = 1ull << 63, GD_HANDED = 1ull << 32;
+__device__ __forceinline__ uint64_t gd_handed(unsigned long long gd) { return (gd >> 32) & 0x7FFFFFFFull; }
+__device__ __forceinline__ uint64_t gd_completed(unsigned long long gd) { return gd & 0xFFFFFFFFull; }
+
 struct NodeState {
-    unsigned long long gd;    // high 32 bits: windows handed out; low 32 bits: windows completed
+    unsigned long long gd;    // GD_FOUND | windows handed out << 32 | windows completed
     unsigned long long best;  // smallest qualifying seed offset found so far (SHGPU_NOT_FOUND: none)
     uint32_t finalized;       // number of waves that claimed finalisation (the first one does it)
     uint32_t pad[3];
@@ -237,12 +243,8 @@ __device__ __forceinline__ unsigned long long wave_fetch_add64(unsigned long lon
     return uniform64(old);
 }
 
-// A relaxed load of the best seed may be served by the local XCD's L2 and lag behind: it is only a
-// hint (skip windows that cannot win). Read-modify-write atomics resolve at the coherence point, so
-// the value that decides a node's code is read with one (atomicOr 0).
-__device__ __forceinline__ uint64_t hint_best(const NodeState *st) {
-    return uniform64(__hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
+// The value that decides a node's code is read with a read-modify-write (atomicOr 0), which resolves at
+// the coherence point.
 __device__ __forceinline__ uint64_t read_best(NodeState *st) {
     return uniform64(atomicOr(&st->best, 0ull));
 }
@@ -282,10 +284,16 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
 }
 
 // Searches windows of a split node or ShockHash1 leaf until its smallest qualifying seed is known.
-// Windows are handed out in increasing order; a window above the best seed found so far is skipped.
-// A node is finalised by a wave whose completion leaves no window outstanding, with a best seed that
-// lies inside the windows completed by then: every window below the winner has been searched, so the
-// result is the sequential minimum.
+// Windows are handed out in increasing order. A wave holds one window at a time; one atomic on the
+// node's gd word completes it and, unless the search is over for this wave, hands out the next, and
+// its return value says whether a qualifying seed is already known (then the window just taken is
+// completed without being searched). A node is finalised by a wave whose completion leaves no window
+// outstanding, with a best seed that lies inside the windows completed by then: the best seed is read
+// after the count, so it may come from a later window whose predecessors are still being searched,
+// and only a seed inside the complete windows is final (otherwise the wave that completes the last
+// outstanding window sees the counts equal again and decides then). Every window below the winner has
+// been searched, so the result is the sequential minimum.
+// Keys are staged in the wave's LDS once per call (leaves, and split nodes of up to STAGE_KEYS keys).
 template <int VARIANT, int SL, int SS>
 __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent) {
     const shgpu_node nd = a.nodes[nid];
@@ -296,42 +304,58 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
     constexpr int SPLIT_HASH = VARIANT == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
     SplitShapeDev sh;
     if (kind != SHGPU_NODE_LEAF) sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
+    unsigned long long old = wave_fetch_add64(&st->gd, GD_HANDED);
+    bool found = (old & GD_FOUND) != 0;
+    uint64_t w = gd_handed(old);
+    // stage the keys once (after the first window is taken: a node already solved costs no loads)
+    uint64_t kv = 0;
+    bool staged = false;
+    if (!found) {
+        if (kind == SHGPU_NODE_LEAF) {
+            if constexpr (VARIANT != SHGPU_SHOCKHASH2)
+                if (m >= 2) kv = sh1_stage(a.keys + o, m, L);
+        } else if (m <= STAGE_KEYS) {
+            split_stage(a.keys + o, m, L.stage);
+            staged = true;
+        }
+    }
     for (;;) {
-        if (hint_best(st) != SHGPU_NOT_FOUND) return;
-        const uint64_t w = wave_fetch_add64(&st->gd, 1ull << 32) >> 32;
         const uint64_t woff = w * W;
-        const bool exhausted = woff >= lim;
+        const bool exhausted = woff >= lim || w >= 0x7FFFFFF0ull;
         uint64_t r = SHGPU_NOT_FOUND;
-        if (!exhausted && hint_best(st) > woff) {
+        if (!exhausted && !found) {
             if (kind == SHGPU_NODE_LEAF) {
                 if constexpr (VARIANT != SHGPU_SHOCKHASH2)  // ShockHash2 leaves never come here
-                    r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND) : sh1_window<SL>(a.keys + o, m, start, woff, L);
+                    r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND) : sh1_window_staged<SL>(kv, m, start, woff, L);
             } else if (kind == SHGPU_NODE_AGGR_LOWER)
-                r = split_window<SS, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage);
+                r = split_window<SS, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
             else
-                r = split_window<1, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage);
+                r = split_window<1, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
             if (r != SHGPU_NOT_FOUND && r >= lim) r = SHGPU_NOT_FOUND;
             if (r != SHGPU_NOT_FOUND) {
-                // same value in every lane; waiting for the returned value orders this atomic before the
-                // completion count below (both resolve at the coherence point)
+                // same value in every lane; waiting for each returned value orders the best seed before the
+                // found flag and the flag before the completion below (all resolve at the coherence point)
                 const unsigned long long prev = atomicMin(&st->best, (unsigned long long)r);
                 asm volatile("" ::"v"(prev));
+                const unsigned long long prevf = atomicOr(&st->gd, GD_FOUND);
+                asm volatile("" ::"v"(prevf));
             }
         }
-        const unsigned long long gd = wave_fetch_add64(&st->gd, 1ull) + 1ull;
-        const uint64_t handed = gd >> 32;
-        if (handed == (gd & 0xFFFFFFFFull)) {
-            // Windows [0, handed) are all complete at the moment of this completion. The best seed is read
-            // afterwards, so it may already come from a later window whose predecessors are still being
-            // searched: only a seed inside the complete windows is final. If none is, the wave that
-            // completes the last outstanding window sees the counts equal again and decides then.
-            const uint64_t b = read_best(st);
+        const bool more = r == SHGPU_NOT_FOUND && !exhausted && !found;
+        old = wave_fetch_add64(&st->gd, 1ull + (more ? GD_HANDED : 0ull));
+        const uint64_t handed = gd_handed(old);  // windows handed out before this operation
+        if (gd_completed(old) + 1 == handed) {
+            // windows [0, handed) are all complete; a seed found in one of them set GD_FOUND before its
+            // completion, so without the flag none of them holds one
+            const uint64_t b = (old & GD_FOUND) ? read_best(st) : SHGPU_NOT_FOUND;
             const uint64_t covered = handed * W;
             if (b < covered || (b == SHGPU_NOT_FOUND && covered >= lim)) {
                 if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, b, persistent);
             }
         }
-        if (r != SHGPU_NOT_FOUND || exhausted) return;
+        if (!more) return;
+        w = handed;  // the window this operation handed out
+        found = (old & GD_FOUND) != 0;
     }
 }
 
@@ -347,7 +371,7 @@ __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLd
                                                   : EMPTY_SLOT;
         // a registered node whose best seed is already known needs no help
         const bool open = v != EMPTY_SLOT &&
-                          __hip_atomic_load(&a.state[v].best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == SHGPU_NOT_FOUND;
+                          !(__hip_atomic_load(&a.state[v].gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & GD_FOUND);
         const uint64_t hit = ballot(open);
         if (hit) {
             const uint32_t nid = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));

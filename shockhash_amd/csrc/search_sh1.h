@@ -76,23 +76,28 @@ __device__ __forceinline__ void sh1_eval(uint64_t key, const uint64_t (&seed)[S]
     }
 }
 
-// One window of a leaf search: seeds start + woff + [0, 64*S), m >= 2 keys (m <= 64) at `keys`.
-// Returns the offset (seed - start) of the smallest qualifying seed in the window or SHGPU_NOT_FOUND.
+// Stages the m <= 64 keys of a leaf in L.stage (read back as broadcasts) and returns this lane's key.
+__device__ __forceinline__ uint64_t sh1_stage(const uint64_t *keys, uint32_t m, PfLds &L) {
+    const uint32_t lane = lane_id();
+    const uint64_t kv = lane < m ? keys[lane] : 0;
+    wave_sync();
+    if (lane < m) L.stage[lane] = kv;
+    wave_sync();
+    return kv;
+}
+
+// One window of a leaf search over keys staged by sh1_stage (kv = this lane's key): seeds
+// start + woff + [0, 64*S), m >= 2 keys (m <= 64). Returns the offset (seed - start) of the smallest
+// qualifying seed in the window or SHGPU_NOT_FOUND.
 template <int S>
-__device__ __forceinline__ uint64_t sh1_window(const uint64_t *keys, uint32_t m, uint64_t start, uint64_t woff,
-                                               PfLds &L) {
+__device__ __forceinline__ uint64_t sh1_window_staged(uint64_t kv, uint32_t m, uint64_t start, uint64_t woff, PfLds &L) {
     const uint32_t lane = lane_id();
     const uint32_t halfA = m / 2, halfB = (m + 1) / 2;
     const uint32_t fullA = mask_bits(halfA), fullB = mask_bits(halfB);
-    const uint64_t kv = lane < m ? keys[lane] : 0;
     uint64_t seed[S];
     uint32_t mA[S], mB[S];
 #pragma unroll
     for (int s = 0; s < S; s++) { seed[s] = start + woff + (uint64_t)(s * 64 + lane); mA[s] = 0; mB[s] = 0; }
-    // keys staged in LDS, read back as broadcasts (two per read)
-    wave_sync();
-    if (lane < m) L.stage[lane] = kv;
-    wave_sync();
     const ulonglong2 *st2 = (const ulonglong2 *)L.stage;
     if (m == 64) {
 #pragma unroll
@@ -138,8 +143,9 @@ template <int S>
 __device__ __forceinline__ uint64_t sh1_leaf_search(const uint64_t *keys, uint32_t m, uint64_t start,
                                                     uint64_t max_trials, PfLds &L) {
     if (m < 2) return start;
+    const uint64_t kv = sh1_stage(keys, m, L);
     for (uint64_t woff = 0; woff < max_trials; woff += 64 * S) {
-        const uint64_t off = sh1_window<S>(keys, m, start, woff, L);
+        const uint64_t off = sh1_window_staged<S>(kv, m, start, woff, L);
         if (off != SHGPU_NOT_FOUND) return off < max_trials ? start + off : SHGPU_NOT_FOUND;
     }
     return SHGPU_NOT_FOUND;

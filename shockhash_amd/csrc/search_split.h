@@ -108,13 +108,20 @@ __device__ __forceinline__ void split_eval(uint64_t key, const uint64_t (&seed)[
     }
 }
 
+// Stages the keys of a node of m <= STAGE_KEYS keys in `stage` for split windows called with staged = true.
+__device__ __forceinline__ void split_stage(const uint64_t *keys, uint32_t m, uint64_t *stage) {
+    wave_sync();
+    for (uint32_t i = lane_id(); i < m; i += 64) stage[i] = keys[i];
+    wave_sync();
+}
+
 // One window: seeds start + woff + [0, 64*S). Returns the offset (seed - start) of the first seed whose
 // parts have the target sizes, or SHGPU_NOT_FOUND. Seeds are tried in (slot, lane) order. The keys are
 // staged STAGE_KEYS at a time in the wave's LDS area `stage` and read back as broadcasts, two per LDS
-// read.
+// read; with staged = true (m <= STAGE_KEYS) the caller has staged all of them already.
 template <int S, int NT, int SIMD, int MODE>
 __device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
-                                                   uint64_t woff, uint64_t *stage) {
+                                                   uint64_t woff, uint64_t *stage, bool staged) {
     const uint32_t lane = lane_id();
     const uint32_t m = sh.m;
     uint64_t seed[S];
@@ -128,9 +135,11 @@ __device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const S
     const ulonglong2 *st2 = (const ulonglong2 *)stage;
     for (uint32_t base = 0; base < m; base += STAGE_KEYS) {
         const uint32_t cnt = m - base < STAGE_KEYS ? m - base : STAGE_KEYS;
-        wave_sync();  // the previous block's reads are done
-        for (uint32_t i = lane; i < cnt; i += 64) stage[i] = keys[base + i];
-        wave_sync();
+        if (!staged) {
+            wave_sync();  // the previous block's reads are done
+            for (uint32_t i = lane; i < cnt; i += 64) stage[i] = keys[base + i];
+            wave_sync();
+        }
         uint32_t k = 0;
         for (; k + 8 <= cnt; k += 8) {
 #pragma unroll
@@ -160,27 +169,29 @@ __device__ __forceinline__ uint64_t split_window_t(const uint64_t *keys, const S
 // flavour's loops are generated), -1 to follow sh.simd.
 template <int S, int HASH = -1>
 __device__ __forceinline__ uint64_t split_window(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
-                                                 uint64_t woff, uint64_t *stage) {
+                                                 uint64_t woff, uint64_t *stage, bool staged = false) {
     if (HASH == SHGPU_SPLIT_SIMD || (HASH < 0 && sh.simd)) {
-        if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 1, SPLIT_POW2_F2>(keys, sh, start, woff, stage);
-        if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 1, SPLIT_POW2_F4>(keys, sh, start, woff, stage);
-        if (sh.nt == 1) return split_window_t<S, 1, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
-        if (sh.nt == 2) return split_window_t<S, 2, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
-        return split_window_t<S, 3, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
+        if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 1, SPLIT_POW2_F2>(keys, sh, start, woff, stage, staged);
+        if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 1, SPLIT_POW2_F4>(keys, sh, start, woff, stage, staged);
+        if (sh.nt == 1) return split_window_t<S, 1, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage, staged);
+        if (sh.nt == 2) return split_window_t<S, 2, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage, staged);
+        return split_window_t<S, 3, 1, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage, staged);
     }
-    if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 0, SPLIT_POW2_F2>(keys, sh, start, woff, stage);
-    if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 0, SPLIT_POW2_F4>(keys, sh, start, woff, stage);
-    if (sh.nt == 1) return split_window_t<S, 1, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
-    if (sh.nt == 2) return split_window_t<S, 2, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
-    return split_window_t<S, 3, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage);
+    if (sh.mode == SPLIT_POW2_F2) return split_window_t<S, 1, 0, SPLIT_POW2_F2>(keys, sh, start, woff, stage, staged);
+    if (sh.mode == SPLIT_POW2_F4) return split_window_t<S, 1, 0, SPLIT_POW2_F4>(keys, sh, start, woff, stage, staged);
+    if (sh.nt == 1) return split_window_t<S, 1, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage, staged);
+    if (sh.nt == 2) return split_window_t<S, 2, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage, staged);
+    return split_window_t<S, 3, 0, SPLIT_THRESHOLDS>(keys, sh, start, woff, stage, staged);
 }
 
 // Sequential search by one wave: the absolute seed, or SHGPU_NOT_FOUND below start + max_trials.
 template <int S>
 __device__ __forceinline__ uint64_t split_search(const uint64_t *keys, const SplitShapeDev &sh, uint64_t start,
                                                  uint64_t max_trials, uint64_t *stage) {
+    const bool staged = sh.m <= STAGE_KEYS;
+    if (staged) split_stage(keys, sh.m, stage);
     for (uint64_t woff = 0; woff < max_trials; woff += 64 * S) {
-        const uint64_t off = split_window<S>(keys, sh, start, woff, stage);
+        const uint64_t off = split_window<S>(keys, sh, start, woff, stage, staged);
         if (off != SHGPU_NOT_FOUND) return off < max_trials ? start + off : SHGPU_NOT_FOUND;
     }
     return SHGPU_NOT_FOUND;
