@@ -69,6 +69,9 @@ typedef struct shgpu_tuning {
 /* shgpu_tuning.flags: the persistent build takes every split node before any leaf (two-part queue)
  * instead of one FIFO over all nodes. */
 #define SHGPU_TUNE_SPLITS_FIRST 2u
+/* shgpu_tuning.flags: after a persistent build, compare the forest planned on the device with the host's
+ * plan node by node (SHGPU_ERR_HIP on a difference); a check for tests, it copies every node back. */
+#define SHGPU_TUNE_CHECK_PLAN 4u
 
 /* Last error message of the calling thread. */
 const char *shgpu_last_error(void);

@@ -75,33 +75,46 @@ struct DevBuf {
 // Device buffers of one construction in flight (the engine's own, or one serving slot's).
 struct BuildBuffers {
     DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl, states, help,
-        sh2_scratch, staging;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        sh2_scratch, staging, plan_tab, plan_sets, plan_bucket, plan_summary;
+    // forest plan: node count tables of plan_variant / plan_leaf (host copy, then the device copy in plan_tab),
+    // node capacity of nodes / codes / queue / states
+    int plan_variant = -1;
+    uint32_t plan_leaf = 0;
+    std::vector<uint32_t> plan_counts;
+    uint64_t plan_capacity = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
     size_t bytes() const {
         const DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
-                               &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging};
+                               &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging, &plan_tab,
+                               &plan_sets, &plan_bucket, &plan_summary};
         size_t t = 0;
         for (const DevBuf *d : all) t += d->bytes;
         return t;
     }
     void release() {
         DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
-                         &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging};
+                         &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging, &plan_tab,
+                         &plan_sets, &plan_bucket, &plan_summary};
         for (DevBuf *d : all) {
             if (d->p) (void)hipFree(d->p);
             d->p = nullptr;
             d->bytes = 0;
         }
+        plan_variant = -1;
+        plan_capacity = 0;
     }
     int create_events() {
-        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
+        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess || hipEventCreate(&evA) != hipSuccess ||
+            hipEventCreate(&evB) != hipSuccess)
             return fail(SHGPU_ERR_HIP, "hipEventCreate failed");
         return SHGPU_OK;
     }
     void destroy_events() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        ev0 = ev1 = nullptr;
+        if (evA) (void)hipEventDestroy(evA);
+        if (evB) (void)hipEventDestroy(evB);
+        ev0 = ev1 = evA = evB = nullptr;
     }
 };
 
@@ -122,10 +135,20 @@ struct shgpu_engine {
 struct shgpu_build {
     uint64_t n = 0, nbuckets = 0;
     std::vector<uint64_t> size_acc;
-    std::vector<shgpu_node> nodes;
-    std::vector<uint64_t> codes;
-    std::vector<uint64_t> keys;  // filled when desc->copy_keys
-    double ms[4] = {0, 0, 0, 0};  // sort+bucketing, tree setup, seed search (host clock), search kernel (HIP events)
+    std::vector<uint64_t> codes;  // one per node, DFS pre-order per bucket, buckets in order
+    std::vector<uint64_t> keys;   // filled when desc->copy_keys
+    // sort+bucketing, forest plan, seed search and copies, search kernel; the persistent form times the first
+    // two and the kernel with HIP events and the third as the rest of the call's wall time
+    double ms[4] = {0, 0, 0, 0};
+    // The nodes follow from the bucket sizes (of the first plan_buckets buckets); a persistent build plans
+    // them on the device, and the host copy is made on first request.
+    int variant = 0;
+    uint32_t leaf = 0;
+    uint64_t max_trials = 0;
+    uint64_t plan_buckets = 0;
+    mutable std::mutex nodes_mu;
+    mutable bool nodes_ready = false;
+    mutable std::vector<shgpu_node> nodes;
 };
 
 // ===================================================================================== kernels
@@ -228,10 +251,10 @@ struct BuildArgs {
     const shgpu_node *nodes;
     uint64_t *codes;
     uint32_t *queue;
-    uint32_t *ctrl;  // [0] queue head, [1] split tail, [2] error flag, [3] nodes finished, [5] leaf tail
+    uint32_t *ctrl;  // [0] queue head, [1] split tail, [2] error flag, [3] nodes finished, [5] leaf tail, [6] nodes
     NodeState *state;
     uint32_t *help;  // per wave slot: node whose windows the slot's wave searches, or EMPTY_SLOT
-    uint32_t total;
+    uint32_t total;  // plain form: nodes (the persistent kernel reads ctrl[6])
     uint32_t nslots;
     uint8_t *scratch;
     int split_hash;
@@ -425,9 +448,10 @@ __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
     const uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
     BuildLds<VARIANT> &L = lds[w];
     uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)slot * SH2_SCRATCH_BYTES : nullptr;
+    const uint32_t total = uniform(a.ctrl[6]);  // written by the forest plan
     for (;;) {
         const uint32_t idx = grab(&a.ctrl[0]);
-        if (idx >= a.total) break;
+        if (idx >= total) break;
         uint32_t nid = EMPTY_SLOT;
         for (uint32_t spin = 0;; spin++) {
             nid = uniform(__hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
@@ -441,7 +465,7 @@ __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
         run_node<VARIANT, SL, SS>(a, nid, slot, L, ws);
     }
     for (uint32_t spin = 0;; spin++) {
-        if (uniform(__hip_atomic_load(&a.ctrl[3], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= a.total) break;
+        if (uniform(__hip_atomic_load(&a.ctrl[3], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= total) break;
         if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
         if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
         __builtin_amdgcn_s_sleep(8);
@@ -471,6 +495,247 @@ __global__ void k_init_states(NodeState *st, uint32_t n, uint32_t *help, uint32_
         st[i].finalized = 0;
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) help[i] = EMPTY_SLOT;
+}
+
+// ------------------------------------------------------------------ forest plan on the device
+// The forest of a persistent build is planned from the bucket sizes where they are computed, so the
+// construction runs from the sort to the codes without a host round trip. A bucket's tree depends only on
+// its size: the host tabulates the node count of a tree of every size once per (variant, leaf size);
+// k_plan_scan turns the bucket sizes into each tree's first node index and its root's queue slot
+// (prefix sums over buckets), k_plan_emit writes every tree's nodes in DFS pre-order (the order of
+// emit_subtree on the host, which recSplit's recursion fixes: ShockHash.h:303-555).
+
+// One key set of a batch: its bucket prefix sums are size_acc[bbase .. bbase + nbk] (nbk buckets and a
+// terminator), its keys start at key_base in the concatenated key array.
+struct PlanSet {
+    uint64_t key_base;
+    uint32_t bbase, nbk;
+};
+
+// summary words written by k_plan_scan
+constexpr uint32_t SUM_TOTAL = 0, SUM_SPLIT_NODES = 1, SUM_SPLIT_ROOTS = 2, SUM_LEAF_ROOTS = 3, SUM_OVERFLOW = 4,
+                   SUM_QSPLIT = 5, SUM_SETS = 8;  // then per set: first node, end node, bucket too large
+constexpr uint32_t PLAN_THREADS = 1024, PLAN_STACK = 64;
+
+struct PlanArgs {
+    const uint64_t *size_acc;
+    const PlanSet *sets;
+    uint32_t nsets;
+    uint32_t nacc;                   // entries of size_acc over all sets
+    const uint32_t *cnt_all;         // nodes of a tree of m keys, m <= MAX_ENCODABLE_BUCKET
+    const uint32_t *cnt_split;       // non-leaf nodes of that tree
+    uint4 *bucket;                   // per size_acc entry: first node, split roots before, leaf roots before
+    uint32_t *summary;
+    uint32_t capacity;               // node capacity of nodes / codes / queue / states
+    int splits_first;
+    uint32_t *ctrl;
+    uint32_t *help;
+    uint32_t nhelp;
+    shgpu_node *nodes;
+    NodeState *state;
+    uint32_t *queue;
+    int variant;
+    uint32_t leaf, lower, upper;
+    uint64_t max_trials;
+    uint64_t start_seed[20];
+};
+
+__device__ __forceinline__ uint32_t plan_set_of(const PlanArgs &p, uint32_t g) {
+    uint32_t lo = 0, hi = p.nsets - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (p.sets[mid].bbase <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Keys in the bucket at size_acc entry g of set r (0 for the set's terminator entry).
+__device__ __forceinline__ uint32_t plan_bucket_keys(const PlanArgs &p, uint32_t r, uint32_t g) {
+    const PlanSet s = p.sets[r];
+    if (g >= s.bbase + s.nbk) return 0;
+    return (uint32_t)(p.size_acc[g + 1] - p.size_acc[g]);
+}
+
+// Inclusive scan of four counters over the block (PLAN_THREADS threads); returns the exclusive prefix and
+// the block totals.
+__device__ __forceinline__ uint4 plan_block_scan(uint4 v, uint4 &total) {
+    __shared__ uint4 wsum[PLAN_THREADS / 64];
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    uint4 inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t x = __shfl_up(inc.x, d), y = __shfl_up(inc.y, d), z = __shfl_up(inc.z, d), q = __shfl_up(inc.w, d);
+        if ((int)lane >= d) { inc.x += x; inc.y += y; inc.z += z; inc.w += q; }
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint4 before = make_uint4(0, 0, 0, 0);
+    total = make_uint4(0, 0, 0, 0);
+    for (uint32_t k = 0; k < PLAN_THREADS / 64; k++) {
+        const uint4 s = wsum[k];
+        if (k < w) { before.x += s.x; before.y += s.y; before.z += s.z; before.w += s.w; }
+        total.x += s.x; total.y += s.y; total.z += s.z; total.w += s.w;
+    }
+    return make_uint4(before.x + inc.x - v.x, before.y + inc.y - v.y, before.z + inc.z - v.z, before.w + inc.w - v.w);
+}
+
+// One block. Per bucket: (nodes, split nodes, root is a split node, root is a leaf), zero for buckets of
+// fewer than 2 keys and for every bucket of a set holding a bucket above MAX_ENCODABLE_BUCKET (that set is
+// not searched). Writes the prefix sums per bucket, the summary, the queue control words and the help
+// registry; a plan larger than the capacity is reported and planned empty.
+__global__ void __launch_bounds__(PLAN_THREADS) k_plan_scan(PlanArgs p) {
+    const uint32_t t = threadIdx.x;
+    uint32_t *set_out = p.summary + SUM_SETS;
+    for (uint32_t r = t; r < p.nsets; r += PLAN_THREADS) set_out[3 * r] = set_out[3 * r + 1] = set_out[3 * r + 2] = 0;
+    __syncthreads();
+    const uint32_t chunk = (p.nacc + PLAN_THREADS - 1) / PLAN_THREADS;
+    const uint32_t c0 = min(p.nacc, t * chunk), c1 = min(p.nacc, c0 + chunk);
+    const uint32_t r0 = c0 < c1 ? plan_set_of(p, c0) : 0;
+    for (uint32_t g = c0, r = r0; g < c1; g++) {
+        while (g > p.sets[r].bbase + p.sets[r].nbk) r++;
+        if (plan_bucket_keys(p, r, g) > MAX_ENCODABLE_BUCKET) atomicOr(&set_out[3 * r + 2], 1u);
+    }
+    __syncthreads();
+    auto values = [&](uint32_t r, uint32_t g) {
+        const uint32_t m = plan_bucket_keys(p, r, g);
+        if (m < 2 || __hip_atomic_load(&set_out[3 * r + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return make_uint4(0, 0, 0, 0);
+        const bool leaf_root = m <= p.leaf;
+        return make_uint4(p.cnt_all[m], p.cnt_split[m], leaf_root ? 0u : 1u, leaf_root ? 1u : 0u);
+    };
+    uint4 sum = make_uint4(0, 0, 0, 0);
+    for (uint32_t g = c0, r = r0; g < c1; g++) {
+        while (g > p.sets[r].bbase + p.sets[r].nbk) r++;
+        const uint4 v = values(r, g);
+        sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
+    }
+    uint4 total;
+    uint4 run = plan_block_scan(sum, total);
+    for (uint32_t g = c0, r = r0; g < c1; g++) {
+        while (g > p.sets[r].bbase + p.sets[r].nbk) r++;
+        p.bucket[g] = make_uint4(run.x, run.z, run.w, 0);
+        if (g == p.sets[r].bbase) set_out[3 * r] = run.x;
+        if (g == p.sets[r].bbase + p.sets[r].nbk) set_out[3 * r + 1] = run.x;
+        const uint4 v = values(r, g);
+        run.x += v.x; run.y += v.y; run.z += v.z; run.w += v.w;
+    }
+    const bool overflow = total.x > p.capacity;
+    const uint32_t n_total = overflow ? 0u : total.x;
+    // queue: one FIFO part of all nodes, or a split part [0, qsplit) and a leaf part [qsplit, total)
+    const uint32_t qsplit = p.splits_first ? total.y : n_total;
+    if (t == 0) {
+        p.summary[SUM_TOTAL] = total.x;
+        p.summary[SUM_SPLIT_NODES] = total.y;
+        p.summary[SUM_SPLIT_ROOTS] = total.z;
+        p.summary[SUM_LEAF_ROOTS] = total.w;
+        p.summary[SUM_OVERFLOW] = overflow ? 1u : 0u;
+        p.summary[SUM_QSPLIT] = qsplit;
+        // head, split tail, error, finished, unused, leaf tail, nodes
+        p.ctrl[0] = 0;
+        p.ctrl[1] = p.splits_first ? total.z : total.z + total.w;
+        p.ctrl[2] = 0;
+        p.ctrl[3] = 0;
+        p.ctrl[4] = 0;
+        p.ctrl[5] = qsplit + (p.splits_first ? total.w : 0u);
+        p.ctrl[6] = n_total;
+    }
+    for (uint32_t i = t; i < p.nhelp; i += PLAN_THREADS) p.help[i] = EMPTY_SLOT;
+}
+
+// The node of m keys at `offset`, depth `level`, the way emit_subtree builds it; its children (sizes >= 2)
+// go to (child_off, child_m), and their number is returned.
+__device__ __forceinline__ uint32_t plan_node(const PlanArgs &p, uint32_t offset, uint32_t m, uint32_t level,
+                                              shgpu_node &nd, uint32_t (&child_off)[4], uint32_t (&child_m)[4]) {
+    nd.offset = offset;
+    nd.m = (uint16_t)m;
+    nd.max_trials = p.max_trials;
+    nd.subtree = p.cnt_all[m];
+    nd.unit = 0;
+    nd.n_children = 0;
+    if (m <= p.leaf) {
+        nd.kind = SHGPU_NODE_LEAF;
+        nd.start_seed = p.variant == SHGPU_SHOCKHASH2 ? 0 : p.variant == SHGPU_SIMD_SHOCKHASH ? p.start_seed[19] : p.start_seed[level];
+        return 0;
+    }
+    uint32_t unit;
+    if (m > p.upper) {
+        nd.kind = SHGPU_NODE_SPLIT2;
+        unit = (uint16_t(m / 2 + p.upper - 1) / p.upper) * p.upper;
+        nd.start_seed = p.start_seed[level];
+    } else if (m > p.lower) {
+        nd.kind = SHGPU_NODE_AGGR_UPPER;
+        unit = p.lower;
+        nd.start_seed = p.variant == SHGPU_SIMD_SHOCKHASH ? p.start_seed[17] : p.start_seed[level];
+    } else {
+        nd.kind = SHGPU_NODE_AGGR_LOWER;
+        unit = p.leaf;
+        nd.start_seed = p.variant == SHGPU_SIMD_SHOCKHASH ? p.start_seed[18] : p.start_seed[level];
+    }
+    nd.unit = unit;
+    uint32_t n = 0, i = 0;
+    if (nd.kind == SHGPU_NODE_SPLIT2) {
+        child_off[n] = offset;
+        child_m[n++] = unit;
+        i = unit;
+    } else {
+        for (; i < m - unit && n < 3; i += unit) {
+            child_off[n] = offset + i;
+            child_m[n++] = unit;
+        }
+    }
+    if (m - i > 1) {
+        child_off[n] = offset + i;
+        child_m[n++] = m - i;
+    }
+    nd.n_children = (uint8_t)n;
+    return n;
+}
+
+// One thread per bucket: the bucket's tree in DFS pre-order from its first node index, the node states,
+// and the queue slots of its node range (its root's slot gets the root, other slots EMPTY_SLOT; every
+// slot of [0, total) belongs to exactly one bucket's range, and the root slots are exactly those the
+// tails of k_plan_scan cover).
+__global__ void __launch_bounds__(256) k_plan_emit(PlanArgs p) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= p.nacc || p.summary[SUM_OVERFLOW]) return;
+    const uint32_t r = plan_set_of(p, g);
+    const uint32_t m = plan_bucket_keys(p, r, g);
+    if (m < 2 || p.summary[SUM_SETS + 3 * r + 2]) return;
+    const uint4 b = p.bucket[g];
+    const uint32_t qsplit = p.summary[SUM_QSPLIT];
+    const uint32_t split_roots = p.summary[SUM_SPLIT_ROOTS], leaf_roots = p.summary[SUM_LEAF_ROOTS];
+    const bool leaf_root = m <= p.leaf;
+    const uint32_t root_slot = p.splits_first ? (leaf_root ? qsplit + b.z : b.y) : b.y + b.z;
+    const uint32_t key_base = (uint32_t)p.sets[r].key_base;
+    uint32_t st_off[PLAN_STACK], st_m[PLAN_STACK], st_level[PLAN_STACK];
+    uint32_t sp = 0, idx = b.x;
+    st_off[sp] = key_base + (uint32_t)p.size_acc[g];
+    st_m[sp] = m;
+    st_level[sp++] = 0;
+    while (sp) {
+        --sp;
+        const uint32_t off = st_off[sp], mm = st_m[sp], level = st_level[sp];
+        shgpu_node nd;
+        uint32_t co[4], cm[4];
+        const uint32_t nc = plan_node(p, off, mm, level, nd, co, cm);
+        p.nodes[idx] = nd;
+        NodeState s;
+        s.gd = 0;
+        s.best = SHGPU_NOT_FOUND;
+        s.finalized = 0;
+        s.pad[0] = s.pad[1] = s.pad[2] = 0;
+        p.state[idx] = s;
+        const bool root_region = p.splits_first ? (idx < split_roots || (idx >= qsplit && idx < qsplit + leaf_roots))
+                                                : idx < split_roots + leaf_roots;
+        if (!root_region) p.queue[idx] = EMPTY_SLOT;
+        idx++;
+        for (uint32_t c = nc; c-- > 0;) {  // reversed, so the first child is emitted next
+            if (sp == PLAN_STACK) { atomicExch(&p.ctrl[2], 1u); return; }
+            st_off[sp] = co[c];
+            st_m[sp] = cm[c];
+            st_level[sp++] = level + 1;
+        }
+    }
+    p.queue[root_slot] = b.x;
 }
 
 // ===================================================================================== host helpers
@@ -690,7 +955,8 @@ static int check_tuning(const shgpu_tuning *t) {
         return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(SHGPU_ERR_ARG, "waves_per_cu must be in [1, 32]");
     if (t->max_trials == 0) return fail(SHGPU_ERR_ARG, "max_trials must be positive");
-    if (t->flags & ~(SHGPU_TUNE_SH2_GENERAL_ONLY | SHGPU_TUNE_SPLITS_FIRST)) return fail(SHGPU_ERR_ARG, "unknown tuning flags");
+    if (t->flags & ~(SHGPU_TUNE_SH2_GENERAL_ONLY | SHGPU_TUNE_SPLITS_FIRST | SHGPU_TUNE_CHECK_PLAN))
+        return fail(SHGPU_ERR_ARG, "unknown tuning flags");
     return SHGPU_OK;
 }
 
@@ -799,10 +1065,60 @@ struct BuildInput {
     uint64_t n;
 };
 
+// Node counts of a tree of every size m <= MAX_ENCODABLE_BUCKET (all nodes, non-leaf nodes): the shape of
+// emit_subtree, by sizes in increasing order (children are smaller than their parent).
+static int subtree_counts(const Strategy &st, std::vector<uint32_t> &all, std::vector<uint32_t> &split) {
+    all.assign(MAX_ENCODABLE_BUCKET + 1, 0);
+    split.assign(MAX_ENCODABLE_BUCKET + 1, 0);
+    all[1] = 1;  // a part of one key (leaf size 1) is a leaf node; roots and last parts need 2 keys
+    for (uint32_t m = 2; m <= MAX_ENCODABLE_BUCKET; m++) {
+        if (m <= st.leaf) {
+            all[m] = 1;
+            continue;
+        }
+        uint32_t a = 1, sp = 1, nc = 0;
+        auto child = [&](uint32_t c) {
+            a += all[c];
+            sp += split[c];
+            nc++;
+        };
+        if (m > st.upper) {
+            const uint32_t unit = (uint16_t(m / 2 + st.upper - 1) / st.upper) * st.upper;
+            child(unit);
+            if (m - unit > 1) child(m - unit);
+        } else {
+            const uint32_t unit = m > st.lower ? st.lower : st.leaf;
+            uint32_t i = 0;
+            for (; i < m - unit; i += unit) child(unit);
+            if (m - i > 1) child(m - i);
+        }
+        if (nc > 4) return fail(SHGPU_ERR_UNSUPPORTED, "tree node with more than 4 children");
+        all[m] = a;
+        split[m] = sp;
+    }
+    return SHGPU_OK;
+}
+
+// Host copy of a build's nodes (DFS pre-order per bucket, offsets within the build's keys).
+static int host_plan(const shgpu_build &b, std::vector<shgpu_node> &out) {
+    const Strategy st(b.variant, b.leaf);
+    out.clear();
+    out.reserve(b.codes.size());
+    const uint64_t nb = b.plan_buckets;
+    for (uint64_t k = 0; k < nb; k++) {
+        const uint64_t m = b.size_acc[k + 1] - b.size_acc[k];
+        if (m > 1) emit_subtree(out, st, (uint32_t)b.size_acc[k], (uint32_t)m, 0, b.max_trials);
+    }
+    if (out.size() != b.codes.size()) return fail(SHGPU_ERR_HIP, "host plan and device plan differ in node count");
+    return SHGPU_OK;
+}
+
 // The construction pipeline for a batch of independent key sets on one buffer set and stream: each
 // set is sorted and bucketed on its own, their forests share one node array, queue and persistent
 // kernel (the trees are independent, so every set's codes equal those of a build of that set alone).
 // The persistent grid uses 1/grid_div of the variant's schedule (serving slots share the device).
+// The persistent form plans the forest on the device and runs from the sort to the codes without a host
+// round trip; the plain form plans on the host and launches one kernel per node.
 // status[i] receives the outcome of set i; the return value is the first failure (SHGPU_OK if none).
 static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t grid_div, const shgpu_build_desc *d,
                      const BuildInput *in, size_t count, shgpu_build **out, int *status) {
@@ -828,6 +1144,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         NB += nbk[r] + 1;
     }
     if (N >= (1ull << 32)) return fail_all(fail(SHGPU_ERR_UNSUPPORTED, "more than 2^32 - 1 keys"));
+    if (NB >= (1ull << 31)) return fail_all(fail(SHGPU_ERR_UNSUPPORTED, "too many buckets"));
     if (hipSetDevice(e->device) != hipSuccess) return fail_all(fail(SHGPU_ERR_HIP, "hipSetDevice failed"));
 #define BATCH_HIP(call)                                                                                        \
     do {                                                                                                       \
@@ -835,11 +1152,17 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         if (err_ != hipSuccess) return fail_all(fail(SHGPU_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_))); \
     } while (0)
     auto t0 = clk::now();
+    const shgpu_tuning &tune = e->tuning_v[d->variant];
+    BATCH_HIP(hipEventRecord(B.evA, s));
     std::vector<std::unique_ptr<shgpu_build>> bs(count);
     for (size_t r = 0; r < count; r++) {
         bs[r].reset(new shgpu_build());
         bs[r]->n = in[r].n;
         bs[r]->nbuckets = nbk[r];
+        bs[r]->plan_buckets = nbk[r];
+        bs[r]->variant = d->variant;
+        bs[r]->leaf = d->leaf_size;
+        bs[r]->max_trials = tune.max_trials;
     }
 
     // 1. partition: sort by .first, emit .second in order (Sorter.hpp:15-36), bucket bounds.
@@ -865,96 +1188,100 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         k_bucket_bounds<<<1024, 256, 0, s>>>(B.firsts_out.as<uint64_t>() + o, n, nbk[r], B.size_acc.as<uint64_t>() + bbase[r]);
         BATCH_HIP(hipGetLastError());
     }
-    std::vector<uint64_t> acc_all(NB);
-    BATCH_HIP(hipMemcpyAsync(acc_all.data(), B.size_acc.p, NB * 8, hipMemcpyDeviceToHost, s));
-    BATCH_HIP(hipStreamSynchronize(s));
-    for (size_t r = 0; r < count; r++)
-        bs[r]->size_acc.assign(acc_all.begin() + bbase[r], acc_all.begin() + bbase[r] + nbk[r] + 1);
-    auto t1 = clk::now();
+    BATCH_HIP(hipEventRecord(B.evB, s));
 
-    // 2. forest of RecSplit trees, DFS pre-order per bucket, buckets in order, sets in order.
-    const Strategy st(d->variant, d->leaf_size);
-    std::vector<shgpu_node> nodes;
-    std::vector<uint32_t> roots;
-    std::vector<uint64_t> node_base(count + 1, 0);
-    nodes.reserve(N / std::max<uint32_t>(1, d->leaf_size) * 3 / 2 + 16 * count);
-    for (size_t r = 0; r < count; r++) {
-        node_base[r] = nodes.size();
-        const auto &acc = bs[r]->size_acc;
-        uint64_t limit_buckets = nbk[r];
-        if (d->mode == SHGPU_MODE_PLAIN && d->plain_bucket_limit) limit_buckets = std::min<uint64_t>(nbk[r], d->plain_bucket_limit);
-        for (uint64_t k = 0; k < limit_buckets; k++) {
-            const uint64_t m = acc[k + 1] - acc[k];
-            if (m > MAX_ENCODABLE_BUCKET) {
-                status[r] = fail(SHGPU_ERR_BUCKET_TOO_LARGE, "bucket " + std::to_string(k) + " holds " + std::to_string(m) + " keys");
-                break;
-            }
-        }
-        if (status[r] != SHGPU_OK) continue;  // this set is not searched
-        for (uint64_t k = 0; k < limit_buckets; k++) {
-            const uint64_t m = acc[k + 1] - acc[k];
-            if (m > 1) {
-                roots.push_back((uint32_t)nodes.size());
-                emit_subtree(nodes, st, (uint32_t)(base[r] + acc[k]), (uint32_t)m, 0, e->tuning_v[d->variant].max_trials);
-            }
-        }
-    }
-    node_base[count] = nodes.size();
-    const uint32_t total = (uint32_t)nodes.size();
-    std::vector<uint64_t> codes(total, 0);
-    const int grid = std::max(1, grid_for(e, e->tuning_v[d->variant]) / (int)std::max(1u, grid_div));
+    const int grid = std::max(1, grid_for(e, tune) / (int)std::max(1u, grid_div));
     const uint32_t nslots = (uint32_t)grid * WAVES_PER_BLOCK;
-    if ((rc = B.nodes.ensure((size_t)total * sizeof(shgpu_node) + 64)) || (rc = B.codes.ensure((size_t)total * 8 + 64)) ||
-        (rc = B.queue.ensure((size_t)total * 4 + 64)) || (rc = B.ctrl.ensure(64)) ||
-        (rc = B.states.ensure((size_t)total * sizeof(NodeState) + 64)) || (rc = B.help.ensure((size_t)nslots * 4 + 64)))
-        return fail_all(rc);
-    if (total) BATCH_HIP(hipMemcpyAsync(B.nodes.p, nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
-    auto t2 = clk::now();
-
-    if ((rc = check_forest(nodes, (uint32_t)roots.size()))) return fail_all(rc);
-
-    // 3. seed search.
+    std::vector<uint64_t> acc_all(NB);
+    std::vector<uint64_t> codes;
+    std::vector<uint64_t> node_begin(count, 0), node_end(count, 0);
     uint32_t ctrl_out[4] = {0, 0, 0, 0};
     bool timed_kernel = false;
     BuildArgs a;
     a.keys = B.keys.as<uint64_t>();
     a.tmp = B.tmp.as<uint64_t>();
-    a.nodes = B.nodes.as<shgpu_node>();
-    a.codes = B.codes.as<uint64_t>();
-    a.queue = B.queue.as<uint32_t>();
-    a.ctrl = B.ctrl.as<uint32_t>();
-    a.total = total;
     a.split_hash = d->variant == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
-    a.flags = e->tuning_v[d->variant].flags;
+    a.flags = tune.flags;
     a.scratch = nullptr;
-    a.state = B.states.as<NodeState>();
-    a.help = B.help.as<uint32_t>();
     a.nslots = nslots;
-    if (total) {
-        k_init_states<<<256, 256, 0, s>>>(a.state, total, a.help, nslots);
-        BATCH_HIP(hipGetLastError());
-        if (d->mode == SHGPU_MODE_PERSISTENT) {
-            if (d->variant == SHGPU_SHOCKHASH2) {
-                if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * SH2_SCRATCH_BYTES))) return fail_all(rc);
-                a.scratch = B.sh2_scratch.as<uint8_t>();
-            }
-            // queue: one FIFO part, or (SHGPU_TUNE_SPLITS_FIRST) split nodes in [0, n_split) and leaves in
-            // [n_split, total); roots first in each part
-            const bool splits_first = (a.flags & SHGPU_TUNE_SPLITS_FIRST) != 0;
-            uint32_t n_split = 0;
-            for (const auto &nd : nodes) n_split += !splits_first || nd.kind != SHGPU_NODE_LEAF;
-            std::vector<uint32_t> split_roots, leaf_roots;
-            for (uint32_t r : roots)
-                (splits_first && nodes[r].kind == SHGPU_NODE_LEAF ? leaf_roots : split_roots).push_back(r);
-            k_fill_u32<<<256, 256, 0, s>>>(a.queue, EMPTY_SLOT, total);
+    const Strategy st(d->variant, d->leaf_size);
+    auto t1 = clk::now(), t2 = t1;
+
+    if (d->mode == SHGPU_MODE_PERSISTENT) {
+        // 2. forest plan on the device (node count tables per variant and leaf size, uploaded once).
+        if (B.plan_variant != d->variant || B.plan_leaf != d->leaf_size) {
+            std::vector<uint32_t> all, split;
+            if ((rc = subtree_counts(st, all, split))) return fail_all(rc);
+            B.plan_counts = all;
+            B.plan_counts.insert(B.plan_counts.end(), split.begin(), split.end());
+            if ((rc = B.plan_tab.ensure(B.plan_counts.size() * 4))) return fail_all(rc);
+            BATCH_HIP(hipMemcpyAsync(B.plan_tab.p, B.plan_counts.data(), B.plan_counts.size() * 4, hipMemcpyHostToDevice, s));
+            B.plan_variant = d->variant;
+            B.plan_leaf = d->leaf_size;
+        }
+        // node capacity: the largest tree of a bucket up to 1.3x the set's mean bucket size, per bucket
+        uint64_t cap = 1024;
+        for (size_t r = 0; r < count; r++) {
+            const uint64_t mean = in[r].n / nbk[r] + 1;
+            const uint64_t hi = std::min<uint64_t>(MAX_ENCODABLE_BUCKET, mean + mean * 3 / 10 + 8);
+            uint32_t mx = 0;
+            for (uint64_t m = 0; m <= hi; m++) mx = std::max(mx, B.plan_counts[m]);
+            cap += nbk[r] * mx;
+        }
+        cap = std::max<uint64_t>(cap, B.plan_capacity);
+        std::vector<PlanSet> sets(count);
+        for (size_t r = 0; r < count; r++) sets[r] = PlanSet{base[r], (uint32_t)bbase[r], (uint32_t)nbk[r]};
+        if ((rc = B.plan_sets.ensure(count * sizeof(PlanSet))) || (rc = B.plan_bucket.ensure(NB * sizeof(uint4))) ||
+            (rc = B.plan_summary.ensure((SUM_SETS + 3 * count) * 4)) || (rc = B.ctrl.ensure(64)) ||
+            (rc = B.help.ensure((size_t)nslots * 4 + 64)))
+            return fail_all(rc);
+        BATCH_HIP(hipMemcpyAsync(B.plan_sets.p, sets.data(), count * sizeof(PlanSet), hipMemcpyHostToDevice, s));
+        if (d->variant == SHGPU_SHOCKHASH2) {
+            if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * SH2_SCRATCH_BYTES))) return fail_all(rc);
+            a.scratch = B.sh2_scratch.as<uint8_t>();
+        }
+        std::vector<uint32_t> summary(SUM_SETS + 3 * count);
+        for (int attempt = 0;; attempt++) {
+            if ((rc = B.nodes.ensure(cap * sizeof(shgpu_node))) || (rc = B.codes.ensure(cap * 8)) ||
+                (rc = B.queue.ensure(cap * 4)) || (rc = B.states.ensure(cap * sizeof(NodeState))))
+                return fail_all(rc);
+            B.plan_capacity = cap;
+            PlanArgs p;
+            p.size_acc = B.size_acc.as<uint64_t>();
+            p.sets = B.plan_sets.as<PlanSet>();
+            p.nsets = (uint32_t)count;
+            p.nacc = (uint32_t)NB;
+            p.cnt_all = B.plan_tab.as<uint32_t>();
+            p.cnt_split = B.plan_tab.as<uint32_t>() + MAX_ENCODABLE_BUCKET + 1;
+            p.bucket = B.plan_bucket.as<uint4>();
+            p.summary = B.plan_summary.as<uint32_t>();
+            p.capacity = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
+            p.splits_first = (tune.flags & SHGPU_TUNE_SPLITS_FIRST) != 0;
+            p.ctrl = B.ctrl.as<uint32_t>();
+            p.help = B.help.as<uint32_t>();
+            p.nhelp = nslots;
+            p.nodes = B.nodes.as<shgpu_node>();
+            p.state = B.states.as<NodeState>();
+            p.queue = B.queue.as<uint32_t>();
+            p.variant = d->variant;
+            p.leaf = st.leaf;
+            p.lower = st.lower;
+            p.upper = st.upper;
+            p.max_trials = tune.max_trials;
+            for (int i = 0; i < 20; i++) p.start_seed[i] = START_SEED[i];
+            k_plan_scan<<<1, PLAN_THREADS, 0, s>>>(p);
             BATCH_HIP(hipGetLastError());
-            if (!split_roots.empty())
-                BATCH_HIP(hipMemcpyAsync(a.queue, split_roots.data(), split_roots.size() * 4, hipMemcpyHostToDevice, s));
-            if (!leaf_roots.empty())
-                BATCH_HIP(hipMemcpyAsync(a.queue + n_split, leaf_roots.data(), leaf_roots.size() * 4, hipMemcpyHostToDevice, s));
-            // head, split tail, error, finished, unused, leaf tail
-            const uint32_t ctrl_init[6] = {0, (uint32_t)split_roots.size(), 0, 0, 0, n_split + (uint32_t)leaf_roots.size()};
-            BATCH_HIP(hipMemcpyAsync(a.ctrl, ctrl_init, sizeof(ctrl_init), hipMemcpyHostToDevice, s));
+            k_plan_emit<<<(unsigned)((NB + 255) / 256), 256, 0, s>>>(p);
+            BATCH_HIP(hipGetLastError());
+
+            // 3. seed search: the persistent kernel reads the node count from the control words.
+            a.nodes = p.nodes;
+            a.codes = B.codes.as<uint64_t>();
+            a.queue = p.queue;
+            a.ctrl = p.ctrl;
+            a.state = p.state;
+            a.help = p.help;
+            a.total = 0;
             BATCH_HIP(hipEventRecord(B.ev0, s));
             if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, a, grid, s);
             else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_build_v<SHGPU_SIMD_SHOCKHASH>(e, a, grid, s);
@@ -962,23 +1289,114 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             if (rc) return fail_all(rc);
             BATCH_HIP(hipEventRecord(B.ev1, s));
             timed_kernel = true;
-        } else {
-            if (d->variant == SHGPU_SHOCKHASH2) {
-                if ((rc = B.sh2_scratch.ensure(SH2_SCRATCH_BYTES))) return fail_all(rc);
-                a.scratch = B.sh2_scratch.as<uint8_t>();
-            }
-            const uint32_t waves = (uint32_t)e->cu_count;  // one wave per CU joins every node's windows
-            for (uint32_t nid = 0; nid < total; nid++) {
-                const bool sh2_leaf = d->variant == SHGPU_SHOCKHASH2 && nodes[nid].kind == SHGPU_NODE_LEAF;
-                const uint32_t wv = sh2_leaf ? 1u : waves;
-                if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, a, nid, wv, s);
-                else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, a, nid, wv, s);
-                else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, a, nid, wv, s);
-                if (rc) return fail_all(rc);
+            BATCH_HIP(hipMemcpyAsync(summary.data(), p.summary, summary.size() * 4, hipMemcpyDeviceToHost, s));
+            BATCH_HIP(hipMemcpyAsync(acc_all.data(), B.size_acc.p, NB * 8, hipMemcpyDeviceToHost, s));
+            BATCH_HIP(hipMemcpyAsync(ctrl_out, a.ctrl, sizeof(ctrl_out), hipMemcpyDeviceToHost, s));
+            BATCH_HIP(hipStreamSynchronize(s));
+            if (!summary[SUM_OVERFLOW]) break;
+            if (attempt > 0) return fail_all(fail(SHGPU_ERR_HIP, "forest plan: node capacity overflow after resizing"));
+            cap = (uint64_t)summary[SUM_TOTAL] + 1024;  // exact size, then run the plan and the search again
+        }
+        const uint32_t total = summary[SUM_TOTAL];
+        codes.resize(total);
+        if (total) BATCH_HIP(hipMemcpyAsync(codes.data(), a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
+        for (size_t r = 0; r < count; r++) {
+            node_begin[r] = summary[SUM_SETS + 3 * r];
+            node_end[r] = summary[SUM_SETS + 3 * r + 1];
+            if (summary[SUM_SETS + 3 * r + 2]) status[r] = fail(SHGPU_ERR_BUCKET_TOO_LARGE, "a bucket holds more than 2999 keys");
+        }
+        if (total && ctrl_out[2]) return fail_all(fail(SHGPU_ERR_HIP, "persistent build: a queue slot was never filled (inconsistent forest)"));
+        if (tune.flags & SHGPU_TUNE_CHECK_PLAN) {  // the device plan against the host plan, node by node
+            std::vector<shgpu_node> dev(total);
+            if (total) BATCH_HIP(hipMemcpyAsync(dev.data(), a.nodes, (size_t)total * sizeof(shgpu_node), hipMemcpyDeviceToHost, s));
+            BATCH_HIP(hipStreamSynchronize(s));
+            for (size_t r = 0; r < count; r++) {
+                if (status[r] != SHGPU_OK) continue;
+                shgpu_build &b = *bs[r];
+                b.size_acc.assign(acc_all.begin() + bbase[r], acc_all.begin() + bbase[r] + nbk[r] + 1);
+                b.codes.assign(node_end[r] - node_begin[r], 0);
+                std::vector<shgpu_node> host;
+                if ((rc = host_plan(b, host))) return fail_all(rc);
+                for (size_t i = 0; i < host.size(); i++) {
+                    shgpu_node dn = dev[node_begin[r] + i];
+                    dn.offset -= (uint32_t)base[r];
+                    if (std::memcmp(&dn, &host[i], sizeof(shgpu_node)) != 0)
+                        return fail_all(fail(SHGPU_ERR_HIP, "device plan differs from the host plan at node " + std::to_string(i)));
+                }
             }
         }
-        BATCH_HIP(hipMemcpyAsync(codes.data(), a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
-        BATCH_HIP(hipMemcpyAsync(ctrl_out, a.ctrl, sizeof(ctrl_out), hipMemcpyDeviceToHost, s));
+    } else {
+        // 2. forest on the host (the plain form launches one kernel per node from it).
+        BATCH_HIP(hipMemcpyAsync(acc_all.data(), B.size_acc.p, NB * 8, hipMemcpyDeviceToHost, s));
+        BATCH_HIP(hipStreamSynchronize(s));
+        t1 = clk::now();
+        std::vector<shgpu_node> nodes;
+        uint32_t roots = 0;
+        for (size_t r = 0; r < count; r++) {
+            node_begin[r] = node_end[r] = nodes.size();
+            const uint64_t *acc = acc_all.data() + bbase[r];
+            uint64_t limit_buckets = nbk[r];
+            if (d->plain_bucket_limit) limit_buckets = std::min<uint64_t>(nbk[r], d->plain_bucket_limit);
+            bs[r]->plan_buckets = limit_buckets;
+            for (uint64_t k = 0; k < nbk[r]; k++) {
+                const uint64_t m = acc[k + 1] - acc[k];
+                if (m > MAX_ENCODABLE_BUCKET) {
+                    status[r] = fail(SHGPU_ERR_BUCKET_TOO_LARGE, "bucket " + std::to_string(k) + " holds " + std::to_string(m) + " keys");
+                    break;
+                }
+            }
+            if (status[r] != SHGPU_OK) continue;  // this set is not searched
+            for (uint64_t k = 0; k < limit_buckets; k++) {
+                const uint64_t m = acc[k + 1] - acc[k];
+                if (m > 1) {
+                    roots++;
+                    emit_subtree(nodes, st, (uint32_t)(base[r] + acc[k]), (uint32_t)m, 0, tune.max_trials);
+                }
+            }
+            node_end[r] = nodes.size();
+        }
+        const uint32_t total = (uint32_t)nodes.size();
+        if ((rc = check_forest(nodes, roots))) return fail_all(rc);
+        if ((rc = B.nodes.ensure((size_t)total * sizeof(shgpu_node) + 64)) || (rc = B.codes.ensure((size_t)total * 8 + 64)) ||
+            (rc = B.ctrl.ensure(64)))
+            return fail_all(rc);
+        if (total) BATCH_HIP(hipMemcpyAsync(B.nodes.p, nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
+        t2 = clk::now();
+        a.nodes = B.nodes.as<shgpu_node>();
+        a.codes = B.codes.as<uint64_t>();
+        a.queue = nullptr;
+        a.ctrl = B.ctrl.as<uint32_t>();
+        a.state = nullptr;
+        a.help = nullptr;
+        a.total = total;
+        if (d->variant == SHGPU_SHOCKHASH2) {
+            if ((rc = B.sh2_scratch.ensure(SH2_SCRATCH_BYTES))) return fail_all(rc);
+            a.scratch = B.sh2_scratch.as<uint8_t>();
+        }
+        // 3. seed search: one launch per node, DFS order (every node's own state is its window counter).
+        if ((rc = B.states.ensure((size_t)total * sizeof(NodeState) + 64))) return fail_all(rc);
+        a.state = B.states.as<NodeState>();
+        if (total) {
+            k_init_states<<<256, 256, 0, s>>>(a.state, total, nullptr, 0);
+            BATCH_HIP(hipGetLastError());
+        }
+        const uint32_t waves = (uint32_t)e->cu_count;  // one wave per CU joins every node's windows
+        for (uint32_t nid = 0; nid < total; nid++) {
+            const bool sh2_leaf = d->variant == SHGPU_SHOCKHASH2 && nodes[nid].kind == SHGPU_NODE_LEAF;
+            const uint32_t wv = sh2_leaf ? 1u : waves;
+            if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, a, nid, wv, s);
+            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, a, nid, wv, s);
+            else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, a, nid, wv, s);
+            if (rc) return fail_all(rc);
+        }
+        codes.resize(total);
+        if (total) BATCH_HIP(hipMemcpyAsync(codes.data(), a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
+        for (size_t r = 0; r < count; r++) {
+            if (status[r] != SHGPU_OK) continue;
+            bs[r]->nodes.assign(nodes.begin() + node_begin[r], nodes.begin() + node_end[r]);
+            for (auto &nd : bs[r]->nodes) nd.offset -= (uint32_t)base[r];  // offsets within this set's keys
+            bs[r]->nodes_ready = true;
+        }
     }
     std::vector<uint64_t> keys_all;
     if (d->copy_keys && N) {
@@ -988,17 +1406,17 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     BATCH_HIP(hipStreamSynchronize(s));
 #undef BATCH_HIP
     auto t3 = clk::now();
-    if (d->mode == SHGPU_MODE_PERSISTENT && total && ctrl_out[2])
-        return fail_all(fail(SHGPU_ERR_HIP, "persistent build: a queue slot was never filled (inconsistent forest)"));
-    float kms = 0;
-    if (timed_kernel && hipEventElapsedTime(&kms, B.ev0, B.ev1) != hipSuccess) return fail_all(fail(SHGPU_ERR_HIP, "hipEventElapsedTime failed"));
+    float kms = 0, sort_ms = 0, plan_ms = 0;
+    if (timed_kernel && (hipEventElapsedTime(&kms, B.ev0, B.ev1) != hipSuccess || hipEventElapsedTime(&sort_ms, B.evA, B.evB) != hipSuccess ||
+                         hipEventElapsedTime(&plan_ms, B.evB, B.ev0) != hipSuccess))
+        return fail_all(fail(SHGPU_ERR_HIP, "hipEventElapsedTime failed"));
+    const double wall_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
     int first_rc = SHGPU_OK;
     for (size_t r = 0; r < count; r++) {
         shgpu_build &b = *bs[r];
         if (status[r] == SHGPU_OK) {
-            b.nodes.assign(nodes.begin() + node_base[r], nodes.begin() + node_base[r + 1]);
-            b.codes.assign(codes.begin() + node_base[r], codes.begin() + node_base[r + 1]);
-            for (auto &nd : b.nodes) nd.offset -= (uint32_t)base[r];  // offsets within this set's keys
+            b.size_acc.assign(acc_all.begin() + bbase[r], acc_all.begin() + bbase[r] + nbk[r] + 1);
+            b.codes.assign(codes.begin() + node_begin[r], codes.begin() + node_end[r]);
             for (size_t i = 0; i < b.codes.size(); i++)
                 if (b.codes[i] == SHGPU_NOT_FOUND) {
                     status[r] = fail(SHGPU_ERR_SEARCH_LIMIT, "node " + std::to_string(i) + " exhausted its seed bound");
@@ -1010,10 +1428,16 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             continue;
         }
         if (!keys_all.empty()) b.keys.assign(keys_all.begin() + base[r], keys_all.begin() + base[r] + b.n);
-        b.ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        b.ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
-        b.ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
-        b.ms[3] = timed_kernel ? kms : 0.0;
+        if (timed_kernel) {
+            b.ms[0] = sort_ms;
+            b.ms[1] = plan_ms;
+            b.ms[2] = std::max(0.0, wall_ms - sort_ms - plan_ms);
+            b.ms[3] = kms;
+        } else {
+            b.ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            b.ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+            b.ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+        }
         out[r] = bs[r].release();
     }
     return first_rc;
@@ -1145,7 +1569,7 @@ int shgpu_copy_to_host(shgpu_engine *e, void *h_dst, const void *d_src, size_t b
 
 void shgpu_build_free(shgpu_build *b) { delete b; }
 uint64_t shgpu_build_nbuckets(const shgpu_build *b) { return b ? b->nbuckets : 0; }
-uint64_t shgpu_build_nnodes(const shgpu_build *b) { return b ? b->nodes.size() : 0; }
+uint64_t shgpu_build_nnodes(const shgpu_build *b) { return b ? b->codes.size() : 0; }
 
 int shgpu_build_bucket_sizes(const shgpu_build *b, uint64_t *size_acc) {
     if (!b || !size_acc) return fail(SHGPU_ERR_ARG, "NULL argument");
@@ -1155,7 +1579,15 @@ int shgpu_build_bucket_sizes(const shgpu_build *b, uint64_t *size_acc) {
 
 int shgpu_build_nodes(const shgpu_build *b, shgpu_node *nodes, uint64_t *codes) {
     if (!b) return fail(SHGPU_ERR_ARG, "NULL argument");
-    if (nodes) std::memcpy(nodes, b->nodes.data(), b->nodes.size() * sizeof(shgpu_node));
+    if (nodes) {
+        std::lock_guard<std::mutex> lock(b->nodes_mu);
+        if (!b->nodes_ready) {
+            int rc = host_plan(*b, b->nodes);
+            if (rc) return rc;
+            b->nodes_ready = true;
+        }
+        std::memcpy(nodes, b->nodes.data(), b->nodes.size() * sizeof(shgpu_node));
+    }
     if (codes) std::memcpy(codes, b->codes.data(), b->codes.size() * 8);
     return SHGPU_OK;
 }

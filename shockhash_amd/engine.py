@@ -213,12 +213,21 @@ class Build:
         nn = lib.shgpu_build_nnodes(handle)
         self.size_acc = np.zeros(self.nbuckets + 1, dtype=np.uint64)
         _check(lib.shgpu_build_bucket_sizes(handle, self.size_acc.ctypes.data_as(P)))
-        self.nodes = np.zeros(nn, dtype=NODE_DTYPE)
         self.codes = np.zeros(nn, dtype=np.uint64)
-        _check(lib.shgpu_build_nodes(handle, self.nodes.ctypes.data_as(P), self.codes.ctypes.data_as(P)))
+        _check(lib.shgpu_build_nodes(handle, None, self.codes.ctypes.data_as(P)))
+        self._nodes = None
         t = np.zeros(4, dtype=np.float64)
         _check(lib.shgpu_build_timings(handle, t.ctypes.data_as(P)))
         self.timings_ms = t
+
+    @property
+    def nodes(self):
+        """The DFS-ordered nodes (a function of the bucket sizes, made on the host on first use)."""
+        if self._nodes is None:
+            nodes = np.zeros(len(self.codes), dtype=NODE_DTYPE)
+            _check(load().shgpu_build_nodes(self.h, nodes.ctypes.data_as(P), None))
+            self._nodes = nodes
+        return self._nodes
 
     def keys(self, n):
         out = np.zeros(n, dtype=np.uint64)
