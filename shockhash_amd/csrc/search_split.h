@@ -22,7 +22,8 @@
 // 46..47 of remix(z), which the final xor-shift (z ^ z >> 31) leaves untouched, so they are bits 46..47
 // of the last 64-bit product; only the low 16 bits of that product's high word are needed, which
 // 24-bit multiplies give exactly. Counting then needs no compares: F = 2 counts the top bit; F = 4
-// adds 1 << (8 * part) into packed 8-bit fields, and the packed sum equals unit * 0x01010101 (mod 2^32)
+// adds 1 << (8 * (3 - part)) into packed 8-bit fields (one v_alignbyte_b32 of a constant, which reads only
+// the low two bits of its shift operand), and the packed sum equals unit * 0x01010101 (mod 2^32)
 // exactly when every part holds unit keys, because each part's deviation from unit lies in
 // [-unit, 3 * unit] and 3 * unit < 256 (the low field forces d0 = 0, then d1 = 0, then d2 = 0, and the
 // counts sum to m).
@@ -79,6 +80,14 @@ __device__ __forceinline__ uint32_t remix_top16(uint64_t z) {
     return mad24(hi, 0x133111ebu, mad24(lo, 0x94d049bbu, __umulhi(lo, 0x133111ebu)));
 }
 
+// x >> 14 as one instruction. Written out because for the SIMD flavour the compiler otherwise distributes
+// the shift over fmix32's last xor-shift ((z >> 14) ^ (z >> 30): three instructions instead of two).
+__device__ __forceinline__ uint32_t shr14(uint32_t x) {
+    uint32_t d;
+    asm("v_lshrrev_b32 %0, 14, %1" : "=v"(d) : "v"(x));
+    return d;
+}
+
 // Value before the final scaling: 48-bit (sux) or 16-bit (SIMD) fraction of the hash.
 __device__ __forceinline__ uint64_t split_value(uint64_t z, int simd) {
     if (simd) return remix32((uint32_t)(z >> 32) ^ (uint32_t)z) & 0xFFFFu;
@@ -95,7 +104,7 @@ __device__ __forceinline__ void split_eval(uint64_t key, const uint64_t (&seed)[
             // the top bits of the w-bit value sit at bits 14..15 of `top`
             const uint32_t top = SIMD ? remix32((uint32_t)(z >> 32) ^ (uint32_t)z) : remix_top16(z);
             if (MODE == SPLIT_POW2_F2) c[s][0] += (top >> 15) & 1u;
-            else c[s][0] += 1u << ((top >> 11) & 0x18u);
+            else c[s][0] += __builtin_amdgcn_alignbyte(0u, 0x01000000u, shr14(top));
         } else if (SIMD) {
             const uint32_t v = remix32((uint32_t)(z >> 32) ^ (uint32_t)z) & 0xFFFFu;
 #pragma unroll

@@ -27,8 +27,8 @@ INT32_PEAK_TOPS = MI355X_CUS * SIMDS_PER_CU * LANES_PER_SIMD * CLOCK_GHZ / 1e3  
 KIND_LEAF = 3
 # VALU instructions per evaluation, per split counting mode and hash flavour
 SPLIT_OPS = {
-    "sux": {"F2": 15.5, "F4": 17.5, "NT1": 20.0, "NT2": 22.0, "NT3": 24.0},
-    "simd": {"F2": 10.5, "F4": 11.0, "NT1": 11.5, "NT2": 13.0, "NT3": 15.0},
+    "sux": {"F2": 15.5, "F4": 16.5, "NT1": 20.0, "NT2": 22.0, "NT3": 24.0},
+    "simd": {"F2": 10.5, "F4": 10.5, "NT1": 11.5, "NT2": 13.0, "NT3": 15.0},
 }
 SH1_LEAF_OPS = 22.25  # remix (18) + two range reductions + two mask updates, plus loop overhead
 SH2_HALF_SEED_OPS = 21.75  # remix (18) + fastrange64 (2) + mask update, plus loop overhead
