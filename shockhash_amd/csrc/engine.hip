@@ -75,7 +75,7 @@ struct DevBuf {
 // Device buffers of one construction in flight (the engine's own, or one serving slot's).
 struct BuildBuffers {
     DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl, states, help,
-        sh2_scratch, staging, plan_tab, plan_sets, plan_bucket, plan_summary;
+        sh2_scratch, staging, plan_tab, plan_sets, plan_bucket, plan_summary, sort_flag;
     // forest plan: node count tables of plan_variant / plan_leaf (host copy, then the device copy in plan_tab),
     // node capacity of nodes / codes / queue / states
     int plan_variant = -1;
@@ -86,7 +86,7 @@ struct BuildBuffers {
     size_t bytes() const {
         const DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
                                &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging, &plan_tab,
-                               &plan_sets, &plan_bucket, &plan_summary};
+                               &plan_sets, &plan_bucket, &plan_summary, &sort_flag};
         size_t t = 0;
         for (const DevBuf *d : all) t += d->bytes;
         return t;
@@ -94,7 +94,7 @@ struct BuildBuffers {
     void release() {
         DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
                          &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging, &plan_tab,
-                         &plan_sets, &plan_bucket, &plan_summary};
+                         &plan_sets, &plan_bucket, &plan_summary, &sort_flag};
         for (DevBuf *d : all) {
             if (d->p) (void)hipFree(d->p);
             d->p = nullptr;
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict_
         const uint32_t t = grab(counter);
         if (t >= n) break;
         const shgpu_leaf_task tk = tasks[t];
-        const uint64_t code = sh2_leaf_search(keys + uniform(tk.offset), uniform(tk.m), ws, L, flags);
+        const uint64_t code = sh2_leaf_search(keys + uniform(tk.offset), uniform(tk.m), ws, L, flags, task_clock());
         out[t] = code;
     }
 }
@@ -214,18 +214,65 @@ __global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restric
         dst[i] = src[i];
 }
 
-__global__ void k_unzip_hashes(const uint64_t *__restrict__ h, uint64_t n, uint64_t *first, uint64_t *second) {
+__device__ __forceinline__ uint64_t rotr64(uint64_t x, uint32_t r) { return r ? (x >> r) | (x << (64 - r)) : x; }
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, uint32_t r) { return r ? (x << r) | (x >> (64 - r)) : x; }
+
+// Splits the fingerprints into .first rotated right by `rot` (so the top 64 - rot bits, the ones sorted on,
+// are the bottom ones) and .second.
+__global__ void k_unzip_hashes(const uint64_t *__restrict__ h, uint64_t n, uint64_t *first, uint64_t *second, uint32_t rot) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        first[i] = h[2 * i];
+        first[i] = rotr64(h[2 * i], rot);
         second[i] = h[2 * i + 1];
     }
 }
 
-// size_acc[b] = number of keys in buckets < b, from firsts sorted ascending (remap128 is monotone).
-__global__ void k_bucket_bounds(const uint64_t *__restrict__ first, uint64_t n, uint64_t nb, uint64_t *size_acc) {
+// The sort orders rotated keys r = rotr(first, rot) on their bottom 64 - rot bits (the top bits of first).
+// Keys that agree on those bits then form runs in input order, and inside a run comparing r compares the
+// remaining bits of first, so a stable insertion sort of every run on r gives the order of a stable sort of
+// first on all 64 bits. With rot chosen so that the expected run length is far below one, the runs are
+// almost all single keys; a run longer than FIXUP_MAX_RUN (crafted or repeated keys) sets *flag and the
+// caller sorts on all 64 bits instead.
+constexpr uint32_t FIXUP_MAX_RUN = 64;
+__global__ void k_sort_fixup(uint64_t *first, uint64_t *second, uint64_t n, uint32_t rot, uint32_t *flag) {
+    const uint64_t mask = (~0ull) >> rot;  // the sorted bits of r
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i + 1 < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t hi = first[i] & mask;
+        if ((first[i + 1] & mask) != hi) continue;           // no run of two or more starts here
+        if (i > 0 && (first[i - 1] & mask) == hi) continue;  // inside a run
+        uint64_t end = i + 2;
+        while (end < n && (first[end] & mask) == hi && end - i <= FIXUP_MAX_RUN) end++;
+        if (end - i > FIXUP_MAX_RUN) {
+            atomicOr(flag, 1u);
+            continue;
+        }
+        for (uint64_t j = i + 1; j < end; j++) {
+            const uint64_t f = first[j], v = second[j];
+            uint64_t k = j;
+            for (; k > i && first[k - 1] > f; k--) {
+                first[k] = first[k - 1];
+                second[k] = second[k - 1];
+            }
+            first[k] = f;
+            second[k] = v;
+        }
+    }
+}
+
+// Top key bits a build sorts on: enough that n keys rarely share them (expected run length n / 2^bits below
+// 1/64), rounded up to whole 8-bit radix digits; 64 when full_sort.
+static int sort_bits(uint64_t n, bool full_sort) {
+    if (full_sort) return 64;
+    int lg = 1;
+    while (lg < 64 && (1ull << lg) < n) lg++;
+    return std::min(64, (lg + 6 + 7) / 8 * 8);
+}
+
+// size_acc[b] = number of keys in buckets < b, from firsts (rotated right by rot) sorted ascending (remap128
+// is monotone).
+__global__ void k_bucket_bounds(const uint64_t *__restrict__ first, uint64_t n, uint64_t nb, uint64_t *size_acc, uint32_t rot) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t b = i < n ? __umul64hi(first[i], nb) : nb;
-        const uint64_t bp = i > 0 ? __umul64hi(first[i - 1], nb) : 0;
+        const uint64_t b = i < n ? __umul64hi(rotl64(first[i], rot), nb) : nb;
+        const uint64_t bp = i > 0 ? __umul64hi(rotl64(first[i - 1], rot), nb) : 0;
         const uint64_t lo = i > 0 ? bp + 1 : 0;
         for (uint64_t k = lo; k <= b && k <= nb; k++) size_acc[k] = i;
     }
@@ -272,6 +319,9 @@ __device__ __forceinline__ uint64_t read_best(NodeState *st) {
     return uniform64(atomicOr(&st->best, 0ull));
 }
 
+// Queue layout: one FIFO over all nodes, or a split part [0, qsplit) and a leaf part [qsplit, total).
+__host__ __device__ __forceinline__ bool two_part_queue(uint32_t flags) { return (flags & SHGPU_TUNE_SPLITS_FIRST) != 0; }
+
 template <int SL, int SS>
 __device__ __forceinline__ uint32_t window_seeds(uint32_t kind) {
     return kind == SHGPU_NODE_LEAF ? 64u * SL : kind == SHGPU_NODE_AGGR_LOWER ? 64u * SS : 64u;
@@ -295,7 +345,7 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
             uint32_t child = nid + 1;
             for (uint32_t c = 0; c < nc; c++) {
                 const shgpu_node cn = a.nodes[child];
-                const bool to_leaf_part = (a.flags & SHGPU_TUNE_SPLITS_FIRST) && uniform(cn.kind) == SHGPU_NODE_LEAF;
+                const bool to_leaf_part = two_part_queue(a.flags) && uniform(cn.kind) == SHGPU_NODE_LEAF;
                 const uint32_t pos = wave_fetch_add(&a.ctrl[to_leaf_part ? 5 : 1], 1u);
                 __hip_atomic_store(&a.queue[pos], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 child = uniform(child + cn.subtree);
@@ -318,7 +368,7 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
 // been searched, so the result is the sequential minimum.
 // Keys are staged in the wave's LDS once per call (leaves, and split nodes of up to STAGE_KEYS keys).
 template <int VARIANT, int SL, int SS>
-__device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent) {
+__device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent, uint64_t t_start) {
     const shgpu_node nd = a.nodes[nid];
     const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
     const uint64_t start = uniform64(nd.start_seed), lim = uniform64(nd.max_trials);
@@ -343,6 +393,7 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
         }
     }
     for (;;) {
+        age_priority(t_start);
         const uint64_t woff = w * W;
         const bool exhausted = woff >= lim || w >= 0x7FFFFFF0ull;
         uint64_t r = SHGPU_NOT_FOUND;
@@ -376,7 +427,10 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
                 if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, b, persistent);
             }
         }
-        if (!more) return;
+        if (!more) {
+            reset_priority();
+            return;
+        }
         w = handed;  // the window this operation handed out
         found = (old & GD_FOUND) != 0;
     }
@@ -399,7 +453,7 @@ __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLd
         if (hit) {
             const uint32_t nid = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));
             __threadfence();
-            work_windows<VARIANT, SL, SS>(a, nid, L, true);
+            work_windows<VARIANT, SL, SS>(a, nid, L, true, task_clock());
             return true;
         }
     }
@@ -420,16 +474,17 @@ __device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint3
                                          uint8_t *ws) {
     const shgpu_node nd = a.nodes[nid];
     const uint32_t kind = uniform(nd.kind);
+    const uint64_t t_start = task_clock();
     if (VARIANT == SHGPU_SHOCKHASH2 && kind == SHGPU_NODE_LEAF) {
         if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            const uint64_t code = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.flags);
+            const uint64_t code = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.flags, t_start);
             a.codes[nid] = code;
             wave_fetch_add(&a.ctrl[3], 1u);
         }
         return;
     }
     __hip_atomic_store(&a.help[slot], nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    work_windows<VARIANT, SL, SS>(a, nid, L.pf, true);
+    work_windows<VARIANT, SL, SS>(a, nid, L.pf, true, t_start);
     __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -481,11 +536,11 @@ __global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
     if (VARIANT == SHGPU_SHOCKHASH2 && uniform(nd.kind) == SHGPU_NODE_LEAF) {
         if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
             if (blockIdx.x == 0)
-                a.codes[nid] = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2, a.flags);
+                a.codes[nid] = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2, a.flags, task_clock());
         }
         return;
     }
-    work_windows<VARIANT, SL, SS>(a, nid, L.pf, false);
+    work_windows<VARIANT, SL, SS>(a, nid, L.pf, false, task_clock());
 }
 
 __global__ void k_init_states(NodeState *st, uint32_t n, uint32_t *help, uint32_t nslots) {
@@ -527,7 +582,7 @@ struct PlanArgs {
     uint4 *bucket;                   // per size_acc entry: first node, split roots before, leaf roots before
     uint32_t *summary;
     uint32_t capacity;               // node capacity of nodes / codes / queue / states
-    int splits_first;
+    int two_part;                    // split part and leaf part (SHGPU_TUNE_SPLITS_FIRST)
     uint32_t *ctrl;
     uint32_t *help;
     uint32_t nhelp;
@@ -621,7 +676,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_scan(PlanArgs p) {
     const bool overflow = total.x > p.capacity;
     const uint32_t n_total = overflow ? 0u : total.x;
     // queue: one FIFO part of all nodes, or a split part [0, qsplit) and a leaf part [qsplit, total)
-    const uint32_t qsplit = p.splits_first ? total.y : n_total;
+    const uint32_t qsplit = p.two_part ? total.y : n_total;
     if (t == 0) {
         p.summary[SUM_TOTAL] = total.x;
         p.summary[SUM_SPLIT_NODES] = total.y;
@@ -631,11 +686,11 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_scan(PlanArgs p) {
         p.summary[SUM_QSPLIT] = qsplit;
         // head, split tail, error, finished, unused, leaf tail, nodes
         p.ctrl[0] = 0;
-        p.ctrl[1] = p.splits_first ? total.z : total.z + total.w;
+        p.ctrl[1] = p.two_part ? total.z : total.z + total.w;
         p.ctrl[2] = 0;
         p.ctrl[3] = 0;
         p.ctrl[4] = 0;
-        p.ctrl[5] = qsplit + (p.splits_first ? total.w : 0u);
+        p.ctrl[5] = qsplit + (p.two_part ? total.w : 0u);
         p.ctrl[6] = n_total;
     }
     for (uint32_t i = t; i < p.nhelp; i += PLAN_THREADS) p.help[i] = EMPTY_SLOT;
@@ -704,7 +759,7 @@ __global__ void __launch_bounds__(256) k_plan_emit(PlanArgs p) {
     const uint32_t qsplit = p.summary[SUM_QSPLIT];
     const uint32_t split_roots = p.summary[SUM_SPLIT_ROOTS], leaf_roots = p.summary[SUM_LEAF_ROOTS];
     const bool leaf_root = m <= p.leaf;
-    const uint32_t root_slot = p.splits_first ? (leaf_root ? qsplit + b.z : b.y) : b.y + b.z;
+    const uint32_t root_slot = p.two_part ? (leaf_root ? qsplit + b.z : b.y) : b.y + b.z;
     const uint32_t key_base = (uint32_t)p.sets[r].key_base;
     uint32_t st_off[PLAN_STACK], st_m[PLAN_STACK], st_level[PLAN_STACK];
     uint32_t sp = 0, idx = b.x;
@@ -724,7 +779,7 @@ __global__ void __launch_bounds__(256) k_plan_emit(PlanArgs p) {
         s.finalized = 0;
         s.pad[0] = s.pad[1] = s.pad[2] = 0;
         p.state[idx] = s;
-        const bool root_region = p.splits_first ? (idx < split_roots || (idx >= qsplit && idx < qsplit + leaf_roots))
+        const bool root_region = p.two_part ? (idx < split_roots || (idx >= qsplit && idx < qsplit + leaf_roots))
                                                 : idx < split_roots + leaf_roots;
         if (!root_region) p.queue[idx] = EMPTY_SLOT;
         idx++;
@@ -1121,7 +1176,7 @@ static int host_plan(const shgpu_build &b, std::vector<shgpu_node> &out) {
 // round trip; the plain form plans on the host and launches one kernel per node.
 // status[i] receives the outcome of set i; the return value is the first failure (SHGPU_OK if none).
 static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t grid_div, const shgpu_build_desc *d,
-                     const BuildInput *in, size_t count, shgpu_build **out, int *status) {
+                     const BuildInput *in, size_t count, shgpu_build **out, int *status, bool full_sort = false) {
     using clk = std::chrono::steady_clock;
     if (!e || !d || !out || !in || !status || count == 0) return fail(SHGPU_ERR_ARG, "NULL argument");
     int rc = SHGPU_OK;
@@ -1165,27 +1220,37 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         bs[r]->max_trials = tune.max_trials;
     }
 
-    // 1. partition: sort by .first, emit .second in order (Sorter.hpp:15-36), bucket bounds.
+    // 1. partition: sort by .first, emit .second in order (Sorter.hpp:15-36), bucket bounds. The radix sort
+    // covers the top key bits only and k_sort_fixup orders the rare runs that share them.
     if ((rc = B.firsts_in.ensure(N * 8 + 8)) || (rc = B.firsts_out.ensure(N * 8 + 8)) ||
         (rc = B.seconds_in.ensure(N * 8 + 8)) || (rc = B.keys.ensure(N * 8 + 64)) || (rc = B.tmp.ensure(N * 8 + 64)) ||
-        (rc = B.size_acc.ensure(NB * 8)))
+        (rc = B.size_acc.ensure(NB * 8)) || (rc = B.sort_flag.ensure(64)))
         return fail_all(rc);
     uint64_t nmax = 0;
     for (size_t r = 0; r < count; r++) nmax = std::max(nmax, in[r].n);
     size_t sort_bytes = 0;
     BATCH_HIP(shgpu_sort_pairs_u64(nullptr, sort_bytes, B.firsts_in.as<uint64_t>(), B.firsts_out.as<uint64_t>(),
-                                   B.seconds_in.as<uint64_t>(), B.keys.as<uint64_t>(), (int)nmax, s));
+                                   B.seconds_in.as<uint64_t>(), B.keys.as<uint64_t>(), (int)nmax, 64, s));
     if ((rc = B.sort_tmp.ensure(sort_bytes + 16))) return fail_all(rc);
+    k_fill_u32<<<1, 64, 0, s>>>(B.sort_flag.as<uint32_t>(), 0u, 1);
+    BATCH_HIP(hipGetLastError());
     for (size_t r = 0; r < count; r++) {
         const uint64_t n = in[r].n, o = base[r];
+        const int bits = sort_bits(n, full_sort);
+        const uint32_t rot = (uint32_t)(64 - bits);
         if (n) {
-            k_unzip_hashes<<<1024, 256, 0, s>>>(in[r].d_hashes, n, B.firsts_in.as<uint64_t>() + o, B.seconds_in.as<uint64_t>() + o);
+            k_unzip_hashes<<<1024, 256, 0, s>>>(in[r].d_hashes, n, B.firsts_in.as<uint64_t>() + o, B.seconds_in.as<uint64_t>() + o, rot);
             BATCH_HIP(hipGetLastError());
             size_t bytes = sort_bytes;
             BATCH_HIP(shgpu_sort_pairs_u64(B.sort_tmp.p, bytes, B.firsts_in.as<uint64_t>() + o, B.firsts_out.as<uint64_t>() + o,
-                                           B.seconds_in.as<uint64_t>() + o, B.keys.as<uint64_t>() + o, (int)n, s));
+                                           B.seconds_in.as<uint64_t>() + o, B.keys.as<uint64_t>() + o, (int)n, bits, s));
+            if (rot > 0) {
+                k_sort_fixup<<<1024, 256, 0, s>>>(B.firsts_out.as<uint64_t>() + o, B.keys.as<uint64_t>() + o, n, rot,
+                                                  B.sort_flag.as<uint32_t>());
+                BATCH_HIP(hipGetLastError());
+            }
         }
-        k_bucket_bounds<<<1024, 256, 0, s>>>(B.firsts_out.as<uint64_t>() + o, n, nbk[r], B.size_acc.as<uint64_t>() + bbase[r]);
+        k_bucket_bounds<<<1024, 256, 0, s>>>(B.firsts_out.as<uint64_t>() + o, n, nbk[r], B.size_acc.as<uint64_t>() + bbase[r], rot);
         BATCH_HIP(hipGetLastError());
     }
     BATCH_HIP(hipEventRecord(B.evB, s));
@@ -1196,6 +1261,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     std::vector<uint64_t> codes;
     std::vector<uint64_t> node_begin(count, 0), node_end(count, 0);
     uint32_t ctrl_out[4] = {0, 0, 0, 0};
+    uint32_t sort_rerun = 0;  // a run of keys sharing the sorted bits was too long: sort on all 64 bits
     bool timed_kernel = false;
     BuildArgs a;
     a.keys = B.keys.as<uint64_t>();
@@ -1256,7 +1322,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             p.bucket = B.plan_bucket.as<uint4>();
             p.summary = B.plan_summary.as<uint32_t>();
             p.capacity = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
-            p.splits_first = (tune.flags & SHGPU_TUNE_SPLITS_FIRST) != 0;
+            p.two_part = two_part_queue(tune.flags);
             p.ctrl = B.ctrl.as<uint32_t>();
             p.help = B.help.as<uint32_t>();
             p.nhelp = nslots;
@@ -1292,7 +1358,9 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             BATCH_HIP(hipMemcpyAsync(summary.data(), p.summary, summary.size() * 4, hipMemcpyDeviceToHost, s));
             BATCH_HIP(hipMemcpyAsync(acc_all.data(), B.size_acc.p, NB * 8, hipMemcpyDeviceToHost, s));
             BATCH_HIP(hipMemcpyAsync(ctrl_out, a.ctrl, sizeof(ctrl_out), hipMemcpyDeviceToHost, s));
+            BATCH_HIP(hipMemcpyAsync(&sort_rerun, B.sort_flag.p, 4, hipMemcpyDeviceToHost, s));
             BATCH_HIP(hipStreamSynchronize(s));
+            if (sort_rerun) return run_batch(e, B, s, grid_div, d, in, count, out, status, true);
             if (!summary[SUM_OVERFLOW]) break;
             if (attempt > 0) return fail_all(fail(SHGPU_ERR_HIP, "forest plan: node capacity overflow after resizing"));
             cap = (uint64_t)summary[SUM_TOTAL] + 1024;  // exact size, then run the plan and the search again
@@ -1328,7 +1396,9 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     } else {
         // 2. forest on the host (the plain form launches one kernel per node from it).
         BATCH_HIP(hipMemcpyAsync(acc_all.data(), B.size_acc.p, NB * 8, hipMemcpyDeviceToHost, s));
+        BATCH_HIP(hipMemcpyAsync(&sort_rerun, B.sort_flag.p, 4, hipMemcpyDeviceToHost, s));
         BATCH_HIP(hipStreamSynchronize(s));
+        if (sort_rerun) return run_batch(e, B, s, grid_div, d, in, count, out, status, true);
         t1 = clk::now();
         std::vector<shgpu_node> nodes;
         uint32_t roots = 0;

@@ -158,7 +158,7 @@ __device__ __forceinline__ void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L
 
 // Returns the ShockHash2 leaf code, or SHGPU_NOT_FOUND when a scratch bound is exceeded.
 __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys, uint32_t m, uint8_t *scratch,
-                                                            Sh2Lds &L) {
+                                                            Sh2Lds &L, uint64_t t_start) {
     const uint32_t lane = lane_id();
     if (m < 2) return 0;
     const Sh2Scratch g(scratch);
@@ -219,6 +219,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys
             mem_sync();
         }
         // Queue pairs (jq, i), i in [iq, iq + 64) passing the isolated-vertex filter.
+        age_priority(t_start);
         {
             const uint64_t iso_j = g.iso[jq];
             const uint32_t i = iq + lane;
@@ -294,7 +295,7 @@ __device__ __forceinline__ uint32_t sh2_half_mask(const uint64_t *stage, uint32_
 // Returns the ShockHash2 leaf code, or SH2_OVERFLOW when the leaf needs more half-seeds or candidates
 // than the registers hold.
 __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, uint32_t m, uint8_t *scratch,
-                                                         Sh2Lds &L) {
+                                                         Sh2Lds &L, uint64_t t_start) {
     const uint32_t lane = lane_id();
     if (m < 2) return 0;
     const Sh2Scratch g(scratch);
@@ -382,6 +383,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
         }
         // Queue the pairs (jq, i), i < jq, that pass the isolated-vertex filter, 64 i at a time, and test
         // whenever 64 are queued.
+        age_priority(t_start);
         const uint64_t iso_j = readlane64(isoR.at(jq >> 6), jq & 63);
         for (uint32_t b = 0; 64 * b < jq; b++) {
             const uint32_t i = 64 * b + lane;
@@ -412,14 +414,15 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
 // Tuning flag: search every ShockHash2 leaf with the general path only (tests compare both paths).
 constexpr uint32_t SH2_FLAG_GENERAL_ONLY = 1;
 
-// The ShockHash2 leaf code, or SHGPU_NOT_FOUND when the general path's bounds are exceeded.
+// The ShockHash2 leaf code, or SHGPU_NOT_FOUND when the general path's bounds are exceeded. The wave's issue
+// priority rises with the time since t_start (task_clock) and is reset on return.
 __device__ __forceinline__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32_t m, uint8_t *scratch, Sh2Lds &L,
-                                                    uint32_t flags) {
-    if (!(flags & SH2_FLAG_GENERAL_ONLY)) {
-        const uint64_t r = sh2_leaf_search_regs(keys, m, scratch, L);
-        if (r != SH2_OVERFLOW) return r;
-    }
-    return sh2_leaf_search_general(keys, m, scratch, L);
+                                                    uint32_t flags, uint64_t t_start) {
+    uint64_t r = SH2_OVERFLOW;
+    if (!(flags & SH2_FLAG_GENERAL_ONLY)) r = sh2_leaf_search_regs(keys, m, scratch, L, t_start);
+    if (r == SH2_OVERFLOW) r = sh2_leaf_search_general(keys, m, scratch, L, t_start);
+    reset_priority();
+    return r;
 }
 
 }  // namespace shgpu

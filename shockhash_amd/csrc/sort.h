@@ -1,8 +1,11 @@
-// Stable LSD radix sort of n (key, value) u64 pairs by all 64 key bits on `stream`. With tmp == nullptr
-// only tmp_bytes is set (the scratch size for n pairs).
+// Stable LSD radix sort of n (key, value) u64 pairs by the key bits [0, end_bit) on `stream`. With
+// tmp == nullptr only tmp_bytes is set (the scratch size for n pairs). (rocPRIM's sort with a nonzero
+// begin_bit does not order the keys on this ROCm release, so callers rotate the bits they sort on down to
+// the bottom.)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 hipError_t shgpu_sort_pairs_u64(void *tmp, size_t &tmp_bytes, const uint64_t *keys_in, uint64_t *keys_out,
-                                const uint64_t *values_in, uint64_t *values_out, int n, hipStream_t stream);
+                                const uint64_t *values_in, uint64_t *values_out, int n, int end_bit,
+                                hipStream_t stream);

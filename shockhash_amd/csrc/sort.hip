@@ -7,6 +7,7 @@
 #include "sort.h"
 
 hipError_t shgpu_sort_pairs_u64(void *tmp, size_t &tmp_bytes, const uint64_t *keys_in, uint64_t *keys_out,
-                                const uint64_t *values_in, uint64_t *values_out, int n, hipStream_t stream) {
-    return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, values_in, values_out, n, 0, 64, stream);
+                                const uint64_t *values_in, uint64_t *values_out, int n, int end_bit, hipStream_t stream) {
+    return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, values_in, values_out, n, 0, end_bit,
+                                              stream);
 }

@@ -56,3 +56,27 @@ def test_bucket_too_large_is_reported(engine):
     ok = workload.fingerprints(5000, seed=33)
     b = engine.build(ok, "sh2", 64, 2000)
     assert np.array_equal(b.codes, O.OracleBuild("sh2", 64, ok, 2000, threads=8).node_code)
+
+
+@pytest.mark.parametrize("groups", [5000, 500])
+def test_sort_runs_sharing_the_sorted_bits(engine, groups):
+    """The radix sort orders on the top 24 key bits at n = 100,000 and a fix-up pass orders the keys that share
+    them. Fingerprints whose .first take only `groups` distinct top-24-bit values (spread over the range) form
+    runs of ~20 keys (fix-up) or ~200 keys (longer than the fix-up handles: the build sorts on all 64 bits).
+    Bucket sizes, leaf-ordered keys and codes equal the oracle's."""
+    log(f"sort runs, {groups} groups")
+    n = 100_000
+    rng = np.random.default_rng(34 + groups)
+    top = rng.integers(0, groups, n, dtype=np.uint64) * np.uint64((1 << 24) // groups)
+    h = workload.fingerprints(n, seed=35)
+    h[:, 0] = (top << np.uint64(40)) | (h[:, 0] >> np.uint64(24))
+    assert len(np.unique(h[:, 0])) == n
+    b = engine.build(h, "sh2", 64, 2000)
+    ob = O.OracleBuild("sh2", 64, h, 2000, threads=8)
+    assert np.array_equal(b.size_acc, ob.size_acc)
+    assert np.array_equal(b.codes, ob.node_code)
+    # ShockHash2 hands the keys of every leaf to the retrieval structure in final leaf order (a part of one key
+    # is no leaf)
+    leaf = b.nodes["kind"] == 3
+    idx = np.concatenate([np.arange(o, o + m) for o, m in zip(b.nodes["offset"][leaf], b.nodes["m"][leaf])])
+    assert np.array_equal(b.keys(n)[idx], ob.ribbon_keys[: len(idx)])
