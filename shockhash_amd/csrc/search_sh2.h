@@ -71,24 +71,15 @@ __device__ __forceinline__ void mem_sync() {
 
 // Per-lane union-find over vertices [0, 64) in LDS, one byte per vertex: byte x of lane l at x * 64 + l
 // (one shift-add per address). A byte below 0x80 is the parent of a non-root vertex; a root holds 0x80,
-// plus 0x40 once its component contains a cycle.
-constexpr uint32_t UF_ROOT = 0x80, UF_CYCLIC = 0x40;
-
-// Root of x; *root_byte receives the root's byte (its cycle flag).
-__device__ __forceinline__ uint32_t uf_find(const uint8_t *lane_par, uint32_t x, uint32_t &root_byte) {
-    uint32_t p = lane_par[x << 6];
-    while (p < UF_ROOT) {
-        x = p;
-        p = lane_par[x << 6];
-    }
-    root_byte = p;
-    return x;
-}
+// plus 0x40 once its component contains a cycle, plus its rank (union by rank: trees stay at most 6 deep).
+constexpr uint32_t UF_ROOT = 0x80, UF_CYCLIC = 0x40, UF_RANK = 0x3F;
 
 // Tests the 64 queued pairs (one per lane). Returns the lane mask of pseudoforest pairs. Each lane
 // streams its two candidates' cell bytes four keys at a time from the scratch area. An edge inside a
 // component with a cycle, or between two components with cycles, kills the lane (UnionFind.h:58-67:
-// a pseudoforest component holds at most one cycle); otherwise the smaller root becomes the parent.
+// a pseudoforest component holds at most one cycle); otherwise the root of higher rank becomes the
+// parent. Both ends of an edge walk to their roots in one loop, so a key costs as many dependent LDS
+// reads as the deeper of its two walks.
 __device__ __forceinline__ uint64_t sh2_uf_batch(const uint8_t *hashes, Sh2Lds &L, uint32_t m) {
     const uint32_t lane = lane_id();
     const uint32_t pr = L.queue[lane];
@@ -109,16 +100,22 @@ __device__ __forceinline__ uint64_t sh2_uf_batch(const uint8_t *hashes, Sh2Lds &
 #pragma unroll
         for (uint32_t r = 0; r < 4; r++) {
             if (4 * q + r < m && alive) {
-                const uint32_t u = ((cj >> (8 * r)) & 0xFF) + shift;
-                const uint32_t v = (ci >> (8 * r)) & 0xFF;
-                uint32_t bu, bv;
-                const uint32_t ru = uf_find(lp, u, bu);
-                const uint32_t rv = uf_find(lp, v, bv);
-                const bool same = ru == rv, cu = (bu & UF_CYCLIC) != 0, cv = (bv & UF_CYCLIC) != 0;
-                const uint32_t root = ru < rv ? ru : rv, child = ru < rv ? rv : ru;
-                lp[child << 6] = (uint8_t)(same ? UF_ROOT | UF_CYCLIC : root);
-                lp[root << 6] = (uint8_t)(same || cu || cv ? UF_ROOT | UF_CYCLIC : UF_ROOT);
-                alive = same ? !cu : !(cu && cv);
+                uint32_t x = ((cj >> (8 * r)) & 0xFF) + shift;
+                uint32_t y = (ci >> (8 * r)) & 0xFF;
+                uint32_t bx = lp[x << 6], by = lp[y << 6];
+                while (bx < UF_ROOT || by < UF_ROOT) {
+                    if (bx < UF_ROOT) { x = bx; bx = lp[x << 6]; }
+                    if (by < UF_ROOT) { y = by; by = lp[y << 6]; }
+                }
+                const bool same = x == y, cx = (bx & UF_CYCLIC) != 0, cy = (by & UF_CYCLIC) != 0;
+                const uint32_t rx = bx & UF_RANK, ry = by & UF_RANK;
+                const bool x_up = rx >= ry;
+                const uint32_t root = x_up ? x : y, child = x_up ? y : x;
+                const uint32_t rank = rx == ry ? rx + 1 : (x_up ? rx : ry);
+                // same component: both stores write the root (x == y) with its cycle flag set
+                lp[child << 6] = (uint8_t)(same ? UF_ROOT | UF_CYCLIC | rx : root);
+                lp[root << 6] = (uint8_t)(same ? UF_ROOT | UF_CYCLIC | rx : UF_ROOT | (cx || cy ? UF_CYCLIC : 0u) | rank);
+                alive = same ? !cx : !(cx && cy);
             }
         }
         if ((q & 1) && ballot(alive) == 0) break;
