@@ -148,6 +148,19 @@ int shgpu_build_nodes(const shgpu_build *b, shgpu_node *nodes, uint64_t *codes);
 int shgpu_build_keys(const shgpu_build *b, uint64_t *keys);
 /* Host-only: the forest shgpu_build_run would search for these bucket sizes (DFS pre-order). Call with
  * out = NULL to get the count. Needs no GPU. */
+/* One-bit retrieval structure of the leaf orientations, built on the device. Replaces the construction of the
+ * reference's SimpleRibbon<1> retrieval (include/ShockHash.h:657-658, include/ShockHash2.h:491-497; the ribbon
+ * library itself is not vendored). Layout: 3 * seg one-bit cells, seg = shgpu_retrieval_cells_per_third(n); key k
+ * uses one cell in each third, from remix of k and the seed, and its bit is the XOR of its three cells
+ * (shockhash::BitRetrieval answers queries on this layout). d_keys: n keys, d_bits: n bytes whose bit 0 is the
+ * key's bit, both in device memory; keys must be distinct. Writes the seed, seg and shgpu_retrieval_words(n)
+ * packed words (bit c of the structure is bit c % 64 of word c / 64) to host memory. Tries seeds 0..63;
+ * SHGPU_ERR_SEARCH_LIMIT when none peels (duplicate keys). */
+uint64_t shgpu_retrieval_cells_per_third(uint64_t n);
+uint64_t shgpu_retrieval_words(uint64_t n);
+int shgpu_retrieval_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
+                          uint64_t *seg, uint64_t *h_words, uint64_t words_capacity);
+
 int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc, uint64_t nbuckets, shgpu_node *out,
                       uint64_t capacity, uint64_t *count);
 /* Milliseconds spent in the build's phases: [0] sort + bucketing, [1] tree setup, [2] seed search

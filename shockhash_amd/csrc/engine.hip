@@ -18,6 +18,7 @@
 #include "search_sh1.h"
 #include "search_sh2.h"
 #include "search_split.h"
+#include "retrieval.h"
 #include "sort.h"
 
 using namespace shgpu;
@@ -1524,6 +1525,17 @@ static int run_build(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
 int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *d_hashes, uint64_t n, shgpu_build **out) {
     if (!e) return fail(SHGPU_ERR_ARG, "NULL argument");
     return run_build(e, e->main, e->stream, 1, d, d_hashes, n, out);
+}
+
+uint64_t shgpu_retrieval_words(uint64_t n) { return (3 * shgpu_retrieval_cells_per_third(n) + 63) / 64; }
+
+int shgpu_retrieval_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
+                          uint64_t *seg, uint64_t *h_words, uint64_t words_capacity) {
+    if (!e || !seed || !seg || !h_words || (n && (!d_keys || !d_bits))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(SHGPU_ERR_HIP, "hipSetDevice failed");
+    std::string err;
+    const int rc = shgpu_retrieval_build_impl(e->stream, d_keys, d_bits, n, seed, seg, h_words, words_capacity, err);
+    return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
 }
 
 int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc, uint64_t nbuckets, shgpu_node *out,

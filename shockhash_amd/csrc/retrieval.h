@@ -1,0 +1,12 @@
+// Device construction of the one-bit retrieval structure (retrieval.hip), called by the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+// (shgpu_retrieval_cells_per_third, the layout's size rule, is part of the C ABI: include/shockhash_amd.h.)
+// Builds the structure for n keys and their bits (device arrays) on stream s; writes the seed, the cells per
+// third and ceil(3 * seg / 64) packed words to host memory. Returns an SHGPU status, message in err.
+int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
+                               uint64_t *seg, uint64_t *h_words, uint64_t words_capacity, std::string &err);

@@ -1,0 +1,220 @@
+// One-bit retrieval construction on the device: the structure that returns, for every key of the set,
+// the orientation bit of the key's leaf cell. It takes the place of the reference's
+// SimpleRibbon<1, ...> construction (include/ShockHash.h:152-153,657-658; include/ShockHash2.h:134,
+// 491-497); query answers depend only on the bits it returns for the set's keys.
+//
+// Layout (shared with the host class shockhash::BitRetrieval, which answers the queries): 3 * seg cells of
+// one bit, seg = floor(1.23 n / 3) + 8; key k uses one cell in each third, derived from remix of k and the
+// seed; its bit is the XOR of its three cells. Construction is hypergraph peeling: repeatedly take a cell
+// with exactly one key, remove that key; then assign bits in reverse peeling order. A peeling that does not
+// take every key (rare, or duplicate keys) retries with the next seed.
+//
+// On the device the peeling runs in rounds. Round r: every cell that reached degree 1 claims its key
+// (compare-and-swap on the key's round; a key reachable through two such cells is claimed once), then every
+// claimed key leaves its three cells (atomic degree decrement and index XOR), and cells that drop to
+// degree 1 form the next round's frontier. Claims only read degrees and index XORs, removals only update
+// them, so each round's two kernels see consistent cells. Keys claimed in one round have distinct cells of
+// degree 1 and none lies on another's cells, so the bits of one round are assigned in parallel, rounds in
+// reverse order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/shockhash_amd.h"
+#include "device_common.h"
+#include "retrieval.h"
+
+namespace {
+
+constexpr uint32_t UNPEELED = 0xFFFFFFFFu;
+constexpr int RT_BLOCK = 256, RT_GRID = 2048;
+constexpr int ROUNDS_PER_CHECK = 32;
+constexpr int MAX_ROUNDS = 1 << 16;
+
+__device__ __forceinline__ void rt_cells(uint64_t key, uint64_t seed, uint64_t seg, uint64_t c[3]) {
+    const uint64_t h0 = shgpu::remix(key + seed * 0x9e3779b97f4a7c15ull + 0x632be59bd9b4e019ull);
+    const uint64_t h1 = shgpu::remix(h0 ^ 0xd6e8feb86659fd93ull);
+    c[0] = __umul64hi(h0, seg);
+    c[1] = seg + __umul64hi(h1, seg);
+    c[2] = 2 * seg + __umul64hi(shgpu::remix(h1 + 0x8cb92ba72f3d8dd7ull), seg);
+}
+
+#define GRID_STRIDE(i, n) for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
+
+__global__ void k_rt_clear(uint32_t *deg, uint32_t *xr, uint8_t *bits, uint64_t ncell, uint32_t *round_of, uint64_t n,
+                           uint32_t *counters, uint32_t *round_begin) {
+    GRID_STRIDE(v, ncell) {
+        deg[v] = 0;
+        xr[v] = 0;
+        bits[v] = 0;
+    }
+    GRID_STRIDE(i, n) round_of[i] = UNPEELED;
+    if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) round_begin[0] = 0;
+}
+
+__global__ void k_rt_degrees(const uint64_t *__restrict__ keys, uint64_t n, uint64_t seed, uint64_t seg, uint32_t *deg,
+                             uint32_t *xr) {
+    GRID_STRIDE(i, n) {
+        uint64_t c[3];
+        rt_cells(keys[i], seed, seg, c);
+        for (int k = 0; k < 3; k++) {
+            atomicAdd(&deg[c[k]], 1u);
+            atomicXor(&xr[c[k]], (uint32_t)i);
+        }
+    }
+}
+
+// counters: [0] frontier length of the current round, [1] next frontier length, [2] keys peeled
+__global__ void k_rt_frontier0(const uint32_t *deg, uint64_t ncell, uint32_t *frontier, uint32_t *counters) {
+    GRID_STRIDE(v, ncell) {
+        if (deg[v] == 1) frontier[atomicAdd(&counters[0], 1u)] = (uint32_t)v;
+    }
+}
+
+// Round r, claims: a frontier cell still of degree 1 claims its only key.
+__global__ void k_rt_claim(const uint32_t *frontier, uint32_t *counters, const uint32_t *deg, const uint32_t *xr,
+                           uint32_t *round_of, uint32_t *assigned, uint32_t *peeled, uint32_t r) {
+    const uint32_t len = counters[0];
+    GRID_STRIDE(f, len) {
+        const uint32_t v = frontier[f];
+        if (deg[v] != 1) continue;
+        const uint32_t i = xr[v];
+        if (atomicCAS(&round_of[i], UNPEELED, r) == UNPEELED) {
+            assigned[i] = v;
+            peeled[atomicAdd(&counters[2], 1u)] = i;
+        }
+    }
+}
+
+// Round r, removals: the keys claimed in round r (peeled[begin, counters[2])) leave their cells.
+__global__ void k_rt_remove(const uint64_t *__restrict__ keys, uint64_t seed, uint64_t seg, const uint32_t *peeled,
+                            const uint32_t *round_begin, uint32_t r, const uint32_t *counters, uint32_t *deg, uint32_t *xr,
+                            uint32_t *next_frontier, uint32_t *next_len) {
+    const uint32_t begin = round_begin[r], end = counters[2];
+    GRID_STRIDE(f, (uint64_t)(end - begin)) {
+        const uint32_t i = peeled[begin + f];
+        uint64_t c[3];
+        rt_cells(keys[i], seed, seg, c);
+        for (int k = 0; k < 3; k++) {
+            atomicXor(&xr[c[k]], i);
+            if (atomicSub(&deg[c[k]], 1u) == 2u) next_frontier[atomicAdd(next_len, 1u)] = (uint32_t)c[k];
+        }
+    }
+}
+
+// Between rounds (one thread): the next frontier becomes the current one, the next round starts where the
+// peeled list ends now.
+__global__ void k_rt_advance(uint32_t *counters, uint32_t *round_begin, uint32_t r) {
+    counters[0] = counters[1];
+    counters[1] = 0;
+    round_begin[r + 1] = counters[2];
+}
+
+// Bits of the keys peeled in round r, given every later round's.
+__global__ void k_rt_assign(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ want, uint64_t seed, uint64_t seg,
+                            const uint32_t *peeled, const uint32_t *round_begin, uint32_t r, const uint32_t *assigned,
+                            uint8_t *bits) {
+    const uint32_t begin = round_begin[r], end = round_begin[r + 1];
+    GRID_STRIDE(f, (uint64_t)(end - begin)) {
+        const uint32_t i = peeled[begin + f];
+        uint64_t c[3];
+        rt_cells(keys[i], seed, seg, c);
+        const uint32_t v = assigned[i];
+        uint32_t x = want[i] & 1u;
+        for (int k = 0; k < 3; k++)
+            if (c[k] != v) x ^= bits[c[k]];
+        bits[v] = (uint8_t)x;
+    }
+}
+
+__global__ void k_rt_pack(const uint8_t *__restrict__ bits, uint64_t ncell, uint64_t *words) {
+    GRID_STRIDE(w, (ncell + 63) / 64) {
+        uint64_t x = 0;
+        const uint64_t base = w * 64;
+        for (uint32_t b = 0; b < 64 && base + b < ncell; b++) x |= (uint64_t)(bits[base + b] & 1u) << b;
+        words[w] = x;
+    }
+}
+
+struct RtBuf {
+    void *p = nullptr;
+    ~RtBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+uint64_t shgpu_retrieval_cells_per_third(uint64_t n) { return (uint64_t)(n * 1.23 / 3.0) + 8; }
+
+int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed_out,
+                               uint64_t *seg_out, uint64_t *h_words, uint64_t words_capacity, std::string &err) {
+    const uint64_t seg = shgpu_retrieval_cells_per_third(n);
+    const uint64_t ncell = 3 * seg, nwords = (ncell + 63) / 64;
+    if (n >= UNPEELED || ncell >= UNPEELED) { err = "retrieval: more than 2^32 - 1 keys or cells"; return SHGPU_ERR_UNSUPPORTED; }
+    if (words_capacity < nwords) { err = "retrieval: output capacity below " + std::to_string(nwords) + " words"; return SHGPU_ERR_ARG; }
+    // one allocation: deg, xr, frontier, next frontier (ncell u32 each), bits (ncell u8), round_of, assigned,
+    // peeled (n u32 each), round starts, counters, packed words; every part starts on a 256-byte boundary
+    const size_t parts[] = {ncell * 4, ncell * 4, ncell * 4, ncell * 4, ncell, (n + 1) * 4, (n + 1) * 4, (n + 1) * 4,
+                            (size_t)(MAX_ROUNDS + 2) * 4, 64, nwords * 8};
+    size_t bytes = 0;
+    for (size_t b : parts) bytes += (b + 255) / 256 * 256;
+    RtBuf buf;
+    if (hipMalloc(&buf.p, bytes) != hipSuccess) { err = "retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed"; return SHGPU_ERR_NOMEM; }
+    char *q = (char *)buf.p;
+    auto take = [&](size_t b) { char *r = q; q += (b + 255) / 256 * 256; return r; };
+    uint32_t *deg = (uint32_t *)take(ncell * 4), *xr = (uint32_t *)take(ncell * 4);
+    uint32_t *fr[2] = {(uint32_t *)take(ncell * 4), (uint32_t *)take(ncell * 4)};
+    uint8_t *bits = (uint8_t *)take(ncell);
+    uint32_t *round_of = (uint32_t *)take((n + 1) * 4), *assigned = (uint32_t *)take((n + 1) * 4), *peeled = (uint32_t *)take((n + 1) * 4);
+    uint32_t *round_begin = (uint32_t *)take((size_t)(MAX_ROUNDS + 2) * 4);
+    uint32_t *counters = (uint32_t *)take(64);
+    uint64_t *words = (uint64_t *)take(nwords * 8);
+    if ((size_t)(q - (char *)buf.p) > bytes) { err = "retrieval: buffer layout"; return SHGPU_ERR_HIP; }
+#define RT_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) { err = std::string(#call) + ": " + hipGetErrorString(e_); return SHGPU_ERR_HIP; } \
+    } while (0)
+    for (uint64_t seed = 0; seed < 64; seed++) {
+        k_rt_clear<<<RT_GRID, RT_BLOCK, 0, s>>>(deg, xr, bits, ncell, round_of, n, counters, round_begin);
+        RT_HIP(hipGetLastError());
+        k_rt_degrees<<<RT_GRID, RT_BLOCK, 0, s>>>(d_keys, n, seed, seg, deg, xr);
+        k_rt_frontier0<<<RT_GRID, RT_BLOCK, 0, s>>>(deg, ncell, fr[0], counters);
+        RT_HIP(hipGetLastError());
+        // round r reads frontier fr[r & 1] (length counters[0]) and fills fr[(r + 1) & 1] (length counters[1])
+        uint32_t rounds = 0, host_ctr[3] = {0, 0, 0};
+        for (;;) {
+            for (int k = 0; k < ROUNDS_PER_CHECK; k++, rounds++) {
+                if (rounds >= (uint32_t)MAX_ROUNDS) { err = "retrieval: peeling did not converge"; return SHGPU_ERR_HIP; }
+                k_rt_claim<<<RT_GRID, RT_BLOCK, 0, s>>>(fr[rounds & 1], counters, deg, xr, round_of, assigned, peeled, rounds);
+                k_rt_remove<<<RT_GRID, RT_BLOCK, 0, s>>>(d_keys, seed, seg, peeled, round_begin, rounds, counters, deg, xr,
+                                                          fr[(rounds + 1) & 1], &counters[1]);
+                k_rt_advance<<<1, 1, 0, s>>>(counters, round_begin, rounds);
+            }
+            RT_HIP(hipGetLastError());
+            RT_HIP(hipMemcpyAsync(host_ctr, counters, sizeof(host_ctr), hipMemcpyDeviceToHost, s));
+            RT_HIP(hipStreamSynchronize(s));
+            if (host_ctr[0] == 0) break;  // empty frontier: no further round can peel
+        }
+        if (host_ctr[2] != n) continue;  // not every key peeled: next seed
+        // round_begin holds the start of every round and the end of the last one; rounds past the last
+        // productive one are empty
+        for (uint32_t r = rounds; r-- > 0;)
+            k_rt_assign<<<RT_GRID, RT_BLOCK, 0, s>>>(d_keys, d_bits, seed, seg, peeled, round_begin, r, assigned, bits);
+        k_rt_pack<<<RT_GRID, RT_BLOCK, 0, s>>>(bits, ncell, words);
+        RT_HIP(hipGetLastError());
+        RT_HIP(hipMemcpyAsync(h_words, words, nwords * 8, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        *seed_out = seed;
+        *seg_out = seg;
+        return SHGPU_OK;
+    }
+#undef RT_HIP
+    err = "retrieval: peeling failed for 64 seeds (duplicate keys?)";
+    return SHGPU_ERR_SEARCH_LIMIT;
+}

@@ -86,6 +86,10 @@ _SIGS = {
     "shgpu_sh2_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_split_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_build_run": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
+    "shgpu_retrieval_cells_per_third": (ctypes.c_uint64, [ctypes.c_uint64]),
+    "shgpu_retrieval_words": (ctypes.c_uint64, [ctypes.c_uint64]),
+    "shgpu_retrieval_build": (ctypes.c_int, [P, P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_uint64), P, ctypes.c_uint64]),
     "shgpu_build_free": (None, [P]),
     "shgpu_build_nbuckets": (ctypes.c_uint64, [P]),
     "shgpu_build_nnodes": (ctypes.c_uint64, [P]),
@@ -348,6 +352,25 @@ class Engine:
         ptr = P((d_hashes.ptr.value or 0) + offset * 16)
         _check(load().shgpu_build_run(self.h, ctypes.byref(desc), ptr, n, ctypes.byref(h)))
         return Build(h)
+
+    def retrieval_build(self, keys, bits):
+        """One-bit retrieval structure of distinct keys and their bits, built on the device
+        (shgpu_retrieval_build): (seed, cells per third, packed words)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        bits = np.ascontiguousarray(bits, dtype=np.uint8)
+        n = len(keys)
+        d_k = DeviceArray.from_host(self, keys if n else np.zeros(1, np.uint64))
+        d_b = DeviceArray.from_host(self, bits if n else np.zeros(1, np.uint8))
+        lib = load()
+        words = np.zeros(lib.shgpu_retrieval_words(n), dtype=np.uint64)
+        seed, seg = ctypes.c_uint64(), ctypes.c_uint64()
+        try:
+            _check(lib.shgpu_retrieval_build(self.h, d_k.ptr, d_b.ptr, n, ctypes.byref(seed), ctypes.byref(seg),
+                                             words.ctypes.data_as(P), len(words)))
+        finally:
+            d_k.free()
+            d_b.free()
+        return seed.value, seg.value, words
 
     def build(self, hashes, variant, leaf_size, bucket_size, **kw):
         hashes = np.ascontiguousarray(hashes, dtype=np.uint64).reshape(-1, 2)

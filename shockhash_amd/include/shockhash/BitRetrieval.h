@@ -4,7 +4,9 @@
 // returned for keys of the set, which this structure returns exactly.
 //
 // Construction: a random 3-hypergraph XOR system (one cell in each third of ~1.23 n cells),
-// solved by peeling; a failed peeling (rare) retries with the next seed.
+// solved by peeling; a failed peeling (rare) retries with the next seed. The class templates build it on the
+// device (shgpu_retrieval_build, same layout) and load the result with the (seed, seg, words) constructor;
+// the host construction remains for the tests that compare the two.
 #pragma once
 #include <cstdint>
 #include <stdexcept>
@@ -70,6 +72,10 @@ class BitRetrieval {
 
   public:
     BitRetrieval() = default;
+    // A structure built elsewhere (shgpu_retrieval_build): its seed, cells per third and packed cell bits.
+    BitRetrieval(uint64_t seed_, uint64_t seg_, std::vector<uint64_t> words) : seed(seed_), seg(seg_), bits(std::move(words)) {
+        if (bits.size() != (3 * seg + 63) / 64) throw std::invalid_argument("BitRetrieval: word count does not match seg");
+    }
     explicit BitRetrieval(const std::vector<std::pair<uint64_t, uint8_t>> &in) {
         seg = (uint64_t)(in.size() * 1.23 / 3.0) + 8;
         for (seed = 0; seed < 64; seed++)

@@ -5,6 +5,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/shockhash_amd.h"
@@ -76,6 +77,32 @@ class Engine {
         shgpu_build_free(b);
         check(rc, "shgpu_build results");
         return f;
+    }
+
+    // The one-bit retrieval structure of (key, bit) pairs, built on the device (shgpu_retrieval_build):
+    // seed, cells per third and packed cell bits in the layout shockhash::BitRetrieval reads.
+    void retrieval(const std::vector<std::pair<uint64_t, uint8_t>> &in, uint64_t &seed, uint64_t &seg,
+                   std::vector<uint64_t> &words) {
+        std::lock_guard<std::mutex> lock(mtx);
+        const uint64_t n = in.size();
+        std::vector<uint64_t> keys(n);
+        std::vector<uint8_t> bits(n);
+        for (uint64_t i = 0; i < n; i++) {
+            keys[i] = in[i].first;
+            bits[i] = in[i].second;
+        }
+        void *d_keys = nullptr, *d_bits = nullptr;
+        int rc = shgpu_device_alloc(e, n * 8 + 8, &d_keys);
+        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, n + 8, &d_bits);
+        if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, d_keys, keys.data(), n * 8);
+        if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, d_bits, bits.data(), n);
+        words.assign(shgpu_retrieval_words(n), 0);
+        if (rc == SHGPU_OK)
+            rc = shgpu_retrieval_build(e, (const uint64_t *)d_keys, (const uint8_t *)d_bits, n, &seed, &seg, words.data(),
+                                       words.size());
+        if (d_keys) shgpu_device_free(e, d_keys);
+        if (d_bits) shgpu_device_free(e, d_bits);
+        check(rc, "shgpu_retrieval_build");
     }
 };
 

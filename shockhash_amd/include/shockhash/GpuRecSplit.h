@@ -48,7 +48,10 @@ class GpuRecSplit {
                      ribbon_input);
         descriptors = RiceBitVector(std::move(words));
         ef = DoubleEF(f.size_acc, pos_acc);
-        ribbon = BitRetrieval(ribbon_input);
+        uint64_t rseed = 0, rseg = 0;
+        std::vector<uint64_t> rwords;
+        gpu::Engine::instance().retrieval(ribbon_input, rseed, rseg, rwords);
+        ribbon = BitRetrieval(rseed, rseg, std::move(rwords));
         f.keys.clear();
         f.keys.shrink_to_fit();
         forest_meta = std::move(f);
