@@ -157,6 +157,12 @@ int shgpu_build_keys(const shgpu_build *b, uint64_t *keys);
  * packed words (bit c of the structure is bit c % 64 of word c / 64) to host memory. Tries seeds 0..63;
  * SHGPU_ERR_SEARCH_LIMIT when none peels (duplicate keys). */
 uint64_t shgpu_retrieval_cells_per_third(uint64_t n);
+/* Fingerprints of string keys: SpookyHash V2 128-bit with seed 0 of each key, as {first, second} = {hash2, hash1}
+ * (first_hash, include/ShockHash.h:129; the string constructors, include/ShockHash.h:159-185,
+ * include/SIMDShockHash.hpp:138, include/ShockHash2.h:140). d_data: the keys' bytes, d_offsets: n + 1 offsets
+ * (key i is d_data[d_offsets[i], d_offsets[i + 1])), d_hashes: 2n u64; all device memory. Runs on the engine's
+ * stream (ordered before a following shgpu_build_run). */
+int shgpu_hash_strings(shgpu_engine *e, const uint8_t *d_data, const uint64_t *d_offsets, uint64_t n, uint64_t *d_hashes);
 uint64_t shgpu_retrieval_words(uint64_t n);
 int shgpu_retrieval_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
                           uint64_t *seg, uint64_t *h_words, uint64_t words_capacity);

@@ -679,10 +679,13 @@ static uint64_t query(const Build &b, uint64_t first, uint64_t second) {
     return cum + (bit == 0 ? cc.c1 : cc.c2);
 }
 
-// ---------------------------------------------------------------- SpookyHash V2 (short messages)
+// ---------------------------------------------------------------- SpookyHash V2
 // sux `spooky(k, len, seed)` = SpookyHash::Hash128 with both state words = seed, returned as
-// {h1, h0} (ShockHash.h:129). Public-domain SpookyV2 (Bob Jenkins); only the < 192-byte path.
-// Parity unpinned: neither the reference nor the image carries a SpookyV2 test vector.
+// {h1, h0} (ShockHash.h:129). Public-domain SpookyV2 (Bob Jenkins): the short path below 192 bytes
+// (ShortMix / ShortEnd over 32-byte rounds) and the long path (Mix per 96-byte block, End over the
+// zero-padded last block that ends in its length, three EndPartial rounds). The short path is pinned by
+// SpookyV2's own self-test table (Hash32 of bytes (i + 128) & 255, lengths 0..63: tests/golden); the long path
+// has no vector source here (parity unpinned: it agrees with the library's two restatements only).
 static inline uint64_t rot64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 static inline void short_mix(uint64_t &h0, uint64_t &h1, uint64_t &h2, uint64_t &h3) {
     h2 = rot64(h2, 50); h2 += h3; h0 ^= h2;  h3 = rot64(h3, 52); h3 += h0; h1 ^= h3;
@@ -733,6 +736,39 @@ static void spooky_short(const uint8_t *p, size_t len, uint64_t &o1, uint64_t &o
     short_end(a, b, c, d);
     o1 = a; o2 = b;
 }
+static void spooky_mix(const uint64_t *data, uint64_t *s) {
+    static const int R[12] = {11, 32, 43, 31, 17, 28, 39, 57, 55, 54, 22, 46};
+    for (int i = 0; i < 12; i++) {
+        s[i] += data[i];
+        s[(i + 2) % 12] ^= s[(i + 10) % 12];
+        s[(i + 11) % 12] ^= s[i];
+        s[i] = rot64(s[i], R[i]);
+        s[(i + 11) % 12] += s[(i + 1) % 12];
+    }
+}
+static void spooky_end_partial(uint64_t *h) {
+    static const int R[12] = {44, 15, 34, 21, 38, 33, 10, 13, 38, 53, 42, 54};
+    for (int i = 0; i < 12; i++) {
+        h[(i + 11) % 12] += h[(i + 1) % 12];
+        h[(i + 2) % 12] ^= h[(i + 11) % 12];
+        h[(i + 1) % 12] = rot64(h[(i + 1) % 12], R[i]);
+    }
+}
+static void spooky128(const uint8_t *p, size_t len, uint64_t &o1, uint64_t &o2) {
+    if (len < 192) { spooky_short(p, len, o1, o2); return; }
+    uint64_t h[12], blk[12];
+    for (int i = 0; i < 12; i += 3) { h[i] = o1; h[i + 1] = o2; h[i + 2] = 0xdeadbeefdeadbeefull; }
+    const size_t whole = len / 96;
+    for (size_t k = 0; k < whole; k++) { std::memcpy(blk, p + 96 * k, 96); spooky_mix(blk, h); }
+    const size_t rem = len - 96 * whole;
+    uint8_t last[96] = {0};
+    std::memcpy(last, p + 96 * whole, rem);
+    last[95] = (uint8_t)rem;
+    std::memcpy(blk, last, 96);
+    for (int i = 0; i < 12; i++) h[i] += blk[i];
+    spooky_end_partial(h); spooky_end_partial(h); spooky_end_partial(h);
+    o1 = h[0]; o2 = h[1];
+}
 
 }  // namespace orc
 
@@ -757,7 +793,7 @@ int orc_float_accurate_elegant(uint64_t x, uint64_t y) { return float_accurate_e
 
 void orc_spooky128(const void *data, size_t len, uint64_t seed, uint64_t *first, uint64_t *second) {
     uint64_t h0 = seed, h1 = seed;
-    spooky_short((const uint8_t *)data, len, h0, h1);
+    spooky128((const uint8_t *)data, len, h0, h1);
     *first = h1; *second = h0;   // sux spooky() returns {h1, h0}
 }
 
@@ -770,7 +806,7 @@ void orc_spooky128_batch(const uint8_t *data, const uint64_t *offs, size_t n, ui
         th.emplace_back([=]() {
             for (size_t i = t; i < n; i += threads) {
                 uint64_t h0 = seed, h1 = seed;
-                spooky_short(data + offs[i], offs[i + 1] - offs[i], h0, h1);
+                spooky128(data + offs[i], offs[i + 1] - offs[i], h0, h1);
                 out[2 * i] = h1;
                 out[2 * i + 1] = h0;
             }

@@ -20,6 +20,7 @@
 #include "search_split.h"
 #include "retrieval.h"
 #include "sort.h"
+#include "strings.h"
 
 using namespace shgpu;
 
@@ -1528,6 +1529,14 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
 }
 
 uint64_t shgpu_retrieval_words(uint64_t n) { return (3 * shgpu_retrieval_cells_per_third(n) + 63) / 64; }
+
+int shgpu_hash_strings(shgpu_engine *e, const uint8_t *d_data, const uint64_t *d_offsets, uint64_t n, uint64_t *d_hashes) {
+    if (!e || (n && (!d_data || !d_offsets || !d_hashes))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(SHGPU_ERR_HIP, "hipSetDevice failed");
+    std::string err;
+    const int rc = shgpu_hash_strings_impl(e->stream, d_data, d_offsets, n, d_hashes, err);
+    return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
+}
 
 int shgpu_retrieval_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
                           uint64_t *seg, uint64_t *h_words, uint64_t words_capacity) {

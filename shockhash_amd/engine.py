@@ -86,6 +86,7 @@ _SIGS = {
     "shgpu_sh2_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_split_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_build_run": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
+    "shgpu_hash_strings": (ctypes.c_int, [P, P, P, ctypes.c_uint64, P]),
     "shgpu_retrieval_cells_per_third": (ctypes.c_uint64, [ctypes.c_uint64]),
     "shgpu_retrieval_words": (ctypes.c_uint64, [ctypes.c_uint64]),
     "shgpu_retrieval_build": (ctypes.c_int, [P, P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
@@ -352,6 +353,26 @@ class Engine:
         ptr = P((d_hashes.ptr.value or 0) + offset * 16)
         _check(load().shgpu_build_run(self.h, ctypes.byref(desc), ptr, n, ctypes.byref(h)))
         return Build(h)
+
+    def hash_strings(self, strings):
+        """SpookyHash V2 fingerprints of byte strings computed on the device (shgpu_hash_strings): an (n, 2)
+        uint64 array of (first, second)."""
+        n = len(strings)
+        lens = np.array([len(x) for x in strings], dtype=np.uint64)
+        offsets = np.zeros(n + 1, dtype=np.uint64)
+        offsets[1:] = np.cumsum(lens)
+        data = np.frombuffer(b"".join(strings) + b"\0" * 8, dtype=np.uint8)
+        d_data = DeviceArray.from_host(self, data)
+        d_off = DeviceArray.from_host(self, offsets)
+        d_out = DeviceArray(self, 2 * n + 2, np.uint64)
+        try:
+            _check(load().shgpu_hash_strings(self.h, d_data.ptr, d_off.ptr, n, d_out.ptr))
+            out = d_out.download()[: 2 * n].reshape(n, 2)
+        finally:
+            d_data.free()
+            d_off.free()
+            d_out.free()
+        return out
 
     def retrieval_build(self, keys, bits):
         """One-bit retrieval structure of distinct keys and their bits, built on the device

@@ -24,8 +24,7 @@ class SIMDShockHash : public detail::GpuRecSplit<SHGPU_SIMD_SHOCKHASH, LEAF_SIZE
     SIMDShockHash() {}
     SIMDShockHash(std::span<const std::string> keys, const size_t bucket_size, size_t num_threads = 1) {
         reject_rotation();
-        const std::vector<hash128_t> h = firstHashes(keys, num_threads);
-        this->construct(h.data(), h.size(), bucket_size, num_threads);
+        this->construct_strings(keys, bucket_size, num_threads);
     }
     SIMDShockHash(std::span<hash128_t> keys, const size_t bucket_size, size_t num_threads = 1) {
         reject_rotation();

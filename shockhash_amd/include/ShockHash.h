@@ -24,8 +24,7 @@ class ShockHash : public detail::GpuRecSplit<SHGPU_SHOCKHASH, LEAF_SIZE> {
     ShockHash() {}
     ShockHash(const std::vector<std::string> &keys, const size_t bucket_size, size_t num_threads = 1) {
         reject_rotation();
-        const std::vector<hash128_t> h = firstHashes(keys, num_threads);
-        this->construct(h.data(), h.size(), bucket_size, num_threads);
+        this->construct_strings(keys, bucket_size, num_threads);
     }
     // The reference sorts the caller's array in place; this engine leaves it unchanged.
     ShockHash(std::vector<hash128_t> &keys, const size_t bucket_size, size_t num_threads = 1) {

@@ -17,8 +17,7 @@ class ShockHash2 : public detail::GpuRecSplit<SHGPU_SHOCKHASH2, LEAF_SIZE> {
   public:
     ShockHash2() {}
     ShockHash2(std::span<const std::string> keys, const size_t bucket_size, size_t num_threads = 1) {
-        const std::vector<hash128_t> h = firstHashes(keys, num_threads);
-        this->construct(h.data(), h.size(), bucket_size, num_threads);
+        this->construct_strings(keys, bucket_size, num_threads);
     }
     ShockHash2(std::span<hash128_t> keys, const size_t bucket_size, size_t num_threads = 1) {
         this->construct(keys.data(), keys.size(), bucket_size, num_threads);

@@ -2,6 +2,7 @@
 // include/shockhash_amd.h). One engine per process and device, created on first use.
 #pragma once
 #include <cstdint>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -55,14 +56,52 @@ class Engine {
     Forest build(int variant, uint32_t leaf_size, const hash128_t *hashes, uint64_t n, uint64_t bucket_size) {
         static_assert(sizeof(hash128_t) == 16, "hash128_t must be two packed 64-bit words");
         std::lock_guard<std::mutex> lock(mtx);
+        const shgpu_build_desc d = desc(variant, leaf_size, bucket_size);
+        shgpu_build *b = nullptr;
+        check(shgpu_build_run_host(e, &d, reinterpret_cast<const uint64_t *>(hashes), n, &b), "shgpu_build_run");
+        return collect(b, n);
+    }
+
+    // String keys: packed and fingerprinted on the device (shgpu_hash_strings, SpookyHash V2 as
+    // first_hash), then built from the device fingerprints.
+    template <class Strings>
+    Forest build_strings(int variant, uint32_t leaf_size, const Strings &keys, uint64_t bucket_size) {
+        std::lock_guard<std::mutex> lock(mtx);
+        const uint64_t n = keys.size();
+        std::vector<uint64_t> offsets(n + 1, 0);
+        for (uint64_t i = 0; i < n; i++) offsets[i + 1] = offsets[i] + keys[i].size();
+        std::vector<uint8_t> data(offsets[n] + 8);
+        for (uint64_t i = 0; i < n; i++) std::memcpy(data.data() + offsets[i], keys[i].data(), keys[i].size());
+        void *d_data = nullptr, *d_off = nullptr, *d_hashes = nullptr;
+        int rc = shgpu_device_alloc(e, data.size(), &d_data);
+        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, offsets.size() * 8, &d_off);
+        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, n * 16 + 16, &d_hashes);
+        if (rc == SHGPU_OK) rc = shgpu_copy_to_device(e, d_data, data.data(), data.size());
+        if (rc == SHGPU_OK) rc = shgpu_copy_to_device(e, d_off, offsets.data(), offsets.size() * 8);
+        if (rc == SHGPU_OK) rc = shgpu_hash_strings(e, (const uint8_t *)d_data, (const uint64_t *)d_off, n, (uint64_t *)d_hashes);
+        shgpu_build *b = nullptr;
+        const shgpu_build_desc d = desc(variant, leaf_size, bucket_size);
+        if (rc == SHGPU_OK) rc = shgpu_build_run(e, &d, (const uint64_t *)d_hashes, n, &b);
+        const std::string msg = rc == SHGPU_OK ? "" : shgpu_last_error();
+        if (d_data) shgpu_device_free(e, d_data);
+        if (d_off) shgpu_device_free(e, d_off);
+        if (d_hashes) shgpu_device_free(e, d_hashes);
+        if (rc != SHGPU_OK) throw Error(rc, std::string("shgpu_build_run (strings): ") + shgpu_status_string(rc) + ": " + msg);
+        return collect(b, n);
+    }
+
+  private:
+    static shgpu_build_desc desc(int variant, uint32_t leaf_size, uint64_t bucket_size) {
         shgpu_build_desc d{};
         d.variant = variant;
         d.leaf_size = leaf_size;
         d.bucket_size = bucket_size;
         d.mode = SHGPU_MODE_PERSISTENT;
         d.copy_keys = 1;
-        shgpu_build *b = nullptr;
-        check(shgpu_build_run_host(e, &d, reinterpret_cast<const uint64_t *>(hashes), n, &b), "shgpu_build_run");
+        return d;
+    }
+
+    Forest collect(shgpu_build *b, uint64_t n) {
         Forest f;
         f.nbuckets = shgpu_build_nbuckets(b);
         f.size_acc.resize(f.nbuckets + 1);
@@ -79,6 +118,7 @@ class Engine {
         return f;
     }
 
+  public:
     // The one-bit retrieval structure of (key, bit) pairs, built on the device (shgpu_retrieval_build):
     // seed, cells per third and packed cell bits in the layout shockhash::BitRetrieval reads.
     void retrieval(const std::vector<std::pair<uint64_t, uint8_t>> &in, uint64_t &seed, uint64_t &seg,

@@ -38,9 +38,19 @@ class GpuRecSplit {
     double host_ms = 0;
 
     void construct(const hash128_t *h, size_t n, size_t bsize, size_t num_threads) {
+        construct_forest(gpu::Engine::instance().build(VARIANT, (uint32_t)LEAF_SIZE, h, n, bsize), n, bsize, num_threads);
+    }
+
+    // String keys: fingerprinted on the device (first_hash, include/ShockHash.h:129,159-185).
+    template <class Strings>
+    void construct_strings(const Strings &keys, size_t bsize, size_t num_threads) {
+        construct_forest(gpu::Engine::instance().build_strings(VARIANT, (uint32_t)LEAF_SIZE, keys, bsize), keys.size(), bsize,
+                         num_threads);
+    }
+
+    void construct_forest(gpu::Forest &&f, size_t n, size_t bsize, size_t num_threads) {
         bucket_size = bsize;
         keys_count = n;
-        gpu::Forest f = gpu::Engine::instance().build(VARIANT, (uint32_t)LEAF_SIZE, h, n, bsize);
         const auto t0 = std::chrono::steady_clock::now();
         nbuckets = f.nbuckets;
         std::vector<uint64_t> words, pos_acc;

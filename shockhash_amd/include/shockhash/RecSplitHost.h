@@ -150,22 +150,5 @@ void encodeForest(const gpu::Forest &f, const MEMO &memo, LeafKind leaf_kind, si
         if (fl) throw std::logic_error("GPU leaf seed does not admit a cuckoo placement");
 }
 
-// Hashes string keys to fingerprints with `num_threads` threads (include/ShockHash.h:159-185).
-template <class Strings>
-std::vector<hash128_t> firstHashes(const Strings &keys, size_t num_threads) {
-    std::vector<hash128_t> h(keys.size());
-    const size_t n = keys.size(), T = std::max<size_t>(1, num_threads);
-    auto run = [&](size_t t) {
-        const size_t per = n / T + 1, from = t * per, to = std::min(n, (t + 1) * per);
-        for (size_t i = from; i < to; i++) h[i] = spooky(keys[i].c_str(), keys[i].size(), 0);
-    };
-    if (T == 1) run(0);
-    else {
-        std::vector<std::thread> th;
-        for (size_t t = 0; t < T; t++) th.emplace_back(run, t);
-        for (auto &x : th) x.join();
-    }
-    return h;
-}
 
 }  // namespace shockhash

@@ -142,3 +142,25 @@ def test_packed_four_way_counts_are_exact(unit):
     packed = (c0 + (c1 << 8) + (c2 << 16) + (c3 << 24)) & 0xFFFFFFFF
     hit = packed == unit * 0x01010101
     assert np.array_equal(hit, (c0 == unit) & (c1 == unit) & (c2 == unit))
+
+
+def test_spookyv2_self_test_table():
+    """SpookyV2's own self-test values (Hash32 of bytes (i + 128) & 255, lengths 0..63) pin the oracle's short
+    path, which fingerprints every string key of the configurations (10..50 bytes)."""
+    want = [int(x, 16) for x in GOLD["spookyv2_hash32"]["expected"]]
+    buf = bytes(((j + 128) & 255) for j in range(len(want)))
+    h = O.spooky128_batch([buf[:i] for i in range(len(want))])
+    assert [int(x) & 0xFFFFFFFF for x in h[:, 1]] == want
+
+
+def test_spookyv2_long_path_blocks():
+    """Long messages (>= 192 bytes) are not pinned by a vector; check what the algorithm fixes: the hash of a
+    message depends on every byte, including the last block's zero padding being distinct from data zeros."""
+    base = bytes(range(256)) * 2
+    hs = O.spooky128_batch([base[:L] for L in range(190, 400)])
+    assert len({tuple(r) for r in hs}) == len(hs)  # every length distinct
+    flipped = [bytearray(base[:300]) for _ in range(300)]
+    for i, b in enumerate(flipped):
+        b[i] ^= 1
+    hf = O.spooky128_batch([bytes(b) for b in flipped])
+    assert len({tuple(r) for r in hf}) == 300 and tuple(O.spooky128_batch([base[:300]])[0]) not in {tuple(r) for r in hf}
