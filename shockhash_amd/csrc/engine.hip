@@ -75,6 +75,26 @@ struct DevBuf {
 }  // namespace
 
 // Device buffers of one construction in flight (the engine's own, or one serving slot's).
+// Page-locked host memory (hipHostMalloc), grown on demand: device-to-host copies into it run at full link
+// speed instead of through the runtime's staging buffers.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want) {
+        if (want <= bytes) return SHGPU_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return fail(SHGPU_ERR_NOMEM, "hipHostMalloc of " + std::to_string(want) + " bytes failed");
+        }
+        bytes = want;
+        return SHGPU_OK;
+    }
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+};
+
 struct BuildBuffers {
     DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl, states, help,
         sh2_scratch, staging, plan_tab, plan_sets, plan_bucket, plan_summary, sort_flag;
@@ -84,6 +104,7 @@ struct BuildBuffers {
     uint32_t plan_leaf = 0;
     std::vector<uint32_t> plan_counts;
     uint64_t plan_capacity = 0;
+    PinnedBuf codes_host;  // codes on their way to the host
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
     size_t bytes() const {
         const DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
@@ -260,13 +281,13 @@ __global__ void k_sort_fixup(uint64_t *first, uint64_t *second, uint64_t n, uint
     }
 }
 
-// Top key bits a build sorts on: enough that n keys rarely share them (expected run length n / 2^bits below
-// 1/64), rounded up to whole 8-bit radix digits; 64 when full_sort.
+// Top key bits a build sorts on: ceil(log2 n) rounded up to whole 8-bit radix digits, so the expected number
+// of other keys sharing a key's sorted bits is at most 1 and the fix-up sorts short runs; 64 when full_sort.
 static int sort_bits(uint64_t n, bool full_sort) {
     if (full_sort) return 64;
     int lg = 1;
     while (lg < 64 && (1ull << lg) < n) lg++;
-    return std::min(64, (lg + 6 + 7) / 8 * 8);
+    return std::min(64, (lg + 7) / 8 * 8);
 }
 
 // size_acc[b] = number of keys in buckets < b, from firsts (rotated right by rot) sorted ascending (remap128
@@ -1264,6 +1285,8 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     std::vector<uint64_t> node_begin(count, 0), node_end(count, 0);
     uint32_t ctrl_out[4] = {0, 0, 0, 0};
     uint32_t sort_rerun = 0;  // a run of keys sharing the sorted bits was too long: sort on all 64 bits
+    const uint64_t *codes_pinned = nullptr;  // persistent form: codes arrive in pinned memory (n_codes of them)
+    uint64_t n_codes = 0;
     bool timed_kernel = false;
     BuildArgs a;
     a.keys = B.keys.as<uint64_t>();
@@ -1368,8 +1391,10 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             cap = (uint64_t)summary[SUM_TOTAL] + 1024;  // exact size, then run the plan and the search again
         }
         const uint32_t total = summary[SUM_TOTAL];
-        codes.resize(total);
-        if (total) BATCH_HIP(hipMemcpyAsync(codes.data(), a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
+        if ((rc = B.codes_host.ensure((size_t)total * 8 + 8))) return fail_all(rc);
+        codes_pinned = (const uint64_t *)B.codes_host.p;
+        n_codes = total;
+        if (total) BATCH_HIP(hipMemcpyAsync(B.codes_host.p, a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
         for (size_t r = 0; r < count; r++) {
             node_begin[r] = summary[SUM_SETS + 3 * r];
             node_end[r] = summary[SUM_SETS + 3 * r + 1];
@@ -1478,6 +1503,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     BATCH_HIP(hipStreamSynchronize(s));
 #undef BATCH_HIP
     auto t3 = clk::now();
+    if (codes_pinned) codes.assign(codes_pinned, codes_pinned + n_codes);
     float kms = 0, sort_ms = 0, plan_ms = 0;
     if (timed_kernel && (hipEventElapsedTime(&kms, B.ev0, B.ev1) != hipSuccess || hipEventElapsedTime(&sort_ms, B.evA, B.evB) != hipSuccess ||
                          hipEventElapsedTime(&plan_ms, B.evB, B.ev0) != hipSuccess))
