@@ -157,6 +157,15 @@ int shgpu_build_keys(const shgpu_build *b, uint64_t *keys);
  * packed words (bit c of the structure is bit c % 64 of word c / 64) to host memory. Tries seeds 0..63;
  * SHGPU_ERR_SEARCH_LIMIT when none peels (duplicate keys). */
 uint64_t shgpu_retrieval_cells_per_third(uint64_t n);
+/* Retrieval input of leaves: replays each leaf's cuckoo insertions in leaf key order (the orientation that
+ * fixes which of its two cells a key occupies) and writes the (key, bit) entries in the reference's order:
+ * ShockHash / SIMDShockHash leaves cell by cell with bit = cell >= m/2 (include/ShockHash.h:425-432,
+ * include/SIMDShockHash.hpp:536-540), ShockHash2 leaves key by key with bit = the key sits in its second cell
+ * (include/shockhash/ShockHash2-internal.h:548-572). variant: SHGPU_*; d_keys: leaf-ordered keys; d_tasks: n
+ * shgpu_orient_task; outputs sized by the tasks' out_offset + m. All device memory; SHGPU_ERR_ARG when a
+ * task's keys cannot be placed with its seed. */
+int shgpu_leaf_orientation(shgpu_engine *e, int variant, const uint64_t *d_keys, const shgpu_orient_task *d_tasks,
+                           uint64_t n, uint64_t *d_out_keys, uint8_t *d_out_bits);
 /* Fingerprints of string keys: SpookyHash V2 128-bit with seed 0 of each key, as {first, second} = {hash2, hash1}
  * (first_hash, include/ShockHash.h:129; the string constructors, include/ShockHash.h:159-185,
  * include/SIMDShockHash.hpp:138, include/ShockHash2.h:140). d_data: the keys' bytes, d_offsets: n + 1 offsets

@@ -1556,6 +1556,18 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
 
 uint64_t shgpu_retrieval_words(uint64_t n) { return (3 * shgpu_retrieval_cells_per_third(n) + 63) / 64; }
 
+int shgpu_leaf_orientation(shgpu_engine *e, int variant, const uint64_t *d_keys, const shgpu_orient_task *d_tasks,
+                           uint64_t n, uint64_t *d_out_keys, uint8_t *d_out_bits) {
+    if (!e || (n && (!d_keys || !d_tasks || !d_out_keys || !d_out_bits))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (variant != SHGPU_SHOCKHASH && variant != SHGPU_SIMD_SHOCKHASH && variant != SHGPU_SHOCKHASH2)
+        return fail(SHGPU_ERR_ARG, "unknown variant");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(SHGPU_ERR_HIP, "hipSetDevice failed");
+    std::string err;
+    const int rc = shgpu_leaf_orientation_impl(e->stream, variant == SHGPU_SHOCKHASH2, d_keys, d_tasks, n, d_out_keys,
+                                               d_out_bits, err);
+    return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
+}
+
 int shgpu_hash_strings(shgpu_engine *e, const uint8_t *d_data, const uint64_t *d_offsets, uint64_t n, uint64_t *d_hashes) {
     if (!e || (n && (!d_data || !d_offsets || !d_hashes))) return fail(SHGPU_ERR_ARG, "NULL argument");
     if (hipSetDevice(e->device) != hipSuccess) return fail(SHGPU_ERR_HIP, "hipSetDevice failed");

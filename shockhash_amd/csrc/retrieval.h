@@ -10,3 +10,8 @@
 // third and ceil(3 * seg / 64) packed words to host memory. Returns an SHGPU status, message in err.
 int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
                                uint64_t *seg, uint64_t *h_words, uint64_t words_capacity, std::string &err);
+
+#include "shgpu_types.h"
+// Retrieval input of n leaves (one thread per leaf replays its cuckoo insertions); bipartite: ShockHash2.
+int shgpu_leaf_orientation_impl(hipStream_t s, int bipartite, const uint64_t *d_keys, const shgpu_orient_task *d_tasks,
+                                uint64_t n, uint64_t *d_out_keys, uint8_t *d_out_bits, std::string &err);

@@ -140,6 +140,103 @@ __global__ void k_rt_pack(const uint8_t *__restrict__ bits, uint64_t ncell, uint
     }
 }
 
+// ------------------------------------------------------------------ leaf orientation (retrieval input)
+// Exact decodes of the pairing functions (PairingFunction.h:52-56,83-91). The reference stores only codes
+// whose double-precision decode is exact (isFloatAccurate*), so for those the integer-corrected estimate
+// below equals the reference's decode.
+__device__ __forceinline__ void unpair_triangular(uint64_t z, uint64_t &x, uint64_t &y) {
+    uint64_t t = (uint64_t)floor(0.5 + sqrt(0.25 + 2.0 * (double)z));
+    while (t > 1 && t * (t - 1) / 2 > z) t--;
+    while ((t + 1) * t / 2 <= z) t++;
+    x = t;
+    y = z - t * (t - 1) / 2;
+}
+__device__ __forceinline__ void unpair_elegant(uint64_t z, uint64_t &x, uint64_t &y) {
+    uint64_t f = (uint64_t)floor(sqrt((double)z));
+    while (f > 0 && f * f > z) f--;
+    while ((f + 1) * (f + 1) <= z) f++;
+    const uint64_t r = z - f * f;
+    if (r < f) { x = r; y = f; } else { x = f; y = r - f; }
+}
+
+// Cuckoo insertion of key k with cells c1, c2 into cell[] (-1 = empty), entries identified by key index; the
+// host's TinyCuckoo (and TinyBinaryCuckooHashTable::insert, TinyBinaryCuckooHashTable.h:169-198) step for
+// step. False when the key cannot be placed.
+__device__ bool orient_insert(int8_t *cell, uint8_t *xr, int32_t k, uint32_t c1, uint32_t c2) {
+    int32_t entry = k;
+    xr[k] = (uint8_t)(c1 ^ c2);
+    if (cell[c1] < 0) { cell[c1] = (int8_t)entry; return true; }
+    if (cell[c2] < 0) { cell[c2] = (int8_t)entry; return true; }
+    uint32_t cur = c1;
+    for (uint32_t steps = 0; steps < 4096; steps++) {
+        const uint32_t alt = xr[entry] ^ cur;
+        const int32_t evicted = cell[alt];
+        cell[alt] = (int8_t)entry;
+        entry = evicted;
+        if (entry < 0) return true;
+        cur = alt;
+        if (cur == c1 && entry == k) return false;
+    }
+    return false;
+}
+
+// One thread per leaf: replay the leaf's insertions in leaf key order and write its retrieval input at
+// out_offset. ShockHash1 / SIMDShockHash (BIPARTITE false): cells c1 < m/2 <= c2 of remix(key + seed) with
+// seed = code + start seed, entries in cell order with bit = (cell >= m/2) (ShockHash.h:425-432,
+// SIMDShockHash.hpp:536-540). ShockHash2: cells sh2(s1) + m/2 and sh2(s2) of the pair code's seeds, entries in
+// key order with bit = (the key sits in its second cell) (ShockHash2-internal.h:548-572).
+template <bool BIPARTITE>
+__global__ void k_leaf_orientation(const uint64_t *__restrict__ keys, const shgpu_orient_task *__restrict__ tasks, uint64_t n,
+                                   uint64_t *out_keys, uint8_t *out_bits, uint32_t *error) {
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+        const shgpu_orient_task tk = tasks[t];
+        const uint64_t *lk = keys + tk.key_offset;
+        const uint32_t m = tk.m;
+        if (m < 1 || m > 64) { atomicOr(error, 1u); continue; }
+        int8_t cell[128];
+        uint8_t xr[64], c2s[64];
+        for (uint32_t c = 0; c < m; c++) cell[c] = -1;
+        bool ok = true;
+        if (!BIPARTITE) {
+            for (uint32_t k = 0; k < m && ok; k++) {
+                const uint64_t r = shgpu::remix(lk[k] + tk.seed);
+                const uint32_t c1 = (uint32_t)(((uint64_t)(uint32_t)r * (m / 2)) >> 32);
+                const uint32_t c2 = (uint32_t)(((r >> 32) * ((m + 1) / 2)) >> 32) + m / 2;
+                ok = orient_insert(cell, xr, (int32_t)k, c1, c2);
+            }
+            if (!ok) { atomicOr(error, 2u); continue; }
+            for (uint32_t c = 0; c < m; c++) {
+                out_keys[tk.out_offset + c] = lk[cell[c]];
+                out_bits[tk.out_offset + c] = c < m / 2 ? 0 : 1;
+            }
+        } else {
+            uint64_t s1, s2;
+            unpair_triangular(tk.seed, s1, s2);
+            uint64_t a1 = s1, b1 = s1, a2 = s2, b2 = s2;  // half-seeds of even / odd keys (m >= 10)
+            if (m >= 10) {
+                unpair_elegant(s1, a1, b1);
+                unpair_elegant(s2, a2, b2);
+            }
+            const uint64_t H = (m + 1) / 2;
+            for (uint32_t k = 0; k < m && ok; k++) {
+                const uint64_t key = lk[k];
+                const bool even = (key & 1) == 0;
+                const uint32_t c1 = (uint32_t)__umul64hi(shgpu::remix(key + (even ? a1 : b1)), H) + m / 2;
+                const uint32_t c2 = (uint32_t)__umul64hi(shgpu::remix(key + (even ? a2 : b2)), H);
+                c2s[k] = (uint8_t)c2;
+                ok = orient_insert(cell, xr, (int32_t)k, c1, c2);
+            }
+            if (!ok) { atomicOr(error, 2u); continue; }
+            for (uint32_t k = 0; k < m; k++) {
+                out_keys[tk.out_offset + k] = lk[k];
+                out_bits[tk.out_offset + k] = cell[c2s[k]] == (int8_t)k ? 1 : 0;
+            }
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p) { p[0] = 0; }
+
 struct RtBuf {
     void *p = nullptr;
     ~RtBuf() {
@@ -217,4 +314,23 @@ int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint
 #undef RT_HIP
     err = "retrieval: peeling failed for 64 seeds (duplicate keys?)";
     return SHGPU_ERR_SEARCH_LIMIT;
+}
+
+int shgpu_leaf_orientation_impl(hipStream_t s, int bipartite, const uint64_t *d_keys, const shgpu_orient_task *d_tasks,
+                                uint64_t n, uint64_t *d_out_keys, uint8_t *d_out_bits, std::string &err) {
+    if (n == 0) return SHGPU_OK;
+    RtBuf flag;
+    if (hipMalloc(&flag.p, 64) != hipSuccess) { err = "orientation: hipMalloc failed"; return SHGPU_ERR_NOMEM; }
+    uint32_t *d_err = (uint32_t *)flag.p;
+    k_zero_u32<<<1, 1, 0, s>>>(d_err);
+    if (bipartite) k_leaf_orientation<true><<<1024, 128, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
+    else k_leaf_orientation<false><<<1024, 128, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
+    uint32_t h = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d_err, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { err = std::string("orientation: ") + hipGetErrorString(e); return SHGPU_ERR_HIP; }
+    if (h & 1) { err = "orientation: a leaf has no keys or more than 64"; return SHGPU_ERR_ARG; }
+    if (h & 2) { err = "orientation: a leaf's keys cannot all be placed with its seed (wrong code?)"; return SHGPU_ERR_ARG; }
+    return SHGPU_OK;
 }

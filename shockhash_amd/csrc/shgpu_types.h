@@ -14,6 +14,16 @@ typedef struct shgpu_leaf_task {
     uint64_t max_trials;
 } shgpu_leaf_task;
 
+/* Orientation of one leaf (retrieval input): keys[key_offset .. key_offset+m) in leaf order, the leaf's seed
+ * (ShockHash1 / SIMDShockHash: code + start seed; ShockHash2: the pair code) and where its m retrieval entries
+ * go in the output. */
+typedef struct shgpu_orient_task {
+    uint64_t out_offset;
+    uint64_t seed;
+    uint32_t key_offset;
+    uint32_t m;
+} shgpu_orient_task;
+
 /* Hash flavour of a split search. */
 enum { SHGPU_SPLIT_SUX = 0, SHGPU_SPLIT_SIMD = 1 };
 

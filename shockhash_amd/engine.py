@@ -87,6 +87,7 @@ _SIGS = {
     "shgpu_split_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_build_run": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
     "shgpu_hash_strings": (ctypes.c_int, [P, P, P, ctypes.c_uint64, P]),
+    "shgpu_leaf_orientation": (ctypes.c_int, [P, ctypes.c_int, P, P, ctypes.c_uint64, P, P]),
     "shgpu_retrieval_cells_per_third": (ctypes.c_uint64, [ctypes.c_uint64]),
     "shgpu_retrieval_words": (ctypes.c_uint64, [ctypes.c_uint64]),
     "shgpu_retrieval_build": (ctypes.c_int, [P, P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),

@@ -25,6 +25,7 @@ _SIGS = {
     "shh_codes": (None, [P, P]),
     "shh_bits": (ctypes.c_size_t, [P]),
     "shh_timings": (None, [P, P]),
+    "shh_keep_retrieval_input": (None, [ctypes.c_int]),
     "shh_memo": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P]),
 }
 VARIANTS = {"sh1": 0, "simd": 1, "sh2": 2}
@@ -66,9 +67,10 @@ def _pack_strings(strings):
 class MPHF:
     """A ShockHash / SIMDShockHash / ShockHash2 instance built by the C++ class template."""
 
-    def __init__(self, variant, leaf, keys, bucket_size, threads=1):
+    def __init__(self, variant, leaf, keys, bucket_size, threads=1, keep_retrieval_input=False):
         lib = load()
         err = ctypes.create_string_buffer(512)
+        lib.shh_keep_retrieval_input(1 if keep_retrieval_input else 0)
         if isinstance(keys, (list, tuple)):
             data, offs = _pack_strings(keys)
             self._strings = True

@@ -19,7 +19,8 @@ struct Handle {
     virtual size_t query(const hash128_t &h) = 0;
     virtual size_t query_string(const std::string &s) = 0;
     virtual const std::vector<uint64_t> &words() const = 0;
-    virtual const std::vector<std::pair<uint64_t, uint8_t>> &ribbon() const = 0;
+    virtual const std::vector<uint64_t> &ribbon_keys() const = 0;
+    virtual const std::vector<uint8_t> &ribbon_bits() const = 0;
     virtual const gpu::Forest &forest() const = 0;
     virtual size_t bits() const = 0;
     virtual double host_ms() const = 0;
@@ -32,7 +33,8 @@ struct HandleT : Handle {
     size_t query(const hash128_t &h) override { return (*obj)(h); }
     size_t query_string(const std::string &s) override { return (*obj)(s); }
     const std::vector<uint64_t> &words() const override { return obj->descriptorWords(); }
-    const std::vector<std::pair<uint64_t, uint8_t>> &ribbon() const override { return obj->ribbonInput(); }
+    const std::vector<uint64_t> &ribbon_keys() const override { return obj->ribbonKeys(); }
+    const std::vector<uint8_t> &ribbon_bits() const override { return obj->ribbonBits(); }
     const gpu::Forest &forest() const override { return obj->forest(); }
     size_t bits() const override { return obj->getBits(); }
     double host_ms() const override { return obj->hostEncodeMs(); }
@@ -131,11 +133,15 @@ void shh_words(void *h, uint64_t *out) {
     const auto &w = ((Handle *)h)->words();
     std::memcpy(out, w.data(), w.size() * 8);
 }
-size_t shh_ribbon_size(void *h) { return ((Handle *)h)->ribbon().size(); }
+size_t shh_ribbon_size(void *h) { return ((Handle *)h)->ribbon_keys().size(); }
 void shh_ribbon(void *h, uint64_t *keys, uint8_t *bits) {
-    const auto &r = ((Handle *)h)->ribbon();
-    for (size_t i = 0; i < r.size(); i++) { keys[i] = r[i].first; bits[i] = r[i].second; }
+    const auto &k = ((Handle *)h)->ribbon_keys();
+    const auto &b = ((Handle *)h)->ribbon_bits();
+    std::memcpy(keys, k.data(), k.size() * 8);
+    std::memcpy(bits, b.data(), b.size());
 }
+// Keep the retrieval input of later builds (for shh_ribbon); off by default.
+void shh_keep_retrieval_input(int keep) { gpu::keep_retrieval_input = keep != 0; }
 size_t shh_nnodes(void *h) { return ((Handle *)h)->forest().codes.size(); }
 void shh_codes(void *h, uint64_t *out) {
     const auto &c = ((Handle *)h)->forest().codes;

@@ -26,6 +26,10 @@ inline void check(int rc, const char *what) {
         throw Error(rc, std::string(what) + ": " + shgpu_status_string(rc) + ": " + shgpu_last_error());
 }
 
+// Whether the class templates keep each build's retrieval input (the leaf orientations in the reference's
+// order) on the host for verification; off by default (the reference keeps none), the tests turn it on.
+inline bool keep_retrieval_input = false;
+
 // Seed-search results of one construction, everything the host encoder needs.
 struct Forest {
     uint64_t nbuckets = 0;
@@ -119,30 +123,45 @@ class Engine {
     }
 
   public:
-    // The one-bit retrieval structure of (key, bit) pairs, built on the device (shgpu_retrieval_build):
-    // seed, cells per third and packed cell bits in the layout shockhash::BitRetrieval reads.
-    void retrieval(const std::vector<std::pair<uint64_t, uint8_t>> &in, uint64_t &seed, uint64_t &seg,
-                   std::vector<uint64_t> &words) {
+    // The retrieval input of a forest (every leaf's orientation, in the reference's order) and its one-bit
+    // retrieval structure, both computed on the device (shgpu_leaf_orientation, shgpu_retrieval_build): the
+    // leaf-ordered keys go up once, the structure comes back, and the input too when `keep` is set.
+    void orientation_retrieval(int variant, const Forest &f, bool keep, std::vector<uint64_t> &input_keys,
+                               std::vector<uint8_t> &input_bits, uint64_t &seed, uint64_t &seg,
+                               std::vector<uint64_t> &words) {
         std::lock_guard<std::mutex> lock(mtx);
-        const uint64_t n = in.size();
-        std::vector<uint64_t> keys(n);
-        std::vector<uint8_t> bits(n);
-        for (uint64_t i = 0; i < n; i++) {
-            keys[i] = in[i].first;
-            bits[i] = in[i].second;
+        std::vector<shgpu_orient_task> tasks;
+        uint64_t total = 0;
+        for (size_t i = 0; i < f.nodes.size(); i++) {
+            const shgpu_node &nd = f.nodes[i];
+            if (nd.kind != SHGPU_NODE_LEAF) continue;
+            tasks.push_back({total, variant == SHGPU_SHOCKHASH2 ? f.codes[i] : f.codes[i] + nd.start_seed, nd.offset, nd.m});
+            total += nd.m;
         }
-        void *d_keys = nullptr, *d_bits = nullptr;
+        const uint64_t n = f.keys.size();
+        void *d_keys = nullptr, *d_tasks = nullptr, *d_out_keys = nullptr, *d_out_bits = nullptr;
         int rc = shgpu_device_alloc(e, n * 8 + 8, &d_keys);
-        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, n + 8, &d_bits);
-        if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, d_keys, keys.data(), n * 8);
-        if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, d_bits, bits.data(), n);
-        words.assign(shgpu_retrieval_words(n), 0);
+        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, tasks.size() * sizeof(shgpu_orient_task) + 8, &d_tasks);
+        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, total * 8 + 8, &d_out_keys);
+        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, total + 8, &d_out_bits);
+        if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, d_keys, f.keys.data(), n * 8);
+        if (rc == SHGPU_OK && !tasks.empty())
+            rc = shgpu_copy_to_device(e, d_tasks, tasks.data(), tasks.size() * sizeof(shgpu_orient_task));
         if (rc == SHGPU_OK)
-            rc = shgpu_retrieval_build(e, (const uint64_t *)d_keys, (const uint8_t *)d_bits, n, &seed, &seg, words.data(),
-                                       words.size());
-        if (d_keys) shgpu_device_free(e, d_keys);
-        if (d_bits) shgpu_device_free(e, d_bits);
-        check(rc, "shgpu_retrieval_build");
+            rc = shgpu_leaf_orientation(e, variant, (const uint64_t *)d_keys, (const shgpu_orient_task *)d_tasks,
+                                        tasks.size(), (uint64_t *)d_out_keys, (uint8_t *)d_out_bits);
+        words.assign(shgpu_retrieval_words(total), 0);
+        if (rc == SHGPU_OK)
+            rc = shgpu_retrieval_build(e, (const uint64_t *)d_out_keys, (const uint8_t *)d_out_bits, total, &seed, &seg,
+                                       words.data(), words.size());
+        input_keys.assign(keep ? total : 0, 0);
+        input_bits.assign(keep ? total : 0, 0);
+        if (rc == SHGPU_OK && keep && total) rc = shgpu_copy_to_host(e, input_keys.data(), d_out_keys, total * 8);
+        if (rc == SHGPU_OK && keep && total) rc = shgpu_copy_to_host(e, input_bits.data(), d_out_bits, total);
+        const std::string msg = rc == SHGPU_OK ? "" : shgpu_last_error();
+        for (void *p : {d_keys, d_tasks, d_out_keys, d_out_bits})
+            if (p) shgpu_device_free(e, p);
+        if (rc != SHGPU_OK) throw Error(rc, std::string("leaf orientation / retrieval: ") + shgpu_status_string(rc) + ": " + msg);
     }
 };
 

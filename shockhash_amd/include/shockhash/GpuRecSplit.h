@@ -33,7 +33,8 @@ class GpuRecSplit {
     RiceBitVector descriptors;
     DoubleEF ef;
     BitRetrieval ribbon;
-    std::vector<std::pair<uint64_t, uint8_t>> ribbon_input;  // kept for verification
+    std::vector<uint64_t> ribbon_keys;  // retrieval input, kept for verification when gpu::keep_retrieval_input
+    std::vector<uint8_t> ribbon_bits;
     gpu::Forest forest_meta;                                  // codes and nodes, kept for verification
     double host_ms = 0;
 
@@ -48,19 +49,21 @@ class GpuRecSplit {
                          num_threads);
     }
 
-    void construct_forest(gpu::Forest &&f, size_t n, size_t bsize, size_t num_threads) {
+    // num_threads (the reference's host parallelism) has no host phase left to split: the per-key work
+    // (sort, seed search, orientation, retrieval) runs on the device; the output never depended on it.
+    void construct_forest(gpu::Forest &&f, size_t n, size_t bsize, size_t /*num_threads*/) {
         bucket_size = bsize;
         keys_count = n;
         const auto t0 = std::chrono::steady_clock::now();
         nbuckets = f.nbuckets;
         std::vector<uint64_t> words, pos_acc;
-        encodeForest(f, memo, BIPARTITE ? LeafKind::ShockHash2 : LeafKind::ShockHash1, num_threads, words, pos_acc,
-                     ribbon_input);
+        encodeForest(f, memo, words, pos_acc);
         descriptors = RiceBitVector(std::move(words));
         ef = DoubleEF(f.size_acc, pos_acc);
         uint64_t rseed = 0, rseg = 0;
         std::vector<uint64_t> rwords;
-        gpu::Engine::instance().retrieval(ribbon_input, rseed, rseg, rwords);
+        gpu::Engine::instance().orientation_retrieval(VARIANT, f, gpu::keep_retrieval_input, ribbon_keys, ribbon_bits, rseed,
+                                                      rseg, rwords);
         ribbon = BitRetrieval(rseed, rseg, std::move(rwords));
         f.keys.clear();
         f.keys.shrink_to_fit();
@@ -139,7 +142,8 @@ class GpuRecSplit {
 
     // Verification accessors (not in the reference interface).
     const std::vector<uint64_t> &descriptorWords() const { return descriptors.words(); }
-    const std::vector<std::pair<uint64_t, uint8_t>> &ribbonInput() const { return ribbon_input; }
+    const std::vector<uint64_t> &ribbonKeys() const { return ribbon_keys; }
+    const std::vector<uint8_t> &ribbonBits() const { return ribbon_bits; }
     const gpu::Forest &forest() const { return forest_meta; }
     size_t bucketCount() const { return nbuckets; }
     double hostEncodeMs() const { return host_ms; }

@@ -5,7 +5,6 @@
 #include <array>
 #include <cstdint>
 #include <stdexcept>
-#include <thread>
 #include <utility>
 #include <vector>
 
@@ -43,45 +42,17 @@ inline uint64_t sh2Cell(uint32_t m, uint64_t code, uint64_t key, uint64_t retrie
     return retrieved == 0 ? sh2HalfHash(m, key, s1) + m / 2 : sh2HalfHash(m, key, s2);
 }
 
-// Cuckoo insertion with entries compared by identity; returns false when a key cannot be placed.
-struct TinyCuckoo {
-    int32_t cell[128];
-    uint32_t xr[128];
-    uint32_t m;
-    explicit TinyCuckoo(uint32_t m_) : m(m_) { std::fill(cell, cell + m, -1); }
-    bool insert(int32_t key, uint32_t c1, uint32_t c2) {
-        int32_t entry = key;
-        xr[key] = c1 ^ c2;
-        if (cell[c1] < 0) { cell[c1] = entry; return true; }
-        if (cell[c2] < 0) { cell[c2] = entry; return true; }
-        uint32_t cur = c1;
-        do {
-            const uint32_t alt = xr[entry] ^ cur;
-            std::swap(entry, cell[alt]);
-            if (entry < 0) return true;
-            cur = alt;
-        } while (cur != c1 || entry != key);
-        return false;
-    }
-};
-
-enum class LeafKind { ShockHash1, ShockHash2 };
-
 // Encodes a GPU forest: descriptors (fixed parts in DFS order then unary parts per bucket, then the
-// sentinel appendFixed(1, 1)), bucket bit positions and the retrieval input in the reference's order
-// (leaf by leaf in DFS order; ShockHash1 leaves cell by cell, ShockHash2 leaves key by key).
-// Mirrors compute_thread + hash_gen (include/ShockHash.h:557-658, include/ShockHash2.h:396-497).
+// sentinel appendFixed(1, 1)) and bucket bit positions. Mirrors compute_thread + hash_gen
+// (include/ShockHash.h:557-658, include/ShockHash2.h:396-497); the retrieval input (leaf orientations) is
+// computed on the device (gpu::Engine::orientation_retrieval).
 template <class MEMO>
-void encodeForest(const gpu::Forest &f, const MEMO &memo, LeafKind leaf_kind, size_t num_threads,
-                  std::vector<uint64_t> &descriptor_words, std::vector<uint64_t> &pos_acc,
-                  std::vector<std::pair<uint64_t, uint8_t>> &ribbon_input) {
+void encodeForest(const gpu::Forest &f, const MEMO &memo, std::vector<uint64_t> &descriptor_words,
+                  std::vector<uint64_t> &pos_acc) {
     auto golomb = [&](size_t m) { return (int)(memo[m] >> 27); };
     RiceBitVector::Builder builder;
     pos_acc.assign(f.nbuckets + 1, 0);
     std::vector<uint32_t> unary;
-    std::vector<size_t> leaves;        // node index of every leaf, DFS order
-    std::vector<uint64_t> leaf_out;    // ribbon offset of every leaf
-    uint64_t ribbon_n = 0;
     size_t i = 0;
     for (uint64_t b = 0; b < f.nbuckets; b++) {
         const uint64_t m = f.size_acc[b + 1] - f.size_acc[b];
@@ -95,11 +66,6 @@ void encodeForest(const gpu::Forest &f, const MEMO &memo, LeafKind leaf_kind, si
                 const int k = golomb(nd.m);
                 builder.appendFixed(f.codes[i], k);
                 unary.push_back((uint32_t)(f.codes[i] >> k));
-                if (nd.kind == SHGPU_NODE_LEAF) {
-                    leaves.push_back(i);
-                    leaf_out.push_back(ribbon_n);
-                    ribbon_n += nd.m;
-                }
             }
             builder.appendUnaryAll(unary);
         }
@@ -107,47 +73,6 @@ void encodeForest(const gpu::Forest &f, const MEMO &memo, LeafKind leaf_kind, si
     }
     builder.appendFixed(1, 1);  // sentinel, include/ShockHash.h:652
     descriptor_words = builder.build();
-
-    ribbon_input.assign(ribbon_n, {0, 0});
-    const size_t T = std::max<size_t>(1, num_threads);
-    std::vector<std::thread> th;
-    std::vector<int> failed(T, 0);
-    auto run = [&](size_t t) {
-        for (size_t l = t; l < leaves.size(); l += T) {
-            const shgpu_node &nd = f.nodes[leaves[l]];
-            const uint64_t *keys = f.keys.data() + nd.offset;
-            const uint32_t m = nd.m;
-            auto *out = ribbon_input.data() + leaf_out[l];
-            TinyCuckoo tc(m);
-            if (leaf_kind == LeafKind::ShockHash1) {
-                const uint64_t x = f.codes[leaves[l]] + nd.start_seed;
-                for (uint32_t k = 0; k < m; k++) {
-                    const CandidateCells c = sh1CandidateCells(keys[k], x, m);
-                    if (!tc.insert((int32_t)k, c.cell1, c.cell2)) { failed[t] = 1; break; }
-                }
-                if (failed[t]) return;
-                for (uint32_t c = 0; c < m; c++) out[c] = {keys[tc.cell[c]], (uint8_t)(c < m / 2 ? 0 : 1)};
-            } else {
-                uint64_t s1, s2;
-                unpairTriangular(f.codes[leaves[l]], s1, s2);
-                uint32_t c2[128];
-                for (uint32_t k = 0; k < m; k++) {
-                    const uint32_t c1 = (uint32_t)sh2HalfHash(m, keys[k], s1) + m / 2;
-                    c2[k] = (uint32_t)sh2HalfHash(m, keys[k], s2);
-                    if (!tc.insert((int32_t)k, c1, c2[k])) { failed[t] = 1; break; }
-                }
-                if (failed[t]) return;
-                for (uint32_t k = 0; k < m; k++) out[k] = {keys[k], (uint8_t)(tc.cell[c2[k]] == (int32_t)k ? 1 : 0)};
-            }
-        }
-    };
-    if (T == 1) run(0);
-    else {
-        for (size_t t = 0; t < T; t++) th.emplace_back(run, t);
-        for (auto &x : th) x.join();
-    }
-    for (int fl : failed)
-        if (fl) throw std::logic_error("GPU leaf seed does not admit a cuckoo placement");
 }
 
 

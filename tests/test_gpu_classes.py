@@ -25,7 +25,7 @@ def test_readme_known_answer():
 def test_class_matches_oracle(variant, leaf, n):
     log(f"class {variant} L={leaf} n={n}")
     h = workload.fingerprints(n, seed=31)
-    f = host.MPHF(variant, leaf, h, 2000, threads=4)
+    f = host.MPHF(variant, leaf, h, 2000, threads=4, keep_retrieval_input=True)
     ob = O.OracleBuild(variant, leaf, h, 2000, threads=8)
     assert np.array_equal(f.codes, ob.node_code)
     assert np.array_equal(f.descriptor_words, ob.words)

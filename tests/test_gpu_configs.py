@@ -20,7 +20,7 @@ def test_config1_shockhash30_strings_1m():
     log("config 1: generating 1M strings")
     strings = workload.random_strings(1_000_000, seed=3)
     log("  building on the GPU (class template)")
-    f = host.MPHF("sh1", 30, strings, 2000, threads=8)
+    f = host.MPHF("sh1", 30, strings, 2000, threads=8, keep_retrieval_input=True)
     log("  oracle")
     h = O.spooky128_batch(strings, threads=16)
     ob = O.OracleBuild("sh1", 30, h, 2000, threads=16)
