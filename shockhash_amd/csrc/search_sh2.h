@@ -22,9 +22,11 @@
 //    in VGPRs (lane l of block b holds entry 64 b + l), so candidate extraction and the isolated-vertex
 //    filter are ballots over registers. Keys are staged in LDS split by parity and read back as
 //    broadcasts. A leaf that needs more half-seeds or candidates (at m = 64 the median is ~100
-//    half-seeds and ~40 candidates) reports SH2_OVERFLOW and is searched again by
+//    half-seeds and ~40 candidates; ~0.14% of leaves need more than 128 candidates) reports
+//    SH2_OVERFLOW and the first j whose pairs are not all tested, and is continued by
 //  * the general path (sh2_leaf_search_general), whose tables live in the wave's global scratch area
-//    and are bounded only by SH2_MAX_HALF_SEEDS / SH2_MAX_CANDIDATES.
+//    and are bounded only by SH2_MAX_HALF_SEEDS / SH2_MAX_CANDIDATES. It rebuilds the half-seed and
+//    candidate tables and tests pairs from (resume j, 0) on (the pairs before it already failed).
 // Both write each candidate's cell indices (64 bytes) to scratch and queue the pairs that pass the
 // isolated filter in LDS in (j, i) order; 64 queued pairs are tested at a time, one pair per lane,
 // with a per-lane union-find (at most one cycle per component, UnionFind.h:58-67 semantics; the smaller
@@ -154,8 +156,9 @@ __device__ __forceinline__ void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L
 }
 
 // Returns the ShockHash2 leaf code, or SHGPU_NOT_FOUND when a scratch bound is exceeded.
+// Pairs (j, i) with j < j_start are not tested: the register path has tested them without success.
 __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys, uint32_t m, uint8_t *scratch,
-                                                            Sh2Lds &L, uint64_t t_start) {
+                                                            Sh2Lds &L, uint64_t t_start, uint32_t j_start) {
     const uint32_t lane = lane_id();
     if (m < 2) return 0;
     const Sh2Scratch g(scratch);
@@ -163,7 +166,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys
     const uint32_t FULL = H >= 32 ? 0xFFFFFFFFu : ((1u << H) - 1u);
     const bool basic = m < 10;
     const uint64_t kv = lane < m ? keys[lane] : 0;
-    uint32_t n_hs = 0, s_next = 0, n_cand = 0, jq = 1, iq = 0, qlen = 0;
+    uint32_t n_hs = 0, s_next = 0, n_cand = 0, jq = j_start > 1 ? j_start : 1, iq = 0, qlen = 0;
     for (;;) {
         // Make candidate jq available.
         while (n_cand <= jq) {
@@ -290,9 +293,9 @@ __device__ __forceinline__ uint32_t sh2_half_mask(const uint64_t *stage, uint32_
 }
 
 // Returns the ShockHash2 leaf code, or SH2_OVERFLOW when the leaf needs more half-seeds or candidates
-// than the registers hold.
+// than the registers hold; then every pair (j, i) with j < resume_j has been tested without success.
 __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, uint32_t m, uint8_t *scratch,
-                                                         Sh2Lds &L, uint64_t t_start) {
+                                                         Sh2Lds &L, uint64_t t_start, uint32_t &resume_j) {
     const uint32_t lane = lane_id();
     if (m < 2) return 0;
     const Sh2Scratch g(scratch);
@@ -313,6 +316,12 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
     Regs4<uint32_t> hsA, hsB;
     Regs2<uint64_t> isoR;
     uint32_t n_blocks = 0, s_next = 0, n_cand = 0, jq = 1, qlen = 0;
+    // Overflow happens while candidate jq is made available: the pairs of every j < jq are queued, and the
+    // queued pairs not yet tested start at the first queue entry's j.
+    auto overflow = [&]() -> uint64_t {
+        resume_j = qlen ? L.queue[0] >> 16 : jq;
+        return SH2_OVERFLOW;
+    };
 
     // Appends candidate (A, B) with the given code; false when the register table is full. The code goes to
     // the scratch table (read once, for the winner).
@@ -329,7 +338,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
         // Make candidate jq available.
         while (n_cand <= jq) {
             if (s_next == 64 * n_blocks) {
-                if (n_blocks == SH2_HS_BLOCKS) return SH2_OVERFLOW;
+                if (n_blocks == SH2_HS_BLOCKS) return overflow();
                 const uint64_t s = 64 * n_blocks + lane;
                 const uint32_t ma = sh2_half_mask(L.keysA, nA, s, H), mb = sh2_half_mask(L.keysB, nB, s, H);
                 hsA.set_if(true, n_blocks, ma);
@@ -341,7 +350,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
             const uint32_t ms_a = (uint32_t)__builtin_amdgcn_readlane((int)hsA.at(sb), (int)sl);
             const uint32_t ms_b = (uint32_t)__builtin_amdgcn_readlane((int)hsB.at(sb), (int)sl);
             if (basic) {
-                if ((ms_a | ms_b) == FULL && !add(s, s, s)) return SH2_OVERFLOW;
+                if ((ms_a | ms_b) == FULL && !add(s, s, s)) return overflow();
                 continue;
             }
             // Hits of (a, s), a < s (code s*s + a) and of (s, b), b <= s (code s*s + s + b), one ballot per
@@ -375,7 +384,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
                 else break;
 #undef SH2_TAKE
                 const bool ok = side == 0 ? add(x, s, s * s + x) : add(s, x, s * s + s + x);
-                if (!ok) return SH2_OVERFLOW;
+                if (!ok) return overflow();
             }
         }
         // Queue the pairs (jq, i), i < jq, that pass the isolated-vertex filter, 64 i at a time, and test
@@ -416,8 +425,9 @@ constexpr uint32_t SH2_FLAG_GENERAL_ONLY = 1;
 __device__ __forceinline__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32_t m, uint8_t *scratch, Sh2Lds &L,
                                                     uint32_t flags, uint64_t t_start) {
     uint64_t r = SH2_OVERFLOW;
-    if (!(flags & SH2_FLAG_GENERAL_ONLY)) r = sh2_leaf_search_regs(keys, m, scratch, L, t_start);
-    if (r == SH2_OVERFLOW) r = sh2_leaf_search_general(keys, m, scratch, L, t_start);
+    uint32_t resume_j = 1;
+    if (!(flags & SH2_FLAG_GENERAL_ONLY)) r = sh2_leaf_search_regs(keys, m, scratch, L, t_start, resume_j);
+    if (r == SH2_OVERFLOW) r = sh2_leaf_search_general(keys, m, scratch, L, t_start, uniform(resume_j));
     reset_priority();
     return r;
 }

@@ -91,6 +91,55 @@ def test_sh2_leaves_m64_many(engine):
     assert np.array_equal(fast[pick], want)
 
 
+def _remix(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+    return z ^ (z >> np.uint64(31))
+
+
+def _sh2_candidates_through(keys, code):
+    """Number of ShockHash2 candidates (QuadSplitCandidateFinder order, m = 64) up to the end of the half-seed
+    that holds the winning pair's larger candidate: how many candidates a search of this leaf tabulates."""
+    cj = int(code)
+    x = int(np.floor((1 + np.sqrt(1 + 8.0 * cj)) / 2))  # unpairTriangular: cj = x(x-1)/2 + y, y < x
+    while x * (x - 1) // 2 > cj:
+        x -= 1
+    while (x + 1) * x // 2 <= cj:
+        x += 1
+    c_j = x
+    s_star = int(np.floor(np.sqrt(c_j)))
+    S = s_star + 1
+    ev, od = keys[(keys & np.uint64(1)) == 0], keys[(keys & np.uint64(1)) == 1]
+    s = np.arange(S, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        ma = np.bitwise_or.reduce(np.uint64(1) << (_remix(ev[None, :] + s[:, None]) >> np.uint64(59)), axis=1)
+        mb = np.bitwise_or.reduce(np.uint64(1) << (_remix(od[None, :] + s[:, None]) >> np.uint64(59)), axis=1)
+    ok = (ma[:, None] | mb[None, :]) == np.uint64(0xFFFFFFFF)  # ok[a, b]: (a, b) covers both halves
+    return int(np.triu(ok).sum() + np.tril(ok, -1).sum())
+
+
+def test_sh2_leaves_past_register_tables(engine):
+    """m = 64 leaves that need more than the register path's 128 candidates: the register path reports them
+    with the first untested j and the scratch path continues from there. Picked on the CPU from 30000 random
+    leaves (about 0.14% of leaves), searched together with ordinary leaves, compared with the oracle."""
+    log("sh2 leaves past the register tables")
+    n = 30000
+    keys = workload.fingerprints(n * 64, seed=31)[:, 1].reshape(n, 64).copy()
+    sizes = np.full(n, 64, dtype=np.uint32)
+    offsets = (np.arange(n) * 64).astype(np.uint32)
+    codes = O.sh2_leaf_batch(keys.reshape(-1), offsets, sizes, threads=8)
+    top = np.argsort(codes)[-300:]
+    big = [int(l) for l in top if _sh2_candidates_through(keys[l], codes[l]) > 128]
+    log(f"  {len(big)} leaves need more than 128 candidates")
+    assert len(big) >= 10
+    pick = np.array(sorted(set(big) | set(range(0, n, 97))), dtype=np.int64)
+    sub = keys[pick].reshape(-1).copy()
+    got = engine.sh2_leaf_search(sub, (np.arange(len(pick)) * 64).astype(np.uint32),
+                                 np.full(len(pick), 64, dtype=np.uint32))
+    bad = np.nonzero(got != codes[pick])[0]
+    assert len(bad) == 0, f"mismatch at leaves {pick[bad][:10]} got {got[bad][:5]} want {codes[pick][bad][:5]}"
+
+
 @pytest.mark.parametrize("variant,leaf", [(0, 64), (1, 58), (0, 30), (0, 16), (1, 16), (1, 24)])
 def test_split_search(engine, variant, leaf):
     """Split nodes of every kind: seeds and stable partitions equal the oracle's. The sizes include
