@@ -124,8 +124,12 @@ typedef struct shgpu_build_desc {
     uint64_t bucket_size;  /* constructor bucket_size */
     int mode;              /* SHGPU_MODE_PERSISTENT or SHGPU_MODE_PLAIN */
     uint32_t plain_bucket_limit; /* plain mode: build only the first N buckets (0 = all) */
-    int copy_keys;         /* keep a host copy of the keys in final leaf order */
+    int copy_keys;         /* SHGPU_KEYS_HOST: keep a host copy of the keys in final leaf order;
+                              SHGPU_KEYS_DEVICE: keep a device copy (shgpu_build_device_keys); or both */
 } shgpu_build_desc;
+
+#define SHGPU_KEYS_HOST 1
+#define SHGPU_KEYS_DEVICE 2
 
 int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
                     shgpu_build **out);
@@ -144,8 +148,12 @@ uint64_t shgpu_build_nnodes(const shgpu_build *b);
 int shgpu_build_bucket_sizes(const shgpu_build *b, uint64_t *size_acc);
 /* nodes (may be NULL) and codes (code = found seed - start_seed), nnodes each, DFS pre-order. */
 int shgpu_build_nodes(const shgpu_build *b, shgpu_node *nodes, uint64_t *codes);
-/* Keys (fingerprint .second) in final leaf order, n of them (needs desc->copy_keys). */
+/* Keys (fingerprint .second) in final leaf order, n of them (needs SHGPU_KEYS_HOST in desc->copy_keys). */
 int shgpu_build_keys(const shgpu_build *b, uint64_t *keys);
+/* The same keys in device memory owned by the build, valid until shgpu_build_free (needs SHGPU_KEYS_DEVICE).
+ * The input of shgpu_leaf_orientation, which the reference fills on the host (leafKeys, ShockHash2.h:273;
+ * ribbonInput, ShockHash.h:425-432): the classes hand it over without a round trip through host memory. */
+int shgpu_build_device_keys(const shgpu_build *b, const uint64_t **d_keys);
 /* Host-only: the forest shgpu_build_run would search for these bucket sizes (DFS pre-order). Call with
  * out = NULL to get the count. Needs no GPU. */
 /* One-bit retrieval structure of the leaf orientations, built on the device. Replaces the construction of the

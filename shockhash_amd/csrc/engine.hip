@@ -172,6 +172,15 @@ struct shgpu_build {
     mutable std::mutex nodes_mu;
     mutable bool nodes_ready = false;
     mutable std::vector<shgpu_node> nodes;
+    // SHGPU_KEYS_DEVICE: the leaf-ordered keys in device memory of its own (freed with the build)
+    void *d_keys = nullptr;
+    int device = 0;
+    shgpu_build() = default;
+    shgpu_build(const shgpu_build &) = delete;
+    shgpu_build &operator=(const shgpu_build &) = delete;
+    ~shgpu_build() {
+        if (d_keys && hipSetDevice(device) == hipSuccess) (void)hipFree(d_keys);
+    }
 };
 
 // ===================================================================================== kernels
@@ -1498,9 +1507,21 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         }
     }
     std::vector<uint64_t> keys_all;
-    if (d->copy_keys && N) {
+    if ((d->copy_keys & SHGPU_KEYS_HOST) && N) {
         keys_all.resize(N);
         BATCH_HIP(hipMemcpyAsync(keys_all.data(), B.keys.p, N * 8, hipMemcpyDeviceToHost, s));
+    }
+    if (d->copy_keys & SHGPU_KEYS_DEVICE) {
+        for (size_t r = 0; r < count; r++) {
+            if (status[r] != SHGPU_OK) continue;
+            shgpu_build &b = *bs[r];
+            b.device = e->device;
+            if (hipMalloc(&b.d_keys, b.n * 8 + 8) != hipSuccess) {
+                b.d_keys = nullptr;
+                return fail_all(fail(SHGPU_ERR_NOMEM, "hipMalloc of the build's device keys failed"));
+            }
+            if (b.n) BATCH_HIP(hipMemcpyAsync(b.d_keys, B.keys.as<uint64_t>() + base[r], b.n * 8, hipMemcpyDeviceToDevice, s));
+        }
     }
     BATCH_HIP(hipStreamSynchronize(s));
 #undef BATCH_HIP
@@ -1727,6 +1748,13 @@ int shgpu_build_keys(const shgpu_build *b, uint64_t *keys) {
     if (!b || !keys) return fail(SHGPU_ERR_ARG, "NULL argument");
     if (b->keys.size() != b->n) return fail(SHGPU_ERR_ARG, "build was run without copy_keys");
     std::memcpy(keys, b->keys.data(), b->n * 8);
+    return SHGPU_OK;
+}
+
+int shgpu_build_device_keys(const shgpu_build *b, const uint64_t **d_keys) {
+    if (!b || !d_keys) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (!b->d_keys) return fail(SHGPU_ERR_ARG, "build was run without SHGPU_KEYS_DEVICE");
+    *d_keys = (const uint64_t *)b->d_keys;
     return SHGPU_OK;
 }
 

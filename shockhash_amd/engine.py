@@ -47,6 +47,9 @@ class Tuning(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("max_trials", ctypes.c_uint64)]
 
 
+KEYS_HOST, KEYS_DEVICE = 1, 2  # shgpu_build_desc.copy_keys bits
+
+
 class BuildDesc(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("leaf_size", ctypes.c_uint32), ("bucket_size", ctypes.c_uint64),
                 ("mode", ctypes.c_int), ("plain_bucket_limit", ctypes.c_uint32), ("copy_keys", ctypes.c_int)]
@@ -98,6 +101,7 @@ _SIGS = {
     "shgpu_build_bucket_sizes": (ctypes.c_int, [P, P]),
     "shgpu_build_nodes": (ctypes.c_int, [P, P, P]),
     "shgpu_build_keys": (ctypes.c_int, [P, P]),
+    "shgpu_build_device_keys": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "shgpu_build_timings": (ctypes.c_int, [P, P]),
     "shgpu_device_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
     "shgpu_device_free": (ctypes.c_int, [P, P]),
@@ -238,6 +242,15 @@ class Build:
     def keys(self, n):
         out = np.zeros(n, dtype=np.uint64)
         _check(load().shgpu_build_keys(self.h, out.ctypes.data_as(P)))
+        return out
+
+    def device_keys(self, engine, n):
+        """The build's device copy of the leaf-ordered keys (copy_keys with KEYS_DEVICE), downloaded."""
+        p = P()
+        _check(load().shgpu_build_device_keys(self.h, ctypes.byref(p)))
+        out = np.zeros(n, dtype=np.uint64)
+        if n:
+            _check(load().shgpu_copy_to_host(engine.h, out.ctypes.data_as(P), p, n * 8))
         return out
 
     def __del__(self):

@@ -3,6 +3,7 @@
 #pragma once
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -36,8 +37,11 @@ struct Forest {
     std::vector<uint64_t> size_acc;  // nbuckets + 1
     std::vector<shgpu_node> nodes;   // DFS pre-order per bucket, buckets in order
     std::vector<uint64_t> codes;     // per node: found seed - start seed (ShockHash2 leaves: pair code)
-    std::vector<uint64_t> keys;      // fingerprint .second in final leaf order
+    std::vector<uint64_t> keys;      // fingerprint .second in final leaf order (host copy, when asked for)
     double ms[4] = {0, 0, 0, 0};     // sort + bucketing, tree setup, seed search, search kernel
+    // The engine's build result while it holds the same keys in device memory (the orientation pass reads
+    // them there); released once the retrieval structure is built.
+    std::shared_ptr<shgpu_build> device;
 };
 
 class Engine {
@@ -101,31 +105,30 @@ class Engine {
         d.leaf_size = leaf_size;
         d.bucket_size = bucket_size;
         d.mode = SHGPU_MODE_PERSISTENT;
-        d.copy_keys = 1;
+        d.copy_keys = SHGPU_KEYS_DEVICE;  // the keys stay on the device for the orientation pass
         return d;
     }
 
-    Forest collect(shgpu_build *b, uint64_t n) {
+    Forest collect(shgpu_build *b, uint64_t /*n*/) {
         Forest f;
+        f.device = std::shared_ptr<shgpu_build>(b, shgpu_build_free);
         f.nbuckets = shgpu_build_nbuckets(b);
         f.size_acc.resize(f.nbuckets + 1);
         const uint64_t nn = shgpu_build_nnodes(b);
         f.nodes.resize(nn);
         f.codes.resize(nn);
-        f.keys.resize(n);
         int rc = shgpu_build_bucket_sizes(b, f.size_acc.data());
         if (rc == SHGPU_OK) rc = shgpu_build_nodes(b, f.nodes.data(), f.codes.data());
-        if (rc == SHGPU_OK && n) rc = shgpu_build_keys(b, f.keys.data());
         if (rc == SHGPU_OK) rc = shgpu_build_timings(b, f.ms);
-        shgpu_build_free(b);
         check(rc, "shgpu_build results");
         return f;
     }
 
   public:
     // The retrieval input of a forest (every leaf's orientation, in the reference's order) and its one-bit
-    // retrieval structure, both computed on the device (shgpu_leaf_orientation, shgpu_retrieval_build): the
-    // leaf-ordered keys go up once, the structure comes back, and the input too when `keep` is set.
+    // retrieval structure, both computed on the device (shgpu_leaf_orientation, shgpu_retrieval_build) from
+    // the build's device-resident leaf-ordered keys (or from f.keys, uploaded, when the forest holds no
+    // device build); the structure comes back, and the input too when `keep` is set.
     void orientation_retrieval(int variant, const Forest &f, bool keep, std::vector<uint64_t> &input_keys,
                                std::vector<uint8_t> &input_bits, uint64_t &seed, uint64_t &seg,
                                std::vector<uint64_t> &words) {
@@ -138,18 +141,25 @@ class Engine {
             tasks.push_back({total, variant == SHGPU_SHOCKHASH2 ? f.codes[i] : f.codes[i] + nd.start_seed, nd.offset, nd.m});
             total += nd.m;
         }
-        const uint64_t n = f.keys.size();
         void *d_keys = nullptr, *d_tasks = nullptr, *d_out_keys = nullptr, *d_out_bits = nullptr;
-        int rc = shgpu_device_alloc(e, n * 8 + 8, &d_keys);
+        const uint64_t *leaf_keys = nullptr;
+        int rc = SHGPU_OK;
+        if (f.device) {
+            rc = shgpu_build_device_keys(f.device.get(), &leaf_keys);
+        } else {
+            const uint64_t n = f.keys.size();
+            rc = shgpu_device_alloc(e, n * 8 + 8, &d_keys);
+            if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, d_keys, f.keys.data(), n * 8);
+            leaf_keys = (const uint64_t *)d_keys;
+        }
         if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, tasks.size() * sizeof(shgpu_orient_task) + 8, &d_tasks);
         if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, total * 8 + 8, &d_out_keys);
         if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, total + 8, &d_out_bits);
-        if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, d_keys, f.keys.data(), n * 8);
         if (rc == SHGPU_OK && !tasks.empty())
             rc = shgpu_copy_to_device(e, d_tasks, tasks.data(), tasks.size() * sizeof(shgpu_orient_task));
         if (rc == SHGPU_OK)
-            rc = shgpu_leaf_orientation(e, variant, (const uint64_t *)d_keys, (const shgpu_orient_task *)d_tasks,
-                                        tasks.size(), (uint64_t *)d_out_keys, (uint8_t *)d_out_bits);
+            rc = shgpu_leaf_orientation(e, variant, leaf_keys, (const shgpu_orient_task *)d_tasks, tasks.size(),
+                                        (uint64_t *)d_out_keys, (uint8_t *)d_out_bits);
         words.assign(shgpu_retrieval_words(total), 0);
         if (rc == SHGPU_OK)
             rc = shgpu_retrieval_build(e, (const uint64_t *)d_out_keys, (const uint8_t *)d_out_bits, total, &seed, &seg,

@@ -65,6 +65,7 @@ class GpuRecSplit {
         gpu::Engine::instance().orientation_retrieval(VARIANT, f, gpu::keep_retrieval_input, ribbon_keys, ribbon_bits, rseed,
                                                       rseg, rwords);
         ribbon = BitRetrieval(rseed, rseg, std::move(rwords));
+        f.device.reset();  // frees the build's device keys
         f.keys.clear();
         f.keys.shrink_to_fit();
         forest_meta = std::move(f);

@@ -4,7 +4,7 @@ import pytest
 
 import oracle_lib as O
 from shockhash_amd import workload
-from shockhash_amd.engine import SPLIT_DTYPE, DeviceArray
+from shockhash_amd.engine import KEYS_DEVICE, KEYS_HOST, SPLIT_DTYPE, DeviceArray
 
 pytestmark = pytest.mark.gpu
 
@@ -191,6 +191,21 @@ def test_build_matches_oracle(engine, variant, leaf, n, mode):
     assert np.array_equal(b.nodes["m"], ob.node_m)
     assert np.array_equal(b.codes, ob.node_code)
     log(f"  ok, timings {b.timings_ms}")
+
+
+@pytest.mark.parametrize("mode", ["persistent", "plain"])
+def test_build_device_keys(engine, mode):
+    """SHGPU_KEYS_DEVICE: the build keeps its leaf-ordered keys in device memory of its own (what the classes
+    hand to the orientation pass); they equal the host copy, and a later build does not overwrite them."""
+    log(f"build device keys, mode {mode}")
+    n = 30000
+    h = workload.fingerprints(n, seed=22)
+    b = engine.build(h, "sh2", 64, 2000, mode=mode, copy_keys=KEYS_HOST | KEYS_DEVICE)
+    host = b.keys(n)
+    assert np.array_equal(np.sort(host), np.sort(h[:, 1]))
+    other = engine.build(workload.fingerprints(n, seed=23), "sh2", 64, 2000, mode=mode, copy_keys=KEYS_DEVICE)
+    assert np.array_equal(b.device_keys(engine, n), host)
+    assert np.array_equal(np.sort(other.device_keys(engine, n)), np.sort(workload.fingerprints(n, seed=23)[:, 1]))
 
 
 @pytest.mark.parametrize("flags", [0, 2])
