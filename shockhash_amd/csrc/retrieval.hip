@@ -68,16 +68,52 @@ __global__ void k_rt_degrees(const uint64_t *__restrict__ keys, uint64_t n, uint
     }
 }
 
-// counters: [0] frontier length of the current round, [1] next frontier length, [2] keys peeled
-__global__ void k_rt_frontier0(const uint32_t *deg, uint64_t ncell, uint32_t *frontier, uint32_t *counters) {
-    GRID_STRIDE(v, ncell) {
-        if (deg[v] == 1) frontier[atomicAdd(&counters[0], 1u)] = (uint32_t)v;
+// Appends to a global list (out, length *len) staged per block in LDS: a block reserves its range with one
+// global atomic when it ends, instead of one per wave and append. Appends to one counter serialise in the
+// memory system (about 10 ns each), which made the frontier and peeled lists of a 10M-key build cost
+// milliseconds. Entries past the stage's capacity go to the list directly. Every thread of the block must
+// call flush.
+constexpr uint32_t STAGE_CAP = 4096;
+struct BlockAppend {
+    uint32_t *buf, *n, *base;  // LDS: staged entries, their count, the reserved global position
+    uint32_t *out, *len;
+    __device__ void init() {
+        if (threadIdx.x == 0) *n = 0;
+        __syncthreads();
     }
+    __device__ void push(uint32_t v) {
+        const uint32_t slot = atomicAdd(n, 1u);
+        if (slot < STAGE_CAP) buf[slot] = v;
+        else out[atomicAdd(len, 1u)] = v;
+    }
+    __device__ void flush() {
+        __syncthreads();
+        const uint32_t cnt = *n < STAGE_CAP ? *n : STAGE_CAP;
+        if (threadIdx.x == 0) *base = cnt ? atomicAdd(len, cnt) : 0u;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) out[*base + j] = buf[j];
+    }
+};
+#define BLOCK_APPEND(name, out_, len_)                                   \
+    __shared__ uint32_t name##_buf[STAGE_CAP], name##_n, name##_base;    \
+    BlockAppend name{name##_buf, &name##_n, &name##_base, (out_), (len_)}; \
+    name.init()
+
+// counters: [0] frontier length of the current round, [1] next frontier length, [2] keys peeled
+__global__ void __launch_bounds__(RT_BLOCK) k_rt_frontier0(const uint32_t *deg, uint64_t ncell, uint32_t *frontier,
+                                                           uint32_t *counters) {
+    BLOCK_APPEND(app, frontier, &counters[0]);
+    GRID_STRIDE(v, ncell) {
+        if (deg[v] == 1) app.push((uint32_t)v);
+    }
+    app.flush();
 }
 
 // Round r, claims: a frontier cell still of degree 1 claims its only key.
-__global__ void k_rt_claim(const uint32_t *frontier, uint32_t *counters, const uint32_t *deg, const uint32_t *xr,
-                           uint32_t *round_of, uint32_t *assigned, uint32_t *peeled, uint32_t r) {
+__global__ void __launch_bounds__(RT_BLOCK) k_rt_claim(const uint32_t *frontier, uint32_t *counters, const uint32_t *deg,
+                                                       const uint32_t *xr, uint32_t *round_of, uint32_t *assigned,
+                                                       uint32_t *peeled, uint32_t r) {
+    BLOCK_APPEND(app, peeled, &counters[2]);
     const uint32_t len = counters[0];
     GRID_STRIDE(f, len) {
         const uint32_t v = frontier[f];
@@ -85,15 +121,18 @@ __global__ void k_rt_claim(const uint32_t *frontier, uint32_t *counters, const u
         const uint32_t i = xr[v];
         if (atomicCAS(&round_of[i], UNPEELED, r) == UNPEELED) {
             assigned[i] = v;
-            peeled[atomicAdd(&counters[2], 1u)] = i;
+            app.push(i);
         }
     }
+    app.flush();
 }
 
 // Round r, removals: the keys claimed in round r (peeled[begin, counters[2])) leave their cells.
-__global__ void k_rt_remove(const uint64_t *__restrict__ keys, uint64_t seed, uint64_t seg, const uint32_t *peeled,
-                            const uint32_t *round_begin, uint32_t r, const uint32_t *counters, uint32_t *deg, uint32_t *xr,
-                            uint32_t *next_frontier, uint32_t *next_len) {
+__global__ void __launch_bounds__(RT_BLOCK) k_rt_remove(const uint64_t *__restrict__ keys, uint64_t seed, uint64_t seg,
+                                                        const uint32_t *peeled, const uint32_t *round_begin, uint32_t r,
+                                                        const uint32_t *counters, uint32_t *deg, uint32_t *xr,
+                                                        uint32_t *next_frontier, uint32_t *next_len) {
+    BLOCK_APPEND(app, next_frontier, next_len);
     const uint32_t begin = round_begin[r], end = counters[2];
     GRID_STRIDE(f, (uint64_t)(end - begin)) {
         const uint32_t i = peeled[begin + f];
@@ -101,9 +140,10 @@ __global__ void k_rt_remove(const uint64_t *__restrict__ keys, uint64_t seed, ui
         rt_cells(keys[i], seed, seg, c);
         for (int k = 0; k < 3; k++) {
             atomicXor(&xr[c[k]], i);
-            if (atomicSub(&deg[c[k]], 1u) == 2u) next_frontier[atomicAdd(next_len, 1u)] = (uint32_t)c[k];
+            if (atomicSub(&deg[c[k]], 1u) == 2u) app.push((uint32_t)c[k]);
         }
     }
+    app.flush();
 }
 
 // Between rounds (one thread): the next frontier becomes the current one, the next round starts where the
@@ -159,21 +199,31 @@ __device__ __forceinline__ void unpair_elegant(uint64_t z, uint64_t &x, uint64_t
     if (r < f) { x = r; y = f; } else { x = f; y = r - f; }
 }
 
-// Cuckoo insertion of key k with cells c1, c2 into cell[] (-1 = empty), entries identified by key index; the
-// host's TinyCuckoo (and TinyBinaryCuckooHashTable::insert, TinyBinaryCuckooHashTable.h:169-198) step for
-// step. False when the key cannot be placed.
-__device__ bool orient_insert(int8_t *cell, uint8_t *xr, int32_t k, uint32_t c1, uint32_t c2) {
-    int32_t entry = k;
+// Per-thread tables of the orientation replay in LDS, one byte column per thread: byte x of thread t at
+// x * ORIENT_BLOCK + t. Cells and key indices of a leaf are below 64.
+constexpr uint32_t ORIENT_BLOCK = 256;
+constexpr uint8_t CELL_EMPTY = 0xFF;
+
+struct OrientCol {
+    uint8_t *p;
+    __device__ uint8_t &operator[](uint32_t x) const { return p[x * ORIENT_BLOCK]; }
+};
+
+// Cuckoo insertion of key k with cells c1, c2 into cell[] (CELL_EMPTY = free), entries identified by key
+// index; the steps of TinyBinaryCuckooHashTable::insert (TinyBinaryCuckooHashTable.h:169-198). False when
+// the key cannot be placed.
+__device__ bool orient_insert(OrientCol cell, OrientCol xr, uint32_t k, uint32_t c1, uint32_t c2) {
+    uint32_t entry = k;
     xr[k] = (uint8_t)(c1 ^ c2);
-    if (cell[c1] < 0) { cell[c1] = (int8_t)entry; return true; }
-    if (cell[c2] < 0) { cell[c2] = (int8_t)entry; return true; }
+    if (cell[c1] == CELL_EMPTY) { cell[c1] = (uint8_t)entry; return true; }
+    if (cell[c2] == CELL_EMPTY) { cell[c2] = (uint8_t)entry; return true; }
     uint32_t cur = c1;
     for (uint32_t steps = 0; steps < 4096; steps++) {
         const uint32_t alt = xr[entry] ^ cur;
-        const int32_t evicted = cell[alt];
-        cell[alt] = (int8_t)entry;
+        const uint32_t evicted = cell[alt];
+        cell[alt] = (uint8_t)entry;
         entry = evicted;
-        if (entry < 0) return true;
+        if (entry == CELL_EMPTY) return true;
         cur = alt;
         if (cur == c1 && entry == k) return false;
     }
@@ -184,25 +234,29 @@ __device__ bool orient_insert(int8_t *cell, uint8_t *xr, int32_t k, uint32_t c1,
 // out_offset. ShockHash1 / SIMDShockHash (BIPARTITE false): cells c1 < m/2 <= c2 of remix(key + seed) with
 // seed = code + start seed, entries in cell order with bit = (cell >= m/2) (ShockHash.h:425-432,
 // SIMDShockHash.hpp:536-540). ShockHash2: cells sh2(s1) + m/2 and sh2(s2) of the pair code's seeds, entries in
-// key order with bit = (the key sits in its second cell) (ShockHash2-internal.h:548-572).
+// key order with bit = (the key sits in its second cell) (ShockHash2-internal.h:548-572). The cell,
+// eviction-XOR and second-cell tables (64 bytes each) live in LDS: the replay is a chain of dependent
+// table reads.
 template <bool BIPARTITE>
-__global__ void k_leaf_orientation(const uint64_t *__restrict__ keys, const shgpu_orient_task *__restrict__ tasks, uint64_t n,
-                                   uint64_t *out_keys, uint8_t *out_bits, uint32_t *error) {
+__global__ void __launch_bounds__(ORIENT_BLOCK) k_leaf_orientation(const uint64_t *__restrict__ keys,
+                                                                   const shgpu_orient_task *__restrict__ tasks, uint64_t n,
+                                                                   uint64_t *out_keys, uint8_t *out_bits, uint32_t *error) {
+    __shared__ uint8_t lds[3 * 64 * ORIENT_BLOCK];
+    const OrientCol cell{lds + threadIdx.x}, xr{lds + 64 * ORIENT_BLOCK + threadIdx.x},
+        c2s{lds + 2 * 64 * ORIENT_BLOCK + threadIdx.x};
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
         const shgpu_orient_task tk = tasks[t];
         const uint64_t *lk = keys + tk.key_offset;
         const uint32_t m = tk.m;
         if (m < 1 || m > 64) { atomicOr(error, 1u); continue; }
-        int8_t cell[128];
-        uint8_t xr[64], c2s[64];
-        for (uint32_t c = 0; c < m; c++) cell[c] = -1;
+        for (uint32_t c = 0; c < m; c++) cell[c] = CELL_EMPTY;
         bool ok = true;
         if (!BIPARTITE) {
             for (uint32_t k = 0; k < m && ok; k++) {
                 const uint64_t r = shgpu::remix(lk[k] + tk.seed);
                 const uint32_t c1 = (uint32_t)(((uint64_t)(uint32_t)r * (m / 2)) >> 32);
                 const uint32_t c2 = (uint32_t)(((r >> 32) * ((m + 1) / 2)) >> 32) + m / 2;
-                ok = orient_insert(cell, xr, (int32_t)k, c1, c2);
+                ok = orient_insert(cell, xr, k, c1, c2);
             }
             if (!ok) { atomicOr(error, 2u); continue; }
             for (uint32_t c = 0; c < m; c++) {
@@ -224,12 +278,12 @@ __global__ void k_leaf_orientation(const uint64_t *__restrict__ keys, const shgp
                 const uint32_t c1 = (uint32_t)__umul64hi(shgpu::remix(key + (even ? a1 : b1)), H) + m / 2;
                 const uint32_t c2 = (uint32_t)__umul64hi(shgpu::remix(key + (even ? a2 : b2)), H);
                 c2s[k] = (uint8_t)c2;
-                ok = orient_insert(cell, xr, (int32_t)k, c1, c2);
+                ok = orient_insert(cell, xr, k, c1, c2);
             }
             if (!ok) { atomicOr(error, 2u); continue; }
             for (uint32_t k = 0; k < m; k++) {
                 out_keys[tk.out_offset + k] = lk[k];
-                out_bits[tk.out_offset + k] = cell[c2s[k]] == (int8_t)k ? 1 : 0;
+                out_bits[tk.out_offset + k] = cell[c2s[k]] == k ? 1 : 0;
             }
         }
     }
@@ -323,8 +377,9 @@ int shgpu_leaf_orientation_impl(hipStream_t s, int bipartite, const uint64_t *d_
     if (hipMalloc(&flag.p, 64) != hipSuccess) { err = "orientation: hipMalloc failed"; return SHGPU_ERR_NOMEM; }
     uint32_t *d_err = (uint32_t *)flag.p;
     k_zero_u32<<<1, 1, 0, s>>>(d_err);
-    if (bipartite) k_leaf_orientation<true><<<1024, 128, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
-    else k_leaf_orientation<false><<<1024, 128, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + ORIENT_BLOCK - 1) / ORIENT_BLOCK, 4096);
+    if (bipartite) k_leaf_orientation<true><<<grid, ORIENT_BLOCK, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
+    else k_leaf_orientation<false><<<grid, ORIENT_BLOCK, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
     uint32_t h = 0;
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&h, d_err, 4, hipMemcpyDeviceToHost, s);

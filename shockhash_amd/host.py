@@ -25,6 +25,7 @@ _SIGS = {
     "shh_codes": (None, [P, P]),
     "shh_bits": (ctypes.c_size_t, [P]),
     "shh_timings": (None, [P, P]),
+    "shh_phase_timings": (None, [P, P]),
     "shh_keep_retrieval_input": (None, [ctypes.c_int]),
     "shh_memo": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P]),
 }
@@ -130,7 +131,10 @@ class MPHF:
     def timings_ms(self):
         t = np.zeros(4, dtype=np.float64)
         load().shh_timings(self.h, t.ctypes.data_as(P))
-        return {"sort": t[0], "tree": t[1], "search": t[2], "host_encode": t[3]}
+        p = np.zeros(4, dtype=np.float64)
+        load().shh_phase_timings(self.h, p.ctypes.data_as(P))
+        return {"sort": t[0], "tree": t[1], "search": t[2], "host_encode": t[3], "rice_ef": p[0],
+                "orientation": p[1], "retrieval": p[2], "constructor": p[3]}
 
     def close(self):
         if getattr(self, "h", None):

@@ -1,6 +1,7 @@
 // libshockhash_host.so: extern "C" handles on instantiations of the drop-in class templates
 // (ShockHash, SIMDShockHash, ShockHash2), used by the Python tests and the benchmark.
 #include <cstdio>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -24,6 +25,8 @@ struct Handle {
     virtual const gpu::Forest &forest() const = 0;
     virtual size_t bits() const = 0;
     virtual double host_ms() const = 0;
+    virtual const double *host_phases() const = 0;
+    double ctor_ms = 0;  // wall time of the class constructor
 };
 
 template <class T>
@@ -38,6 +41,7 @@ struct HandleT : Handle {
     const gpu::Forest &forest() const override { return obj->forest(); }
     size_t bits() const override { return obj->getBits(); }
     double host_ms() const override { return obj->hostEncodeMs(); }
+    const double *host_phases() const override { return obj->hostPhaseMs(); }
 };
 
 template <class T>
@@ -97,7 +101,10 @@ void *shh_build(int variant, int leaf, const uint64_t *hashes, size_t n, size_t 
     try {
         std::vector<hash128_t> h(n);
         std::memcpy(h.data(), hashes, n * 16);
-        return dispatch<std::vector<hash128_t>, ByHash, ByHash, ByHash>(variant, leaf, h, bucket, (size_t)threads);
+        const auto t0 = std::chrono::steady_clock::now();
+        Handle *hd = dispatch<std::vector<hash128_t>, ByHash, ByHash, ByHash>(variant, leaf, h, bucket, (size_t)threads);
+        hd->ctor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return hd;
     } catch (const std::exception &e) {
         set_err(err, errlen, e.what());
         return nullptr;
@@ -109,7 +116,10 @@ void *shh_build_strings(int variant, int leaf, const char *data, const uint64_t 
     try {
         std::vector<std::string> s(n);
         for (size_t i = 0; i < n; i++) s[i].assign(data + offsets[i], offsets[i + 1] - offsets[i]);
-        return dispatch<std::vector<std::string>, ByString, ByString, ByString>(variant, leaf, s, bucket, (size_t)threads);
+        const auto t0 = std::chrono::steady_clock::now();
+        Handle *hd = dispatch<std::vector<std::string>, ByString, ByString, ByString>(variant, leaf, s, bucket, (size_t)threads);
+        hd->ctor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return hd;
     } catch (const std::exception &e) {
         set_err(err, errlen, e.what());
         return nullptr;
@@ -148,6 +158,13 @@ void shh_codes(void *h, uint64_t *out) {
     std::memcpy(out, c.data(), c.size() * 8);
 }
 size_t shh_bits(void *h) { return ((Handle *)h)->bits(); }
+// After the search: Golomb-Rice + Elias-Fano encoding, leaf orientation and retrieval construction; then the
+// whole constructor (ms).
+void shh_phase_timings(void *h, double *out4) {
+    const double *p = ((Handle *)h)->host_phases();
+    out4[0] = p[0]; out4[1] = p[1]; out4[2] = p[2];
+    out4[3] = ((Handle *)h)->ctor_ms;
+}
 void shh_timings(void *h, double *out4) {
     const auto &f = ((Handle *)h)->forest();
     out4[0] = f.ms[0]; out4[1] = f.ms[1]; out4[2] = f.ms[2];

@@ -1,6 +1,7 @@
 // Host-side handle of the MI355X seed-search engine (libshockhash_amd.so, C ABI in
 // include/shockhash_amd.h). One engine per process and device, created on first use.
 #pragma once
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -131,8 +132,9 @@ class Engine {
     // device build); the structure comes back, and the input too when `keep` is set.
     void orientation_retrieval(int variant, const Forest &f, bool keep, std::vector<uint64_t> &input_keys,
                                std::vector<uint8_t> &input_bits, uint64_t &seed, uint64_t &seg,
-                               std::vector<uint64_t> &words) {
+                               std::vector<uint64_t> &words, double *ms2 = nullptr) {
         std::lock_guard<std::mutex> lock(mtx);
+        const auto t0 = std::chrono::steady_clock::now();
         std::vector<shgpu_orient_task> tasks;
         uint64_t total = 0;
         for (size_t i = 0; i < f.nodes.size(); i++) {
@@ -160,6 +162,7 @@ class Engine {
         if (rc == SHGPU_OK)
             rc = shgpu_leaf_orientation(e, variant, leaf_keys, (const shgpu_orient_task *)d_tasks, tasks.size(),
                                         (uint64_t *)d_out_keys, (uint8_t *)d_out_bits);
+        const auto t1 = std::chrono::steady_clock::now();
         words.assign(shgpu_retrieval_words(total), 0);
         if (rc == SHGPU_OK)
             rc = shgpu_retrieval_build(e, (const uint64_t *)d_out_keys, (const uint8_t *)d_out_bits, total, &seed, &seg,
@@ -171,6 +174,10 @@ class Engine {
         const std::string msg = rc == SHGPU_OK ? "" : shgpu_last_error();
         for (void *p : {d_keys, d_tasks, d_out_keys, d_out_bits})
             if (p) shgpu_device_free(e, p);
+        if (ms2) {
+            ms2[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            ms2[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+        }
         if (rc != SHGPU_OK) throw Error(rc, std::string("leaf orientation / retrieval: ") + shgpu_status_string(rc) + ": " + msg);
     }
 };

@@ -37,6 +37,7 @@ class GpuRecSplit {
     std::vector<uint8_t> ribbon_bits;
     gpu::Forest forest_meta;                                  // codes and nodes, kept for verification
     double host_ms = 0;
+    double phase_ms[3] = {0, 0, 0};  // Golomb-Rice + Elias-Fano, leaf orientation (device), retrieval (device)
 
     void construct(const hash128_t *h, size_t n, size_t bsize, size_t num_threads) {
         construct_forest(gpu::Engine::instance().build(VARIANT, (uint32_t)LEAF_SIZE, h, n, bsize), n, bsize, num_threads);
@@ -60,10 +61,11 @@ class GpuRecSplit {
         encodeForest(f, memo, words, pos_acc);
         descriptors = RiceBitVector(std::move(words));
         ef = DoubleEF(f.size_acc, pos_acc);
+        phase_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         uint64_t rseed = 0, rseg = 0;
         std::vector<uint64_t> rwords;
         gpu::Engine::instance().orientation_retrieval(VARIANT, f, gpu::keep_retrieval_input, ribbon_keys, ribbon_bits, rseed,
-                                                      rseg, rwords);
+                                                      rseg, rwords, phase_ms + 1);
         ribbon = BitRetrieval(rseed, rseg, std::move(rwords));
         f.device.reset();  // frees the build's device keys
         f.keys.clear();
@@ -148,6 +150,7 @@ class GpuRecSplit {
     const gpu::Forest &forest() const { return forest_meta; }
     size_t bucketCount() const { return nbuckets; }
     double hostEncodeMs() const { return host_ms; }
+    const double *hostPhaseMs() const { return phase_ms; }
 };
 
 }  // namespace detail
