@@ -11,9 +11,9 @@
 //
 // On the device the peeling runs in rounds. Round r: every cell that reached degree 1 claims its key
 // (compare-and-swap on the key's round; a key reachable through two such cells is claimed once), then every
-// claimed key leaves its three cells (atomic degree decrement and index XOR), and cells that drop to
-// degree 1 form the next round's frontier. Claims only read degrees and index XORs, removals only update
-// them, so each round's two kernels see consistent cells. Keys claimed in one round have distinct cells of
+// claimed key leaves its three cells (one 64-bit atomic subtracts it from the cell's degree and index sum),
+// and cells that drop to degree 1 form the next round's frontier. Claims only read cells, removals only
+// update them, so each round's two kernels see consistent cells. Keys claimed in one round have distinct cells of
 // degree 1 and none lies on another's cells, so the bits of one round are assigned in parallel, rounds in
 // reverse order.
 #include <hip/hip_runtime.h>
@@ -44,11 +44,19 @@ __device__ __forceinline__ void rt_cells(uint64_t key, uint64_t seed, uint64_t s
 
 #define GRID_STRIDE(i, n) for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
 
-__global__ void k_rt_clear(uint32_t *deg, uint32_t *xr, uint8_t *bits, uint64_t ncell, uint32_t *round_of, uint64_t n,
+// A cell's word: its degree (number of keys not yet peeled) in bits 40..63 and the sum of those keys'
+// indices in bits 0..39. One 64-bit atomic add or subtract updates both; a cell of degree 1 holds its key's
+// index. Sums fit while a cell holds fewer than 256 keys (a degree of 255 or more is reported as an error:
+// at 1.23 cells per key only repeated keys get there).
+constexpr int CELL_DEG_SHIFT = 40;
+constexpr uint64_t CELL_ONE = 1ull << CELL_DEG_SHIFT, CELL_SUM_MASK = CELL_ONE - 1;
+constexpr uint64_t CELL_MAX_DEG = 255;
+__device__ __forceinline__ uint64_t cell_deg(uint64_t w) { return w >> CELL_DEG_SHIFT; }
+
+__global__ void k_rt_clear(uint64_t *cw, uint8_t *bits, uint64_t ncell, uint32_t *round_of, uint64_t n,
                            uint32_t *counters, uint32_t *round_begin) {
     GRID_STRIDE(v, ncell) {
-        deg[v] = 0;
-        xr[v] = 0;
+        cw[v] = 0;
         bits[v] = 0;
     }
     GRID_STRIDE(i, n) round_of[i] = UNPEELED;
@@ -56,14 +64,15 @@ __global__ void k_rt_clear(uint32_t *deg, uint32_t *xr, uint8_t *bits, uint64_t 
     if (blockIdx.x == 0 && threadIdx.x == 0) round_begin[0] = 0;
 }
 
-__global__ void k_rt_degrees(const uint64_t *__restrict__ keys, uint64_t n, uint64_t seed, uint64_t seg, uint32_t *deg,
-                             uint32_t *xr) {
+// counters[3] is set when a cell reaches CELL_MAX_DEG keys.
+__global__ void k_rt_degrees(const uint64_t *__restrict__ keys, uint64_t n, uint64_t seed, uint64_t seg, uint64_t *cw,
+                             uint32_t *counters) {
     GRID_STRIDE(i, n) {
         uint64_t c[3];
         rt_cells(keys[i], seed, seg, c);
         for (int k = 0; k < 3; k++) {
-            atomicAdd(&deg[c[k]], 1u);
-            atomicXor(&xr[c[k]], (uint32_t)i);
+            const uint64_t old = atomicAdd((unsigned long long *)&cw[c[k]], (unsigned long long)(CELL_ONE + i));
+            if (cell_deg(old) + 1 >= CELL_MAX_DEG) counters[3] = 1;
         }
     }
 }
@@ -100,25 +109,28 @@ struct BlockAppend {
     name.init()
 
 // counters: [0] frontier length of the current round, [1] next frontier length, [2] keys peeled
-__global__ void __launch_bounds__(RT_BLOCK) k_rt_frontier0(const uint32_t *deg, uint64_t ncell, uint32_t *frontier,
+__global__ void __launch_bounds__(RT_BLOCK) k_rt_frontier0(const uint64_t *cw, uint64_t ncell, uint32_t *frontier,
                                                            uint32_t *counters) {
     BLOCK_APPEND(app, frontier, &counters[0]);
     GRID_STRIDE(v, ncell) {
-        if (deg[v] == 1) app.push((uint32_t)v);
+        if (cell_deg(cw[v]) == 1) app.push((uint32_t)v);
     }
     app.flush();
 }
 
 // Round r, claims: a frontier cell still of degree 1 claims its only key.
-__global__ void __launch_bounds__(RT_BLOCK) k_rt_claim(const uint32_t *frontier, uint32_t *counters, const uint32_t *deg,
-                                                       const uint32_t *xr, uint32_t *round_of, uint32_t *assigned,
-                                                       uint32_t *peeled, uint32_t r) {
+// A cell whose index sum has overflowed (degree 255 or more, reported through counters[3]) can show a degree of
+// 1 with any sum: indices at or past n are not claimed.
+__global__ void __launch_bounds__(RT_BLOCK) k_rt_claim(const uint32_t *frontier, uint32_t *counters, const uint64_t *cw,
+                                                       uint64_t n, uint32_t *round_of, uint32_t *assigned, uint32_t *peeled,
+                                                       uint32_t r) {
     BLOCK_APPEND(app, peeled, &counters[2]);
     const uint32_t len = counters[0];
     GRID_STRIDE(f, len) {
         const uint32_t v = frontier[f];
-        if (deg[v] != 1) continue;
-        const uint32_t i = xr[v];
+        const uint64_t w = cw[v];
+        if (cell_deg(w) != 1 || (w & CELL_SUM_MASK) >= n) continue;
+        const uint32_t i = (uint32_t)(w & CELL_SUM_MASK);
         if (atomicCAS(&round_of[i], UNPEELED, r) == UNPEELED) {
             assigned[i] = v;
             app.push(i);
@@ -130,8 +142,8 @@ __global__ void __launch_bounds__(RT_BLOCK) k_rt_claim(const uint32_t *frontier,
 // Round r, removals: the keys claimed in round r (peeled[begin, counters[2])) leave their cells.
 __global__ void __launch_bounds__(RT_BLOCK) k_rt_remove(const uint64_t *__restrict__ keys, uint64_t seed, uint64_t seg,
                                                         const uint32_t *peeled, const uint32_t *round_begin, uint32_t r,
-                                                        const uint32_t *counters, uint32_t *deg, uint32_t *xr,
-                                                        uint32_t *next_frontier, uint32_t *next_len) {
+                                                        const uint32_t *counters, uint64_t *cw, uint32_t *next_frontier,
+                                                        uint32_t *next_len) {
     BLOCK_APPEND(app, next_frontier, next_len);
     const uint32_t begin = round_begin[r], end = counters[2];
     GRID_STRIDE(f, (uint64_t)(end - begin)) {
@@ -139,8 +151,8 @@ __global__ void __launch_bounds__(RT_BLOCK) k_rt_remove(const uint64_t *__restri
         uint64_t c[3];
         rt_cells(keys[i], seed, seg, c);
         for (int k = 0; k < 3; k++) {
-            atomicXor(&xr[c[k]], i);
-            if (atomicSub(&deg[c[k]], 1u) == 2u) app.push((uint32_t)c[k]);
+            const uint64_t old = atomicAdd((unsigned long long *)&cw[c[k]], (unsigned long long)(0 - (CELL_ONE + i)));
+            if (cell_deg(old) == 2) app.push((uint32_t)c[k]);
         }
     }
     app.flush();
@@ -308,9 +320,10 @@ int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint
     const uint64_t ncell = 3 * seg, nwords = (ncell + 63) / 64;
     if (n >= UNPEELED || ncell >= UNPEELED) { err = "retrieval: more than 2^32 - 1 keys or cells"; return SHGPU_ERR_UNSUPPORTED; }
     if (words_capacity < nwords) { err = "retrieval: output capacity below " + std::to_string(nwords) + " words"; return SHGPU_ERR_ARG; }
-    // one allocation: deg, xr, frontier, next frontier (ncell u32 each), bits (ncell u8), round_of, assigned,
-    // peeled (n u32 each), round starts, counters, packed words; every part starts on a 256-byte boundary
-    const size_t parts[] = {ncell * 4, ncell * 4, ncell * 4, ncell * 4, ncell, (n + 1) * 4, (n + 1) * 4, (n + 1) * 4,
+    // one allocation: cell words (ncell u64), frontier, next frontier (ncell u32 each), bits (ncell u8),
+    // round_of, assigned, peeled (n u32 each), round starts, counters, packed words; every part starts on a
+    // 256-byte boundary
+    const size_t parts[] = {ncell * 8, ncell * 4, ncell * 4, ncell, (n + 1) * 4, (n + 1) * 4, (n + 1) * 4,
                             (size_t)(MAX_ROUNDS + 2) * 4, 64, nwords * 8};
     size_t bytes = 0;
     for (size_t b : parts) bytes += (b + 255) / 256 * 256;
@@ -318,7 +331,7 @@ int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint
     if (hipMalloc(&buf.p, bytes) != hipSuccess) { err = "retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed"; return SHGPU_ERR_NOMEM; }
     char *q = (char *)buf.p;
     auto take = [&](size_t b) { char *r = q; q += (b + 255) / 256 * 256; return r; };
-    uint32_t *deg = (uint32_t *)take(ncell * 4), *xr = (uint32_t *)take(ncell * 4);
+    uint64_t *cw = (uint64_t *)take(ncell * 8);
     uint32_t *fr[2] = {(uint32_t *)take(ncell * 4), (uint32_t *)take(ncell * 4)};
     uint8_t *bits = (uint8_t *)take(ncell);
     uint32_t *round_of = (uint32_t *)take((n + 1) * 4), *assigned = (uint32_t *)take((n + 1) * 4), *peeled = (uint32_t *)take((n + 1) * 4);
@@ -332,24 +345,25 @@ int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint
         if (e_ != hipSuccess) { err = std::string(#call) + ": " + hipGetErrorString(e_); return SHGPU_ERR_HIP; } \
     } while (0)
     for (uint64_t seed = 0; seed < 64; seed++) {
-        k_rt_clear<<<RT_GRID, RT_BLOCK, 0, s>>>(deg, xr, bits, ncell, round_of, n, counters, round_begin);
+        k_rt_clear<<<RT_GRID, RT_BLOCK, 0, s>>>(cw, bits, ncell, round_of, n, counters, round_begin);
         RT_HIP(hipGetLastError());
-        k_rt_degrees<<<RT_GRID, RT_BLOCK, 0, s>>>(d_keys, n, seed, seg, deg, xr);
-        k_rt_frontier0<<<RT_GRID, RT_BLOCK, 0, s>>>(deg, ncell, fr[0], counters);
+        k_rt_degrees<<<RT_GRID, RT_BLOCK, 0, s>>>(d_keys, n, seed, seg, cw, counters);
+        k_rt_frontier0<<<RT_GRID, RT_BLOCK, 0, s>>>(cw, ncell, fr[0], counters);
         RT_HIP(hipGetLastError());
         // round r reads frontier fr[r & 1] (length counters[0]) and fills fr[(r + 1) & 1] (length counters[1])
-        uint32_t rounds = 0, host_ctr[3] = {0, 0, 0};
+        uint32_t rounds = 0, host_ctr[4] = {0, 0, 0, 0};
         for (;;) {
             for (int k = 0; k < ROUNDS_PER_CHECK; k++, rounds++) {
                 if (rounds >= (uint32_t)MAX_ROUNDS) { err = "retrieval: peeling did not converge"; return SHGPU_ERR_HIP; }
-                k_rt_claim<<<RT_GRID, RT_BLOCK, 0, s>>>(fr[rounds & 1], counters, deg, xr, round_of, assigned, peeled, rounds);
-                k_rt_remove<<<RT_GRID, RT_BLOCK, 0, s>>>(d_keys, seed, seg, peeled, round_begin, rounds, counters, deg, xr,
+                k_rt_claim<<<RT_GRID, RT_BLOCK, 0, s>>>(fr[rounds & 1], counters, cw, n, round_of, assigned, peeled, rounds);
+                k_rt_remove<<<RT_GRID, RT_BLOCK, 0, s>>>(d_keys, seed, seg, peeled, round_begin, rounds, counters, cw,
                                                           fr[(rounds + 1) & 1], &counters[1]);
                 k_rt_advance<<<1, 1, 0, s>>>(counters, round_begin, rounds);
             }
             RT_HIP(hipGetLastError());
             RT_HIP(hipMemcpyAsync(host_ctr, counters, sizeof(host_ctr), hipMemcpyDeviceToHost, s));
             RT_HIP(hipStreamSynchronize(s));
+            if (host_ctr[3]) { err = "retrieval: a cell holds 255 or more keys (repeated keys?)"; return SHGPU_ERR_ARG; }
             if (host_ctr[0] == 0) break;  // empty frontier: no further round can peel
         }
         if (host_ctr[2] != n) continue;  // not every key peeled: next seed

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from shockhash_amd import workload
-from shockhash_amd.engine import ERR_SEARCH_LIMIT, ShgpuError, load
+from shockhash_amd.engine import ERR_ARG, ERR_SEARCH_LIMIT, ShgpuError, load
 
 pytestmark = pytest.mark.gpu
 
@@ -74,3 +74,13 @@ def test_duplicate_keys_are_reported(engine):
     with pytest.raises(ShgpuError) as ei:
         engine.retrieval_build(keys, np.zeros(1000, dtype=np.uint8))
     assert ei.value.status == ERR_SEARCH_LIMIT
+
+
+def test_key_repeated_past_the_cell_limit_is_reported(engine):
+    """A key repeated 300 times puts 300 keys in its cells, past the 255 that a cell's index sum can count:
+    reported as an argument error at once, not searched over every seed."""
+    keys = workload.fingerprints(1000, seed=44)[:, 1].copy()
+    keys[:300] = keys[500]
+    with pytest.raises(ShgpuError) as ei:
+        engine.retrieval_build(keys, np.zeros(1000, dtype=np.uint8))
+    assert ei.value.status == ERR_ARG
