@@ -102,6 +102,33 @@ def serving_mix(eng, args):
     return out
 
 
+def class_build(h, args, ob):
+    """The drop-in class constructor ShockHash2<64>(fingerprints, 2000) on the same 10M host fingerprints:
+    upload, device sort/plan/search, orientation and retrieval, host Golomb-Rice + Elias-Fano. The second of
+    two builds is reported; every key is queried, and the descriptor words are compared with the oracle's
+    when the CPU baseline built the same keys."""
+    from shockhash_amd import host
+    for rep in range(2):
+        f = host.MPHF("sh2", 64, h, 2000, threads=1)
+        t = f.timings_ms
+        if rep == 0:
+            f.close()
+    t0 = time.perf_counter()
+    q = f(h)
+    query_s = time.perf_counter() - t0
+    bij = bool(len(q) == len(h) and int(q.max()) < len(h) and len(np.unique(q)) == len(h))
+    desc_eq = None if ob is None else bool(np.array_equal(f.descriptor_words, ob.words))
+    out = {"constructor_ms": t["constructor"], "keys_per_s": len(h) / (t["constructor"] / 1e3),
+           "phases_ms": {k: float(t[k]) for k in ("sort", "tree", "search", "rice_ef", "orientation", "retrieval")},
+           "bits_per_key": f.bits / len(h), "queries_bijective": bij, "query_ns_per_key": 1e9 * query_s / len(h),
+           "descriptors_equal_oracle": desc_eq}
+    f.close()
+    log(f"class build: ShockHash2<64> constructor {t['constructor']:.1f} ms ({out['keys_per_s'] / 1e6:.0f} M keys/s), "
+        f"phases {out['phases_ms']}, {out['bits_per_key']:.3f} bits/key, queries bijective {bij}, "
+        f"descriptors equal the oracle's: {desc_eq}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +148,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--serve-batch-keys", type=int, default=1 << 22, help="keys a slot builds together at most")
+    ap.add_argument("--no-class-build", action="store_true",
+                    help="skip the ShockHash2<64> class-constructor timing (host fingerprints in, queries out)")
     ap.add_argument("--copy-probe-mib", type=int, default=256,
                     help="size of the plain copy kernel run once as the HBM-counter calibration reference")
     args = ap.parse_args()
@@ -208,6 +237,11 @@ def main():
                          f"{args.cpu_threads} threads, {dt:.1f} s"}
         log(f"cpu oracle: {cpu['value']:.0f} keys/s; GPU codes equal the oracle's: {bit_exact}")
 
+    cls = None
+    if not args.no_class_build:
+        cls = class_build(h, args, ob if (not args.no_cpu_baseline and (args.cpu_sample or args.n) == args.n) else None)
+        bit_exact = bit_exact and cls["queries_bijective"] and cls["descriptors_equal_oracle"] is not False
+
     serving = None if args.no_serving else serving_mix(eng, args)
     if serving is not None:
         bit_exact = bit_exact and serving["identical"]
@@ -235,6 +269,7 @@ def main():
         "search_kernel_ms": kms,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "class_build": cls,
         "p99_build_latency_ms": serving["p99_ms"] if serving else None,
         "serving": serving,
     }
