@@ -145,7 +145,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2048)
     ap.add_argument("--serve-rate", type=float, default=800.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-slots", type=int, default=1,
+                    help="concurrent builds; two or more overlap kernels that then stretch each other's tails")
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--serve-batch-keys", type=int, default=1 << 22, help="keys a slot builds together at most")
     ap.add_argument("--no-class-build", action="store_true",
