@@ -149,7 +149,8 @@ struct shgpu_engine {
     char arch[64] = {0};
     // Schedule per variant (ShockHash, SIMDShockHash, ShockHash2); the defaults are the measured best on
     // MI355X for leaf 30 / 58 / 64 builds (see shgpu_autotune).
-    shgpu_tuning tuning_v[3] = {{4, 2, 24, 0, 1ull << 40}, {4, 2, 24, 0, 1ull << 40}, {4, 2, 24, 0, 1ull << 40}};
+    // ShockHash2 builds size their grid by batch (waves_per_cu 0, see waves_for)
+    shgpu_tuning tuning_v[3] = {{4, 2, 24, 0, 1ull << 40}, {4, 2, 24, 0, 1ull << 40}, {4, 2, 0, 0, 1ull << 40}};
     shgpu_tuning tuning{4, 2, 24, 0, 1ull << 40};  // batched per-node entry points
     DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
     BuildBuffers main;                          // shgpu_build_run
@@ -923,8 +924,18 @@ int check_variant(int variant, uint32_t leaf) {
     return fail(SHGPU_ERR_ARG, "unknown variant");
 }
 
-int grid_for(const shgpu_engine *e, const shgpu_tuning &t) {
-    return std::max(1, e->cu_count * (int)t.waves_per_cu / WAVES_PER_BLOCK);
+// Resident waves per CU of a persistent build: the schedule's value, or by the batch's key count when it is 0.
+// Measured on ShockHash2 leaf 64, bucket 2000 (kernel ms at 24 / 16 / 12 waves per CU): 200k keys 6.3 /
+// 4.7 / 3.7, 1M 7.6 / 5.8 / 5.3, 2M 10.2 / 8.0 / 7.9, 4M 14.4 / 13.4 / 14.0, 6M 19.1 / 19.2 / 20.1, 10M
+// 29.7 / 30.9 / 33.0. Small builds are set by their longest chain of dependent searches, and fewer
+// resident waves leave each searching wave more of its SIMD; large ones need the waves to hide latency.
+uint32_t waves_for(const shgpu_tuning &t, uint64_t n_keys) {
+    if (t.waves_per_cu) return t.waves_per_cu;
+    return n_keys < 3000000 ? 12u : n_keys < 5000000 ? 16u : 24u;
+}
+
+int grid_for(const shgpu_engine *e, const shgpu_tuning &t, uint64_t n_keys = ~0ull) {
+    return std::max(1, e->cu_count * (int)waves_for(t, n_keys) / WAVES_PER_BLOCK);
 }
 int grid_for(const shgpu_engine *e) { return grid_for(e, e->tuning); }
 
@@ -1042,7 +1053,7 @@ static int check_tuning(const shgpu_tuning *t) {
     auto okS = [](uint32_t s) { return s == 1 || s == 2 || s == 4; };
     if (!okS(t->sh1_seeds_per_lane) || !okS(t->split_seeds_per_lane))
         return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
-    if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(SHGPU_ERR_ARG, "waves_per_cu must be in [1, 32]");
+    if (t->waves_per_cu > 32) return fail(SHGPU_ERR_ARG, "waves_per_cu must be in [0, 32] (0: by batch size)");
     if (t->max_trials == 0) return fail(SHGPU_ERR_ARG, "max_trials must be positive");
     if (t->flags & ~(SHGPU_TUNE_SH2_GENERAL_ONLY | SHGPU_TUNE_SPLITS_FIRST | SHGPU_TUNE_CHECK_PLAN))
         return fail(SHGPU_ERR_ARG, "unknown tuning flags");
@@ -1289,7 +1300,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     }
     BATCH_HIP(hipEventRecord(B.evB, s));
 
-    const int grid = std::max(1, grid_for(e, tune) / (int)std::max(1u, grid_div));
+    const int grid = std::max(1, grid_for(e, tune, N) / (int)std::max(1u, grid_div));
     const uint32_t nslots = (uint32_t)grid * WAVES_PER_BLOCK;
     std::vector<uint64_t> acc_all(NB);
     std::vector<uint64_t> codes;
