@@ -925,13 +925,15 @@ int check_variant(int variant, uint32_t leaf) {
 }
 
 // Resident waves per CU of a persistent build: the schedule's value, or by the batch's key count when it is 0.
-// Measured on ShockHash2 leaf 64, bucket 2000 (kernel ms at 24 / 16 / 12 waves per CU): 200k keys 6.3 /
-// 4.7 / 3.7, 1M 7.6 / 5.8 / 5.3, 2M 10.2 / 8.0 / 7.9, 4M 14.4 / 13.4 / 14.0, 6M 19.1 / 19.2 / 20.1, 10M
-// 29.7 / 30.9 / 33.0. Small builds are set by their longest chain of dependent searches, and fewer
-// resident waves leave each searching wave more of its SIMD; large ones need the waves to hide latency.
+// Measured on ShockHash2 leaf 64, bucket 2000, kernel ms at 4 / 8 / 12 / 16 / 24 waves per CU: 200k keys
+// 2.5 / 2.8 / 3.7 / 4.7 / 6.3, 500k 4.2 / 3.8 / - / - / -, 1M 6.6 / 5.1 / 5.3 / 5.8 / 7.6, 2M 12.6 / 8.7 /
+// 7.9 / 8.0 / 10.2, 4M - / - / 14.0 / 13.4 / 14.4, 6M - / - / 20.1 / 19.2 / 19.1, 10M - / 38.2 / 33.0 /
+// 30.9 / 29.7. Small builds are set by their longest chain of dependent searches, and fewer resident
+// waves (fewer idle pollers) leave each searching wave more of its SIMD and of the memory system; large
+// ones need the waves to hide latency.
 uint32_t waves_for(const shgpu_tuning &t, uint64_t n_keys) {
     if (t.waves_per_cu) return t.waves_per_cu;
-    return n_keys < 3000000 ? 12u : n_keys < 5000000 ? 16u : 24u;
+    return n_keys < 350000 ? 4u : n_keys < 1500000 ? 8u : n_keys < 3000000 ? 12u : n_keys < 5000000 ? 16u : 24u;
 }
 
 int grid_for(const shgpu_engine *e, const shgpu_tuning &t, uint64_t n_keys = ~0ull) {
