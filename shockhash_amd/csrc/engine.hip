@@ -518,6 +518,7 @@ __device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint3
         }
         return;
     }
+    wave_fetch_add(&a.ctrl[8], 1u);  // windowed nodes started
     __hip_atomic_store(&a.help[slot], nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     work_windows<VARIANT, SL, SS>(a, nid, L.pf, true, t_start);
     __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -554,9 +555,17 @@ __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
         __threadfence();
         run_node<VARIANT, SL, SS>(a, nid, slot, L, ws);
     }
+    // Queue exhausted: help until every node is finished. Once every windowed node has started (ctrl[8]
+    // reaches ctrl[7]) no node will register for help any more, and a wave that finds none open leaves
+    // instead of polling beside the waves still searching (checked every 8th poll: ~6000 waves polling one
+    // word load a single L2 channel).
+    const uint32_t windowed = uniform(a.ctrl[7]);
     for (uint32_t spin = 0;; spin++) {
         if (uniform(__hip_atomic_load(&a.ctrl[3], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= total) break;
         if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
+        if ((spin & 7) == 0 &&
+            uniform(__hip_atomic_load(&a.ctrl[8], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= windowed)
+            break;
         if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
         __builtin_amdgcn_s_sleep(8);
     }
@@ -719,7 +728,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_scan(PlanArgs p) {
         p.summary[SUM_LEAF_ROOTS] = total.w;
         p.summary[SUM_OVERFLOW] = overflow ? 1u : 0u;
         p.summary[SUM_QSPLIT] = qsplit;
-        // head, split tail, error, finished, unused, leaf tail, nodes
+        // head, split tail, error, finished, unused, leaf tail, nodes, windowed nodes, windowed nodes started
         p.ctrl[0] = 0;
         p.ctrl[1] = p.two_part ? total.z : total.z + total.w;
         p.ctrl[2] = 0;
@@ -727,6 +736,8 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_scan(PlanArgs p) {
         p.ctrl[4] = 0;
         p.ctrl[5] = qsplit + (p.two_part ? total.w : 0u);
         p.ctrl[6] = n_total;
+        p.ctrl[7] = p.variant == SHGPU_SHOCKHASH2 ? (overflow ? 0u : total.y) : n_total;
+        p.ctrl[8] = 0;
     }
     for (uint32_t i = t; i < p.nhelp; i += PLAN_THREADS) p.help[i] = EMPTY_SLOT;
 }
