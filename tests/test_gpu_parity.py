@@ -193,6 +193,22 @@ def test_build_matches_oracle(engine, variant, leaf, n, mode):
     log(f"  ok, timings {b.timings_ms}")
 
 
+@pytest.mark.parametrize("waves", [24, 12, 4])
+def test_build_fixed_grid(engine, waves):
+    """The ShockHash2 schedule sizes its grid by batch (waves_per_cu 0: 4 waves per CU at this size); the
+    other grids, 24 waves per CU as at 10M keys included, give the same codes."""
+    log(f"sh2 build with {waves} waves per CU")
+    n = 30000
+    h = workload.fingerprints(n, seed=24)
+    ob = O.OracleBuild("sh2", 64, h, 2000, threads=8)
+    engine.set_variant_tuning("sh2", waves_per_cu=waves)
+    try:
+        b = engine.build(h, "sh2", 64, 2000)
+    finally:
+        engine.set_variant_tuning("sh2", waves_per_cu=0)
+    assert np.array_equal(b.codes, ob.node_code)
+
+
 @pytest.mark.parametrize("mode", ["persistent", "plain"])
 def test_build_device_keys(engine, mode):
     """SHGPU_KEYS_DEVICE: the build keeps its leaf-ordered keys in device memory of its own (what the classes
