@@ -179,8 +179,13 @@ struct shgpu_build {
     shgpu_build() = default;
     shgpu_build(const shgpu_build &) = delete;
     shgpu_build &operator=(const shgpu_build &) = delete;
-    ~shgpu_build() {
-        if (d_keys && hipSetDevice(device) == hipSuccess) (void)hipFree(d_keys);
+    ~shgpu_build() {  // frees on the build's device and leaves the caller's current device as it was
+        if (!d_keys) return;
+        int cur = device;
+        (void)hipGetDevice(&cur);
+        if (cur != device && hipSetDevice(device) != hipSuccess) return;
+        (void)hipFree(d_keys);
+        if (cur != device) (void)hipSetDevice(cur);
     }
 };
 
