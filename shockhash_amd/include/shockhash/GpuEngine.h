@@ -38,10 +38,9 @@ struct Forest {
     std::vector<uint64_t> size_acc;  // nbuckets + 1
     std::vector<shgpu_node> nodes;   // DFS pre-order per bucket, buckets in order
     std::vector<uint64_t> codes;     // per node: found seed - start seed (ShockHash2 leaves: pair code)
-    std::vector<uint64_t> keys;      // fingerprint .second in final leaf order (host copy, when asked for)
     double ms[4] = {0, 0, 0, 0};     // sort + bucketing, tree setup, seed search, search kernel
-    // The engine's build result while it holds the same keys in device memory (the orientation pass reads
-    // them there); released once the retrieval structure is built.
+    // The engine's build result while it holds the fingerprint .second values in final leaf order in device
+    // memory (the orientation pass reads them there); released once the retrieval structure is built.
     std::shared_ptr<shgpu_build> device;
 };
 
@@ -128,8 +127,8 @@ class Engine {
   public:
     // The retrieval input of a forest (every leaf's orientation, in the reference's order) and its one-bit
     // retrieval structure, both computed on the device (shgpu_leaf_orientation, shgpu_retrieval_build) from
-    // the build's device-resident leaf-ordered keys (or from f.keys, uploaded, when the forest holds no
-    // device build); the structure comes back, and the input too when `keep` is set.
+    // the build's device-resident leaf-ordered keys; the structure comes back, and the input too when `keep`
+    // is set.
     void orientation_retrieval(int variant, const Forest &f, bool keep, std::vector<uint64_t> &input_keys,
                                std::vector<uint8_t> &input_bits, uint64_t &seed, uint64_t &seg,
                                std::vector<uint64_t> &words, double *ms2 = nullptr) {
@@ -143,17 +142,10 @@ class Engine {
             tasks.push_back({total, variant == SHGPU_SHOCKHASH2 ? f.codes[i] : f.codes[i] + nd.start_seed, nd.offset, nd.m});
             total += nd.m;
         }
-        void *d_keys = nullptr, *d_tasks = nullptr, *d_out_keys = nullptr, *d_out_bits = nullptr;
+        if (!f.device) throw Error(SHGPU_ERR_ARG, "leaf orientation: the forest holds no device build");
+        void *d_tasks = nullptr, *d_out_keys = nullptr, *d_out_bits = nullptr;
         const uint64_t *leaf_keys = nullptr;
-        int rc = SHGPU_OK;
-        if (f.device) {
-            rc = shgpu_build_device_keys(f.device.get(), &leaf_keys);
-        } else {
-            const uint64_t n = f.keys.size();
-            rc = shgpu_device_alloc(e, n * 8 + 8, &d_keys);
-            if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, d_keys, f.keys.data(), n * 8);
-            leaf_keys = (const uint64_t *)d_keys;
-        }
+        int rc = shgpu_build_device_keys(f.device.get(), &leaf_keys);
         if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, tasks.size() * sizeof(shgpu_orient_task) + 8, &d_tasks);
         if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, total * 8 + 8, &d_out_keys);
         if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, total + 8, &d_out_bits);
@@ -172,7 +164,7 @@ class Engine {
         if (rc == SHGPU_OK && keep && total) rc = shgpu_copy_to_host(e, input_keys.data(), d_out_keys, total * 8);
         if (rc == SHGPU_OK && keep && total) rc = shgpu_copy_to_host(e, input_bits.data(), d_out_bits, total);
         const std::string msg = rc == SHGPU_OK ? "" : shgpu_last_error();
-        for (void *p : {d_keys, d_tasks, d_out_keys, d_out_bits})
+        for (void *p : {d_tasks, d_out_keys, d_out_bits})
             if (p) shgpu_device_free(e, p);
         if (ms2) {
             ms2[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();

@@ -68,8 +68,6 @@ class GpuRecSplit {
                                                       rseg, rwords, phase_ms + 1);
         ribbon = BitRetrieval(rseed, rseg, std::move(rwords));
         f.device.reset();  // frees the build's device keys
-        f.keys.clear();
-        f.keys.shrink_to_fit();
         forest_meta = std::move(f);
         host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
