@@ -6,6 +6,10 @@ fingerprints, buckets them, plans every RecSplit tree and searches every split a
 GPU (one persistent kernel with a device work queue); the per-node codes come back to the host. The
 same step in the plain form (one kernel launch per tree node) is timed on the first buckets, the CPU
 oracle is timed on a bounded sample, and the GPU codes are checked against the oracle on that sample.
+Two more legs follow: the ShockHash2<64> class constructor over the same keys from host memory (upload,
+device search, orientation and retrieval, host encoding; every key queried, descriptor words compared
+with the oracle's), and a serving mix of 2048 Poisson-arriving requests on the continuous-batching
+server (p50/p99 latency).
 
 Prints one JSON line (see the keys below). One GPU only: --gpus N > 1 exits with a message.
 """
