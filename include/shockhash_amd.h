@@ -127,11 +127,14 @@ typedef struct shgpu_build_desc {
     int mode;              /* SHGPU_MODE_PERSISTENT or SHGPU_MODE_PLAIN */
     uint32_t plain_bucket_limit; /* plain mode: build only the first N buckets (0 = all) */
     int copy_keys;         /* SHGPU_KEYS_HOST: keep a host copy of the keys in final leaf order;
-                              SHGPU_KEYS_DEVICE: keep a device copy (shgpu_build_device_keys); or both */
+                              SHGPU_KEYS_DEVICE: keep a device copy (shgpu_build_device_keys);
+                              SHGPU_NODES_HOST: copy the device-planned nodes back with the codes (otherwise
+                              shgpu_build_nodes plans them again on the host on first request); any OR */
 } shgpu_build_desc;
 
 #define SHGPU_KEYS_HOST 1
 #define SHGPU_KEYS_DEVICE 2
+#define SHGPU_NODES_HOST 4
 
 int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
                     shgpu_build **out);

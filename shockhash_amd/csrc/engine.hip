@@ -105,6 +105,7 @@ struct BuildBuffers {
     std::vector<uint32_t> plan_counts;
     uint64_t plan_capacity = 0;
     PinnedBuf codes_host;  // codes on their way to the host
+    PinnedBuf nodes_host;  // device-planned nodes on their way to the host (SHGPU_NODES_HOST)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
     size_t bytes() const {
         const DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
@@ -1326,6 +1327,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     uint32_t ctrl_out[4] = {0, 0, 0, 0};
     uint32_t sort_rerun = 0;  // a run of keys sharing the sorted bits was too long: sort on all 64 bits
     const uint64_t *codes_pinned = nullptr;  // persistent form: codes arrive in pinned memory (n_codes of them)
+    const shgpu_node *nodes_pinned = nullptr;  // ... and with SHGPU_NODES_HOST the device-planned nodes
     uint64_t n_codes = 0;
     bool timed_kernel = false;
     BuildArgs a;
@@ -1435,6 +1437,11 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         codes_pinned = (const uint64_t *)B.codes_host.p;
         n_codes = total;
         if (total) BATCH_HIP(hipMemcpyAsync(B.codes_host.p, a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
+        if ((d->copy_keys & SHGPU_NODES_HOST) && total) {
+            if ((rc = B.nodes_host.ensure((size_t)total * sizeof(shgpu_node)))) return fail_all(rc);
+            nodes_pinned = (const shgpu_node *)B.nodes_host.p;
+            BATCH_HIP(hipMemcpyAsync(B.nodes_host.p, a.nodes, (size_t)total * sizeof(shgpu_node), hipMemcpyDeviceToHost, s));
+        }
         for (size_t r = 0; r < count; r++) {
             node_begin[r] = summary[SUM_SETS + 3 * r];
             node_end[r] = summary[SUM_SETS + 3 * r + 1];
@@ -1578,6 +1585,11 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             continue;
         }
         if (!keys_all.empty()) b.keys.assign(keys_all.begin() + base[r], keys_all.begin() + base[r] + b.n);
+        if (nodes_pinned) {  // the device plan, offsets made relative to this set's keys
+            b.nodes.assign(nodes_pinned + node_begin[r], nodes_pinned + node_end[r]);
+            for (auto &nd : b.nodes) nd.offset -= (uint32_t)base[r];
+            b.nodes_ready = true;
+        }
         if (timed_kernel) {
             b.ms[0] = sort_ms;
             b.ms[1] = plan_ms;

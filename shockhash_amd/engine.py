@@ -47,7 +47,7 @@ class Tuning(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("max_trials", ctypes.c_uint64)]
 
 
-KEYS_HOST, KEYS_DEVICE = 1, 2  # shgpu_build_desc.copy_keys bits
+KEYS_HOST, KEYS_DEVICE, NODES_HOST = 1, 2, 4  # shgpu_build_desc.copy_keys bits
 
 
 class BuildDesc(ctypes.Structure):

@@ -105,7 +105,7 @@ class Engine {
         d.leaf_size = leaf_size;
         d.bucket_size = bucket_size;
         d.mode = SHGPU_MODE_PERSISTENT;
-        d.copy_keys = SHGPU_KEYS_DEVICE;  // the keys stay on the device for the orientation pass
+        d.copy_keys = SHGPU_KEYS_DEVICE | SHGPU_NODES_HOST;  // keys stay on the device for the orientation pass
         return d;
     }
 

@@ -4,7 +4,7 @@ import pytest
 
 import oracle_lib as O
 from shockhash_amd import workload
-from shockhash_amd.engine import KEYS_DEVICE, KEYS_HOST, SPLIT_DTYPE, DeviceArray
+from shockhash_amd.engine import KEYS_DEVICE, KEYS_HOST, NODES_HOST, SPLIT_DTYPE, DeviceArray, plan_forest
 
 pytestmark = pytest.mark.gpu
 
@@ -207,6 +207,23 @@ def test_build_fixed_grid(engine, waves):
     finally:
         engine.set_variant_tuning("sh2", waves_per_cu=0)
     assert np.array_equal(b.codes, ob.node_code)
+
+
+@pytest.mark.parametrize("variant,leaf,n", [("sh2", 64, 60000), ("sh1", 30, 30000), ("simd", 16, 20000)])
+def test_build_nodes_from_device(engine, variant, leaf, n):
+    """SHGPU_NODES_HOST: the nodes come back from the device plan and equal the host plan of the same
+    bucket sizes (offsets relative to the build's keys)."""
+    log(f"nodes from the device plan, {variant} L={leaf} n={n}")
+    h = workload.fingerprints(n, seed=25)
+    d = DeviceArray.from_host(engine, h.reshape(-1))
+    dev = engine.build_dev(d, n, variant, leaf, 2000, copy_keys=NODES_HOST)
+    host = engine.build_dev(d, n, variant, leaf, 2000, copy_keys=0)
+    d.free()
+    assert np.array_equal(dev.codes, host.codes)
+    assert np.array_equal(dev.nodes, host.nodes)
+    plan = plan_forest(variant, leaf, dev.size_acc)
+    for field in ("offset", "m", "kind", "n_children", "unit", "subtree", "start_seed"):  # max_trials: the schedule's
+        assert np.array_equal(dev.nodes[field], plan[field]), field
 
 
 @pytest.mark.parametrize("mode", ["persistent", "plain"])
