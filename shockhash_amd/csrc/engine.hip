@@ -155,6 +155,7 @@ struct shgpu_engine {
     shgpu_tuning tuning{4, 2, 24, 0, 1ull << 40};  // batched per-node entry points
     DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
     BuildBuffers main;                          // shgpu_build_run
+    DevBuf retrieval;                           // shgpu_retrieval_build working memory (~33 B per key)
 };
 
 struct shgpu_build {
@@ -1645,7 +1646,8 @@ int shgpu_retrieval_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t
     if (!e || !seed || !seg || !h_words || (n && (!d_keys || !d_bits))) return fail(SHGPU_ERR_ARG, "NULL argument");
     if (hipSetDevice(e->device) != hipSuccess) return fail(SHGPU_ERR_HIP, "hipSetDevice failed");
     std::string err;
-    const int rc = shgpu_retrieval_build_impl(e->stream, d_keys, d_bits, n, seed, seg, h_words, words_capacity, err);
+    const int rc = shgpu_retrieval_build_impl(e->stream, d_keys, d_bits, n, seed, seg, h_words, words_capacity, err,
+                                              &e->retrieval.p, &e->retrieval.bytes);
     return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
 }
 

@@ -8,8 +8,11 @@
 // (shgpu_retrieval_cells_per_third, the layout's size rule, is part of the C ABI: include/shockhash_amd.h.)
 // Builds the structure for n keys and their bits (device arrays) on stream s; writes the seed, the cells per
 // third and ceil(3 * seg / 64) packed words to host memory. Returns an SHGPU status, message in err.
+// Working memory: *cache (*cache_bytes bytes, grown as needed and kept by the caller) when cache is not
+// NULL, else an allocation of this call.
 int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
-                               uint64_t *seg, uint64_t *h_words, uint64_t words_capacity, std::string &err);
+                               uint64_t *seg, uint64_t *h_words, uint64_t words_capacity, std::string &err,
+                               void **cache = nullptr, size_t *cache_bytes = nullptr);
 
 #include "shgpu_types.h"
 // Retrieval input of n leaves (one thread per leaf replays its cuckoo insertions); bipartite: ShockHash2.

@@ -315,7 +315,8 @@ struct RtBuf {
 uint64_t shgpu_retrieval_cells_per_third(uint64_t n) { return (uint64_t)(n * 1.23 / 3.0) + 8; }
 
 int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed_out,
-                               uint64_t *seg_out, uint64_t *h_words, uint64_t words_capacity, std::string &err) {
+                               uint64_t *seg_out, uint64_t *h_words, uint64_t words_capacity, std::string &err,
+                               void **cache, size_t *cache_bytes) {
     const uint64_t seg = shgpu_retrieval_cells_per_third(n);
     const uint64_t ncell = 3 * seg, nwords = (ncell + 63) / 64;
     if (n >= UNPEELED || ncell >= UNPEELED) { err = "retrieval: more than 2^32 - 1 keys or cells"; return SHGPU_ERR_UNSUPPORTED; }
@@ -328,8 +329,25 @@ int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint
     size_t bytes = 0;
     for (size_t b : parts) bytes += (b + 255) / 256 * 256;
     RtBuf buf;
-    if (hipMalloc(&buf.p, bytes) != hipSuccess) { err = "retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed"; return SHGPU_ERR_NOMEM; }
-    char *q = (char *)buf.p;
+    void *base = nullptr;
+    if (cache && cache_bytes) {
+        if (*cache_bytes < bytes) {
+            if (*cache) (void)hipFree(*cache);
+            *cache = nullptr;
+            *cache_bytes = 0;
+            if (hipMalloc(cache, bytes) != hipSuccess) {
+                *cache = nullptr;
+                err = "retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed";
+                return SHGPU_ERR_NOMEM;
+            }
+            *cache_bytes = bytes;
+        }
+        base = *cache;
+    } else {
+        if (hipMalloc(&buf.p, bytes) != hipSuccess) { err = "retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed"; return SHGPU_ERR_NOMEM; }
+        base = buf.p;
+    }
+    char *q = (char *)base;
     auto take = [&](size_t b) { char *r = q; q += (b + 255) / 256 * 256; return r; };
     uint64_t *cw = (uint64_t *)take(ncell * 8);
     uint32_t *fr[2] = {(uint32_t *)take(ncell * 4), (uint32_t *)take(ncell * 4)};
@@ -338,7 +356,7 @@ int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint
     uint32_t *round_begin = (uint32_t *)take((size_t)(MAX_ROUNDS + 2) * 4);
     uint32_t *counters = (uint32_t *)take(64);
     uint64_t *words = (uint64_t *)take(nwords * 8);
-    if ((size_t)(q - (char *)buf.p) > bytes) { err = "retrieval: buffer layout"; return SHGPU_ERR_HIP; }
+    if ((size_t)(q - (char *)base) > bytes) { err = "retrieval: buffer layout"; return SHGPU_ERR_HIP; }
 #define RT_HIP(call)                                                                  \
     do {                                                                              \
         hipError_t e_ = (call);                                                       \

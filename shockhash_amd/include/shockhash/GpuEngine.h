@@ -47,10 +47,29 @@ struct Forest {
 class Engine {
     shgpu_engine *e = nullptr;
     std::mutex mtx;
+    // Device buffers of the orientation pass, grown as needed and kept between constructions.
+    struct Buf {
+        void *p = nullptr;
+        size_t bytes = 0;
+    };
+    Buf tasks_buf, out_keys_buf, out_bits_buf;
+    int ensure(Buf &b, size_t bytes) {
+        if (b.bytes >= bytes) return SHGPU_OK;
+        if (b.p) shgpu_device_free(e, b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        const int rc = shgpu_device_alloc(e, bytes, &b.p);
+        if (rc == SHGPU_OK) b.bytes = bytes;
+        return rc;
+    }
 
   public:
     explicit Engine(int device = 0) { check(shgpu_engine_create(device, &e), "shgpu_engine_create"); }
-    ~Engine() { shgpu_engine_destroy(e); }
+    ~Engine() {
+        for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf})
+            if (b->p) shgpu_device_free(e, b->p);
+        shgpu_engine_destroy(e);
+    }
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
 
@@ -143,12 +162,12 @@ class Engine {
             total += nd.m;
         }
         if (!f.device) throw Error(SHGPU_ERR_ARG, "leaf orientation: the forest holds no device build");
-        void *d_tasks = nullptr, *d_out_keys = nullptr, *d_out_bits = nullptr;
         const uint64_t *leaf_keys = nullptr;
         int rc = shgpu_build_device_keys(f.device.get(), &leaf_keys);
-        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, tasks.size() * sizeof(shgpu_orient_task) + 8, &d_tasks);
-        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, total * 8 + 8, &d_out_keys);
-        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, total + 8, &d_out_bits);
+        if (rc == SHGPU_OK) rc = ensure(tasks_buf, tasks.size() * sizeof(shgpu_orient_task) + 8);
+        if (rc == SHGPU_OK) rc = ensure(out_keys_buf, total * 8 + 8);
+        if (rc == SHGPU_OK) rc = ensure(out_bits_buf, total + 8);
+        void *d_tasks = tasks_buf.p, *d_out_keys = out_keys_buf.p, *d_out_bits = out_bits_buf.p;
         if (rc == SHGPU_OK && !tasks.empty())
             rc = shgpu_copy_to_device(e, d_tasks, tasks.data(), tasks.size() * sizeof(shgpu_orient_task));
         if (rc == SHGPU_OK)
@@ -164,8 +183,6 @@ class Engine {
         if (rc == SHGPU_OK && keep && total) rc = shgpu_copy_to_host(e, input_keys.data(), d_out_keys, total * 8);
         if (rc == SHGPU_OK && keep && total) rc = shgpu_copy_to_host(e, input_bits.data(), d_out_bits, total);
         const std::string msg = rc == SHGPU_OK ? "" : shgpu_last_error();
-        for (void *p : {d_tasks, d_out_keys, d_out_bits})
-            if (p) shgpu_device_free(e, p);
         if (ms2) {
             ms2[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
             ms2[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
