@@ -154,6 +154,7 @@ class Engine {
         std::lock_guard<std::mutex> lock(mtx);
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<shgpu_orient_task> tasks;
+        tasks.reserve(f.nodes.size());
         uint64_t total = 0;
         for (size_t i = 0; i < f.nodes.size(); i++) {
             const shgpu_node &nd = f.nodes[i];
