@@ -37,6 +37,25 @@ def test_class_matches_oracle(variant, leaf, n):
     log(f"  ok {f.timings_ms}")
 
 
+@pytest.mark.parametrize("variant,leaf", [("sh2", 64), ("sh1", 30), ("simd", 16)])
+@pytest.mark.parametrize("n,bucket", [(1, 2000), (2, 2000), (3, 2000), (65, 2000), (500, 2000), (1000, 8),
+                                      (3000, 100)])
+def test_class_small_and_ragged(variant, leaf, n, bucket):
+    """Tiny key sets and small buckets (many buckets of 0, 1 or 2 keys): codes, descriptor words, retrieval
+    input and queries equal the oracle's."""
+    log(f"class {variant} L={leaf} n={n} bucket={bucket}")
+    h = workload.fingerprints(n, seed=32)
+    f = host.MPHF(variant, leaf, h, bucket, threads=1, keep_retrieval_input=True)
+    ob = O.OracleBuild(variant, leaf, h, bucket, threads=1)
+    assert np.array_equal(f.codes, ob.node_code)
+    assert np.array_equal(f.descriptor_words, ob.words)
+    k, b = f.ribbon_input
+    assert np.array_equal(k, ob.ribbon_keys[: len(k)]) and np.array_equal(b, ob.ribbon_bits[: len(b)])
+    q = f(h)
+    assert np.array_equal(q, ob.query(h))
+    assert len(np.unique(q)) == n and int(q.max()) < n
+
+
 def test_string_keys_shockhash2():
     log("strings sh2")
     keys = workload.random_strings(20000, seed=4)
