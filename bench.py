@@ -33,19 +33,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def O_build(h, threads):
+VARIANT_NAMES = {"sh2": "ShockHash2", "sh1": "ShockHash", "simd": "SIMDShockHash"}
+
+
+def O_build(h, threads, variant="sh2", leaf=64):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    return O.OracleBuild("sh2", 64, h, 2000, threads=threads)
+    return O.OracleBuild(variant, leaf, h, 2000, threads=threads)
 
 
-def cpu_baseline(n_sample, threads):
+def cpu_baseline(n_sample, threads, variant="sh2", leaf=64):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from shockhash_amd import workload
     h = workload.fingerprints(n_sample, seed=7)
     t0 = time.perf_counter()
-    ob = O.OracleBuild("sh2", 64, h, 2000, threads=threads)
+    ob = O.OracleBuild(variant, leaf, h, 2000, threads=threads)
     dt = time.perf_counter() - t0
     return h, ob, dt
 
@@ -139,7 +142,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=10_000_000)
-    ap.add_argument("--plain-buckets", type=int, default=100)
+    ap.add_argument("--variant", choices=sorted(VARIANT_NAMES), default="sh2",
+                    help="class variant of the build (the headline workload is sh2 with leaf 64)")
+    ap.add_argument("--leaf", type=int, default=64, help="leaf size")
+    ap.add_argument("--plain-buckets", type=int, default=100, help="plain-form buckets (0: no plain-form run)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="CPU oracle sample size (0: the whole workload)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -171,10 +177,12 @@ def main():
     tune = {k: v for k, v in (("split_seeds_per_lane", args.split_seeds_per_lane),
                                 ("sh1_seeds_per_lane", args.leaf_seeds_per_lane),
                                 ("waves_per_cu", args.waves_per_cu)) if v is not None}
+    V, L = args.variant, args.leaf
+    headline = V == "sh2" and L == 64
     if tune:
-        eng.set_variant_tuning("sh2", **tune)
+        eng.set_variant_tuning(V, **tune)
     log(f"device {eng.arch} CUs {eng.cu_count} clock {eng.clock_khz / 1e6:.2f} GHz; "
-        f"ShockHash2 schedule {eng.variant_tuning('sh2')}")
+        f"{VARIANT_NAMES[V]} schedule {eng.variant_tuning(V)}")
     h = workload.fingerprints(args.n, seed=1)
     d_h = eng.array(h.reshape(-1))  # fingerprints resident in HBM
     log(f"{args.n} fingerprints resident on the device")
@@ -190,13 +198,13 @@ def main():
         dst.free()
 
     for i in range(args.warmup):
-        b = eng.build_dev(d_h, args.n, "sh2", 64, 2000, copy_keys=False)
+        b = eng.build_dev(d_h, args.n, V, L, 2000, copy_keys=False)
         log(f"warmup {i}: timings {np.round(b.timings_ms, 2)}")
     step_s, kernel_ms = [], []
     for i in range(args.steps):
         eng.sync()
         t0 = time.perf_counter()
-        b = eng.build_dev(d_h, args.n, "sh2", 64, 2000, copy_keys=False)
+        b = eng.build_dev(d_h, args.n, V, L, 2000, copy_keys=False)
         eng.sync()
         step_s.append(time.perf_counter() - t0)
         kernel_ms.append(b.timings_ms[3])
@@ -205,20 +213,22 @@ def main():
     ms_per_step = 1e3 * float(np.mean(step_s))
     value = args.n / (ms_per_step / 1e3)
     kms = float(np.mean(kernel_ms))
-    roof = roofline(b.nodes, b.codes, "sh2", kms, MEASURED_TRAFFIC_BYTES if args.n == 10_000_000 else None)
+    roof = roofline(b.nodes, b.codes, V, kms, MEASURED_TRAFFIC_BYTES if headline and args.n == 10_000_000 else None)
 
     # plain form: one launch per node over the first buckets, same device input
-    t0 = time.perf_counter()
-    bp = eng.build_dev(d_h, args.n, "sh2", 64, 2000, mode="plain", plain_bucket_limit=args.plain_buckets,
-                       copy_keys=False)
-    plain_s = time.perf_counter() - t0
-    n_plain_keys = int(b.size_acc[args.plain_buckets])
-    nn = len(bp.codes)
-    plain_match = bool(np.array_equal(bp.codes, b.codes[:nn]))
-    # the plain run also sorts all keys; charge it only for the searched buckets' share of sort time
-    plain_search_s = (bp.timings_ms[1] + bp.timings_ms[2]) / 1e3 + bp.timings_ms[0] / 1e3 * n_plain_keys / args.n
-    plain_value = n_plain_keys / plain_search_s
-    log(f"plain form: {args.plain_buckets} buckets ({n_plain_keys} keys) in {plain_s:.2f} s, codes equal: {plain_match}")
+    plain_value, n_plain_keys, plain_match = None, 0, True
+    if args.plain_buckets:
+        t0 = time.perf_counter()
+        bp = eng.build_dev(d_h, args.n, V, L, 2000, mode="plain", plain_bucket_limit=args.plain_buckets,
+                           copy_keys=False)
+        plain_s = time.perf_counter() - t0
+        n_plain_keys = int(b.size_acc[args.plain_buckets])
+        nn = len(bp.codes)
+        plain_match = bool(np.array_equal(bp.codes, b.codes[:nn]))
+        # the plain run also sorts all keys; charge it only for the searched buckets' share of sort time
+        plain_search_s = (bp.timings_ms[1] + bp.timings_ms[2]) / 1e3 + bp.timings_ms[0] / 1e3 * n_plain_keys / args.n
+        plain_value = n_plain_keys / plain_search_s
+        log(f"plain form: {args.plain_buckets} buckets ({n_plain_keys} keys) in {plain_s:.2f} s, codes equal: {plain_match}")
 
     cpu = None
     bit_exact = plain_match
@@ -227,27 +237,29 @@ def main():
         if n_cpu == args.n:
             hs = h
             t0 = time.perf_counter()
-            ob = O_build(hs, args.cpu_threads)
+            ob = O_build(hs, args.cpu_threads, V, L)
             dt = time.perf_counter() - t0
             bs = b  # the timed GPU build of the same keys
             sample = f"the whole workload ({args.n} keys, seed 1)"
         else:
-            hs, ob, dt = cpu_baseline(n_cpu, args.cpu_threads)
-            bs = eng.build(hs, "sh2", 64, 2000)
+            hs, ob, dt = cpu_baseline(n_cpu, args.cpu_threads, V, L)
+            bs = eng.build(hs, V, L, 2000)
             sample = f"{n_cpu} fingerprints (same generator, seed 7)"
         bit_exact = bit_exact and bool(np.array_equal(bs.codes, ob.node_code)) and \
             bool(np.array_equal(bs.size_acc, ob.size_acc))
         cpu = {"value": n_cpu / dt, "unit": "keys/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"{sample}, ShockHash2 leaf 64, bucket 2000, oracle C++ restatement, "
+               "sample": f"{sample}, {VARIANT_NAMES[V]} leaf {L}, bucket 2000, oracle C++ restatement, "
                          f"{args.cpu_threads} threads, {dt:.1f} s"}
         log(f"cpu oracle: {cpu['value']:.0f} keys/s; GPU codes equal the oracle's: {bit_exact}")
 
     cls = None
-    if not args.no_class_build:
+    if args.no_class_build or not headline:
+        log("class-constructor leg: skipped" + ("" if headline else " (ShockHash2 leaf 64 only)"))
+    else:
         cls = class_build(h, args, ob if (not args.no_cpu_baseline and (args.cpu_sample or args.n) == args.n) else None)
         bit_exact = bit_exact and cls["queries_bijective"] and cls["descriptors_equal_oracle"] is not False
 
-    serving = None if args.no_serving else serving_mix(eng, args)
+    serving = None if args.no_serving or not headline else serving_mix(eng, args)
     if serving is not None:
         bit_exact = bit_exact and serving["identical"]
 
@@ -264,12 +276,13 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": "ShockHash2 leaf 64, bucket 2000, 10M uniform 128-bit fingerprints (SplitMix64, seed 1), "
-                               "persistent device-queue build", "keys": args.n, "leaf_size": 64, "bucket_size": 2000,
-                   "variant": "ShockHash2"},
+        "config": {"workload": f"{VARIANT_NAMES[V]} leaf {L}, bucket 2000, "
+                               f"{f'{args.n // 1_000_000}M' if args.n % 1_000_000 == 0 else args.n} uniform 128-bit fingerprints "
+                               "(SplitMix64, seed 1), persistent device-queue build", "keys": args.n, "leaf_size": L,
+                   "bucket_size": 2000, "variant": VARIANT_NAMES[V]},
         "plain_form": {"keys_per_s": plain_value, "buckets": args.plain_buckets, "keys": n_plain_keys,
                        "codes_equal_persistent": plain_match},
-        "speedup_vs_plain": value / plain_value,
+        "speedup_vs_plain": value / plain_value if plain_value else None,
         "bit_exact": bit_exact,
         "search_kernel_ms": kms,
         "roofline": roof,
