@@ -60,7 +60,7 @@ typedef struct shgpu_tuning {
     uint32_t split_seeds_per_lane;  /* 1, 2 or 4: split window for aggregation nodes */
     uint32_t waves_per_cu;          /* persistent grid = CUs * this; 0: by the batch's key count (4 below
                                        350k keys, 8 below 1.5M, 12 below 3M, 16 below 5M, else 24; the
-                                       ShockHash2 default) */
+                                       ShockHash and ShockHash2 default, 24 for SIMDShockHash) */
     uint32_t flags;                 /* SHGPU_TUNE_* bits, 0 by default */
     uint64_t max_trials;            /* seed bound per split node / ShockHash1 leaf */
 } shgpu_tuning;

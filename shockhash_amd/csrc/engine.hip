@@ -150,8 +150,9 @@ struct shgpu_engine {
     char arch[64] = {0};
     // Schedule per variant (ShockHash, SIMDShockHash, ShockHash2); the defaults are the measured best on
     // MI355X for leaf 30 / 58 / 64 builds (see shgpu_autotune).
-    // ShockHash2 builds size their grid by batch (waves_per_cu 0, see waves_for)
-    shgpu_tuning tuning_v[3] = {{4, 2, 24, 0, 1ull << 40}, {4, 2, 24, 0, 1ull << 40}, {4, 2, 0, 0, 1ull << 40}};
+    // ShockHash and ShockHash2 builds size their grid by batch (waves_per_cu 0, see waves_for); SIMDShockHash
+    // leaves carry ~1000x the work per key, and 24 waves per CU are best from 1M keys on
+    shgpu_tuning tuning_v[3] = {{4, 2, 0, 0, 1ull << 40}, {4, 2, 24, 0, 1ull << 40}, {4, 2, 0, 0, 1ull << 40}};
     shgpu_tuning tuning{4, 2, 24, 0, 1ull << 40};  // batched per-node entry points
     DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
     BuildBuffers main;                          // shgpu_build_run
@@ -946,7 +947,9 @@ int check_variant(int variant, uint32_t leaf) {
 // Measured on ShockHash2 leaf 64, bucket 2000, kernel ms at 4 / 8 / 12 / 16 / 24 waves per CU: 200k keys
 // 2.5 / 2.8 / 3.7 / 4.7 / 6.3, 500k 4.2 / 3.8 / - / - / -, 1M 6.6 / 5.1 / 5.3 / 5.8 / 7.6, 2M 12.6 / 8.7 /
 // 7.9 / 8.0 / 10.2, 4M - / - / 14.0 / 13.4 / 14.4, 6M - / - / 20.1 / 19.2 / 19.1, 10M - / 38.2 / 33.0 /
-// 30.9 / 29.7. Small builds are set by their longest chain of dependent searches, and fewer resident
+// 30.9 / 29.7. ShockHash leaf 30 has the same optimum at every size (200k: 1.9 ms at 4 waves, 3.7 at 12;
+// 1M: 5.0 at 8, 8.1 at 24; 2M: 8.1 at 12; 4M: 14.0 at 16; 10M: 32.5 at 24, 38.4 at 8). Small builds are set
+// by their longest chain of dependent searches, and fewer resident
 // waves (fewer idle pollers) leave each searching wave more of its SIMD and of the memory system; large
 // ones need the waves to hide latency.
 uint32_t waves_for(const shgpu_tuning &t, uint64_t n_keys) {
