@@ -93,6 +93,8 @@ int shgpu_set_variant_tuning(shgpu_engine *e, int variant, const shgpu_tuning *t
 /* Synchronises the engine's stream. */
 int shgpu_sync(shgpu_engine *e);
 
+/* Free and total bytes of the engine's device (hipMemGetInfo), e.g. to size a serving HBM budget. */
+int shgpu_device_memory(shgpu_engine *e, size_t *free_bytes, size_t *total_bytes);
 /* Device memory helpers (ordered on the engine's stream; the copies are synchronous). */
 int shgpu_device_alloc(shgpu_engine *e, size_t bytes, void **d_out);
 int shgpu_device_free(shgpu_engine *e, void *d);

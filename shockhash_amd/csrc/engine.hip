@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/shockhash_amd.h"
+#include "../include/shockhash/GolombMemo.h"
 #include "search_sh1.h"
 #include "search_sh2.h"
 #include "search_split.h"
@@ -39,6 +40,7 @@ constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
 constexpr int BLOCK = 256;
 constexpr int WAVES_PER_BLOCK = BLOCK / 64;
 constexpr uint32_t MAX_ENCODABLE_BUCKET = 2999;  // memo tables cover sizes < MAX_BUCKET_SIZE = 3000
+constexpr uint64_t DEFAULT_MAX_TRIALS = 1ull << 40;  // schedule cap of every node's seed bound
 
 thread_local std::string g_err;
 
@@ -149,11 +151,12 @@ struct shgpu_engine {
     int clock_khz = 0;
     char arch[64] = {0};
     // Schedule per variant (ShockHash, SIMDShockHash, ShockHash2); the defaults are the measured best on
-    // MI355X for leaf 30 / 58 / 64 builds (see shgpu_autotune).
-    // ShockHash and ShockHash2 builds size their grid by batch (waves_per_cu 0, see waves_for); SIMDShockHash
-    // leaves carry ~1000x the work per key, and 24 waves per CU are best from 1M keys on
-    shgpu_tuning tuning_v[3] = {{4, 2, 0, 0, 1ull << 40}, {4, 2, 24, 0, 1ull << 40}, {4, 2, 0, 0, 1ull << 40}};
-    shgpu_tuning tuning{4, 2, 24, 0, 1ull << 40};  // batched per-node entry points
+    // MI355X for leaf 30 / 58 / 64 builds. waves_per_cu 0 selects the variant's rule (waves_for): by the
+    // batch's key count for ShockHash and ShockHash2, 24 for SIMDShockHash, whose leaves carry ~1000x the
+    // work per key.
+    shgpu_tuning tuning_v[3] = {{4, 2, 0, 0, DEFAULT_MAX_TRIALS}, {4, 2, 0, 0, DEFAULT_MAX_TRIALS},
+                                {4, 2, 0, 0, DEFAULT_MAX_TRIALS}};
+    shgpu_tuning tuning{4, 2, 0, 0, DEFAULT_MAX_TRIALS};  // batched per-node entry points (waves 0: 24 per CU)
     DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
     BuildBuffers main;                          // shgpu_build_run
     DevBuf retrieval;                           // shgpu_retrieval_build working memory (~33 B per key)
@@ -325,6 +328,9 @@ __global__ void k_bucket_bounds(const uint64_t *__restrict__ first, uint64_t n, 
 // gd: bit 63 set once a qualifying seed is known (GD_FOUND), bits 32..62 windows handed out, bits 0..31
 // windows completed. One read-modify-write on it both completes a window and hands out the next.
 constexpr unsigned long long GD_FOUND = 1ull << 63, GD_HANDED = 1ull << 32;
+// Windows a node's search covers at most. The handed-out count has 31 bits; past the cap, every wave that joins
+// a node still takes (and completes unsearched) one window, so the cap leaves 2^24 windows of margin.
+constexpr uint64_t WINDOW_CAP = 0x7F000000ull;
 __device__ __forceinline__ uint64_t gd_handed(unsigned long long gd) { return (gd >> 32) & 0x7FFFFFFFull; }
 __device__ __forceinline__ uint64_t gd_completed(unsigned long long gd) { return gd & 0xFFFFFFFFull; }
 
@@ -414,9 +420,11 @@ template <int VARIANT, int SL, int SS>
 __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent, uint64_t t_start) {
     const shgpu_node nd = a.nodes[nid];
     const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
-    const uint64_t start = uniform64(nd.start_seed), lim = uniform64(nd.max_trials);
     NodeState *st = &a.state[nid];
     const uint64_t W = window_seeds<SL, SS>(kind);
+    // The window count has 31 bits: a bound beyond WINDOW_CAP windows ends the search at the cap, and
+    // the finish rule below uses the same limit, so a node that fails is finalised as not found.
+    const uint64_t start = uniform64(nd.start_seed), lim = min(uniform64(nd.max_trials), WINDOW_CAP * W);
     constexpr int SPLIT_HASH = VARIANT == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
     SplitShapeDev sh;
     if (kind != SHGPU_NODE_LEAF) sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
@@ -438,7 +446,7 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
     for (;;) {
         age_priority(t_start);
         const uint64_t woff = w * W;
-        const bool exhausted = woff >= lim || w >= 0x7FFFFFF0ull;
+        const bool exhausted = woff >= lim;
         uint64_t r = SHGPU_NOT_FOUND;
         if (!exhausted && !found) {
             if (kind == SHGPU_NODE_LEAF) {
@@ -595,11 +603,12 @@ __global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
     work_windows<VARIANT, SL, SS>(a, nid, L.pf, false, task_clock());
 }
 
-__global__ void k_init_states(NodeState *st, uint32_t n, uint32_t *help, uint32_t nslots) {
+__global__ void k_init_states(NodeState *st, uint64_t *codes, uint32_t n, uint32_t *help, uint32_t nslots) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         st[i].gd = 0;
         st[i].best = SHGPU_NOT_FOUND;
         st[i].finalized = 0;
+        codes[i] = SHGPU_NOT_FOUND;  // a node that is never finalised reads as not found
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) help[i] = EMPTY_SLOT;
 }
@@ -631,6 +640,7 @@ struct PlanArgs {
     uint32_t nacc;                   // entries of size_acc over all sets
     const uint32_t *cnt_all;         // nodes of a tree of m keys, m <= MAX_ENCODABLE_BUCKET
     const uint32_t *cnt_split;       // non-leaf nodes of that tree
+    const uint32_t *bound_lg;        // log2 of the default seed bound of a node of m keys (seed_bound_log2)
     uint4 *bucket;                   // per size_acc entry: first node, split roots before, leaf roots before
     uint32_t *summary;
     uint32_t capacity;               // node capacity of nodes / codes / queue / states
@@ -639,6 +649,7 @@ struct PlanArgs {
     uint32_t *help;
     uint32_t nhelp;
     shgpu_node *nodes;
+    uint64_t *codes;
     NodeState *state;
     uint32_t *queue;
     int variant;
@@ -756,7 +767,7 @@ __device__ __forceinline__ uint32_t plan_node(const PlanArgs &p, uint32_t offset
                                               shgpu_node &nd, uint32_t (&child_off)[4], uint32_t (&child_m)[4]) {
     nd.offset = offset;
     nd.m = (uint16_t)m;
-    nd.max_trials = p.max_trials;
+    nd.max_trials = min(p.max_trials, 1ull << p.bound_lg[m]);
     nd.subtree = p.cnt_all[m];
     nd.unit = 0;
     nd.n_children = 0;
@@ -827,6 +838,7 @@ __global__ void __launch_bounds__(256) k_plan_emit(PlanArgs p) {
         uint32_t co[4], cm[4];
         const uint32_t nc = plan_node(p, off, mm, level, nd, co, cm);
         p.nodes[idx] = nd;
+        p.codes[idx] = SHGPU_NOT_FOUND;  // a node that is never finalised reads as not found
         NodeState s;
         s.gd = 0;
         s.best = SHGPU_NOT_FOUND;
@@ -860,16 +872,55 @@ struct Strategy {
     }
 };
 
+// Default seed bound of a node of m keys: 2^(k(m) + SEED_BOUND_SLACK) seeds, where k(m) is the Golomb-Rice
+// parameter of the node's code (include/ShockHash.h:53-61,79-127; include/ShockHash2.h:33-47,63-115). Codes
+// are geometric with mean about 2^k(m), so a search that runs 2^24 times its mean has failed for good (its
+// keys repeat: three equal keys in one ShockHash leaf never form a pseudoforest, and the reference then loops
+// forever, include/SIMDShockHash.hpp:131,171). The schedule's max_trials caps it further. Returned as log2
+// per node size, evaluated with libm at run time (a bound, not part of the stored format).
+constexpr uint32_t SEED_BOUND_SLACK = 24;
+std::vector<uint32_t> seed_bound_log2(const Strategy &st) {
+    std::vector<uint32_t> lg(MAX_ENCODABLE_BUCKET + 1, 0);
+    for (uint32_t m = 0; m <= MAX_ENCODABLE_BUCKET; m++) {
+        uint32_t k;
+        if (m <= st.leaf) {
+            k = st.variant == SHGPU_SHOCKHASH2 ? shockhash::bij_memo2[std::min<uint32_t>(m, shockhash::MAX_LEAF_SIZE2)]
+                                               : shockhash::bij_memo[std::min<uint32_t>(m, shockhash::MAX_LEAF_SIZE)];
+        } else {
+            uint32_t fanout, unit;
+            if (m > st.upper) { unit = st.upper * (uint16_t(m / 2 + st.upper - 1) / st.upper); fanout = 2; }
+            else if (m > st.lower) { unit = st.lower; fanout = uint16_t(m + st.lower - 1) / st.lower; }
+            else { unit = st.leaf; fanout = uint16_t(m + st.leaf - 1) / st.leaf; }
+            double sqrt_prod = std::sqrt((double)(m - (fanout - 1) * unit));
+            for (uint32_t i = 0; i + 1 < fanout; i++) sqrt_prod *= std::sqrt((double)unit);
+            const double p = std::sqrt((double)m) / (std::pow(2 * M_PI, (fanout - 1.) / 2) * sqrt_prod);
+            const double len = std::ceil(std::log2(-std::log((std::sqrt(5) + 1) / 2) / std::log1p(-p)));
+            k = len > 0 ? (uint32_t)std::min(len, 40.0) : 0u;
+        }
+        lg[m] = std::min<uint32_t>(63, k + SEED_BOUND_SLACK);
+    }
+    return lg;
+}
+
+// Per node size: min(cap, 2^seed_bound_log2), the max_trials of every node of that size.
+std::vector<uint64_t> node_bounds(const Strategy &st, uint64_t cap) {
+    const std::vector<uint32_t> lg = seed_bound_log2(st);
+    std::vector<uint64_t> b(lg.size());
+    for (size_t m = 0; m < lg.size(); m++) b[m] = std::min<uint64_t>(cap, 1ull << lg[m]);
+    return b;
+}
+
 // Emits the subtree of a node of m keys at `offset` in DFS pre-order (recSplit order,
-// ShockHash.h:303-555 / SIMDShockHash.hpp:563-617 / ShockHash2.h:259-394).
+// ShockHash.h:303-555 / SIMDShockHash.hpp:563-617 / ShockHash2.h:259-394). bounds[m]: the max_trials of a
+// node of m keys.
 void emit_subtree(std::vector<shgpu_node> &out, const Strategy &st, uint32_t offset, uint32_t m, int level,
-                  uint64_t max_trials) {
+                  const std::vector<uint64_t> &bounds) {
     const size_t me = out.size();
     out.push_back(shgpu_node{});
     shgpu_node nd{};
     nd.offset = offset;
     nd.m = (uint16_t)m;
-    nd.max_trials = max_trials;
+    nd.max_trials = bounds[m];
     if (m <= st.leaf) {
         nd.kind = SHGPU_NODE_LEAF;
         nd.start_seed = st.variant == SHGPU_SHOCKHASH2 ? 0
@@ -895,17 +946,17 @@ void emit_subtree(std::vector<shgpu_node> &out, const Strategy &st, uint32_t off
     nd.unit = unit;
     uint32_t nchild = 0, i = 0;
     if (nd.kind == SHGPU_NODE_SPLIT2) {  // ShockHash.h:479-480
-        emit_subtree(out, st, offset, unit, level + 1, max_trials);
+        emit_subtree(out, st, offset, unit, level + 1, bounds);
         nchild++;
         i = unit;
     } else {  // ShockHash.h:510-514,543-547
         for (; i < m - unit; i += unit) {
-            emit_subtree(out, st, offset + i, unit, level + 1, max_trials);
+            emit_subtree(out, st, offset + i, unit, level + 1, bounds);
             nchild++;
         }
     }
     if (m - i > 1) {  // the last part only when it has at least 2 keys
-        emit_subtree(out, st, offset + i, m - i, level + 1, max_trials);
+        emit_subtree(out, st, offset + i, m - i, level + 1, bounds);
         nchild++;
     }
     nd.n_children = (uint8_t)nchild;
@@ -952,15 +1003,17 @@ int check_variant(int variant, uint32_t leaf) {
 // by their longest chain of dependent searches, and fewer resident
 // waves (fewer idle pollers) leave each searching wave more of its SIMD and of the memory system; large
 // ones need the waves to hide latency.
-uint32_t waves_for(const shgpu_tuning &t, uint64_t n_keys) {
+uint32_t waves_for(const shgpu_tuning &t, int variant, uint64_t n_keys) {
     if (t.waves_per_cu) return t.waves_per_cu;
+    if (variant == SHGPU_SIMD_SHOCKHASH) return 24u;
     return n_keys < 350000 ? 4u : n_keys < 1500000 ? 8u : n_keys < 3000000 ? 12u : n_keys < 5000000 ? 16u : 24u;
 }
 
-int grid_for(const shgpu_engine *e, const shgpu_tuning &t, uint64_t n_keys = ~0ull) {
-    return std::max(1, e->cu_count * (int)waves_for(t, n_keys) / WAVES_PER_BLOCK);
+int grid_for(const shgpu_engine *e, const shgpu_tuning &t, int variant, uint64_t n_keys) {
+    return std::max(1, e->cu_count * (int)waves_for(t, variant, n_keys) / WAVES_PER_BLOCK);
 }
-int grid_for(const shgpu_engine *e) { return grid_for(e, e->tuning); }
+// Batched per-node entry points: the engine-wide schedule, 24 waves per CU unless it names a count.
+int grid_for(const shgpu_engine *e) { return grid_for(e, e->tuning, -1, ~0ull); }
 
 hipStream_t pick(shgpu_engine *e, void *stream) { return stream ? (hipStream_t)stream : e->stream; }
 
@@ -1225,12 +1278,13 @@ static int subtree_counts(const Strategy &st, std::vector<uint32_t> &all, std::v
 // Host copy of a build's nodes (DFS pre-order per bucket, offsets within the build's keys).
 static int host_plan(const shgpu_build &b, std::vector<shgpu_node> &out) {
     const Strategy st(b.variant, b.leaf);
+    const std::vector<uint64_t> bounds = node_bounds(st, b.max_trials);
     out.clear();
     out.reserve(b.codes.size());
     const uint64_t nb = b.plan_buckets;
     for (uint64_t k = 0; k < nb; k++) {
         const uint64_t m = b.size_acc[k + 1] - b.size_acc[k];
-        if (m > 1) emit_subtree(out, st, (uint32_t)b.size_acc[k], (uint32_t)m, 0, b.max_trials);
+        if (m > 1) emit_subtree(out, st, (uint32_t)b.size_acc[k], (uint32_t)m, 0, bounds);
     }
     if (out.size() != b.codes.size()) return fail(SHGPU_ERR_HIP, "host plan and device plan differ in node count");
     return SHGPU_OK;
@@ -1323,7 +1377,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     }
     BATCH_HIP(hipEventRecord(B.evB, s));
 
-    const int grid = std::max(1, grid_for(e, tune, N) / (int)std::max(1u, grid_div));
+    const int grid = std::max(1, grid_for(e, tune, d->variant, N) / (int)std::max(1u, grid_div));
     const uint32_t nslots = (uint32_t)grid * WAVES_PER_BLOCK;
     std::vector<uint64_t> acc_all(NB);
     std::vector<uint64_t> codes;
@@ -1349,8 +1403,10 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         if (B.plan_variant != d->variant || B.plan_leaf != d->leaf_size) {
             std::vector<uint32_t> all, split;
             if ((rc = subtree_counts(st, all, split))) return fail_all(rc);
+            const std::vector<uint32_t> lg = seed_bound_log2(st);
             B.plan_counts = all;
             B.plan_counts.insert(B.plan_counts.end(), split.begin(), split.end());
+            B.plan_counts.insert(B.plan_counts.end(), lg.begin(), lg.end());
             if ((rc = B.plan_tab.ensure(B.plan_counts.size() * 4))) return fail_all(rc);
             BATCH_HIP(hipMemcpyAsync(B.plan_tab.p, B.plan_counts.data(), B.plan_counts.size() * 4, hipMemcpyHostToDevice, s));
             B.plan_variant = d->variant;
@@ -1390,6 +1446,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             p.nacc = (uint32_t)NB;
             p.cnt_all = B.plan_tab.as<uint32_t>();
             p.cnt_split = B.plan_tab.as<uint32_t>() + MAX_ENCODABLE_BUCKET + 1;
+            p.bound_lg = B.plan_tab.as<uint32_t>() + 2 * (MAX_ENCODABLE_BUCKET + 1);
             p.bucket = B.plan_bucket.as<uint4>();
             p.summary = B.plan_summary.as<uint32_t>();
             p.capacity = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
@@ -1398,6 +1455,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             p.help = B.help.as<uint32_t>();
             p.nhelp = nslots;
             p.nodes = B.nodes.as<shgpu_node>();
+            p.codes = B.codes.as<uint64_t>();
             p.state = B.states.as<NodeState>();
             p.queue = B.queue.as<uint32_t>();
             p.variant = d->variant;
@@ -1479,6 +1537,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         if (sort_rerun) return run_batch(e, B, s, grid_div, d, in, count, out, status, true);
         t1 = clk::now();
         std::vector<shgpu_node> nodes;
+        const std::vector<uint64_t> bounds = node_bounds(st, tune.max_trials);
         uint32_t roots = 0;
         for (size_t r = 0; r < count; r++) {
             node_begin[r] = node_end[r] = nodes.size();
@@ -1498,7 +1557,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
                 const uint64_t m = acc[k + 1] - acc[k];
                 if (m > 1) {
                     roots++;
-                    emit_subtree(nodes, st, (uint32_t)(base[r] + acc[k]), (uint32_t)m, 0, tune.max_trials);
+                    emit_subtree(nodes, st, (uint32_t)(base[r] + acc[k]), (uint32_t)m, 0, bounds);
                 }
             }
             node_end[r] = nodes.size();
@@ -1525,7 +1584,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         if ((rc = B.states.ensure((size_t)total * sizeof(NodeState) + 64))) return fail_all(rc);
         a.state = B.states.as<NodeState>();
         if (total) {
-            k_init_states<<<256, 256, 0, s>>>(a.state, total, nullptr, 0);
+            k_init_states<<<256, 256, 0, s>>>(a.state, a.codes, total, nullptr, 0);
             BATCH_HIP(hipGetLastError());
         }
         const uint32_t waves = (uint32_t)e->cu_count;  // one wave per CU joins every node's windows
@@ -1660,12 +1719,13 @@ int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc,
     int rc = check_variant(variant, leaf_size);
     if (rc) return rc;
     const Strategy st(variant, leaf_size);
+    const std::vector<uint64_t> bounds = node_bounds(st, DEFAULT_MAX_TRIALS);
     std::vector<shgpu_node> nodes;
     uint32_t roots = 0;
     for (uint64_t k = 0; k < nbuckets; k++) {
         const uint64_t m = size_acc[k + 1] - size_acc[k];
         if (m > MAX_ENCODABLE_BUCKET) return fail(SHGPU_ERR_BUCKET_TOO_LARGE, "bucket too large");
-        if (m > 1) { roots++; emit_subtree(nodes, st, (uint32_t)size_acc[k], (uint32_t)m, 0, 1ull << 40); }
+        if (m > 1) { roots++; emit_subtree(nodes, st, (uint32_t)size_acc[k], (uint32_t)m, 0, bounds); }
     }
     if ((rc = check_forest(nodes, roots))) return rc;
     *count = nodes.size();
@@ -1721,6 +1781,13 @@ int shgpu_build_run_host(shgpu_engine *e, const shgpu_build_desc *desc, const ui
     if (rc) return rc;
     if (n) HIP_TRY(hipMemcpyAsync(e->staging.p, h_hashes, n * 16, hipMemcpyHostToDevice, e->stream));
     return shgpu_build_run(e, desc, e->staging.as<uint64_t>(), n, out);
+}
+
+int shgpu_device_memory(shgpu_engine *e, size_t *free_bytes, size_t *total_bytes) {
+    if (!e || !free_bytes || !total_bytes) return fail(SHGPU_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+    return SHGPU_OK;
 }
 
 int shgpu_device_alloc(shgpu_engine *e, size_t bytes, void **d_out) {
@@ -1859,7 +1926,7 @@ struct shgpu_server {
         return cfg.variant == SHGPU_SHOCKHASH2 ? cfg.slots * slot_waves() * SH2_SCRATCH_BYTES : 0;
     }
     size_t slot_waves() const {
-        const int grid = std::max(1, grid_for(e, e->tuning_v[cfg.variant]) / (int)cfg.slots);
+        const int grid = std::max(1, grid_for(e, e->tuning_v[cfg.variant], cfg.variant, ~0ull) / (int)cfg.slots);
         return (size_t)grid * WAVES_PER_BLOCK;
     }
     size_t allocated() const {
@@ -1982,11 +2049,17 @@ int shgpu_server_create(shgpu_engine *e, const shgpu_server_config *cfg, shgpu_s
     }
     for (uint32_t i = 0; i < cfg->slots; i++) {
         auto sl = std::make_unique<shgpu_server::Slot>();
-        if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess || sl->B.create_events()) {
-            sv->stop = true;
-            for (auto &x : sv->slots) x->worker.join();
+        const bool stream_ok = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) == hipSuccess;
+        if (!stream_ok || sl->B.create_events()) {
+            // no worker has started yet (they start below): release what the slots created so far
+            if (stream_ok) (void)hipStreamDestroy(sl->stream);
+            sl->B.destroy_events();
+            for (auto &x : sv->slots) {
+                (void)hipStreamDestroy(x->stream);
+                x->B.destroy_events();
+            }
             delete sv;
-            return fail(SHGPU_ERR_HIP, "stream creation failed");
+            return fail(SHGPU_ERR_HIP, "stream or event creation failed");
         }
         sv->slots.push_back(std::move(sl));
     }

@@ -103,6 +103,7 @@ _SIGS = {
     "shgpu_build_keys": (ctypes.c_int, [P, P]),
     "shgpu_build_device_keys": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "shgpu_build_timings": (ctypes.c_int, [P, P]),
+    "shgpu_device_memory": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "shgpu_device_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
     "shgpu_device_free": (ctypes.c_int, [P, P]),
     "shgpu_copy_to_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
@@ -311,6 +312,12 @@ class Engine:
 
     def sync(self):
         _check(load().shgpu_sync(self.h))
+
+    def memory(self):
+        """(free, total) bytes of the engine's device."""
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(load().shgpu_device_memory(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
 
     # ---- batched searches on device arrays (DeviceArray)
     def sh1_leaf_search_dev(self, d_keys, d_tasks, n, d_out):

@@ -28,6 +28,7 @@ _SIGS = {
     "shh_phase_timings": (None, [P, P]),
     "shh_keep_retrieval_input": (None, [ctypes.c_int]),
     "shh_memo": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P]),
+    "shh_last_status": (ctypes.c_int, []),
 }
 VARIANTS = {"sh1": 0, "simd": 1, "sh2": 2}
 _lib = None
@@ -65,6 +66,14 @@ def _pack_strings(strings):
     return data, offs
 
 
+class ConstructionError(RuntimeError):
+    """A class constructor threw; `status` is the engine's SHGPU_* code (-1: another exception)."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"construction failed (status {status}): {msg}")
+        self.status = status
+
+
 class MPHF:
     """A ShockHash / SIMDShockHash / ShockHash2 instance built by the C++ class template."""
 
@@ -85,7 +94,7 @@ class MPHF:
                               512)
             self.n = arr.shape[0]
         if not h:
-            raise RuntimeError(f"construction failed: {err.value.decode(errors='replace')}")
+            raise ConstructionError(lib.shh_last_status(), err.value.decode(errors="replace"))
         self.h = h
 
     def __call__(self, keys):

@@ -92,6 +92,14 @@ void set_err(char *err, size_t len, const char *msg) {
     if (err && len) std::snprintf(err, len, "%s", msg);
 }
 
+// Status of the calling thread's last failed construction: the engine's status code for a gpu::Error, -1 for
+// any other exception.
+thread_local int g_status = 0;
+void set_status(const std::exception &e) {
+    const auto *ge = dynamic_cast<const gpu::Error *>(&e);
+    g_status = ge ? ge->status : -1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -107,6 +115,7 @@ void *shh_build(int variant, int leaf, const uint64_t *hashes, size_t n, size_t 
         return hd;
     } catch (const std::exception &e) {
         set_err(err, errlen, e.what());
+        set_status(e);
         return nullptr;
     }
 }
@@ -122,11 +131,15 @@ void *shh_build_strings(int variant, int leaf, const char *data, const uint64_t 
         return hd;
     } catch (const std::exception &e) {
         set_err(err, errlen, e.what());
+        set_status(e);
         return nullptr;
     }
 }
 
 void shh_free(void *h) { delete (Handle *)h; }
+
+// Status of the calling thread's last failed shh_build / shh_build_strings (SHGPU_* code, -1 otherwise).
+int shh_last_status(void) { return g_status; }
 
 void shh_query(void *h, const uint64_t *hashes, size_t n, uint64_t *out) {
     Handle *x = (Handle *)h;

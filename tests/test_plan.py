@@ -44,3 +44,16 @@ def test_oversized_bucket_rejected():
     with pytest.raises(E.ShgpuError) as ei:
         E.plan_forest("sh2", 64, np.array([0, 3000], dtype=np.uint64))
     assert ei.value.status == E.ERR_BUCKET_TOO_LARGE
+
+
+@pytest.mark.parametrize("variant,leaf", [("sh2", 64), ("sh1", 30), ("simd", 58), ("sh1", 8), ("sh2", 16)])
+def test_node_seed_bounds_follow_the_golomb_parameter(variant, leaf):
+    """Every planned node's max_trials is min(2^40, 2^(k(m) + 24)), k(m) the Golomb-Rice parameter of a node of
+    m keys in the class templates' compile-time memo (the engine evaluates the same formula at run time)."""
+    from shockhash_amd import host as H
+    size_acc = np.array([0, 2000, 3999, 6000, 6100, 6102, 8999], dtype=np.uint64)
+    nodes = E.plan_forest(variant, leaf, size_acc)
+    memo = H.memo(variant, leaf)
+    k = (memo[nodes["m"].astype(np.int64)] >> np.uint64(27)).astype(np.int64)
+    want = np.minimum(np.uint64(1 << 40), np.left_shift(np.uint64(1), (k + 24).astype(np.uint64)))
+    assert np.array_equal(nodes["max_trials"], want)
