@@ -7,6 +7,21 @@
 
 namespace shgpu {
 
+// hipMalloc that leaves no error behind when it fails: a failed allocation sets the runtime's last error, and
+// the next hipGetLastError (after a later kernel launch) would report it as that launch's failure.
+inline bool device_malloc(void **p, size_t bytes) {
+    if (hipMalloc(p, bytes) == hipSuccess) return true;
+    *p = nullptr;
+    (void)hipGetLastError();
+    return false;
+}
+inline bool host_malloc_pinned(void **p, size_t bytes) {
+    if (hipHostMalloc(p, bytes, hipHostMallocDefault) == hipSuccess) return true;
+    *p = nullptr;
+    (void)hipGetLastError();
+    return false;
+}
+
 // Stafford Mix13, the 64-bit finaliser the reference calls `remix`
 // (in-tree text: include/shockhash/SimdUtils.h:42-46).
 __device__ __forceinline__ uint64_t remix(uint64_t z) {

@@ -63,7 +63,7 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
-        if (hipMalloc(&p, want) != hipSuccess) {
+        if (!device_malloc(&p, want)) {
             p = nullptr;
             return fail(SHGPU_ERR_NOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
         }
@@ -87,7 +87,7 @@ struct PinnedBuf {
         if (p) (void)hipHostFree(p);
         p = nullptr;
         bytes = 0;
-        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        if (!host_malloc_pinned(&p, want)) {
             p = nullptr;
             return fail(SHGPU_ERR_NOMEM, "hipHostMalloc of " + std::to_string(want) + " bytes failed");
         }
@@ -1615,7 +1615,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             if (status[r] != SHGPU_OK) continue;
             shgpu_build &b = *bs[r];
             b.device = e->device;
-            if (hipMalloc(&b.d_keys, b.n * 8 + 8) != hipSuccess) {
+            if (!device_malloc(&b.d_keys, b.n * 8 + 8)) {
                 b.d_keys = nullptr;
                 return fail_all(fail(SHGPU_ERR_NOMEM, "hipMalloc of the build's device keys failed"));
             }
@@ -1793,7 +1793,7 @@ int shgpu_device_memory(shgpu_engine *e, size_t *free_bytes, size_t *total_bytes
 int shgpu_device_alloc(shgpu_engine *e, size_t bytes, void **d_out) {
     if (!e || !d_out) return fail(SHGPU_ERR_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(e->device));
-    if (hipMalloc(d_out, bytes ? bytes : 1) != hipSuccess) return fail(SHGPU_ERR_NOMEM, "hipMalloc failed");
+    if (!device_malloc(d_out, bytes ? bytes : 1)) return fail(SHGPU_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     return SHGPU_OK;
 }
 

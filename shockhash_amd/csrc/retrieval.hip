@@ -335,7 +335,7 @@ int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint
             if (*cache) (void)hipFree(*cache);
             *cache = nullptr;
             *cache_bytes = 0;
-            if (hipMalloc(cache, bytes) != hipSuccess) {
+            if (!shgpu::device_malloc(cache, bytes)) {
                 *cache = nullptr;
                 err = "retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed";
                 return SHGPU_ERR_NOMEM;
@@ -344,7 +344,7 @@ int shgpu_retrieval_build_impl(hipStream_t s, const uint64_t *d_keys, const uint
         }
         base = *cache;
     } else {
-        if (hipMalloc(&buf.p, bytes) != hipSuccess) { err = "retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed"; return SHGPU_ERR_NOMEM; }
+        if (!shgpu::device_malloc(&buf.p, bytes)) { err = "retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed"; return SHGPU_ERR_NOMEM; }
         base = buf.p;
     }
     char *q = (char *)base;
@@ -406,7 +406,7 @@ int shgpu_leaf_orientation_impl(hipStream_t s, int bipartite, const uint64_t *d_
                                 uint64_t n, uint64_t *d_out_keys, uint8_t *d_out_bits, std::string &err) {
     if (n == 0) return SHGPU_OK;
     RtBuf flag;
-    if (hipMalloc(&flag.p, 64) != hipSuccess) { err = "orientation: hipMalloc failed"; return SHGPU_ERR_NOMEM; }
+    if (!shgpu::device_malloc(&flag.p, 64)) { err = "orientation: hipMalloc failed"; return SHGPU_ERR_NOMEM; }
     uint32_t *d_err = (uint32_t *)flag.p;
     k_zero_u32<<<1, 1, 0, s>>>(d_err);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + ORIENT_BLOCK - 1) / ORIENT_BLOCK, 4096);
