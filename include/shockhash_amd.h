@@ -119,6 +119,19 @@ int shgpu_sh2_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_l
 int shgpu_split_search(shgpu_engine *e, uint64_t *d_keys, const shgpu_split_task *d_tasks, uint32_t n_tasks,
                        uint64_t *d_seeds, void *stream);
 
+/* shgpu_split_search without any allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its partition staging and work counter live in a fixed per-device scratch array of the library
+ * (SHGPU_NOALLOC_MAX_KEYS keys, 8 MiB per GPU, held until the process ends); it calls no hipMalloc/hipFree and
+ * does not synchronise. n_keys: length of the d_keys array (at most SHGPU_NOALLOC_MAX_KEYS, else SHGPU_ERR_ARG).
+ * The tasks are validated on the device instead of the host: a task that shgpu_split_search would reject (or one
+ * reaching past n_keys) leaves its keys untouched and gets d_seeds[i] = SHGPU_NOT_FOUND, like a search that
+ * exhausts its bound. Results equal shgpu_split_search's. Calls on one GPU share the scratch: calls that overlap
+ * on one GPU (other streams, other engines, other threads) race and must be serialised by the caller. */
+#define SHGPU_NOALLOC_MAX_KEYS (1u << 20)
+int shgpu_split_search_noalloc(shgpu_engine *e, uint64_t *d_keys, uint64_t n_keys, const shgpu_split_task *d_tasks,
+                               uint32_t n_tasks, uint64_t *d_seeds, void *stream);
+
 /* Whole-construction seed search. d_hashes: n 128-bit fingerprints as (first, second) pairs. Sorts by
  * first, buckets by remap128(first, nbuckets), builds the RecSplit tree of every bucket and searches
  * every node's seed. Synchronous; the result owns copies of everything the encoder needs. */

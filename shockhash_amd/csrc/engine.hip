@@ -144,8 +144,37 @@ struct BuildBuffers {
     }
 };
 
+// Scratch of shgpu_split_search_noalloc, which allocates nothing: the partition's staging copy of up to
+// SHGPU_NOALLOC_MAX_KEYS keys, then the work counter (16 words). Every GPU the library runs on holds its own copy
+// (8 MiB) until the process ends; host code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (noalloc_scratch_of).
+static __device__ __align__(256) uint64_t scratch[SHGPU_NOALLOC_MAX_KEYS + 8];
+
+namespace {
+std::mutex g_scratch_mu;
+std::unordered_map<int, void *> g_scratch_ptr;  // device -> its copy of `scratch`
+
+// The device address of `scratch` on `device` (current), looked up once per device.
+int noalloc_scratch_of(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return SHGPU_OK;
+    }
+    void *ptr = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr)
+        return fail(SHGPU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+    g_scratch_ptr[device] = ptr;
+    *out = ptr;
+    return SHGPU_OK;
+}
+}  // namespace
+
 struct shgpu_engine {
     int device = 0;
+    void *noalloc_scratch = nullptr;  // this device's copy of `scratch` (device address)
     hipStream_t stream = nullptr;
     int cu_count = 0;
     int clock_khz = 0;
@@ -235,15 +264,32 @@ __global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict_
     }
 }
 
-template <int S>
+// A split task the kernels can run on a key array of n_keys keys: 2 <= fanout <= 4, unit < m <= 65535, the
+// node inside the array, a known kind and hash flavour (the checks shgpu_split_search makes on the host).
+__device__ __forceinline__ bool split_task_valid(const shgpu_split_task &tk, uint64_t n_keys) {
+    const uint32_t m = uniform(tk.m), unit = uniform(tk.unit), kind = uniform(tk.kind);
+    if (m > 65535 || unit == 0 || unit >= m || kind > SHGPU_NODE_AGGR_LOWER || uniform(tk.hash) > SHGPU_SPLIT_SIMD)
+        return false;
+    const uint32_t fanout = kind == SHGPU_NODE_SPLIT2 ? 2u : (m + unit - 1) / unit;
+    return fanout >= 2 && fanout <= 4 && (uint64_t)uniform(tk.offset) + m <= n_keys;
+}
+
+// CHECK: validate every task on the device (split_task_valid against n_keys); an invalid task leaves its keys
+// untouched and reports SHGPU_NOT_FOUND.
+template <int S, bool CHECK>
 __global__ void __launch_bounds__(BLOCK) k_split_batch(uint64_t *keys, uint64_t *tmp, const shgpu_split_task *__restrict__ tasks,
-                                                       uint32_t n, uint64_t *__restrict__ out, uint32_t *counter) {
+                                                       uint32_t n, uint64_t *__restrict__ out, uint32_t *counter,
+                                                       uint64_t n_keys) {
     __shared__ PfLds lds[WAVES_PER_BLOCK];
     uint64_t *stage = lds[threadIdx.x / 64].stage;
     for (;;) {
         const uint32_t t = grab(counter);
         if (t >= n) break;
         const shgpu_split_task tk = tasks[t];
+        if (CHECK && !split_task_valid(tk, n_keys)) {
+            out[t] = SHGPU_NOT_FOUND;
+            continue;
+        }
         const uint32_t off = uniform(tk.offset), m = uniform(tk.m);
         const SplitShapeDev sh = make_split_shape(m, uniform(tk.unit), uniform(tk.kind), uniform(tk.hash));
         const uint64_t x = split_search<S>(keys + off, sh, uniform64(tk.start_seed), uniform64(tk.max_trials), stage);
@@ -1027,6 +1073,16 @@ int reset_counter(shgpu_engine *e, hipStream_t s) {
 
 int ensure_sh2_scratch(shgpu_engine *e, size_t waves) { return e->sh2_scratch.ensure(waves * SH2_SCRATCH_BYTES); }
 
+template <bool CHECK>
+void launch_split_batch(uint32_t seeds_per_lane, int grid, hipStream_t s, uint64_t *keys, uint64_t *tmp,
+                        const shgpu_split_task *tasks, uint32_t n, uint64_t *out, uint32_t *ctr, uint64_t n_keys) {
+    switch (seeds_per_lane) {
+        case 1: k_split_batch<1, CHECK><<<grid, BLOCK, 0, s>>>(keys, tmp, tasks, n, out, ctr, n_keys); break;
+        case 2: k_split_batch<2, CHECK><<<grid, BLOCK, 0, s>>>(keys, tmp, tasks, n, out, ctr, n_keys); break;
+        default: k_split_batch<4, CHECK><<<grid, BLOCK, 0, s>>>(keys, tmp, tasks, n, out, ctr, n_keys); break;
+    }
+}
+
 template <int VARIANT, int SL, int SS>
 int launch_build(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
     k_build<VARIANT, SL, SS><<<grid, BLOCK, 0, s>>>(a);
@@ -1089,8 +1145,12 @@ int shgpu_engine_create(int device, shgpu_engine **out) {
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(SHGPU_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+    void *scratch_dev = nullptr;
+    int rc = noalloc_scratch_of(device, &scratch_dev);
+    if (rc) return rc;
     shgpu_engine *e = new shgpu_engine();
     e->device = device;
+    e->noalloc_scratch = scratch_dev;
     e->cu_count = prop.multiProcessorCount;
     e->clock_khz = prop.clockRate;
     std::snprintf(e->arch, sizeof(e->arch), "%s", prop.gcnArchName);
@@ -1227,11 +1287,28 @@ int shgpu_split_search(shgpu_engine *e, uint64_t *d_keys, const shgpu_split_task
     if (rc) return rc;
     const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
     uint32_t *ctr = e->counter.as<uint32_t>();
-    switch (e->tuning.split_seeds_per_lane) {
-        case 1: k_split_batch<1><<<grid, BLOCK, 0, s>>>(d_keys, e->tmp.as<uint64_t>(), d_tasks, n, d_seeds, ctr); break;
-        case 2: k_split_batch<2><<<grid, BLOCK, 0, s>>>(d_keys, e->tmp.as<uint64_t>(), d_tasks, n, d_seeds, ctr); break;
-        default: k_split_batch<4><<<grid, BLOCK, 0, s>>>(d_keys, e->tmp.as<uint64_t>(), d_tasks, n, d_seeds, ctr); break;
-    }
+    launch_split_batch<false>(e->tuning.split_seeds_per_lane, grid, s, d_keys, e->tmp.as<uint64_t>(), d_tasks, n, d_seeds,
+                              ctr, extent);
+    HIP_TRY(hipGetLastError());
+    return SHGPU_OK;
+}
+
+int shgpu_split_search_noalloc(shgpu_engine *e, uint64_t *d_keys, uint64_t n_keys, const shgpu_split_task *d_tasks,
+                               uint32_t n, uint64_t *d_seeds, void *stream) {
+    if (!e || (n && (!d_keys || !d_tasks || !d_seeds))) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (n_keys > SHGPU_NOALLOC_MAX_KEYS)
+        return fail(SHGPU_ERR_ARG, "shgpu_split_search_noalloc: " + std::to_string(n_keys) + " keys exceed its scratch of " +
+                                       std::to_string(SHGPU_NOALLOC_MAX_KEYS) + " keys");
+    if (n == 0) return SHGPU_OK;
+    HIP_TRY(hipSetDevice(e->device));
+    uint64_t *scratch_dev = (uint64_t *)e->noalloc_scratch;  // this device's copy of `scratch`
+    if (!scratch_dev) return fail(SHGPU_ERR_HIP, "no-allocation scratch of this device is unavailable");
+    uint32_t *ctr = (uint32_t *)(scratch_dev + SHGPU_NOALLOC_MAX_KEYS);
+    hipStream_t s = pick(e, stream);
+    k_fill_u32<<<1, 64, 0, s>>>(ctr, 0, 16);  // a kernel, not a memset: callers may capture this stream in a graph
+    HIP_TRY(hipGetLastError());
+    const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
+    launch_split_batch<true>(e->tuning.split_seeds_per_lane, grid, s, d_keys, scratch_dev, d_tasks, n, d_seeds, ctr, n_keys);
     HIP_TRY(hipGetLastError());
     return SHGPU_OK;
 }

@@ -88,6 +88,7 @@ _SIGS = {
     "shgpu_sh1_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_sh2_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_split_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
+    "shgpu_split_search_noalloc": (ctypes.c_int, [P, P, ctypes.c_uint64, P, ctypes.c_uint32, P, P]),
     "shgpu_build_run": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
     "shgpu_hash_strings": (ctypes.c_int, [P, P, P, ctypes.c_uint64, P]),
     "shgpu_leaf_orientation": (ctypes.c_int, [P, ctypes.c_int, P, P, ctypes.c_uint64, P, P]),
@@ -362,6 +363,17 @@ class Engine:
         """tasks: SPLIT_DTYPE array. Returns (seeds, permuted keys)."""
         d_keys, d_tasks, d_out = self._stage(keys, tasks)
         self.split_search_dev(d_keys, d_tasks, len(tasks), d_out)
+        self.sync()
+        return d_out.download(), d_keys.download()
+
+    def split_search_noalloc_dev(self, d_keys, n_keys, d_tasks, n, d_out):
+        """shgpu_split_search_noalloc on device arrays (no allocation inside the call)."""
+        _check(load().shgpu_split_search_noalloc(self.h, d_keys.ptr, n_keys, d_tasks.ptr, n, d_out.ptr, None))
+
+    def split_search_noalloc(self, keys, tasks, n_keys=None):
+        """The same as split_search through the entry point that allocates nothing; n_keys defaults to len(keys)."""
+        d_keys, d_tasks, d_out = self._stage(keys, tasks)
+        self.split_search_noalloc_dev(d_keys, len(keys) if n_keys is None else n_keys, d_tasks, len(tasks), d_out)
         self.sync()
         return d_out.download(), d_keys.download()
 
