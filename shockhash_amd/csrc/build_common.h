@@ -1,0 +1,72 @@
+// Types, constants and launchers shared by the engine's host code (engine.hip) and its kernels, which are
+// compiled in translation units of their own: the persistent build per variant (build_sh1.hip,
+// build_simd.hip, build_sh2.hip) and the batched per-node searches (batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/shockhash_amd.h"
+#include "device_common.h"
+
+namespace shgpu {
+
+constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
+constexpr int BLOCK = 256;
+constexpr int WAVES_PER_BLOCK = BLOCK / 64;
+
+// Next task index for the whole wave (no divergent branch, see wave_fetch_add).
+__device__ __forceinline__ uint32_t grab(uint32_t *counter) { return wave_fetch_add(counter, 1u); }
+
+// Per-node search state of a build (windows of seeds shared by every wave that works on the node).
+// gd: bit 63 set once a qualifying seed is known (GD_FOUND), bits 32..62 windows handed out, bits 0..31
+// windows completed. One read-modify-write on it both completes a window and hands out the next.
+constexpr unsigned long long GD_FOUND = 1ull << 63, GD_HANDED = 1ull << 32;
+// Windows a node's search covers at most. The handed-out count has 31 bits; past the cap, every wave that joins
+// a node still takes (and completes unsearched) one window, so the cap leaves 2^24 windows of margin.
+constexpr uint64_t WINDOW_CAP = 0x7F000000ull;
+__device__ __forceinline__ uint64_t gd_handed(unsigned long long gd) { return (gd >> 32) & 0x7FFFFFFFull; }
+__device__ __forceinline__ uint64_t gd_completed(unsigned long long gd) { return gd & 0xFFFFFFFFull; }
+
+struct NodeState {
+    unsigned long long gd;    // GD_FOUND | windows handed out << 32 | windows completed
+    unsigned long long best;  // smallest qualifying seed offset found so far (SHGPU_NOT_FOUND: none)
+    uint32_t finalized;       // number of waves that claimed finalisation (the first one does it)
+    uint32_t pad[3];
+};
+
+struct BuildArgs {
+    uint64_t *keys;
+    uint64_t *tmp;
+    const shgpu_node *nodes;
+    uint64_t *codes;
+    uint32_t *queue;
+    uint32_t *ctrl;  // [0] queue head, [1] split tail, [2] error flag, [3] nodes finished, [5] leaf tail, [6] nodes
+    NodeState *state;
+    uint32_t *help;  // per wave slot: node whose windows the slot's wave searches, or EMPTY_SLOT
+    uint32_t total;  // plain form: nodes (the persistent kernel reads ctrl[6])
+    uint32_t nslots;
+    uint8_t *scratch;
+    int split_hash;
+    uint32_t flags;  // shgpu_tuning.flags of the variant
+};
+
+// Queue layout: one FIFO over all nodes, or a split part [0, qsplit) and a leaf part [qsplit, total).
+__host__ __device__ __forceinline__ bool two_part_queue(uint32_t flags) { return (flags & SHGPU_TUNE_SPLITS_FIRST) != 0; }
+
+// Persistent build of one variant (k_build, grid x BLOCK threads) with sl / ss seeds per lane for leaf / split
+// windows (1, 2 or 4; ShockHash2 leaves ignore sl), and the plain form's launch for node nid (k_node).
+template <int VARIANT>
+hipError_t launch_build_variant(uint32_t sl, uint32_t ss, const BuildArgs &a, int grid, hipStream_t s);
+template <int VARIANT>
+hipError_t launch_node_variant(const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s);
+
+// Batched per-node searches (batch.hip); every call returns the launch's status.
+hipError_t launch_sh1_batch(uint32_t seeds_per_lane, int grid, hipStream_t s, const uint64_t *keys,
+                            const shgpu_leaf_task *tasks, uint32_t n, uint64_t *out, uint32_t *ctr);
+hipError_t launch_sh2_batch(int grid, hipStream_t s, const uint64_t *keys, const shgpu_leaf_task *tasks, uint32_t n,
+                            uint64_t *out, uint32_t *ctr, uint8_t *scratch, uint32_t flags);
+// check: validate every task on the device against n_keys (the no-allocation entry point)
+hipError_t launch_split_batch(bool check, uint32_t seeds_per_lane, int grid, hipStream_t s, uint64_t *keys, uint64_t *tmp,
+                              const shgpu_split_task *tasks, uint32_t n, uint64_t *out, uint32_t *ctr, uint64_t n_keys);
+
+}  // namespace shgpu

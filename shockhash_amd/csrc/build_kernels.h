@@ -1,0 +1,284 @@
+// Persistent construction kernel (k_build) and the plain one-launch-per-node form (k_node): the window
+// protocol shared by split nodes and ShockHash1 leaves, the help registry, and ShockHash2 leaves searched by
+// one wave. Included by one translation unit per variant, which instantiates that variant's launchers.
+#pragma once
+#include <type_traits>
+
+#include "build_common.h"
+#include "search_sh1.h"
+#include "search_sh2.h"
+#include "search_split.h"
+
+namespace shgpu {
+
+__device__ __forceinline__ unsigned long long wave_fetch_add64(unsigned long long *p, unsigned long long add) {
+    const unsigned long long old = atomicAdd(p, __lane_id() == 0 ? add : 0ull);
+    return uniform64(old);
+}
+
+// The value that decides a node's code is read with a read-modify-write (atomicOr 0), which resolves at
+// the coherence point.
+__device__ __forceinline__ uint64_t read_best(NodeState *st) {
+    return uniform64(atomicOr(&st->best, 0ull));
+}
+
+
+template <int SL, int SS>
+__device__ __forceinline__ uint32_t window_seeds(uint32_t kind) {
+    return kind == SHGPU_NODE_LEAF ? 64u * SL : kind == SHGPU_NODE_AGGR_LOWER ? 64u * SS : 64u;
+}
+
+// Stores the node's code; for a split node also partitions its keys by the found seed and (in the
+// persistent form) publishes the children. An exhausted search still publishes the children so that
+// the forest drains; the host then reports the error.
+template <int VARIANT>
+__device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, uint64_t off, bool persistent) {
+    const shgpu_node nd = a.nodes[nid];
+    const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
+    if (kind != SHGPU_NODE_LEAF) {
+        const SplitShapeDev sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
+        if (off != SHGPU_NOT_FOUND) split_partition(a.keys + o, a.tmp + o, sh, uniform64(nd.start_seed) + off);
+        if (persistent) {
+            __threadfence();  // children's keys visible device-wide before they are queued
+            const uint32_t nc = uniform(nd.n_children);
+            // every lane computes the same child ids and stores the same slots (no divergence); split
+            // children go to the split part of the queue, leaves to the leaf part
+            uint32_t child = nid + 1;
+            for (uint32_t c = 0; c < nc; c++) {
+                const shgpu_node cn = a.nodes[child];
+                const bool to_leaf_part = two_part_queue(a.flags) && uniform(cn.kind) == SHGPU_NODE_LEAF;
+                const uint32_t pos = wave_fetch_add(&a.ctrl[to_leaf_part ? 5 : 1], 1u);
+                __hip_atomic_store(&a.queue[pos], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                child = uniform(child + cn.subtree);
+            }
+        }
+    }
+    a.codes[nid] = off;  // read by the host after the kernel ends
+    if (persistent) wave_fetch_add(&a.ctrl[3], 1u);
+}
+
+// Searches windows of a split node or ShockHash1 leaf until its smallest qualifying seed is known.
+// Windows are handed out in increasing order. A wave holds one window at a time; one atomic on the
+// node's gd word completes it and, unless the search is over for this wave, hands out the next, and
+// its return value says whether a qualifying seed is already known (then the window just taken is
+// completed without being searched). A node is finalised by a wave whose completion leaves no window
+// outstanding, with a best seed that lies inside the windows completed by then: the best seed is read
+// after the count, so it may come from a later window whose predecessors are still being searched,
+// and only a seed inside the complete windows is final (otherwise the wave that completes the last
+// outstanding window sees the counts equal again and decides then). Every window below the winner has
+// been searched, so the result is the sequential minimum.
+// Keys are staged in the wave's LDS once per call (leaves, and split nodes of up to STAGE_KEYS keys).
+template <int VARIANT, int SL, int SS>
+__device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent, uint64_t t_start) {
+    const shgpu_node nd = a.nodes[nid];
+    const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
+    NodeState *st = &a.state[nid];
+    const uint64_t W = window_seeds<SL, SS>(kind);
+    // The window count has 31 bits: a bound beyond WINDOW_CAP windows ends the search at the cap, and
+    // the finish rule below uses the same limit, so a node that fails is finalised as not found.
+    const uint64_t start = uniform64(nd.start_seed), lim = min(uniform64(nd.max_trials), WINDOW_CAP * W);
+    constexpr int SPLIT_HASH = VARIANT == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
+    SplitShapeDev sh;
+    if (kind != SHGPU_NODE_LEAF) sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
+    unsigned long long old = wave_fetch_add64(&st->gd, GD_HANDED);
+    bool found = (old & GD_FOUND) != 0;
+    uint64_t w = gd_handed(old);
+    // stage the keys once (after the first window is taken: a node already solved costs no loads)
+    uint64_t kv = 0;
+    bool staged = false;
+    if (!found) {
+        if (kind == SHGPU_NODE_LEAF) {
+            if constexpr (VARIANT != SHGPU_SHOCKHASH2)
+                if (m >= 2) kv = sh1_stage(a.keys + o, m, L);
+        } else if (m <= STAGE_KEYS) {
+            split_stage(a.keys + o, m, L.stage);
+            staged = true;
+        }
+    }
+    for (;;) {
+        age_priority(t_start);
+        const uint64_t woff = w * W;
+        const bool exhausted = woff >= lim;
+        uint64_t r = SHGPU_NOT_FOUND;
+        if (!exhausted && !found) {
+            if (kind == SHGPU_NODE_LEAF) {
+                if constexpr (VARIANT != SHGPU_SHOCKHASH2)  // ShockHash2 leaves never come here
+                    r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND) : sh1_window_staged<SL>(kv, m, start, woff, L);
+            } else if (kind == SHGPU_NODE_AGGR_LOWER)
+                r = split_window<SS, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
+            else
+                r = split_window<1, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
+            if (r != SHGPU_NOT_FOUND && r >= lim) r = SHGPU_NOT_FOUND;
+            if (r != SHGPU_NOT_FOUND) {
+                // same value in every lane; waiting for each returned value orders the best seed before the
+                // found flag and the flag before the completion below (all resolve at the coherence point)
+                const unsigned long long prev = atomicMin(&st->best, (unsigned long long)r);
+                asm volatile("" ::"v"(prev));
+                const unsigned long long prevf = atomicOr(&st->gd, GD_FOUND);
+                asm volatile("" ::"v"(prevf));
+            }
+        }
+        const bool more = r == SHGPU_NOT_FOUND && !exhausted && !found;
+        old = wave_fetch_add64(&st->gd, 1ull + (more ? GD_HANDED : 0ull));
+        const uint64_t handed = gd_handed(old);  // windows handed out before this operation
+        if (gd_completed(old) + 1 == handed) {
+            // windows [0, handed) are all complete; a seed found in one of them set GD_FOUND before its
+            // completion, so without the flag none of them holds one
+            const uint64_t b = (old & GD_FOUND) ? read_best(st) : SHGPU_NOT_FOUND;
+            const uint64_t covered = handed * W;
+            if (b < covered || (b == SHGPU_NOT_FOUND && covered >= lim)) {
+                if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, b, persistent);
+            }
+        }
+        if (!more) {
+            reset_priority();
+            return;
+        }
+        w = handed;  // the window this operation handed out
+        found = (old & GD_FOUND) != 0;
+    }
+}
+
+// Joins one node another wave is searching in windows, if any. Returns true when it did some work.
+template <int VARIANT, int SL, int SS>
+__device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLds &L) {
+    const uint32_t lane = lane_id();
+    const uint32_t first = (slot * 97u) % a.nslots;
+    for (uint32_t base = 0; base < a.nslots; base += 64) {
+        uint32_t i = first + base + lane;
+        if (i >= a.nslots) i -= a.nslots;
+        const uint32_t v = base + lane < a.nslots ? __hip_atomic_load(&a.help[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                                  : EMPTY_SLOT;
+        // a registered node whose best seed is already known needs no help
+        const bool open = v != EMPTY_SLOT &&
+                          !(__hip_atomic_load(&a.state[v].gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & GD_FOUND);
+        const uint64_t hit = ballot(open);
+        if (hit) {
+            const uint32_t nid = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));
+            __threadfence();
+            work_windows<VARIANT, SL, SS>(a, nid, L, true, task_clock());
+            return true;
+        }
+    }
+    return false;
+}
+
+// A wave runs either a windowed search (pf) or a ShockHash2 leaf (sh2), never both at once.
+template <int VARIANT>
+union BuildLds {
+    PfLds pf;
+    typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, char>::type sh2;
+};
+
+// One node taken from the queue: ShockHash2 leaves are searched by this wave alone; splits and
+// ShockHash1 leaves are searched in windows that idle waves may join (help registry).
+template <int VARIANT, int SL, int SS>
+__device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint32_t slot, BuildLds<VARIANT> &L,
+                                         uint8_t *ws) {
+    const shgpu_node nd = a.nodes[nid];
+    const uint32_t kind = uniform(nd.kind);
+    const uint64_t t_start = task_clock();
+    if (VARIANT == SHGPU_SHOCKHASH2 && kind == SHGPU_NODE_LEAF) {
+        if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
+            const uint64_t code = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.flags, t_start);
+            a.codes[nid] = code;
+            wave_fetch_add(&a.ctrl[3], 1u);
+        }
+        return;
+    }
+    wave_fetch_add(&a.ctrl[8], 1u);  // windowed nodes started
+    __hip_atomic_store(&a.help[slot], nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    work_windows<VARIANT, SL, SS>(a, nid, L.pf, true, t_start);
+    __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr uint32_t SPIN_LIMIT = 1u << 27;  // sleeps before a wait is declared stuck (well over ten seconds)
+
+// Persistent construction kernel. Every wave takes queue slots in order; a finished split node
+// publishes its children. With SHGPU_TUNE_SPLITS_FIRST the queue has a split part (slots [0, n_split))
+// and a leaf part (slots [n_split, total)) and children go to the part of their kind, so all split
+// nodes are taken before any leaf; otherwise it is one FIFO. While a slot is not yet published (its
+// parent is still being split) the wave helps windowed searches. When the queue is exhausted, waves
+// keep helping until every node is finished.
+template <int VARIANT, int SL, int SS>
+__global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
+    __shared__ BuildLds<VARIANT> lds[WAVES_PER_BLOCK];
+    const uint32_t w = threadIdx.x / 64;
+    const uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
+    BuildLds<VARIANT> &L = lds[w];
+    uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)slot * SH2_SCRATCH_BYTES : nullptr;
+    const uint32_t total = uniform(a.ctrl[6]);  // written by the forest plan
+    for (;;) {
+        const uint32_t idx = grab(&a.ctrl[0]);
+        if (idx >= total) break;
+        uint32_t nid = EMPTY_SLOT;
+        for (uint32_t spin = 0;; spin++) {
+            nid = uniform(__hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+            if (nid != EMPTY_SLOT) break;
+            if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
+            if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
+            __builtin_amdgcn_s_sleep(4);
+        }
+        if (nid == EMPTY_SLOT) return;
+        __threadfence();
+        run_node<VARIANT, SL, SS>(a, nid, slot, L, ws);
+    }
+    // Queue exhausted: help until every node is finished. Once every windowed node has started (ctrl[8]
+    // reaches ctrl[7]) no node will register for help any more, and a wave that finds none open leaves
+    // instead of polling beside the waves still searching (checked every 8th poll: ~6000 waves polling one
+    // word load a single L2 channel).
+    const uint32_t windowed = uniform(a.ctrl[7]);
+    for (uint32_t spin = 0;; spin++) {
+        if (uniform(__hip_atomic_load(&a.ctrl[3], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= total) break;
+        if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
+        if ((spin & 7) == 0 &&
+            uniform(__hip_atomic_load(&a.ctrl[8], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= windowed)
+            break;
+        if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
+        __builtin_amdgcn_s_sleep(8);
+    }
+}
+
+// Plain form: one launch per node (the host walks the forest in DFS order on one stream). Every
+// wave of the launch joins the node's windowed search; a ShockHash2 leaf is searched by wave 0.
+template <int VARIANT, int SL, int SS>
+__global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
+    __shared__ BuildLds<VARIANT> L;
+    const shgpu_node nd = a.nodes[nid];
+    if (VARIANT == SHGPU_SHOCKHASH2 && uniform(nd.kind) == SHGPU_NODE_LEAF) {
+        if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
+            if (blockIdx.x == 0)
+                a.codes[nid] = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2, a.flags, task_clock());
+        }
+        return;
+    }
+    work_windows<VARIANT, SL, SS>(a, nid, L.pf, false, task_clock());
+}
+
+
+template <int VARIANT, int SL, int SS>
+hipError_t launch_build_t(const BuildArgs &a, int grid, hipStream_t s) {
+    k_build<VARIANT, SL, SS><<<grid, BLOCK, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int VARIANT>
+hipError_t launch_build_variant(uint32_t sl, uint32_t ss, const BuildArgs &a, int grid, hipStream_t s) {
+    // ShockHash2 leaves do not use the ShockHash1 window: one SL instantiation serves every setting
+#define SHGPU_CASE(A, B)                                                                            \
+    if constexpr (VARIANT != SHGPU_SHOCKHASH2 || A == 1) {                                          \
+        if ((VARIANT == SHGPU_SHOCKHASH2 || sl == A) && ss == B) return launch_build_t<VARIANT, A, B>(a, grid, s); \
+    }
+    SHGPU_CASE(1, 1) SHGPU_CASE(1, 2) SHGPU_CASE(1, 4) SHGPU_CASE(2, 1) SHGPU_CASE(2, 2) SHGPU_CASE(2, 4)
+    SHGPU_CASE(4, 1) SHGPU_CASE(4, 2) SHGPU_CASE(4, 4)
+#undef SHGPU_CASE
+    return hipErrorInvalidValue;  // seeds per lane outside {1, 2, 4}
+}
+
+template <int VARIANT>
+hipError_t launch_node_variant(const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
+    k_node<VARIANT, VARIANT == SHGPU_SHOCKHASH2 ? 1 : 4, 4><<<waves, 64, 0, s>>>(a, nid);
+    return hipGetLastError();
+}
+
+}  // namespace shgpu

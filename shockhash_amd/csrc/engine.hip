@@ -16,9 +16,8 @@
 
 #include "../../include/shockhash_amd.h"
 #include "../include/shockhash/GolombMemo.h"
-#include "search_sh1.h"
+#include "build_common.h"
 #include "search_sh2.h"
-#include "search_split.h"
 #include "retrieval.h"
 #include "sort.h"
 #include "strings.h"
@@ -36,9 +35,6 @@ const uint64_t START_SEED[20] = {
 
 static_assert(SHGPU_TUNE_SH2_GENERAL_ONLY == SH2_FLAG_GENERAL_ONLY, "tuning flag of the ShockHash2 leaf search");
 
-constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
-constexpr int BLOCK = 256;
-constexpr int WAVES_PER_BLOCK = BLOCK / 64;
 constexpr uint32_t MAX_ENCODABLE_BUCKET = 2999;  // memo tables cover sizes < MAX_BUCKET_SIZE = 3000
 constexpr uint64_t DEFAULT_MAX_TRIALS = 1ull << 40;  // schedule cap of every node's seed bound
 
@@ -230,74 +226,6 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Next task index for the whole wave (no divergent branch, see wave_fetch_add).
-__device__ __forceinline__ uint32_t grab(uint32_t *counter) { return wave_fetch_add(counter, 1u); }
-
-template <int S>
-__global__ void __launch_bounds__(BLOCK) k_sh1_batch(const uint64_t *__restrict__ keys, const shgpu_leaf_task *__restrict__ tasks,
-                                                     uint32_t n, uint64_t *__restrict__ out, uint32_t *counter) {
-    __shared__ PfLds lds[WAVES_PER_BLOCK];
-    PfLds &L = lds[threadIdx.x / 64];
-    for (;;) {
-        const uint32_t t = grab(counter);
-        if (t >= n) break;
-        const shgpu_leaf_task tk = tasks[t];
-        const uint64_t x = sh1_leaf_search<S>(keys + uniform(tk.offset), uniform(tk.m), uniform64(tk.start_seed),
-                                              uniform64(tk.max_trials), L);
-        out[t] = x;  // wave-uniform value, stored by every lane
-    }
-}
-
-__global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict__ keys, const shgpu_leaf_task *__restrict__ tasks,
-                                                     uint32_t n, uint64_t *__restrict__ out, uint32_t *counter, uint8_t *scratch,
-                                                     uint32_t flags) {
-    __shared__ Sh2Lds lds[WAVES_PER_BLOCK];
-    const uint32_t w = threadIdx.x / 64;
-    Sh2Lds &L = lds[w];
-    uint8_t *ws = scratch + (size_t)(blockIdx.x * WAVES_PER_BLOCK + w) * SH2_SCRATCH_BYTES;
-    for (;;) {
-        const uint32_t t = grab(counter);
-        if (t >= n) break;
-        const shgpu_leaf_task tk = tasks[t];
-        const uint64_t code = sh2_leaf_search(keys + uniform(tk.offset), uniform(tk.m), ws, L, flags, task_clock());
-        out[t] = code;
-    }
-}
-
-// A split task the kernels can run on a key array of n_keys keys: 2 <= fanout <= 4, unit < m <= 65535, the
-// node inside the array, a known kind and hash flavour (the checks shgpu_split_search makes on the host).
-__device__ __forceinline__ bool split_task_valid(const shgpu_split_task &tk, uint64_t n_keys) {
-    const uint32_t m = uniform(tk.m), unit = uniform(tk.unit), kind = uniform(tk.kind);
-    if (m > 65535 || unit == 0 || unit >= m || kind > SHGPU_NODE_AGGR_LOWER || uniform(tk.hash) > SHGPU_SPLIT_SIMD)
-        return false;
-    const uint32_t fanout = kind == SHGPU_NODE_SPLIT2 ? 2u : (m + unit - 1) / unit;
-    return fanout >= 2 && fanout <= 4 && (uint64_t)uniform(tk.offset) + m <= n_keys;
-}
-
-// CHECK: validate every task on the device (split_task_valid against n_keys); an invalid task leaves its keys
-// untouched and reports SHGPU_NOT_FOUND.
-template <int S, bool CHECK>
-__global__ void __launch_bounds__(BLOCK) k_split_batch(uint64_t *keys, uint64_t *tmp, const shgpu_split_task *__restrict__ tasks,
-                                                       uint32_t n, uint64_t *__restrict__ out, uint32_t *counter,
-                                                       uint64_t n_keys) {
-    __shared__ PfLds lds[WAVES_PER_BLOCK];
-    uint64_t *stage = lds[threadIdx.x / 64].stage;
-    for (;;) {
-        const uint32_t t = grab(counter);
-        if (t >= n) break;
-        const shgpu_split_task tk = tasks[t];
-        if (CHECK && !split_task_valid(tk, n_keys)) {
-            out[t] = SHGPU_NOT_FOUND;
-            continue;
-        }
-        const uint32_t off = uniform(tk.offset), m = uniform(tk.m);
-        const SplitShapeDev sh = make_split_shape(m, uniform(tk.unit), uniform(tk.kind), uniform(tk.hash));
-        const uint64_t x = split_search<S>(keys + off, sh, uniform64(tk.start_seed), uniform64(tk.max_trials), stage);
-        if (x != SHGPU_NOT_FOUND) split_partition(keys + off, tmp + off, sh, x);
-        out[t] = x;
-    }
-}
-
 // Plain streaming copy (bandwidth / counter calibration probe).
 __global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -368,285 +296,6 @@ __global__ void k_bucket_bounds(const uint64_t *__restrict__ first, uint64_t n, 
         const uint64_t lo = i > 0 ? bp + 1 : 0;
         for (uint64_t k = lo; k <= b && k <= nb; k++) size_acc[k] = i;
     }
-}
-
-// Per-node search state of a build (windows of seeds shared by every wave that works on the node).
-// gd: bit 63 set once a qualifying seed is known (GD_FOUND), bits 32..62 windows handed out, bits 0..31
-// windows completed. One read-modify-write on it both completes a window and hands out the next.
-constexpr unsigned long long GD_FOUND = 1ull << 63, GD_HANDED = 1ull << 32;
-// Windows a node's search covers at most. The handed-out count has 31 bits; past the cap, every wave that joins
-// a node still takes (and completes unsearched) one window, so the cap leaves 2^24 windows of margin.
-constexpr uint64_t WINDOW_CAP = 0x7F000000ull;
-__device__ __forceinline__ uint64_t gd_handed(unsigned long long gd) { return (gd >> 32) & 0x7FFFFFFFull; }
-__device__ __forceinline__ uint64_t gd_completed(unsigned long long gd) { return gd & 0xFFFFFFFFull; }
-
-struct NodeState {
-    unsigned long long gd;    // GD_FOUND | windows handed out << 32 | windows completed
-    unsigned long long best;  // smallest qualifying seed offset found so far (SHGPU_NOT_FOUND: none)
-    uint32_t finalized;       // number of waves that claimed finalisation (the first one does it)
-    uint32_t pad[3];
-};
-
-struct BuildArgs {
-    uint64_t *keys;
-    uint64_t *tmp;
-    const shgpu_node *nodes;
-    uint64_t *codes;
-    uint32_t *queue;
-    uint32_t *ctrl;  // [0] queue head, [1] split tail, [2] error flag, [3] nodes finished, [5] leaf tail, [6] nodes
-    NodeState *state;
-    uint32_t *help;  // per wave slot: node whose windows the slot's wave searches, or EMPTY_SLOT
-    uint32_t total;  // plain form: nodes (the persistent kernel reads ctrl[6])
-    uint32_t nslots;
-    uint8_t *scratch;
-    int split_hash;
-    uint32_t flags;  // shgpu_tuning.flags of the variant
-};
-
-__device__ __forceinline__ unsigned long long wave_fetch_add64(unsigned long long *p, unsigned long long add) {
-    const unsigned long long old = atomicAdd(p, __lane_id() == 0 ? add : 0ull);
-    return uniform64(old);
-}
-
-// The value that decides a node's code is read with a read-modify-write (atomicOr 0), which resolves at
-// the coherence point.
-__device__ __forceinline__ uint64_t read_best(NodeState *st) {
-    return uniform64(atomicOr(&st->best, 0ull));
-}
-
-// Queue layout: one FIFO over all nodes, or a split part [0, qsplit) and a leaf part [qsplit, total).
-__host__ __device__ __forceinline__ bool two_part_queue(uint32_t flags) { return (flags & SHGPU_TUNE_SPLITS_FIRST) != 0; }
-
-template <int SL, int SS>
-__device__ __forceinline__ uint32_t window_seeds(uint32_t kind) {
-    return kind == SHGPU_NODE_LEAF ? 64u * SL : kind == SHGPU_NODE_AGGR_LOWER ? 64u * SS : 64u;
-}
-
-// Stores the node's code; for a split node also partitions its keys by the found seed and (in the
-// persistent form) publishes the children. An exhausted search still publishes the children so that
-// the forest drains; the host then reports the error.
-template <int VARIANT>
-__device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, uint64_t off, bool persistent) {
-    const shgpu_node nd = a.nodes[nid];
-    const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
-    if (kind != SHGPU_NODE_LEAF) {
-        const SplitShapeDev sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
-        if (off != SHGPU_NOT_FOUND) split_partition(a.keys + o, a.tmp + o, sh, uniform64(nd.start_seed) + off);
-        if (persistent) {
-            __threadfence();  // children's keys visible device-wide before they are queued
-            const uint32_t nc = uniform(nd.n_children);
-            // every lane computes the same child ids and stores the same slots (no divergence); split
-            // children go to the split part of the queue, leaves to the leaf part
-            uint32_t child = nid + 1;
-            for (uint32_t c = 0; c < nc; c++) {
-                const shgpu_node cn = a.nodes[child];
-                const bool to_leaf_part = two_part_queue(a.flags) && uniform(cn.kind) == SHGPU_NODE_LEAF;
-                const uint32_t pos = wave_fetch_add(&a.ctrl[to_leaf_part ? 5 : 1], 1u);
-                __hip_atomic_store(&a.queue[pos], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                child = uniform(child + cn.subtree);
-            }
-        }
-    }
-    a.codes[nid] = off;  // read by the host after the kernel ends
-    if (persistent) wave_fetch_add(&a.ctrl[3], 1u);
-}
-
-// Searches windows of a split node or ShockHash1 leaf until its smallest qualifying seed is known.
-// Windows are handed out in increasing order. A wave holds one window at a time; one atomic on the
-// node's gd word completes it and, unless the search is over for this wave, hands out the next, and
-// its return value says whether a qualifying seed is already known (then the window just taken is
-// completed without being searched). A node is finalised by a wave whose completion leaves no window
-// outstanding, with a best seed that lies inside the windows completed by then: the best seed is read
-// after the count, so it may come from a later window whose predecessors are still being searched,
-// and only a seed inside the complete windows is final (otherwise the wave that completes the last
-// outstanding window sees the counts equal again and decides then). Every window below the winner has
-// been searched, so the result is the sequential minimum.
-// Keys are staged in the wave's LDS once per call (leaves, and split nodes of up to STAGE_KEYS keys).
-template <int VARIANT, int SL, int SS>
-__device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent, uint64_t t_start) {
-    const shgpu_node nd = a.nodes[nid];
-    const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
-    NodeState *st = &a.state[nid];
-    const uint64_t W = window_seeds<SL, SS>(kind);
-    // The window count has 31 bits: a bound beyond WINDOW_CAP windows ends the search at the cap, and
-    // the finish rule below uses the same limit, so a node that fails is finalised as not found.
-    const uint64_t start = uniform64(nd.start_seed), lim = min(uniform64(nd.max_trials), WINDOW_CAP * W);
-    constexpr int SPLIT_HASH = VARIANT == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
-    SplitShapeDev sh;
-    if (kind != SHGPU_NODE_LEAF) sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
-    unsigned long long old = wave_fetch_add64(&st->gd, GD_HANDED);
-    bool found = (old & GD_FOUND) != 0;
-    uint64_t w = gd_handed(old);
-    // stage the keys once (after the first window is taken: a node already solved costs no loads)
-    uint64_t kv = 0;
-    bool staged = false;
-    if (!found) {
-        if (kind == SHGPU_NODE_LEAF) {
-            if constexpr (VARIANT != SHGPU_SHOCKHASH2)
-                if (m >= 2) kv = sh1_stage(a.keys + o, m, L);
-        } else if (m <= STAGE_KEYS) {
-            split_stage(a.keys + o, m, L.stage);
-            staged = true;
-        }
-    }
-    for (;;) {
-        age_priority(t_start);
-        const uint64_t woff = w * W;
-        const bool exhausted = woff >= lim;
-        uint64_t r = SHGPU_NOT_FOUND;
-        if (!exhausted && !found) {
-            if (kind == SHGPU_NODE_LEAF) {
-                if constexpr (VARIANT != SHGPU_SHOCKHASH2)  // ShockHash2 leaves never come here
-                    r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND) : sh1_window_staged<SL>(kv, m, start, woff, L);
-            } else if (kind == SHGPU_NODE_AGGR_LOWER)
-                r = split_window<SS, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
-            else
-                r = split_window<1, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
-            if (r != SHGPU_NOT_FOUND && r >= lim) r = SHGPU_NOT_FOUND;
-            if (r != SHGPU_NOT_FOUND) {
-                // same value in every lane; waiting for each returned value orders the best seed before the
-                // found flag and the flag before the completion below (all resolve at the coherence point)
-                const unsigned long long prev = atomicMin(&st->best, (unsigned long long)r);
-                asm volatile("" ::"v"(prev));
-                const unsigned long long prevf = atomicOr(&st->gd, GD_FOUND);
-                asm volatile("" ::"v"(prevf));
-            }
-        }
-        const bool more = r == SHGPU_NOT_FOUND && !exhausted && !found;
-        old = wave_fetch_add64(&st->gd, 1ull + (more ? GD_HANDED : 0ull));
-        const uint64_t handed = gd_handed(old);  // windows handed out before this operation
-        if (gd_completed(old) + 1 == handed) {
-            // windows [0, handed) are all complete; a seed found in one of them set GD_FOUND before its
-            // completion, so without the flag none of them holds one
-            const uint64_t b = (old & GD_FOUND) ? read_best(st) : SHGPU_NOT_FOUND;
-            const uint64_t covered = handed * W;
-            if (b < covered || (b == SHGPU_NOT_FOUND && covered >= lim)) {
-                if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, b, persistent);
-            }
-        }
-        if (!more) {
-            reset_priority();
-            return;
-        }
-        w = handed;  // the window this operation handed out
-        found = (old & GD_FOUND) != 0;
-    }
-}
-
-// Joins one node another wave is searching in windows, if any. Returns true when it did some work.
-template <int VARIANT, int SL, int SS>
-__device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLds &L) {
-    const uint32_t lane = lane_id();
-    const uint32_t first = (slot * 97u) % a.nslots;
-    for (uint32_t base = 0; base < a.nslots; base += 64) {
-        uint32_t i = first + base + lane;
-        if (i >= a.nslots) i -= a.nslots;
-        const uint32_t v = base + lane < a.nslots ? __hip_atomic_load(&a.help[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                                  : EMPTY_SLOT;
-        // a registered node whose best seed is already known needs no help
-        const bool open = v != EMPTY_SLOT &&
-                          !(__hip_atomic_load(&a.state[v].gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & GD_FOUND);
-        const uint64_t hit = ballot(open);
-        if (hit) {
-            const uint32_t nid = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));
-            __threadfence();
-            work_windows<VARIANT, SL, SS>(a, nid, L, true, task_clock());
-            return true;
-        }
-    }
-    return false;
-}
-
-// A wave runs either a windowed search (pf) or a ShockHash2 leaf (sh2), never both at once.
-template <int VARIANT>
-union BuildLds {
-    PfLds pf;
-    typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, char>::type sh2;
-};
-
-// One node taken from the queue: ShockHash2 leaves are searched by this wave alone; splits and
-// ShockHash1 leaves are searched in windows that idle waves may join (help registry).
-template <int VARIANT, int SL, int SS>
-__device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint32_t slot, BuildLds<VARIANT> &L,
-                                         uint8_t *ws) {
-    const shgpu_node nd = a.nodes[nid];
-    const uint32_t kind = uniform(nd.kind);
-    const uint64_t t_start = task_clock();
-    if (VARIANT == SHGPU_SHOCKHASH2 && kind == SHGPU_NODE_LEAF) {
-        if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            const uint64_t code = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.flags, t_start);
-            a.codes[nid] = code;
-            wave_fetch_add(&a.ctrl[3], 1u);
-        }
-        return;
-    }
-    wave_fetch_add(&a.ctrl[8], 1u);  // windowed nodes started
-    __hip_atomic_store(&a.help[slot], nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    work_windows<VARIANT, SL, SS>(a, nid, L.pf, true, t_start);
-    __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-constexpr uint32_t SPIN_LIMIT = 1u << 27;  // sleeps before a wait is declared stuck (well over ten seconds)
-
-// Persistent construction kernel. Every wave takes queue slots in order; a finished split node
-// publishes its children. With SHGPU_TUNE_SPLITS_FIRST the queue has a split part (slots [0, n_split))
-// and a leaf part (slots [n_split, total)) and children go to the part of their kind, so all split
-// nodes are taken before any leaf; otherwise it is one FIFO. While a slot is not yet published (its
-// parent is still being split) the wave helps windowed searches. When the queue is exhausted, waves
-// keep helping until every node is finished.
-template <int VARIANT, int SL, int SS>
-__global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
-    __shared__ BuildLds<VARIANT> lds[WAVES_PER_BLOCK];
-    const uint32_t w = threadIdx.x / 64;
-    const uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
-    BuildLds<VARIANT> &L = lds[w];
-    uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)slot * SH2_SCRATCH_BYTES : nullptr;
-    const uint32_t total = uniform(a.ctrl[6]);  // written by the forest plan
-    for (;;) {
-        const uint32_t idx = grab(&a.ctrl[0]);
-        if (idx >= total) break;
-        uint32_t nid = EMPTY_SLOT;
-        for (uint32_t spin = 0;; spin++) {
-            nid = uniform(__hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-            if (nid != EMPTY_SLOT) break;
-            if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
-            if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
-            __builtin_amdgcn_s_sleep(4);
-        }
-        if (nid == EMPTY_SLOT) return;
-        __threadfence();
-        run_node<VARIANT, SL, SS>(a, nid, slot, L, ws);
-    }
-    // Queue exhausted: help until every node is finished. Once every windowed node has started (ctrl[8]
-    // reaches ctrl[7]) no node will register for help any more, and a wave that finds none open leaves
-    // instead of polling beside the waves still searching (checked every 8th poll: ~6000 waves polling one
-    // word load a single L2 channel).
-    const uint32_t windowed = uniform(a.ctrl[7]);
-    for (uint32_t spin = 0;; spin++) {
-        if (uniform(__hip_atomic_load(&a.ctrl[3], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= total) break;
-        if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
-        if ((spin & 7) == 0 &&
-            uniform(__hip_atomic_load(&a.ctrl[8], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= windowed)
-            break;
-        if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
-        __builtin_amdgcn_s_sleep(8);
-    }
-}
-
-// Plain form: one launch per node (the host walks the forest in DFS order on one stream). Every
-// wave of the launch joins the node's windowed search; a ShockHash2 leaf is searched by wave 0.
-template <int VARIANT, int SL, int SS>
-__global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
-    __shared__ BuildLds<VARIANT> L;
-    const shgpu_node nd = a.nodes[nid];
-    if (VARIANT == SHGPU_SHOCKHASH2 && uniform(nd.kind) == SHGPU_NODE_LEAF) {
-        if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            if (blockIdx.x == 0)
-                a.codes[nid] = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2, a.flags, task_clock());
-        }
-        return;
-    }
-    work_windows<VARIANT, SL, SS>(a, nid, L.pf, false, task_clock());
 }
 
 __global__ void k_init_states(NodeState *st, uint64_t *codes, uint32_t n, uint32_t *help, uint32_t nslots) {
@@ -1073,41 +722,18 @@ int reset_counter(shgpu_engine *e, hipStream_t s) {
 
 int ensure_sh2_scratch(shgpu_engine *e, size_t waves) { return e->sh2_scratch.ensure(waves * SH2_SCRATCH_BYTES); }
 
-template <bool CHECK>
-void launch_split_batch(uint32_t seeds_per_lane, int grid, hipStream_t s, uint64_t *keys, uint64_t *tmp,
-                        const shgpu_split_task *tasks, uint32_t n, uint64_t *out, uint32_t *ctr, uint64_t n_keys) {
-    switch (seeds_per_lane) {
-        case 1: k_split_batch<1, CHECK><<<grid, BLOCK, 0, s>>>(keys, tmp, tasks, n, out, ctr, n_keys); break;
-        case 2: k_split_batch<2, CHECK><<<grid, BLOCK, 0, s>>>(keys, tmp, tasks, n, out, ctr, n_keys); break;
-        default: k_split_batch<4, CHECK><<<grid, BLOCK, 0, s>>>(keys, tmp, tasks, n, out, ctr, n_keys); break;
-    }
-}
-
-template <int VARIANT, int SL, int SS>
-int launch_build(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
-    k_build<VARIANT, SL, SS><<<grid, BLOCK, 0, s>>>(a);
-    HIP_TRY(hipGetLastError());
+template <int VARIANT>
+int launch_build_v(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
+    const hipError_t err = launch_build_variant<VARIANT>(e->tuning_v[VARIANT].sh1_seeds_per_lane,
+                                                         e->tuning_v[VARIANT].split_seeds_per_lane, a, grid, s);
+    if (err != hipSuccess) return fail(SHGPU_ERR_HIP, std::string("k_build launch: ") + hipGetErrorString(err));
     return SHGPU_OK;
 }
 
 template <int VARIANT>
-int launch_build_v(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
-    const uint32_t sl = e->tuning_v[VARIANT].sh1_seeds_per_lane, ss = e->tuning_v[VARIANT].split_seeds_per_lane;
-    // ShockHash2 leaves do not use the ShockHash1 window: one SL instantiation serves every setting
-#define SHGPU_CASE(A, B)                                                         \
-    if constexpr (VARIANT != SHGPU_SHOCKHASH2 || A == 1) {                       \
-        if ((VARIANT == SHGPU_SHOCKHASH2 || sl == A) && ss == B) return launch_build<VARIANT, A, B>(e, a, grid, s); \
-    }
-    SHGPU_CASE(1, 1) SHGPU_CASE(1, 2) SHGPU_CASE(1, 4) SHGPU_CASE(2, 1) SHGPU_CASE(2, 2) SHGPU_CASE(2, 4)
-    SHGPU_CASE(4, 1) SHGPU_CASE(4, 2) SHGPU_CASE(4, 4)
-#undef SHGPU_CASE
-    return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
-}
-
-template <int VARIANT>
-int launch_node_v(shgpu_engine *e, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
-    k_node<VARIANT, VARIANT == SHGPU_SHOCKHASH2 ? 1 : 4, 4><<<waves, 64, 0, s>>>(a, nid);
-    HIP_TRY(hipGetLastError());
+int launch_node_v(shgpu_engine *, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
+    const hipError_t err = launch_node_variant<VARIANT>(a, nid, waves, s);
+    if (err != hipSuccess) return fail(SHGPU_ERR_HIP, std::string("k_node launch: ") + hipGetErrorString(err));
     return SHGPU_OK;
 }
 
@@ -1236,12 +862,7 @@ int shgpu_sh1_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_l
     if (rc) return rc;
     const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
     uint32_t *ctr = e->counter.as<uint32_t>();
-    switch (e->tuning.sh1_seeds_per_lane) {
-        case 1: k_sh1_batch<1><<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_seeds, ctr); break;
-        case 2: k_sh1_batch<2><<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_seeds, ctr); break;
-        default: k_sh1_batch<4><<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_seeds, ctr); break;
-    }
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_sh1_batch(e->tuning.sh1_seeds_per_lane, grid, s, d_keys, d_tasks, n, d_seeds, ctr));
     return SHGPU_OK;
 }
 
@@ -1256,9 +877,8 @@ int shgpu_sh2_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_l
     const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
     rc = ensure_sh2_scratch(e, (size_t)grid * WAVES_PER_BLOCK);
     if (rc) return rc;
-    k_sh2_batch<<<grid, BLOCK, 0, s>>>(d_keys, d_tasks, n, d_codes, e->counter.as<uint32_t>(), e->sh2_scratch.as<uint8_t>(),
-                                       e->tuning.flags);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_sh2_batch(grid, s, d_keys, d_tasks, n, d_codes, e->counter.as<uint32_t>(), e->sh2_scratch.as<uint8_t>(),
+                             e->tuning.flags));
     return SHGPU_OK;
 }
 
@@ -1287,9 +907,8 @@ int shgpu_split_search(shgpu_engine *e, uint64_t *d_keys, const shgpu_split_task
     if (rc) return rc;
     const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
     uint32_t *ctr = e->counter.as<uint32_t>();
-    launch_split_batch<false>(e->tuning.split_seeds_per_lane, grid, s, d_keys, e->tmp.as<uint64_t>(), d_tasks, n, d_seeds,
-                              ctr, extent);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_split_batch(false, e->tuning.split_seeds_per_lane, grid, s, d_keys, e->tmp.as<uint64_t>(), d_tasks, n,
+                               d_seeds, ctr, extent));
     return SHGPU_OK;
 }
 
@@ -1308,8 +927,8 @@ int shgpu_split_search_noalloc(shgpu_engine *e, uint64_t *d_keys, uint64_t n_key
     k_fill_u32<<<1, 64, 0, s>>>(ctr, 0, 16);  // a kernel, not a memset: callers may capture this stream in a graph
     HIP_TRY(hipGetLastError());
     const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
-    launch_split_batch<true>(e->tuning.split_seeds_per_lane, grid, s, d_keys, scratch_dev, d_tasks, n, d_seeds, ctr, n_keys);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_split_batch(true, e->tuning.split_seeds_per_lane, grid, s, d_keys, scratch_dev, d_tasks, n, d_seeds, ctr,
+                               n_keys));
     return SHGPU_OK;
 }
 
