@@ -573,16 +573,15 @@ static void rec_split(Ctx &c, uint64_t *keys, uint32_t m, int level) {
     if (m - i > 1) rec_split(c, keys + i, m - i, level + 1);
 }
 
-// hash_gen + compute_thread (ShockHash.h:557-688; ShockHash2.h:396-498; SIMDShockHash.hpp:619-760):
-// partition by .first (Sorter.hpp:15-36), contiguous bucket ranges per thread, tid-ordered merge.
-static Build *build(Variant v, uint32_t L, const uint64_t *h, size_t n, size_t bucket_size, int threads) {
-    Build *b = new Build(v, L);
+// parallelPartition (Sorter.hpp:15-36): sort by .first, bucket b = remap128(first, nbuckets), the
+// .second values in that order into `sorted`, bucket prefix sums into b->size_acc.
+static void partition(Build *b, const uint64_t *h, size_t n, size_t bucket_size, std::vector<uint64_t> &sorted) {
     b->n = n;
     b->nbuckets = std::max<size_t>(1, (n + bucket_size - 1) / bucket_size);
     std::vector<std::pair<uint64_t, uint64_t>> kv(n);
     for (size_t i = 0; i < n; i++) kv[i] = {h[2 * i], h[2 * i + 1]};
     std::sort(kv.begin(), kv.end());
-    std::vector<uint64_t> sorted(n);
+    sorted.assign(n, 0);
     b->size_acc.assign(b->nbuckets + 1, 0);
     b->pos_acc.assign(b->nbuckets + 1, 0);
     size_t i = 0;
@@ -591,6 +590,54 @@ static Build *build(Variant v, uint32_t L, const uint64_t *h, size_t n, size_t b
         while (i < n && remap128(kv[i].first, b->nbuckets) == bk) { sorted[i] = kv[i].second; i++; }
     }
     b->size_acc[b->nbuckets] = n;
+}
+
+// compute_thread's per-bucket work (ShockHash.h:566-579) over a chosen set of buckets only, for a partition
+// too large to build whole on the CPU: the nodes of those buckets' trees (codes, sizes, levels), in the order
+// of `sel`. Descriptors, positions and retrieval input are not produced.
+static Build *build_buckets(Variant v, uint32_t L, const uint64_t *h, size_t n, size_t bucket_size, const uint32_t *sel,
+                            size_t nsel, int threads) {
+    Build *b = new Build(v, L);
+    std::vector<uint64_t> sorted;
+    partition(b, h, n, bucket_size, sorted);
+    if (threads < 1) threads = 1;
+    std::vector<std::vector<NodeRec>> per(nsel);
+    std::vector<std::thread> th;
+    std::mutex mu;
+    size_t next = 0;
+    for (int t = 0; t < threads; t++) {
+        th.emplace_back([&]() {
+            Ctx c(*b);
+            for (;;) {
+                size_t k;
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    if (next >= nsel) return;
+                    k = next++;
+                }
+                const uint32_t bk = sel[k];
+                if (bk >= b->nbuckets) continue;
+                const size_t s = b->size_acc[bk + 1] - b->size_acc[bk];
+                c.nodes.clear();
+                if (s > 1) {
+                    c.bucket = bk;
+                    rec_split(c, &sorted[b->size_acc[bk]], (uint32_t)s, 0);
+                }
+                per[k] = c.nodes;
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+    for (auto &p : per) b->nodes.insert(b->nodes.end(), p.begin(), p.end());
+    return b;
+}
+
+// hash_gen + compute_thread (ShockHash.h:557-688; ShockHash2.h:396-498; SIMDShockHash.hpp:619-760):
+// partition by .first (Sorter.hpp:15-36), contiguous bucket ranges per thread, tid-ordered merge.
+static Build *build(Variant v, uint32_t L, const uint64_t *h, size_t n, size_t bucket_size, int threads) {
+    Build *b = new Build(v, L);
+    std::vector<uint64_t> sorted;
+    partition(b, h, n, bucket_size, sorted);
     if (threads < 1) threads = 1;
     std::vector<Ctx *> ctx(threads);
     std::vector<std::thread> th;
@@ -918,6 +965,11 @@ void orc_golomb_memo(int variant, int leaf, uint64_t *out /* 3000 */) {
 void *orc_build(int variant, int leaf, const uint64_t *hashes, size_t n, size_t bucket_size, int threads) {
     try { return build((Variant)variant, (uint32_t)leaf, hashes, n, bucket_size, threads); }
     catch (...) { return nullptr; }
+}
+// Nodes of the chosen buckets only (orc_build_nnodes / orc_build_nodes read them; nothing else is built).
+void *orc_build_buckets(int variant, int leaf, const uint64_t *hashes, size_t n, size_t bucket_size, const uint32_t *sel,
+                        size_t nsel, int threads) {
+    return build_buckets((Variant)variant, (uint32_t)leaf, hashes, n, bucket_size, sel, nsel, threads);
 }
 void orc_build_free(void *b) { delete (Build *)b; }
 size_t orc_build_nbuckets(void *b) { return ((Build *)b)->nbuckets; }

@@ -57,3 +57,18 @@ def random_strings(n, seed=3, min_len=10, max_len=50):
                 seen.add(s)
                 out.append(s)
     return out
+
+
+def bucket_sizes(hashes, bucket_size):
+    """Cumulative bucket sizes (nbuckets + 1 entries) of fingerprints partitioned as the constructors do:
+    nbuckets = max(1, ceil(n / bucket_size)), bucket = floor(first * nbuckets / 2^64) (remap128). Bucket b holds
+    the fingerprints with first >= ceil(b * 2^64 / nbuckets), so the bounds are found by searching the sorted
+    .first values for those thresholds (exact integer arithmetic, no 128-bit products over the keys)."""
+    first = np.sort(np.asarray(hashes, dtype=np.uint64).reshape(-1, 2)[:, 0])
+    n = len(first)
+    nb = max(1, -(-n // bucket_size))
+    acc = np.zeros(nb + 1, dtype=np.uint64)
+    thr = np.array([-(-(b << 64) // nb) for b in range(1, nb)], dtype=np.uint64)
+    acc[1:nb] = np.searchsorted(first, thr, side="left")
+    acc[nb] = n
+    return acc

@@ -42,6 +42,8 @@ _SIGS = {
     "orc_split_ok": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, U32, U64]),
     "orc_golomb_memo": (None, [ctypes.c_int, ctypes.c_int, P]),
     "orc_build": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]),
+    "orc_build_buckets": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, P, ctypes.c_size_t,
+                              ctypes.c_int]),
     "orc_build_free": (None, [P]),
     "orc_build_nbuckets": (ctypes.c_size_t, [P]),
     "orc_build_nwords": (ctypes.c_size_t, [P]),
@@ -204,3 +206,29 @@ class OracleBuild:
         if getattr(self, "h", None):
             lib().orc_build_free(self.h)
             self.h = None
+
+
+class OracleBucketBuild:
+    """The oracle's trees of chosen buckets of a partition (the rest is not searched): per node, in the order of
+    the chosen buckets and DFS pre-order within each, its bucket, size, level, kind and code."""
+
+    def __init__(self, variant, leaf, hashes, bucket_size, buckets, threads=8):
+        hashes = u64(hashes).reshape(-1, 2)
+        sel = np.ascontiguousarray(buckets, dtype=np.uint32)
+        L = lib()
+        h = L.orc_build_buckets(VARIANT[variant], leaf, ptr(hashes), hashes.shape[0], bucket_size, ptr(sel), len(sel),
+                                threads)
+        if not h:
+            raise RuntimeError("oracle bucket build failed")
+        try:
+            self.nbuckets = L.orc_build_nbuckets(h)
+            nn = L.orc_build_nnodes(h)
+            self.node_bucket = np.zeros(nn, dtype=np.uint32)
+            self.node_m = np.zeros(nn, dtype=np.uint32)
+            self.node_level = np.zeros(nn, dtype=np.int32)
+            self.node_kind = np.zeros(nn, dtype=np.uint8)
+            self.node_code = np.zeros(nn, dtype=np.uint64)
+            L.orc_build_nodes(h, ptr(self.node_bucket), ptr(self.node_m), ptr(self.node_level), ptr(self.node_kind),
+                              ptr(self.node_code))
+        finally:
+            L.orc_build_free(h)

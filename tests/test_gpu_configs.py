@@ -1,5 +1,7 @@
 """The workload configurations other than the bench's, at their stated sizes where the CPU oracle can check
 them in seconds, otherwise on a stated subset: GPU engine / class templates against the oracle."""
+import os
+
 import numpy as np
 import pytest
 
@@ -7,6 +9,8 @@ import oracle_lib as O
 from shockhash_amd import host, workload
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def log(msg):
@@ -36,19 +40,16 @@ def test_config1_shockhash30_strings_1m():
 
 def test_config2_isolated_leaves_200k(engine):
     """200,000 isolated ShockHash1 leaves of 40 keys (the .second of uniform fingerprints, seed 11), seeds from
-    0 (benchmark/bijections/ShockHash1.h): every GPU seed satisfies the leaf predicate, and on a sample of
-    2,000 leaves it is the oracle's minimal seed."""
+    0 (benchmark/bijections/ShockHash1.h:13-35): every GPU seed equals the oracle's minimal seed, all 200,000 of
+    them (tests/golden/config2_seeds.npz, written by tests/golden/make_config2_seeds.py)."""
     log("config 2: 200k leaves of 40")
-    n, m = 200_000, 40
-    keys = workload.isolated_leaves(n, m, seed=11)
+    gold = np.load(os.path.join(GOLDEN, "config2_seeds.npz"))
+    n, m = int(gold["n_leaves"]), int(gold["m"])
+    keys = workload.isolated_leaves(n, m, seed=int(gold["seed"]))
     engine.set_tuning(max_trials=1 << 40)
     got = engine.sh1_leaf_search(keys.reshape(-1), np.arange(n) * m, np.full(n, m), np.zeros(n, np.uint64))
-    log("  validity of every seed")
-    assert O.sh1_valid_batch(keys, got, threads=16).all()
-    pick = np.random.default_rng(12).choice(n, 2000, replace=False)
-    log("  oracle on 2000 leaves")
-    want = O.sh1_leaf_batch(keys[pick], start=0, threads=16)
-    assert np.array_equal(got[pick], want)
+    bad = np.nonzero(got != gold["seeds"])[0]
+    assert len(bad) == 0, f"{len(bad)} leaves differ, first {bad[:5]}: got {got[bad[:5]]} want {gold['seeds'][bad[:5]]}"
 
 
 def test_config3_simd58_two_buckets():
@@ -64,3 +65,39 @@ def test_config3_simd58_two_buckets():
     q = f(h)
     assert np.array_equal(q, ob.query(h))
     assert len(np.unique(q)) == len(h)
+
+
+@pytest.mark.timeout(900)
+def test_config3_simd58_10m_full_build():
+    """SIMDShockHash<58,false>, bucket 2000, all 10,000,000 fingerprints of the config-3 workload (SplitMix64
+    seed 1), ~10^14 hash evaluations on the GPU: the node codes of 32 buckets spread over the partition equal
+    the oracle's (tests/golden/config3_simd58_buckets.npz, written by tests/golden/make_config3_buckets.py, a few
+    core-minutes per bucket), and the 10,000,000 queries are distinct and below n."""
+    from shockhash_amd.engine import plan_forest
+    gold = np.load(os.path.join(GOLDEN, "config3_simd58_buckets.npz"))
+    n, leaf, bsize = int(gold["n"]), int(gold["leaf"]), int(gold["bucket_size"])
+    log(f"config 3: SIMDShockHash L{leaf}, {n} keys (about a minute on the GPU)")
+    h = workload.fingerprints(n, seed=int(gold["seed"]))
+    f = host.MPHF("simd", leaf, h, bsize, threads=1)
+    log(f"  built in {f.timings_ms['constructor'] / 1e3:.1f} s; comparing {len(gold['buckets'])} buckets")
+    acc = workload.bucket_sizes(h, bsize)
+    nodes = plan_forest("simd", leaf, acc)
+    codes = f.codes
+    assert len(codes) == len(nodes)
+    root = {}
+    i = 0
+    for b in range(len(acc) - 1):
+        if acc[b + 1] - acc[b] > 1:
+            root[b] = (i, int(nodes["subtree"][i]))
+            i += int(nodes["subtree"][i])
+    assert i == len(nodes)
+    for b in gold["buckets"]:
+        sel = gold["node_bucket"] == b
+        start, cnt = root[int(b)]
+        assert cnt == int(sel.sum()), f"bucket {b}: node count"
+        assert np.array_equal(nodes["m"][start:start + cnt], gold["node_m"][sel]), f"bucket {b}: tree shape"
+        assert np.array_equal(codes[start:start + cnt], gold["node_code"][sel]), f"bucket {b}: codes"
+    log("  queries")
+    q = f(h)
+    assert int(q.max()) < n and len(np.unique(q)) == n
+
