@@ -74,6 +74,10 @@ typedef struct shgpu_tuning {
 /* shgpu_tuning.flags: after a persistent build, compare the forest planned on the device with the host's
  * plan node by node (SHGPU_ERR_HIP on a difference); a check for tests, it copies every node back. */
 #define SHGPU_TUNE_CHECK_PLAN 4u
+/* shgpu_tuning.flags: ShockHash2 leaves above 64 keys first search with tables sized for a quarter of the
+ * half-seeds they need on average, so most of them outgrow the tables and are finished by the second pass with
+ * larger tables, which otherwise runs for about one leaf in 10^7; a check for tests, the codes are unchanged. */
+#define SHGPU_TUNE_SH2_TIGHT_TABLES 8u
 
 /* Last error message of the calling thread. */
 const char *shgpu_last_error(void);
@@ -110,7 +114,9 @@ int shgpu_sh1_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_l
                           uint64_t *d_seeds, void *stream);
 
 /* Batched ShockHash2 leaf search: d_codes[i] = the findSeed code of task i (start_seed and
- * max_trials are ignored). m <= 64. */
+ * max_trials are ignored). m <= 128; reads the tasks back to size the search tables by the largest leaf and
+ * synchronises when a leaf exceeds 64 keys. SHGPU_NOT_FOUND when a leaf's search exhausts its tables (repeated
+ * keys). */
 int shgpu_sh2_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_leaf_task *d_tasks, uint32_t n_tasks,
                           uint64_t *d_codes, void *stream);
 

@@ -22,18 +22,19 @@ __global__ void __launch_bounds__(BLOCK) k_sh1_batch(const uint64_t *__restrict_
     }
 }
 
+template <bool WIDE>
 __global__ void __launch_bounds__(BLOCK) k_sh2_batch(const uint64_t *__restrict__ keys, const shgpu_leaf_task *__restrict__ tasks,
                                                      uint32_t n, uint64_t *__restrict__ out, uint32_t *counter, uint8_t *scratch,
-                                                     uint32_t flags) {
-    __shared__ Sh2Lds lds[WAVES_PER_BLOCK];
+                                                     uint64_t wave_bytes, Sh2Caps caps, uint32_t flags) {
+    __shared__ Sh2LdsAny<WIDE> lds[WAVES_PER_BLOCK];
     const uint32_t w = threadIdx.x / 64;
-    Sh2Lds &L = lds[w];
-    uint8_t *ws = scratch + (size_t)(blockIdx.x * WAVES_PER_BLOCK + w) * SH2_SCRATCH_BYTES;
+    Sh2LdsAny<WIDE> &L = lds[w];
+    uint8_t *ws = scratch + (size_t)(blockIdx.x * WAVES_PER_BLOCK + w) * wave_bytes;
     for (;;) {
         const uint32_t t = grab(counter);
         if (t >= n) break;
         const shgpu_leaf_task tk = tasks[t];
-        const uint64_t code = sh2_leaf_search(keys + uniform(tk.offset), uniform(tk.m), ws, L, flags, task_clock());
+        const uint64_t code = sh2_leaf_search_any<WIDE>(keys + uniform(tk.offset), uniform(tk.m), ws, L, caps, flags, task_clock());
         out[t] = code;
     }
 }
@@ -83,9 +84,11 @@ hipError_t launch_sh1_batch(uint32_t seeds_per_lane, int grid, hipStream_t s, co
     return hipGetLastError();
 }
 
-hipError_t launch_sh2_batch(int grid, hipStream_t s, const uint64_t *keys, const shgpu_leaf_task *tasks, uint32_t n,
-                            uint64_t *out, uint32_t *ctr, uint8_t *scratch, uint32_t flags) {
-    k_sh2_batch<<<grid, BLOCK, 0, s>>>(keys, tasks, n, out, ctr, scratch, flags);
+hipError_t launch_sh2_batch(bool wide, int grid, hipStream_t s, const uint64_t *keys, const shgpu_leaf_task *tasks, uint32_t n,
+                            uint64_t *out, uint32_t *ctr, uint8_t *scratch, uint64_t wave_bytes, Sh2Caps caps,
+                            uint32_t flags) {
+    if (wide) k_sh2_batch<true><<<grid, BLOCK, 0, s>>>(keys, tasks, n, out, ctr, scratch, wave_bytes, caps, flags);
+    else k_sh2_batch<false><<<grid, BLOCK, 0, s>>>(keys, tasks, n, out, ctr, scratch, wave_bytes, caps, flags);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 
 #include "../../include/shockhash_amd.h"
 #include "device_common.h"
+#include "search_sh2.h"
 
 namespace shgpu {
 
@@ -45,7 +46,9 @@ struct BuildArgs {
     uint32_t *help;  // per wave slot: node whose windows the slot's wave searches, or EMPTY_SLOT
     uint32_t total;  // plain form: nodes (the persistent kernel reads ctrl[6])
     uint32_t nslots;
-    uint8_t *scratch;
+    uint8_t *scratch;          // ShockHash2 leaves: sh2_wave_bytes per wave slot
+    uint64_t sh2_wave_bytes;
+    Sh2Caps sh2_caps;          // table capacities of leaves above 64 keys
     int split_hash;
     uint32_t flags;  // shgpu_tuning.flags of the variant
 };
@@ -54,17 +57,20 @@ struct BuildArgs {
 __host__ __device__ __forceinline__ bool two_part_queue(uint32_t flags) { return (flags & SHGPU_TUNE_SPLITS_FIRST) != 0; }
 
 // Persistent build of one variant (k_build, grid x BLOCK threads) with sl / ss seeds per lane for leaf / split
-// windows (1, 2 or 4; ShockHash2 leaves ignore sl), and the plain form's launch for node nid (k_node).
+// windows (1, 2 or 4; ShockHash2 leaves ignore sl), and the plain form's launch for node nid (k_node). wide:
+// ShockHash2 leaves of more than 64 keys occur (the kernels then carry the two-keys-per-lane search).
 template <int VARIANT>
-hipError_t launch_build_variant(uint32_t sl, uint32_t ss, const BuildArgs &a, int grid, hipStream_t s);
+hipError_t launch_build_variant(uint32_t sl, uint32_t ss, bool wide, const BuildArgs &a, int grid, hipStream_t s);
 template <int VARIANT>
-hipError_t launch_node_variant(const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s);
+hipError_t launch_node_variant(bool wide, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s);
 
 // Batched per-node searches (batch.hip); every call returns the launch's status.
 hipError_t launch_sh1_batch(uint32_t seeds_per_lane, int grid, hipStream_t s, const uint64_t *keys,
                             const shgpu_leaf_task *tasks, uint32_t n, uint64_t *out, uint32_t *ctr);
-hipError_t launch_sh2_batch(int grid, hipStream_t s, const uint64_t *keys, const shgpu_leaf_task *tasks, uint32_t n,
-                            uint64_t *out, uint32_t *ctr, uint8_t *scratch, uint32_t flags);
+// wide: some task has more than 64 keys; each wave's scratch is wave_bytes, caps sizes the wide tables
+hipError_t launch_sh2_batch(bool wide, int grid, hipStream_t s, const uint64_t *keys, const shgpu_leaf_task *tasks, uint32_t n,
+                            uint64_t *out, uint32_t *ctr, uint8_t *scratch, uint64_t wave_bytes, Sh2Caps caps,
+                            uint32_t flags);
 // check: validate every task on the device against n_keys (the no-allocation entry point)
 hipError_t launch_split_batch(bool check, uint32_t seeds_per_lane, int grid, hipStream_t s, uint64_t *keys, uint64_t *tmp,
                               const shgpu_split_task *tasks, uint32_t n, uint64_t *out, uint32_t *ctr, uint64_t n_keys);

@@ -164,23 +164,24 @@ __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLd
 }
 
 // A wave runs either a windowed search (pf) or a ShockHash2 leaf (sh2), never both at once.
-template <int VARIANT>
+template <int VARIANT, bool WIDE>
 union BuildLds {
     PfLds pf;
-    typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2Lds, char>::type sh2;
+    typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2LdsAny<WIDE>, char>::type sh2;
 };
 
 // One node taken from the queue: ShockHash2 leaves are searched by this wave alone; splits and
 // ShockHash1 leaves are searched in windows that idle waves may join (help registry).
-template <int VARIANT, int SL, int SS>
-__device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint32_t slot, BuildLds<VARIANT> &L,
+template <int VARIANT, int SL, int SS, bool WIDE>
+__device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint32_t slot, BuildLds<VARIANT, WIDE> &L,
                                          uint8_t *ws) {
     const shgpu_node nd = a.nodes[nid];
     const uint32_t kind = uniform(nd.kind);
     const uint64_t t_start = task_clock();
     if (VARIANT == SHGPU_SHOCKHASH2 && kind == SHGPU_NODE_LEAF) {
         if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            const uint64_t code = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.flags, t_start);
+            const uint64_t code = sh2_leaf_search_any<WIDE>(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.sh2_caps,
+                                                            a.flags, t_start);
             a.codes[nid] = code;
             wave_fetch_add(&a.ctrl[3], 1u);
         }
@@ -200,13 +201,13 @@ constexpr uint32_t SPIN_LIMIT = 1u << 27;  // sleeps before a wait is declared s
 // nodes are taken before any leaf; otherwise it is one FIFO. While a slot is not yet published (its
 // parent is still being split) the wave helps windowed searches. When the queue is exhausted, waves
 // keep helping until every node is finished.
-template <int VARIANT, int SL, int SS>
+template <int VARIANT, int SL, int SS, bool WIDE>
 __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
-    __shared__ BuildLds<VARIANT> lds[WAVES_PER_BLOCK];
+    __shared__ BuildLds<VARIANT, WIDE> lds[WAVES_PER_BLOCK];
     const uint32_t w = threadIdx.x / 64;
     const uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
-    BuildLds<VARIANT> &L = lds[w];
-    uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)slot * SH2_SCRATCH_BYTES : nullptr;
+    BuildLds<VARIANT, WIDE> &L = lds[w];
+    uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)slot * a.sh2_wave_bytes : nullptr;
     const uint32_t total = uniform(a.ctrl[6]);  // written by the forest plan
     for (;;) {
         const uint32_t idx = grab(&a.ctrl[0]);
@@ -221,7 +222,7 @@ __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
         }
         if (nid == EMPTY_SLOT) return;
         __threadfence();
-        run_node<VARIANT, SL, SS>(a, nid, slot, L, ws);
+        run_node<VARIANT, SL, SS, WIDE>(a, nid, slot, L, ws);
     }
     // Queue exhausted: help until every node is finished. Once every windowed node has started (ctrl[8]
     // reaches ctrl[7]) no node will register for help any more, and a wave that finds none open leaves
@@ -241,14 +242,15 @@ __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
 
 // Plain form: one launch per node (the host walks the forest in DFS order on one stream). Every
 // wave of the launch joins the node's windowed search; a ShockHash2 leaf is searched by wave 0.
-template <int VARIANT, int SL, int SS>
+template <int VARIANT, int SL, int SS, bool WIDE>
 __global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
-    __shared__ BuildLds<VARIANT> L;
+    __shared__ BuildLds<VARIANT, WIDE> L;
     const shgpu_node nd = a.nodes[nid];
     if (VARIANT == SHGPU_SHOCKHASH2 && uniform(nd.kind) == SHGPU_NODE_LEAF) {
         if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
             if (blockIdx.x == 0)
-                a.codes[nid] = sh2_leaf_search(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2, a.flags, task_clock());
+                a.codes[nid] = sh2_leaf_search_any<WIDE>(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2,
+                                                         a.sh2_caps, a.flags, task_clock());
         }
         return;
     }
@@ -256,18 +258,18 @@ __global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
 }
 
 
-template <int VARIANT, int SL, int SS>
+template <int VARIANT, int SL, int SS, bool WIDE>
 hipError_t launch_build_t(const BuildArgs &a, int grid, hipStream_t s) {
-    k_build<VARIANT, SL, SS><<<grid, BLOCK, 0, s>>>(a);
+    k_build<VARIANT, SL, SS, WIDE><<<grid, BLOCK, 0, s>>>(a);
     return hipGetLastError();
 }
 
-template <int VARIANT>
-hipError_t launch_build_variant(uint32_t sl, uint32_t ss, const BuildArgs &a, int grid, hipStream_t s) {
+template <int VARIANT, bool WIDE>
+hipError_t launch_build_w(uint32_t sl, uint32_t ss, const BuildArgs &a, int grid, hipStream_t s) {
     // ShockHash2 leaves do not use the ShockHash1 window: one SL instantiation serves every setting
-#define SHGPU_CASE(A, B)                                                                            \
-    if constexpr (VARIANT != SHGPU_SHOCKHASH2 || A == 1) {                                          \
-        if ((VARIANT == SHGPU_SHOCKHASH2 || sl == A) && ss == B) return launch_build_t<VARIANT, A, B>(a, grid, s); \
+#define SHGPU_CASE(A, B)                                                                                      \
+    if constexpr (VARIANT != SHGPU_SHOCKHASH2 || A == 1) {                                                    \
+        if ((VARIANT == SHGPU_SHOCKHASH2 || sl == A) && ss == B) return launch_build_t<VARIANT, A, B, WIDE>(a, grid, s); \
     }
     SHGPU_CASE(1, 1) SHGPU_CASE(1, 2) SHGPU_CASE(1, 4) SHGPU_CASE(2, 1) SHGPU_CASE(2, 2) SHGPU_CASE(2, 4)
     SHGPU_CASE(4, 1) SHGPU_CASE(4, 2) SHGPU_CASE(4, 4)
@@ -276,8 +278,23 @@ hipError_t launch_build_variant(uint32_t sl, uint32_t ss, const BuildArgs &a, in
 }
 
 template <int VARIANT>
-hipError_t launch_node_variant(const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
-    k_node<VARIANT, VARIANT == SHGPU_SHOCKHASH2 ? 1 : 4, 4><<<waves, 64, 0, s>>>(a, nid);
+hipError_t launch_build_variant(uint32_t sl, uint32_t ss, bool wide, const BuildArgs &a, int grid, hipStream_t s) {
+    if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
+        if (wide) return launch_build_w<VARIANT, true>(sl, ss, a, grid, s);
+    }
+    return launch_build_w<VARIANT, false>(sl, ss, a, grid, s);
+}
+
+template <int VARIANT>
+hipError_t launch_node_variant(bool wide, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
+    constexpr int SL = VARIANT == SHGPU_SHOCKHASH2 ? 1 : 4;
+    if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
+        if (wide) {
+            k_node<VARIANT, SL, 4, true><<<waves, 64, 0, s>>>(a, nid);
+            return hipGetLastError();
+        }
+    }
+    k_node<VARIANT, SL, 4, false><<<waves, 64, 0, s>>>(a, nid);
     return hipGetLastError();
 }
 

@@ -3,6 +3,6 @@
 #include "build_kernels.h"
 
 namespace shgpu {
-template hipError_t launch_build_variant<SHGPU_SIMD_SHOCKHASH>(uint32_t, uint32_t, const BuildArgs &, int, hipStream_t);
-template hipError_t launch_node_variant<SHGPU_SIMD_SHOCKHASH>(const BuildArgs &, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_build_variant<SHGPU_SIMD_SHOCKHASH>(uint32_t, uint32_t, bool, const BuildArgs &, int, hipStream_t);
+template hipError_t launch_node_variant<SHGPU_SIMD_SHOCKHASH>(bool, const BuildArgs &, uint32_t, uint32_t, hipStream_t);
 }  // namespace shgpu

@@ -95,7 +95,7 @@ struct PinnedBuf {
 
 struct BuildBuffers {
     DevBuf firsts_in, firsts_out, seconds_in, tmp, keys, sort_tmp, size_acc, nodes, codes, queue, ctrl, states, help,
-        sh2_scratch, staging, plan_tab, plan_sets, plan_bucket, plan_summary, sort_flag;
+        sh2_scratch, staging, plan_tab, plan_sets, plan_bucket, plan_summary, sort_flag, sh2_retry;
     // forest plan: node count tables of plan_variant / plan_leaf (host copy, then the device copy in plan_tab),
     // node capacity of nodes / codes / queue / states
     int plan_variant = -1;
@@ -108,7 +108,7 @@ struct BuildBuffers {
     size_t bytes() const {
         const DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
                                &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging, &plan_tab,
-                               &plan_sets, &plan_bucket, &plan_summary, &sort_flag};
+                               &plan_sets, &plan_bucket, &plan_summary, &sort_flag, &sh2_retry};
         size_t t = 0;
         for (const DevBuf *d : all) t += d->bytes;
         return t;
@@ -116,7 +116,7 @@ struct BuildBuffers {
     void release() {
         DevBuf *all[] = {&firsts_in, &firsts_out, &seconds_in, &tmp, &keys, &sort_tmp, &size_acc, &nodes,
                          &codes, &queue, &ctrl, &states, &help, &sh2_scratch, &staging, &plan_tab,
-                         &plan_sets, &plan_bucket, &plan_summary, &sort_flag};
+                         &plan_sets, &plan_bucket, &plan_summary, &sort_flag, &sh2_retry};
         for (DevBuf *d : all) {
             if (d->p) (void)hipFree(d->p);
             d->p = nullptr;
@@ -306,6 +306,23 @@ __global__ void k_init_states(NodeState *st, uint64_t *codes, uint32_t n, uint32
         codes[i] = SHGPU_NOT_FOUND;  // a node that is never finalised reads as not found
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) help[i] = EMPTY_SLOT;
+}
+
+// ShockHash2 leaves above 64 keys that outgrew their tables (code SHGPU_NOT_FOUND after the build) as leaf
+// tasks, with their node ids; count[0] receives how many.
+__global__ void k_sh2_collect_overflow(const shgpu_node *__restrict__ nodes, const uint64_t *__restrict__ codes, uint32_t total,
+                                       shgpu_leaf_task *tasks, uint32_t *ids, uint32_t *count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const shgpu_node nd = nodes[i];
+        if (nd.kind != SHGPU_NODE_LEAF || nd.m <= 64 || codes[i] != SHGPU_NOT_FOUND) continue;
+        const uint32_t k = atomicAdd(count, 1u);  // k < total: each node is collected once
+        tasks[k] = shgpu_leaf_task{nd.offset, nd.m, 0, 0};
+        ids[k] = i;
+    }
+}
+
+__global__ void k_scatter_codes(const uint64_t *__restrict__ src, const uint32_t *__restrict__ ids, uint32_t n, uint64_t *codes) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) codes[ids[t]] = src[t];
 }
 
 // ------------------------------------------------------------------ forest plan on the device
@@ -683,7 +700,8 @@ int check_variant(int variant, uint32_t leaf) {
         return SHGPU_OK;
     }
     if (variant == SHGPU_SHOCKHASH2) {
-        if (leaf < 1 || leaf > 64) return fail(SHGPU_ERR_UNSUPPORTED, "ShockHash2 GPU leaf search supports leaf sizes <= 64");
+        if (leaf < 1 || leaf > SH2_MAX_LEAF)
+            return fail(SHGPU_ERR_UNSUPPORTED, "ShockHash2 leaf size must be in [1, 128] (MAX_LEAF_SIZE2)");
         return SHGPU_OK;
     }
     return fail(SHGPU_ERR_ARG, "unknown variant");
@@ -720,19 +738,64 @@ int reset_counter(shgpu_engine *e, hipStream_t s) {
     return SHGPU_OK;
 }
 
-int ensure_sh2_scratch(shgpu_engine *e, size_t waves) { return e->sh2_scratch.ensure(waves * SH2_SCRATCH_BYTES); }
+// P(`balls` keys hashed uniformly into `bins` cells leave no cell empty), by inclusion-exclusion.
+double cover_probability(uint32_t balls, uint32_t bins) {
+    double sum = 0, binom = 1;
+    for (uint32_t i = 0; i <= bins; i++) {
+        sum += (i & 1 ? -1.0 : 1.0) * binom * std::pow(1.0 - (double)i / bins, (double)balls);
+        binom = binom * (double)(bins - i) / (double)(i + 1);
+    }
+    return std::max(sum, 0.0);
+}
+
+// Scratch layout of the ShockHash2 leaf searches of a batch whose largest leaf has max_m keys. Up to 64 keys:
+// the fixed tables (1024 half-seeds, 4096 candidates). Above: two keys per lane, with tables sized from the
+// largest leaf. Its code (pairTriangular of two pairElegant codes) is about s^4 / 2 for s half-seeds and is
+// geometric with mean about 2^k / ln 2, k = bij_memo2[m] the Golomb parameter the reference chose for it, so
+// P(s > x * s_mean) = exp(-x^4) with s_mean = (2^(k+1) / ln 2)^(1/4) (~110 at m = 64, ~15,000 at m = 128).
+// The tables hold x = 2 * scale mean half-seeds (a leaf needs more with probability exp(-16) for scale 1,
+// and the host searches those again with scale 2, exp(-256)) and the candidates among their pairs: each pair of
+// half-seeds covers all ceil(m/2) cells with the probability that m keys cover them.
+struct Sh2Layout {
+    bool wide = false;
+    Sh2Caps caps = SH2_NARROW_CAPS;
+    size_t wave_bytes = SH2_SCRATCH_BYTES;
+};
+Sh2Layout sh2_layout(uint32_t max_m, uint32_t scale = 1, bool tight = false) {
+    Sh2Layout l;
+    if (max_m <= 64) return l;
+    l.wide = true;
+    const uint32_t k = shockhash::bij_memo2[std::min<uint32_t>(max_m, SH2_MAX_LEAF)];
+    const double s_mean = std::pow(std::ldexp(1.0, (int)k + 1) / std::log(2.0), 0.25);
+    const double x = tight ? 0.25 : 2.0 * scale;  // tight: a test knob that makes leaves overflow
+    const double hs = std::min(65472.0, std::max(1024.0, std::ceil(x * s_mean / 64) * 64));
+    const double lambda = hs * hs * cover_probability(max_m, (max_m + 1) / 2);
+    const double cand = std::min(4194304.0, std::max(4096.0, std::ceil((1.1 * lambda + 64) / 64) * 64));
+    l.caps = Sh2Caps{(uint32_t)hs, (uint32_t)cand};
+    l.wave_bytes = sh2_scratch_bytes(2, l.caps);
+    return l;
+}
+// Waves that search wide leaves share at most SH2_WIDE_BUDGET bytes of scratch.
+constexpr size_t SH2_WIDE_BUDGET = 16ull << 30;
+int sh2_grid_cap(const Sh2Layout &l, int grid) {
+    if (!l.wide) return grid;
+    const size_t blocks = SH2_WIDE_BUDGET / (l.wave_bytes * WAVES_PER_BLOCK);
+    return std::max(1, std::min(grid, (int)std::min<size_t>(blocks, 1u << 30)));
+}
+
+int ensure_sh2_scratch(shgpu_engine *e, size_t waves, const Sh2Layout &l) { return e->sh2_scratch.ensure(waves * l.wave_bytes); }
 
 template <int VARIANT>
-int launch_build_v(shgpu_engine *e, const BuildArgs &a, int grid, hipStream_t s) {
+int launch_build_v(shgpu_engine *e, bool wide, const BuildArgs &a, int grid, hipStream_t s) {
     const hipError_t err = launch_build_variant<VARIANT>(e->tuning_v[VARIANT].sh1_seeds_per_lane,
-                                                         e->tuning_v[VARIANT].split_seeds_per_lane, a, grid, s);
+                                                         e->tuning_v[VARIANT].split_seeds_per_lane, wide, a, grid, s);
     if (err != hipSuccess) return fail(SHGPU_ERR_HIP, std::string("k_build launch: ") + hipGetErrorString(err));
     return SHGPU_OK;
 }
 
 template <int VARIANT>
-int launch_node_v(shgpu_engine *, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
-    const hipError_t err = launch_node_variant<VARIANT>(a, nid, waves, s);
+int launch_node_v(shgpu_engine *, bool wide, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
+    const hipError_t err = launch_node_variant<VARIANT>(wide, a, nid, waves, s);
     if (err != hipSuccess) return fail(SHGPU_ERR_HIP, std::string("k_node launch: ") + hipGetErrorString(err));
     return SHGPU_OK;
 }
@@ -817,7 +880,7 @@ static int check_tuning(const shgpu_tuning *t) {
         return fail(SHGPU_ERR_ARG, "seeds per lane must be 1, 2 or 4");
     if (t->waves_per_cu > 32) return fail(SHGPU_ERR_ARG, "waves_per_cu must be in [0, 32] (0: by batch size)");
     if (t->max_trials == 0) return fail(SHGPU_ERR_ARG, "max_trials must be positive");
-    if (t->flags & ~(SHGPU_TUNE_SH2_GENERAL_ONLY | SHGPU_TUNE_SPLITS_FIRST | SHGPU_TUNE_CHECK_PLAN))
+    if (t->flags & ~(SHGPU_TUNE_SH2_GENERAL_ONLY | SHGPU_TUNE_SPLITS_FIRST | SHGPU_TUNE_CHECK_PLAN | SHGPU_TUNE_SH2_TIGHT_TABLES))
         return fail(SHGPU_ERR_ARG, "unknown tuning flags");
     return SHGPU_OK;
 }
@@ -872,13 +935,47 @@ int shgpu_sh2_leaf_search(shgpu_engine *e, const uint64_t *d_keys, const shgpu_l
     if (n == 0) return SHGPU_OK;
     HIP_TRY(hipSetDevice(e->device));
     hipStream_t s = pick(e, stream);
+    // the table layout follows the largest leaf: read the tasks' sizes
+    std::vector<shgpu_leaf_task> h(n);
+    HIP_TRY(hipMemcpyAsync(h.data(), d_tasks, n * sizeof(shgpu_leaf_task), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint32_t max_m = 0;
+    for (const auto &t : h) max_m = std::max(max_m, t.m);
+    if (max_m > SH2_MAX_LEAF) return fail(SHGPU_ERR_ARG, "ShockHash2 leaf task with more than 128 keys");
+    const Sh2Layout l = sh2_layout(max_m, 1, (e->tuning.flags & SHGPU_TUNE_SH2_TIGHT_TABLES) != 0);
     int rc = reset_counter(e, s);
     if (rc) return rc;
-    const int grid = std::min<int>(grid_for(e), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
-    rc = ensure_sh2_scratch(e, (size_t)grid * WAVES_PER_BLOCK);
+    const int grid = std::min<int>(sh2_grid_cap(l, grid_for(e)), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
+    rc = ensure_sh2_scratch(e, (size_t)grid * WAVES_PER_BLOCK, l);
     if (rc) return rc;
-    HIP_TRY(launch_sh2_batch(grid, s, d_keys, d_tasks, n, d_codes, e->counter.as<uint32_t>(), e->sh2_scratch.as<uint8_t>(),
-                             e->tuning.flags));
+    HIP_TRY(launch_sh2_batch(l.wide, grid, s, d_keys, d_tasks, n, d_codes, e->counter.as<uint32_t>(), e->sh2_scratch.as<uint8_t>(),
+                             l.wave_bytes, l.caps, e->tuning.flags));
+    if (!l.wide) return SHGPU_OK;
+    // leaves above 64 keys that outgrew the tables: searched again with tables twice as large
+    std::vector<uint64_t> codes(n);
+    HIP_TRY(hipMemcpyAsync(codes.data(), d_codes, n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<shgpu_leaf_task> again;
+    std::vector<uint32_t> ids;
+    for (uint32_t i = 0; i < n; i++)
+        if (codes[i] == SHGPU_NOT_FOUND && h[i].m > 64) { again.push_back(h[i]); ids.push_back(i); }
+    if (again.empty()) return SHGPU_OK;
+    const uint32_t k = (uint32_t)again.size();
+    const Sh2Layout l2 = sh2_layout(max_m, 2);
+    const size_t off_out = sh2_align256((size_t)k * sizeof(shgpu_leaf_task));
+    if ((rc = e->staging.ensure(off_out + (size_t)k * 8))) return rc;
+    HIP_TRY(hipMemcpyAsync(e->staging.p, again.data(), k * sizeof(shgpu_leaf_task), hipMemcpyHostToDevice, s));
+    if ((rc = reset_counter(e, s))) return rc;
+    const int grid2 = std::min<int>(sh2_grid_cap(l2, grid_for(e)), (int)((k + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
+    if ((rc = ensure_sh2_scratch(e, (size_t)grid2 * WAVES_PER_BLOCK, l2))) return rc;
+    uint64_t *out2 = (uint64_t *)(e->staging.as<uint8_t>() + off_out);
+    HIP_TRY(launch_sh2_batch(true, grid2, s, d_keys, (const shgpu_leaf_task *)e->staging.p, k, out2, e->counter.as<uint32_t>(),
+                             e->sh2_scratch.as<uint8_t>(), l2.wave_bytes, l2.caps, e->tuning.flags & SHGPU_TUNE_SH2_GENERAL_ONLY));
+    std::vector<uint64_t> codes2(k);
+    HIP_TRY(hipMemcpyAsync(codes2.data(), out2, k * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (uint32_t t = 0; t < k; t++) HIP_TRY(hipMemcpyAsync(d_codes + ids[t], &codes2[t], 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return SHGPU_OK;
 }
 
@@ -986,6 +1083,38 @@ static int host_plan(const shgpu_build &b, std::vector<shgpu_node> &out) {
     return SHGPU_OK;
 }
 
+// Searches again, with tables twice as large (sh2_layout scale 2), the ShockHash2 leaves above 64 keys whose
+// search outgrew the build's tables, and writes their codes into a.codes. The stream is idle on entry (the
+// caller has synchronised it). A leaf that fails again keeps SHGPU_NOT_FOUND (repeated keys).
+static int sh2_retry_overflowed(shgpu_engine *e, BuildBuffers &B, hipStream_t s, const BuildArgs &a, uint32_t total,
+                                uint32_t leaf, uint32_t flags) {
+    const size_t off_tasks = 256, off_ids = off_tasks + sh2_align256((size_t)total * sizeof(shgpu_leaf_task)),
+                 off_out = off_ids + sh2_align256((size_t)total * 4), bytes = off_out + (size_t)total * 8 + 64;
+    int rc = B.sh2_retry.ensure(bytes);
+    if (rc) return rc;
+    uint8_t *base = B.sh2_retry.as<uint8_t>();
+    uint32_t *count = (uint32_t *)base;  // [0] overflowed leaves, [16] the batch's work counter
+    shgpu_leaf_task *tasks = (shgpu_leaf_task *)(base + off_tasks);
+    uint32_t *ids = (uint32_t *)(base + off_ids);
+    uint64_t *out = (uint64_t *)(base + off_out);
+    k_fill_u32<<<1, 64, 0, s>>>(count, 0, 64);
+    HIP_TRY(hipGetLastError());
+    k_sh2_collect_overflow<<<std::max(1u, std::min(1024u, (total + 255) / 256)), 256, 0, s>>>(a.nodes, a.codes, total, tasks, ids, count);
+    HIP_TRY(hipGetLastError());
+    uint32_t n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, count, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (n == 0) return SHGPU_OK;
+    const Sh2Layout l = sh2_layout(leaf, 2);
+    const int grid = std::min(sh2_grid_cap(l, grid_for(e)), (int)((n + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK));
+    if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * l.wave_bytes))) return rc;
+    HIP_TRY(launch_sh2_batch(true, grid, s, a.keys, tasks, n, out, count + 16, B.sh2_scratch.as<uint8_t>(), l.wave_bytes, l.caps,
+                             flags & SHGPU_TUNE_SH2_GENERAL_ONLY));
+    k_scatter_codes<<<std::max(1u, std::min(1024u, (n + 255) / 256)), 256, 0, s>>>(out, ids, n, a.codes);
+    HIP_TRY(hipGetLastError());
+    return SHGPU_OK;
+}
+
 // The construction pipeline for a batch of independent key sets on one buffer set and stream: each
 // set is sorted and bucketed on its own, their forests share one node array, queue and persistent
 // kernel (the trees are independent, so every set's codes equal those of a build of that set alone).
@@ -1073,7 +1202,11 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     }
     BATCH_HIP(hipEventRecord(B.evB, s));
 
-    const int grid = std::max(1, grid_for(e, tune, d->variant, N) / (int)std::max(1u, grid_div));
+    // ShockHash2 leaves above 64 keys need tables that grow with the leaf size (fewer waves)
+    const Sh2Layout sh2l = d->variant == SHGPU_SHOCKHASH2
+                               ? sh2_layout(d->leaf_size, 1, (tune.flags & SHGPU_TUNE_SH2_TIGHT_TABLES) != 0)
+                               : Sh2Layout{};
+    const int grid = sh2_grid_cap(sh2l, std::max(1, grid_for(e, tune, d->variant, N) / (int)std::max(1u, grid_div)));
     const uint32_t nslots = (uint32_t)grid * WAVES_PER_BLOCK;
     std::vector<uint64_t> acc_all(NB);
     std::vector<uint64_t> codes;
@@ -1090,6 +1223,8 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     a.split_hash = d->variant == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
     a.flags = tune.flags;
     a.scratch = nullptr;
+    a.sh2_caps = sh2l.caps;
+    a.sh2_wave_bytes = sh2l.wave_bytes;
     a.nslots = nslots;
     const Strategy st(d->variant, d->leaf_size);
     auto t1 = clk::now(), t2 = t1;
@@ -1126,7 +1261,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             return fail_all(rc);
         BATCH_HIP(hipMemcpyAsync(B.plan_sets.p, sets.data(), count * sizeof(PlanSet), hipMemcpyHostToDevice, s));
         if (d->variant == SHGPU_SHOCKHASH2) {
-            if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * SH2_SCRATCH_BYTES))) return fail_all(rc);
+            if ((rc = B.sh2_scratch.ensure((size_t)grid * WAVES_PER_BLOCK * sh2l.wave_bytes))) return fail_all(rc);
             a.scratch = B.sh2_scratch.as<uint8_t>();
         }
         std::vector<uint32_t> summary(SUM_SETS + 3 * count);
@@ -1174,9 +1309,9 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             a.help = p.help;
             a.total = 0;
             BATCH_HIP(hipEventRecord(B.ev0, s));
-            if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, a, grid, s);
-            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_build_v<SHGPU_SIMD_SHOCKHASH>(e, a, grid, s);
-            else rc = launch_build_v<SHGPU_SHOCKHASH2>(e, a, grid, s);
+            if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, false, a, grid, s);
+            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_build_v<SHGPU_SIMD_SHOCKHASH>(e, false, a, grid, s);
+            else rc = launch_build_v<SHGPU_SHOCKHASH2>(e, sh2l.wide, a, grid, s);
             if (rc) return fail_all(rc);
             BATCH_HIP(hipEventRecord(B.ev1, s));
             timed_kernel = true;
@@ -1191,6 +1326,9 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             cap = (uint64_t)summary[SUM_TOTAL] + 1024;  // exact size, then run the plan and the search again
         }
         const uint32_t total = summary[SUM_TOTAL];
+        if (sh2l.wide && total && !ctrl_out[2] &&
+            (rc = sh2_retry_overflowed(e, B, s, a, total, d->leaf_size, tune.flags)))
+            return fail_all(rc);
         if ((rc = B.codes_host.ensure((size_t)total * 8 + 8))) return fail_all(rc);
         codes_pinned = (const uint64_t *)B.codes_host.p;
         n_codes = total;
@@ -1273,7 +1411,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         a.help = nullptr;
         a.total = total;
         if (d->variant == SHGPU_SHOCKHASH2) {
-            if ((rc = B.sh2_scratch.ensure(SH2_SCRATCH_BYTES))) return fail_all(rc);
+            if ((rc = B.sh2_scratch.ensure(sh2l.wave_bytes))) return fail_all(rc);
             a.scratch = B.sh2_scratch.as<uint8_t>();
         }
         // 3. seed search: one launch per node, DFS order (every node's own state is its window counter).
@@ -1287,10 +1425,14 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         for (uint32_t nid = 0; nid < total; nid++) {
             const bool sh2_leaf = d->variant == SHGPU_SHOCKHASH2 && nodes[nid].kind == SHGPU_NODE_LEAF;
             const uint32_t wv = sh2_leaf ? 1u : waves;
-            if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, a, nid, wv, s);
-            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, a, nid, wv, s);
-            else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, a, nid, wv, s);
+            if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, false, a, nid, wv, s);
+            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, false, a, nid, wv, s);
+            else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, sh2l.wide, a, nid, wv, s);
             if (rc) return fail_all(rc);
+        }
+        if (sh2l.wide && total) {
+            BATCH_HIP(hipStreamSynchronize(s));
+            if ((rc = sh2_retry_overflowed(e, B, s, a, total, d->leaf_size, tune.flags))) return fail_all(rc);
         }
         codes.resize(total);
         if (total) BATCH_HIP(hipMemcpyAsync(codes.data(), a.codes, (size_t)total * 8, hipMemcpyDeviceToHost, s));
@@ -1619,11 +1761,11 @@ struct shgpu_server {
     }
     // Bytes every slot holds regardless of the request: the ShockHash2 per-wave leaf scratch.
     size_t fixed_bytes() const {
-        return cfg.variant == SHGPU_SHOCKHASH2 ? cfg.slots * slot_waves() * SH2_SCRATCH_BYTES : 0;
+        return cfg.variant == SHGPU_SHOCKHASH2 ? cfg.slots * slot_waves() * sh2_layout(cfg.leaf_size).wave_bytes : 0;
     }
     size_t slot_waves() const {
         const int grid = std::max(1, grid_for(e, e->tuning_v[cfg.variant], cfg.variant, ~0ull) / (int)cfg.slots);
-        return (size_t)grid * WAVES_PER_BLOCK;
+        return (size_t)(cfg.variant == SHGPU_SHOCKHASH2 ? sh2_grid_cap(sh2_layout(cfg.leaf_size), grid) : grid) * WAVES_PER_BLOCK;
     }
     size_t allocated() const {
         size_t t = 0;

@@ -212,19 +212,22 @@ __device__ __forceinline__ void unpair_elegant(uint64_t z, uint64_t &x, uint64_t
 }
 
 // Per-thread tables of the orientation replay in LDS, one byte column per thread: byte x of thread t at
-// x * ORIENT_BLOCK + t. Cells and key indices of a leaf are below 64.
+// x * BS + t for a block of BS threads. Cells and key indices of a leaf are below MAXM: 64 in blocks of 256
+// threads, 128 (ShockHash2 leaves above 64 keys) in blocks of 128 threads, 48 KB of LDS either way.
 constexpr uint32_t ORIENT_BLOCK = 256;
 constexpr uint8_t CELL_EMPTY = 0xFF;
 
+template <uint32_t BS>
 struct OrientCol {
     uint8_t *p;
-    __device__ uint8_t &operator[](uint32_t x) const { return p[x * ORIENT_BLOCK]; }
+    __device__ uint8_t &operator[](uint32_t x) const { return p[x * BS]; }
 };
 
 // Cuckoo insertion of key k with cells c1, c2 into cell[] (CELL_EMPTY = free), entries identified by key
 // index; the steps of TinyBinaryCuckooHashTable::insert (TinyBinaryCuckooHashTable.h:169-198). False when
 // the key cannot be placed.
-__device__ bool orient_insert(OrientCol cell, OrientCol xr, uint32_t k, uint32_t c1, uint32_t c2) {
+template <uint32_t BS>
+__device__ bool orient_insert(OrientCol<BS> cell, OrientCol<BS> xr, uint32_t k, uint32_t c1, uint32_t c2) {
     uint32_t entry = k;
     xr[k] = (uint8_t)(c1 ^ c2);
     if (cell[c1] == CELL_EMPTY) { cell[c1] = (uint8_t)entry; return true; }
@@ -249,18 +252,21 @@ __device__ bool orient_insert(OrientCol cell, OrientCol xr, uint32_t k, uint32_t
 // key order with bit = (the key sits in its second cell) (ShockHash2-internal.h:548-572). The cell,
 // eviction-XOR and second-cell tables (64 bytes each) live in LDS: the replay is a chain of dependent
 // table reads.
-template <bool BIPARTITE>
-__global__ void __launch_bounds__(ORIENT_BLOCK) k_leaf_orientation(const uint64_t *__restrict__ keys,
-                                                                   const shgpu_orient_task *__restrict__ tasks, uint64_t n,
-                                                                   uint64_t *out_keys, uint8_t *out_bits, uint32_t *error) {
-    __shared__ uint8_t lds[3 * 64 * ORIENT_BLOCK];
-    const OrientCol cell{lds + threadIdx.x}, xr{lds + 64 * ORIENT_BLOCK + threadIdx.x},
-        c2s{lds + 2 * 64 * ORIENT_BLOCK + threadIdx.x};
+// MAXM 64: leaves above 64 keys are left to the MAXM 128 launch (error bit 4 says there are some); MAXM 128: only
+// the leaves above 64 keys.
+template <bool BIPARTITE, uint32_t MAXM, uint32_t BS>
+__global__ void __launch_bounds__(BS) k_leaf_orientation(const uint64_t *__restrict__ keys,
+                                                         const shgpu_orient_task *__restrict__ tasks, uint64_t n,
+                                                         uint64_t *out_keys, uint8_t *out_bits, uint32_t *error) {
+    __shared__ uint8_t lds[3 * MAXM * BS];
+    const OrientCol<BS> cell{lds + threadIdx.x}, xr{lds + MAXM * BS + threadIdx.x}, c2s{lds + 2 * MAXM * BS + threadIdx.x};
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
         const shgpu_orient_task tk = tasks[t];
         const uint64_t *lk = keys + tk.key_offset;
         const uint32_t m = tk.m;
-        if (m < 1 || m > 64) { atomicOr(error, 1u); continue; }
+        if (m < 1 || m > 128 || (!BIPARTITE && m > 64)) { atomicOr(error, 1u); continue; }
+        if (MAXM == 64 && m > 64) { atomicOr(error, 4u); continue; }
+        if (MAXM == 128 && m <= 64) continue;
         for (uint32_t c = 0; c < m; c++) cell[c] = CELL_EMPTY;
         bool ok = true;
         if (!BIPARTITE) {
@@ -410,14 +416,21 @@ int shgpu_leaf_orientation_impl(hipStream_t s, int bipartite, const uint64_t *d_
     uint32_t *d_err = (uint32_t *)flag.p;
     k_zero_u32<<<1, 1, 0, s>>>(d_err);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + ORIENT_BLOCK - 1) / ORIENT_BLOCK, 4096);
-    if (bipartite) k_leaf_orientation<true><<<grid, ORIENT_BLOCK, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
-    else k_leaf_orientation<false><<<grid, ORIENT_BLOCK, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
+    if (bipartite) k_leaf_orientation<true, 64, ORIENT_BLOCK><<<grid, ORIENT_BLOCK, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
+    else k_leaf_orientation<false, 64, ORIENT_BLOCK><<<grid, ORIENT_BLOCK, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
     uint32_t h = 0;
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&h, d_err, 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && bipartite && (h & 4)) {  // ShockHash2 leaves above 64 keys
+        const uint32_t grid2 = (uint32_t)std::min<uint64_t>((n + 127) / 128, 4096);
+        k_leaf_orientation<true, 128, 128><<<grid2, 128, 0, s>>>(d_keys, d_tasks, n, d_out_keys, d_out_bits, d_err);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&h, d_err, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
     if (e != hipSuccess) { err = std::string("orientation: ") + hipGetErrorString(e); return SHGPU_ERR_HIP; }
-    if (h & 1) { err = "orientation: a leaf has no keys or more than 64"; return SHGPU_ERR_ARG; }
+    if (h & 1) { err = "orientation: a leaf has no keys or more than 64 (128 for ShockHash2)"; return SHGPU_ERR_ARG; }
     if (h & 2) { err = "orientation: a leaf's keys cannot all be placed with its seed (wrong code?)"; return SHGPU_ERR_ARG; }
     return SHGPU_OK;
 }

@@ -27,44 +27,90 @@
 //  * the general path (sh2_leaf_search_general), whose tables live in the wave's global scratch area
 //    and are bounded only by SH2_MAX_HALF_SEEDS / SH2_MAX_CANDIDATES. It rebuilds the half-seed and
 //    candidate tables and tests pairs from (resume j, 0) on (the pairs before it already failed).
-// Both write each candidate's cell indices (64 bytes) to scratch and queue the pairs that pass the
+// Leaves of 65..128 keys (ShockHash2<L> with L up to MAX_LEAF_SIZE2 = 128, include/ShockHash2.h:29) run the
+// general path with two keys per lane (KPL = 2): 64-bit half-seed masks (H <= 64 cells per half), two-word
+// isolated-key masks, 128-byte cell rows and a union-find over up to 128 vertices. Their tables grow with the
+// leaf size (about (2^k(m) * 2)^(1/4) half-seeds, k(m) = bij_memo2[m]: ~110 at m = 64, ~14,000 at m = 128), so
+// their capacities are run-time values the host derives from the largest leaf (Sh2Caps).
+// Both write each candidate's cell indices (64 bytes per 64 keys) to scratch and queue the pairs that pass the
 // isolated filter in LDS in (j, i) order; 64 queued pairs are tested at a time, one pair per lane,
 // with a per-lane union-find (at most one cycle per component, UnionFind.h:58-67 semantics; the smaller
 // root becomes the parent, a root's byte carries its cycle flag) whose bytes are interleaved in LDS. Any exact pseudoforest predicate gives
 // the reference's result: CuckooUnionFind, UnionFind and the textbook component count agree.
 #pragma once
+#include <type_traits>
+
 #include "device_common.h"
 
 namespace shgpu {
 
-constexpr uint32_t SH2_MAX_HALF_SEEDS = 1024;  // half-seed bound (reference expects ~2^7 at m = 64)
+constexpr uint32_t SH2_MAX_HALF_SEEDS = 1024;  // half-seed bound of leaves up to 64 keys (~2^7 needed at m = 64)
 constexpr uint32_t SH2_MAX_CANDIDATES = 4096;
-constexpr size_t SH2_SCRATCH_BYTES =
-    SH2_MAX_HALF_SEEDS * 4 * 2 + SH2_MAX_CANDIDATES * (4 + 8 + 64);
 constexpr uint32_t SH2_HS_BLOCKS = 4;    // register path: half-seeds [0, 256)
 constexpr uint32_t SH2_CAND_BLOCKS = 2;  // register path: candidates [0, 128)
 constexpr uint64_t SH2_OVERFLOW = SHGPU_NOT_FOUND - 1;
+constexpr uint32_t SH2_MAX_LEAF = 128;   // MAX_LEAF_SIZE2 (include/ShockHash2.h:29)
 
-struct Sh2Lds {
-    uint32_t par[16 * 64];  // union-find bytes: vertex x of lane l at byte x * 64 + l
-    uint32_t queue[128];    // pending pairs j << 16 | i
+// Capacities of the general path's tables: half-seeds hashed and candidates kept per leaf.
+struct Sh2Caps {
+    uint32_t max_hs, max_cand;
+};
+constexpr Sh2Caps SH2_NARROW_CAPS{SH2_MAX_HALF_SEEDS, SH2_MAX_CANDIDATES};
+
+__host__ __device__ constexpr size_t sh2_align256(size_t b) { return (b + 255) / 256 * 256; }
+// Scratch bytes of one wave: half-seed masks A and B, candidate codes, isolated-key masks, cell rows.
+__host__ __device__ constexpr size_t sh2_scratch_bytes(int kpl, Sh2Caps c) {
+    return sh2_align256((size_t)c.max_hs * 2 * (kpl == 1 ? 4 : 8)) + sh2_align256((size_t)c.max_cand * 4) +
+           sh2_align256((size_t)c.max_cand * 8 * kpl) + (size_t)c.max_cand * 64 * kpl;
+}
+constexpr size_t SH2_SCRATCH_BYTES = sh2_scratch_bytes(1, SH2_NARROW_CAPS);
+
+template <int KPL> struct Sh2Types;
+template <> struct Sh2Types<1> {
+    using Mask = uint32_t;   // half-seed mask, H <= 32 cells
+    using Queued = uint32_t; // pending pair j << 16 | i (candidates < 2^16)
+};
+template <> struct Sh2Types<2> {
+    using Mask = uint64_t;   // H <= 64 cells
+    using Queued = uint64_t; // j << 32 | i
+};
+
+template <int KPL>
+struct Sh2LdsT {
+    uint32_t par[16 * KPL * 64];  // union-find bytes: vertex x of lane l at byte x * 64 + l
+    typename Sh2Types<KPL>::Queued queue[128];  // pending pairs (j, i)
     uint32_t hist[64];
-    __align__(16) uint64_t keysA[64];  // register path: even keys, then
+    __align__(16) uint64_t keysA[64];  // register path (KPL 1): even keys, then
     __align__(16) uint64_t keysB[64];  // odd keys, in leaf order
 };
+using Sh2Lds = Sh2LdsT<1>;
 
-struct Sh2Scratch {
-    uint32_t *hsA, *hsB, *code;
-    uint64_t *iso;
-    uint8_t *hashes;
-    __device__ explicit Sh2Scratch(uint8_t *base) {
-        hsA = (uint32_t *)base;
-        hsB = hsA + SH2_MAX_HALF_SEEDS;
-        code = hsB + SH2_MAX_HALF_SEEDS;
-        iso = (uint64_t *)(code + SH2_MAX_CANDIDATES);
-        hashes = (uint8_t *)(iso + SH2_MAX_CANDIDATES);
+template <int KPL>
+struct Sh2ScratchT {
+    using Mask = typename Sh2Types<KPL>::Mask;
+    Mask *hsA, *hsB;
+    uint32_t *code;
+    uint64_t *iso;    // KPL words per candidate
+    uint8_t *hashes;  // 64 * KPL bytes per candidate
+    Sh2Caps caps;
+    __device__ Sh2ScratchT(uint8_t *base, Sh2Caps c) : caps(c) {
+        hsA = (Mask *)base;
+        hsB = hsA + c.max_hs;
+        code = (uint32_t *)(base + sh2_align256((size_t)c.max_hs * 2 * sizeof(Mask)));
+        iso = (uint64_t *)((uint8_t *)code + sh2_align256((size_t)c.max_cand * 4));
+        hashes = (uint8_t *)iso + sh2_align256((size_t)c.max_cand * 8 * KPL);
     }
 };
+using Sh2Scratch = Sh2ScratchT<1>;
+
+template <int KPL>
+__device__ __forceinline__ typename Sh2Types<KPL>::Queued sh2_pack(uint32_t j, uint32_t i) {
+    return KPL == 1 ? (typename Sh2Types<KPL>::Queued)((j << 16) | i) : (typename Sh2Types<KPL>::Queued)(((uint64_t)j << 32) | i);
+}
+template <int KPL>
+__device__ __forceinline__ uint32_t sh2_pj(typename Sh2Types<KPL>::Queued q) { return KPL == 1 ? (uint32_t)(q >> 16) : (uint32_t)((uint64_t)q >> 32); }
+template <int KPL>
+__device__ __forceinline__ uint32_t sh2_pi(typename Sh2Types<KPL>::Queued q) { return KPL == 1 ? (uint32_t)(q & 0xFFFF) : (uint32_t)q; }
 
 __device__ __forceinline__ void mem_sync() {
     __threadfence_block();
@@ -81,15 +127,16 @@ constexpr uint32_t UF_ROOT = 0x80, UF_CYCLIC = 0x40, UF_RANK = 0x3F;
 // component with a cycle, or between two components with cycles, kills the lane (UnionFind.h:58-67:
 // a pseudoforest component holds at most one cycle); otherwise the root of higher rank becomes the
 // parent. Both ends of an edge walk to their roots in one loop, so a key costs as many dependent LDS
-// reads as the deeper of its two walks.
-__device__ __forceinline__ uint64_t sh2_uf_batch(const uint8_t *hashes, Sh2Lds &L, uint32_t m) {
+// reads as the deeper of its two walks. Vertices are below m/2 + ceil(m/2) = m <= 64 * KPL.
+template <int KPL>
+__device__ __forceinline__ uint64_t sh2_uf_batch(const uint8_t *hashes, Sh2LdsT<KPL> &L, uint32_t m) {
     const uint32_t lane = lane_id();
-    const uint32_t pr = L.queue[lane];
-    const uint32_t j = pr >> 16, i = pr & 0xFFFF;
-    const uint32_t *pj = (const uint32_t *)(hashes + (size_t)j * 64);
-    const uint32_t *pi = (const uint32_t *)(hashes + (size_t)i * 64);
+    const auto pr = L.queue[lane];
+    const uint32_t j = sh2_pj<KPL>(pr), i = sh2_pi<KPL>(pr);
+    const uint32_t *pj = (const uint32_t *)(hashes + (size_t)j * 64 * KPL);
+    const uint32_t *pi = (const uint32_t *)(hashes + (size_t)i * 64 * KPL);
 #pragma unroll
-    for (uint32_t q = 0; q < 16; q++) L.par[q * 64 + lane] = UF_ROOT * 0x01010101u;
+    for (uint32_t q = 0; q < 16 * KPL; q++) L.par[q * 64 + lane] = UF_ROOT * 0x01010101u;
     wave_sync();
     uint8_t *lp = (uint8_t *)L.par + lane;
     const uint32_t shift = m / 2;
@@ -127,57 +174,75 @@ __device__ __forceinline__ uint64_t sh2_uf_batch(const uint8_t *hashes, Sh2Lds &
     return res;
 }
 
-// Cell of this lane's key (lane < m) in candidate (A, B) and the candidate's isolated-key mask; the cells
-// go to row c of the scratch cell table.
-__device__ __forceinline__ uint64_t sh2_candidate_cells(uint8_t *hashes, Sh2Lds &L, uint32_t c, uint64_t kv, uint32_t m,
-                                                        uint32_t H, uint32_t A, uint32_t B) {
+// Cells of this lane's keys (lane + 64 q < m) in candidate (A, B) and the candidate's isolated-key mask
+// (iso[q]: keys 64 q .. 64 q + 63); the cells go to row c of the scratch cell table.
+template <int KPL>
+__device__ __forceinline__ void sh2_candidate_cells(uint8_t *hashes, Sh2LdsT<KPL> &L, uint32_t c, const uint64_t (&kv)[KPL],
+                                                    uint32_t m, uint32_t H, uint32_t A, uint32_t B, uint64_t (&iso)[KPL]) {
     const uint32_t lane = lane_id();
-    const bool act = lane < m;
-    uint32_t h = 0;
-    if (act) h = fastrange64(remix(kv + (uint64_t)(((kv & 1) == 0) ? A : B)), H);
+    uint32_t h[KPL];
+#pragma unroll
+    for (int q = 0; q < KPL; q++)
+        h[q] = lane + 64 * q < m ? fastrange64(remix(kv[q] + (uint64_t)(((kv[q] & 1) == 0) ? A : B)), H) : 0;
     L.hist[lane] = 0;
     wave_sync();
-    if (act) atomicAdd(&L.hist[h], 1u);
+#pragma unroll
+    for (int q = 0; q < KPL; q++)
+        if (lane + 64 * q < m) atomicAdd(&L.hist[h[q]], 1u);
     wave_sync();
-    const uint64_t iso = ballot(act && L.hist[h] == 1);
+#pragma unroll
+    for (int q = 0; q < KPL; q++) iso[q] = ballot(lane + 64 * q < m && L.hist[h[q]] == 1);
     wave_sync();
-    hashes[(size_t)c * 64 + lane] = act ? (uint8_t)h : 0;
-    return iso;
+#pragma unroll
+    for (int q = 0; q < KPL; q++) hashes[(size_t)c * 64 * KPL + 64 * q + lane] = lane + 64 * q < m ? (uint8_t)h[q] : 0;
 }
 
 // ------------------------------------------------------------------ general path (scratch tables)
 
 // Appends candidate (A, B) with the given code: cells of every key and its isolated-key mask.
-__device__ __forceinline__ void sh2_add_candidate(const Sh2Scratch &g, Sh2Lds &L, uint32_t c, uint64_t kv, uint32_t m,
-                                                  uint32_t H, uint32_t A, uint32_t B, uint32_t code) {
-    const uint64_t iso = sh2_candidate_cells(g.hashes, L, c, kv, m, H, A, B);
+template <int KPL>
+__device__ __forceinline__ void sh2_add_candidate(const Sh2ScratchT<KPL> &g, Sh2LdsT<KPL> &L, uint32_t c,
+                                                  const uint64_t (&kv)[KPL], uint32_t m, uint32_t H, uint32_t A, uint32_t B,
+                                                  uint32_t code) {
+    uint64_t iso[KPL];
+    sh2_candidate_cells<KPL>(g.hashes, L, c, kv, m, H, A, B, iso);
     g.code[c] = code;  // every lane stores the same wave-uniform values
-    g.iso[c] = iso;
+#pragma unroll
+    for (int q = 0; q < KPL; q++) g.iso[(size_t)c * KPL + q] = iso[q];
 }
 
-// Returns the ShockHash2 leaf code, or SHGPU_NOT_FOUND when a scratch bound is exceeded.
+// Returns the ShockHash2 leaf code, or SHGPU_NOT_FOUND when a table capacity is exceeded.
 // Pairs (j, i) with j < j_start are not tested: the register path has tested them without success.
+template <int KPL>
 __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys, uint32_t m, uint8_t *scratch,
-                                                            Sh2Lds &L, uint64_t t_start, uint32_t j_start) {
+                                                            Sh2LdsT<KPL> &L, Sh2Caps caps, uint64_t t_start,
+                                                            uint32_t j_start) {
+    using Mask = typename Sh2Types<KPL>::Mask;
     const uint32_t lane = lane_id();
     if (m < 2) return 0;
-    const Sh2Scratch g(scratch);
+    const Sh2ScratchT<KPL> g(scratch, caps);
     const uint32_t H = (m + 1) / 2;
-    const uint32_t FULL = H >= 32 ? 0xFFFFFFFFu : ((1u << H) - 1u);
+    const Mask FULL = H >= 8 * sizeof(Mask) ? ~(Mask)0 : (((Mask)1 << H) - 1);
     const bool basic = m < 10;
-    const uint64_t kv = lane < m ? keys[lane] : 0;
+    uint64_t kv[KPL];
+#pragma unroll
+    for (int q = 0; q < KPL; q++) kv[q] = lane + 64 * q < m ? keys[lane + 64 * q] : 0;
     uint32_t n_hs = 0, s_next = 0, n_cand = 0, jq = j_start > 1 ? j_start : 1, iq = 0, qlen = 0;
     for (;;) {
         // Make candidate jq available.
         while (n_cand <= jq) {
             if (s_next == n_hs) {
-                if (n_hs + 64 > SH2_MAX_HALF_SEEDS) return SHGPU_NOT_FOUND;
+                if (n_hs + 64 > caps.max_hs) return SHGPU_NOT_FOUND;
                 const uint64_t s = n_hs + lane;
-                uint32_t ma = 0, mb = 0;
-                for (uint32_t k = 0; k < m; k++) {
-                    const uint64_t key = readlane64(kv, k);
-                    const uint32_t bit = 1u << fastrange64(remix(key + s), H);
-                    if ((key & 1) == 0) ma |= bit; else mb |= bit;
+                Mask ma = 0, mb = 0;
+#pragma unroll
+                for (int q = 0; q < KPL; q++) {
+                    const uint32_t kn = m - 64 * q < 64 ? m - 64 * q : 64;
+                    for (uint32_t k = 0; k < kn && 64 * q < m; k++) {
+                        const uint64_t key = readlane64(kv[q], k);
+                        const Mask bit = (Mask)1 << fastrange64(remix(key + s), H);
+                        if ((key & 1) == 0) ma |= bit; else mb |= bit;
+                    }
                 }
                 g.hsA[n_hs + lane] = ma;
                 g.hsB[n_hs + lane] = mb;
@@ -185,11 +250,11 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys
                 mem_sync();
             }
             const uint32_t s = s_next++;
-            const uint32_t ms_a = g.hsA[s], ms_b = g.hsB[s];
+            const Mask ms_a = g.hsA[s], ms_b = g.hsB[s];
             if (basic) {
                 if ((ms_a | ms_b) == FULL) {
-                    if (n_cand >= SH2_MAX_CANDIDATES) return SHGPU_NOT_FOUND;
-                    sh2_add_candidate(g, L, n_cand++, kv, m, H, s, s, s);
+                    if (n_cand >= caps.max_cand) return SHGPU_NOT_FOUND;
+                    sh2_add_candidate<KPL>(g, L, n_cand++, kv, m, H, s, s, s);
                     mem_sync();
                 }
                 continue;
@@ -201,8 +266,8 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys
                 while (hit) {
                     const uint32_t l = (uint32_t)__builtin_ctzll(hit);
                     hit &= hit - 1;
-                    if (n_cand >= SH2_MAX_CANDIDATES) return SHGPU_NOT_FOUND;
-                    sh2_add_candidate(g, L, n_cand++, kv, m, H, c0 + l, s, s * s + c0 + l);
+                    if (n_cand >= caps.max_cand) return SHGPU_NOT_FOUND;
+                    sh2_add_candidate<KPL>(g, L, n_cand++, kv, m, H, c0 + l, s, s * s + c0 + l);
                 }
             }
             // (s, b), b <= s: code s*s + s + b
@@ -212,8 +277,8 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys
                 while (hit) {
                     const uint32_t l = (uint32_t)__builtin_ctzll(hit);
                     hit &= hit - 1;
-                    if (n_cand >= SH2_MAX_CANDIDATES) return SHGPU_NOT_FOUND;
-                    sh2_add_candidate(g, L, n_cand++, kv, m, H, s, c0 + l, s * s + s + c0 + l);
+                    if (n_cand >= caps.max_cand) return SHGPU_NOT_FOUND;
+                    sh2_add_candidate<KPL>(g, L, n_cand++, kv, m, H, s, c0 + l, s * s + s + c0 + l);
                 }
             }
             mem_sync();
@@ -221,26 +286,28 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_general(const uint64_t *keys
         // Queue pairs (jq, i), i in [iq, iq + 64) passing the isolated-vertex filter.
         age_priority(t_start);
         {
-            const uint64_t iso_j = g.iso[jq];
             const uint32_t i = iq + lane;
-            const bool pass = i < jq && (g.iso[i] & iso_j) == 0;
+            bool pass = i < jq;
+#pragma unroll
+            for (int q = 0; q < KPL; q++)
+                pass = pass && (g.iso[(size_t)i * KPL + q] & g.iso[(size_t)jq * KPL + q]) == 0;
             const uint64_t b = ballot(pass);
-            if (pass) L.queue[qlen + lanes_below(b)] = (jq << 16) | i;
+            if (pass) L.queue[qlen + lanes_below(b)] = sh2_pack<KPL>(jq, i);
             qlen += (uint32_t)__builtin_popcountll(b);
             iq += 64;
             if (iq >= jq) { jq++; iq = 0; }
             wave_sync();
         }
         if (qlen >= 64) {
-            const uint64_t ok = sh2_uf_batch(g.hashes, L, m);
+            const uint64_t ok = sh2_uf_batch<KPL>(g.hashes, L, m);
             if (ok) {
                 const uint32_t l = (uint32_t)__builtin_ctzll(ok);
-                const uint32_t pr = L.queue[l];
-                const uint64_t cj = g.code[pr >> 16], ci = g.code[pr & 0xFFFF];
+                const auto pr = L.queue[l];
+                const uint64_t cj = g.code[sh2_pj<KPL>(pr)], ci = g.code[sh2_pi<KPL>(pr)];
                 return cj * (cj - 1) / 2 + ci;  // pairTriangular (PairingFunction.h:47-50)
             }
             const uint32_t rest = qlen - 64;
-            const uint32_t moved = lane < rest ? L.queue[64 + lane] : 0;
+            const auto moved = lane < rest ? L.queue[64 + lane] : L.queue[0];
             wave_sync();
             if (lane < rest) L.queue[lane] = moved;
             qlen = rest;
@@ -298,7 +365,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
                                                          Sh2Lds &L, uint64_t t_start, uint32_t &resume_j) {
     const uint32_t lane = lane_id();
     if (m < 2) return 0;
-    const Sh2Scratch g(scratch);
+    const Sh2Scratch g(scratch, SH2_NARROW_CAPS);
     const uint32_t H = (m + 1) / 2;
     const uint32_t FULL = H >= 32 ? 0xFFFFFFFFu : ((1u << H) - 1u);
     const bool basic = m < 10;
@@ -319,7 +386,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
     // Overflow happens while candidate jq is made available: the pairs of every j < jq are queued, and the
     // queued pairs not yet tested start at the first queue entry's j.
     auto overflow = [&]() -> uint64_t {
-        resume_j = qlen ? L.queue[0] >> 16 : jq;
+        resume_j = qlen ? sh2_pj<1>(L.queue[0]) : jq;
         return SH2_OVERFLOW;
     };
 
@@ -327,8 +394,10 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
     // the scratch table (read once, for the winner).
     auto add = [&](uint32_t A, uint32_t B, uint32_t code) -> bool {
         if (n_cand >= SH2_CAND_BLOCKS * 64) return false;
-        const uint64_t iso = sh2_candidate_cells(g.hashes, L, n_cand, kv, m, H, A, B);
-        isoR.set_if(lane == (n_cand & 63), n_cand >> 6, iso);
+        uint64_t iso[1];
+        const uint64_t kvs[1] = {kv};
+        sh2_candidate_cells<1>(g.hashes, L, n_cand, kvs, m, H, A, B, iso);
+        isoR.set_if(lane == (n_cand & 63), n_cand >> 6, iso[0]);
         g.code[n_cand] = code;  // same value in every lane
         n_cand++;
         return true;
@@ -395,14 +464,14 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
             const uint32_t i = 64 * b + lane;
             const bool pass = i < jq && (isoR.at(b) & iso_j) == 0;
             const uint64_t bits = ballot(pass);
-            if (pass) L.queue[qlen + lanes_below(bits)] = (jq << 16) | i;
+            if (pass) L.queue[qlen + lanes_below(bits)] = sh2_pack<1>(jq, i);
             qlen += (uint32_t)__builtin_popcountll(bits);
             wave_sync();
             if (qlen >= 64) {
-                const uint64_t ok = sh2_uf_batch(g.hashes, L, m);
+                const uint64_t ok = sh2_uf_batch<1>(g.hashes, L, m);
                 if (ok) {
                     const uint32_t pr = L.queue[__builtin_ctzll(ok)];
-                    const uint64_t cj = g.code[pr >> 16], ci = g.code[pr & 0xFFFF];
+                    const uint64_t cj = g.code[sh2_pj<1>(pr)], ci = g.code[sh2_pi<1>(pr)];
                     return cj * (cj - 1) / 2 + ci;  // pairTriangular (PairingFunction.h:47-50)
                 }
                 const uint32_t rest = qlen - 64;
@@ -420,16 +489,42 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
 // Tuning flag: search every ShockHash2 leaf with the general path only (tests compare both paths).
 constexpr uint32_t SH2_FLAG_GENERAL_ONLY = 1;
 
-// The ShockHash2 leaf code, or SHGPU_NOT_FOUND when the general path's bounds are exceeded. The wave's issue
-// priority rises with the time since t_start (task_clock) and is reset on return.
+// The ShockHash2 leaf code of m <= 64 keys, or SHGPU_NOT_FOUND when the general path's bounds are exceeded. The
+// wave's issue priority rises with the time since t_start (task_clock) and is reset on return.
 __device__ __forceinline__ uint64_t sh2_leaf_search(const uint64_t *keys, uint32_t m, uint8_t *scratch, Sh2Lds &L,
                                                     uint32_t flags, uint64_t t_start) {
     uint64_t r = SH2_OVERFLOW;
     uint32_t resume_j = 1;
     if (!(flags & SH2_FLAG_GENERAL_ONLY)) r = sh2_leaf_search_regs(keys, m, scratch, L, t_start, resume_j);
-    if (r == SH2_OVERFLOW) r = sh2_leaf_search_general(keys, m, scratch, L, t_start, uniform(resume_j));
+    if (r == SH2_OVERFLOW) r = sh2_leaf_search_general<1>(keys, m, scratch, L, SH2_NARROW_CAPS, t_start, uniform(resume_j));
     reset_priority();
     return r;
+}
+
+// The same for 64 < m <= 128 keys (two keys per lane, tables of the given capacities).
+__device__ __forceinline__ uint64_t sh2_leaf_search_wide(const uint64_t *keys, uint32_t m, uint8_t *scratch, Sh2LdsT<2> &L,
+                                                         Sh2Caps caps, uint64_t t_start) {
+    const uint64_t r = sh2_leaf_search_general<2>(keys, m, scratch, L, caps, t_start, 1);
+    reset_priority();
+    return r;
+}
+
+// LDS of one wave that searches leaves of up to 64 keys, or up to 128 (WIDE) with either path.
+template <bool WIDE>
+union Sh2LdsAny {
+    Sh2Lds narrow;
+    typename std::conditional<WIDE, Sh2LdsT<2>, char>::type wide;
+};
+
+// Either search by the leaf's size; `caps` and the scratch size are those of the widest leaf (narrow leaves use the
+// start of the wave's area).
+template <bool WIDE>
+__device__ __forceinline__ uint64_t sh2_leaf_search_any(const uint64_t *keys, uint32_t m, uint8_t *scratch,
+                                                        Sh2LdsAny<WIDE> &L, Sh2Caps caps, uint32_t flags, uint64_t t_start) {
+    if constexpr (WIDE) {
+        if (m > 64) return sh2_leaf_search_wide(keys, m, scratch, L.wide, caps, t_start);
+    }
+    return sh2_leaf_search(keys, m, scratch, L.narrow, flags, t_start);
 }
 
 }  // namespace shgpu

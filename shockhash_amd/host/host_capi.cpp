@@ -55,7 +55,7 @@ Handle *make_s(const std::vector<std::string> &s, size_t bucket, size_t threads)
 
 #define SHH_LEAVES_SH1(X) X(2) X(4) X(8) X(16) X(24) X(30)
 #define SHH_LEAVES_SIMD(X) X(8) X(16) X(24) X(58)
-#define SHH_LEAVES_SH2(X) X(4) X(8) X(16) X(24) X(32) X(64)
+#define SHH_LEAVES_SH2(X) X(4) X(8) X(16) X(24) X(32) X(64) X(100) X(128)
 
 template <class Keys, class MakeSH1, class MakeSIMD, class MakeSH2>
 Handle *dispatch(int variant, int leaf, Keys &keys, size_t bucket, size_t threads) {

@@ -21,7 +21,8 @@ def test_readme_known_answer():
 
 
 @pytest.mark.parametrize("variant,leaf,n", [("sh2", 64, 40000), ("sh1", 30, 30000), ("simd", 24, 30000),
-                                            ("sh2", 16, 25000), ("sh1", 8, 9000), ("simd", 8, 9000)])
+                                            ("sh2", 16, 25000), ("sh1", 8, 9000), ("simd", 8, 9000),
+                                            ("sh2", 100, 30000), ("sh2", 128, 30000)])
 def test_class_matches_oracle(variant, leaf, n):
     log(f"class {variant} L={leaf} n={n}")
     h = workload.fingerprints(n, seed=31)

@@ -53,12 +53,14 @@ def test_sh1_leaves_all_sizes(engine):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("flags", [0, 1])
+@pytest.mark.parametrize("flags", [0, 1, 8])
 def test_sh2_leaves_all_sizes(engine, flags):
-    """ShockHash2 leaves of every size 2..64 (Basic finder below 10, QuadSplit from 10), on the register
-    path (flags 0) and on the scratch-memory path alone (flags 1, SHGPU_TUNE_SH2_GENERAL_ONLY)."""
+    """ShockHash2 leaves of every size 2..128 (Basic finder below 10, QuadSplit from 10; up to MAX_LEAF_SIZE2, two
+    keys per lane above 64), on the register path (flags 0), on the scratch-memory path alone (flags 1,
+    SHGPU_TUNE_SH2_GENERAL_ONLY), and with tables too small for most leaves above 64 keys, which the second pass
+    then finishes (flags 8, SHGPU_TUNE_SH2_TIGHT_TABLES)."""
     log(f"sh2 all sizes, flags {flags}")
-    sizes = np.array(list(range(2, 65)) * 2 + [64] * 32, dtype=np.uint32)
+    sizes = np.array(list(range(2, 129)) * 2 + [64] * 32 + [128] * 16, dtype=np.uint32)
     offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint32)
     keys = workload.fingerprints(int(sizes.sum()), seed=13)[:, 1]
     want = O.sh2_leaf_batch(keys, offsets, sizes, threads=8)
@@ -178,7 +180,8 @@ def test_split_search(engine, variant, leaf):
 
 
 @pytest.mark.parametrize("variant,leaf,n", [("sh2", 64, 30000), ("sh1", 30, 20000), ("simd", 16, 20000),
-                                            ("sh2", 16, 20000), ("sh1", 8, 5000)])
+                                            ("sh2", 16, 20000), ("sh1", 8, 5000), ("sh2", 100, 30000),
+                                            ("sh2", 128, 20000)])
 @pytest.mark.parametrize("mode", ["persistent", "plain"])
 def test_build_matches_oracle(engine, variant, leaf, n, mode):
     """Whole construction: bucket sizes, every node code (DFS order) and the final key order."""
@@ -264,3 +267,20 @@ def test_cooperative_windows_repeated(engine, variant, leaf, bucket, n, reps, fl
     d.free()
     engine.set_tuning(flags=0)
     assert not bad, "; ".join(bad[:5])
+
+
+@pytest.mark.parametrize("leaf", [100, 128])
+def test_wide_leaf_build_second_pass(engine, leaf):
+    """ShockHash2 leaf 100 / 128 builds with tables sized for a quarter of the expected half-seeds
+    (SHGPU_TUNE_SH2_TIGHT_TABLES): most leaves outgrow them in the persistent kernel and the host's second pass with
+    larger tables finishes them; the codes equal the oracle's."""
+    log(f"sh2 L={leaf} build, tight tables")
+    h = workload.fingerprints(20000, seed=27)
+    ob = O.OracleBuild("sh2", leaf, h, 2000, threads=8)
+    engine.set_variant_tuning("sh2", flags=8)
+    try:
+        b = engine.build(h, "sh2", leaf, 2000)
+    finally:
+        engine.set_variant_tuning("sh2", flags=0)
+    assert np.array_equal(b.codes, ob.node_code)
+
