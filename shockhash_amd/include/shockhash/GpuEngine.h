@@ -53,6 +53,7 @@ class Engine {
         size_t bytes = 0;
     };
     Buf tasks_buf, out_keys_buf, out_bits_buf;
+    Buf leaf_keys_buf, leaf_task_buf, leaf_code_buf;  // one ShockHash2 leaf at a time (sh2LeafCode)
     int ensure(Buf &b, size_t bytes) {
         if (b.bytes >= bytes) return SHGPU_OK;
         if (b.p) shgpu_device_free(e, b.p);
@@ -66,7 +67,7 @@ class Engine {
   public:
     explicit Engine(int device = 0) { check(shgpu_engine_create(device, &e), "shgpu_engine_create"); }
     ~Engine() {
-        for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf})
+        for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf, &leaf_keys_buf, &leaf_task_buf, &leaf_code_buf})
             if (b->p) shgpu_device_free(e, b->p);
         shgpu_engine_destroy(e);
     }
@@ -115,6 +116,26 @@ class Engine {
         if (d_hashes) shgpu_device_free(e, d_hashes);
         if (rc != SHGPU_OK) throw Error(rc, std::string("shgpu_build_run (strings): ") + shgpu_status_string(rc) + ": " + msg);
         return collect(b, n);
+    }
+
+    // The ShockHash2 leaf code of m keys (BijectionsShockHash2::findSeed, ShockHash2-internal.h:501-546),
+    // searched on the device by shgpu_sh2_leaf_search: the compiled seam shockhash2construct runs one leaf per call.
+    uint64_t sh2LeafCode(const uint64_t *keys, uint32_t m) {
+        std::lock_guard<std::mutex> lock(mtx);
+        const shgpu_leaf_task task{0, m, 0, 0};
+        uint64_t code = SHGPU_NOT_FOUND;
+        int rc = ensure(leaf_keys_buf, (size_t)m * 8 + 8);
+        if (rc == SHGPU_OK) rc = ensure(leaf_task_buf, sizeof(task));
+        if (rc == SHGPU_OK) rc = ensure(leaf_code_buf, 8);
+        if (rc == SHGPU_OK && m) rc = shgpu_copy_to_device(e, leaf_keys_buf.p, keys, (size_t)m * 8);
+        if (rc == SHGPU_OK) rc = shgpu_copy_to_device(e, leaf_task_buf.p, &task, sizeof(task));
+        if (rc == SHGPU_OK)
+            rc = shgpu_sh2_leaf_search(e, (const uint64_t *)leaf_keys_buf.p, (const shgpu_leaf_task *)leaf_task_buf.p, 1,
+                                       (uint64_t *)leaf_code_buf.p, nullptr);
+        if (rc == SHGPU_OK) rc = shgpu_copy_to_host(e, &code, leaf_code_buf.p, 8);
+        check(rc, "shgpu_sh2_leaf_search (one leaf)");
+        if (code == SHGPU_NOT_FOUND) throw Error(SHGPU_ERR_SEARCH_LIMIT, "ShockHash2 leaf search exhausted its tables (duplicate keys?)");
+        return code;
     }
 
   private:
