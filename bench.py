@@ -11,12 +11,19 @@ device search, orientation and retrieval, host encoding; every key queried, desc
 with the oracle's), and a serving mix of 2048 Poisson-arriving requests on the continuous-batching
 server (p50/p99 latency).
 
+Two more workloads print lines of their own (not the headline): --workload strings times the ShockHash<30>
+class constructor over 1,000,000 random strings (device fingerprinting, build, orientation, retrieval, host
+encoding), --workload leaves times 200,000 isolated ShockHash leaves of 40 keys searched from seed 0 (hash
+evaluations per second). --leaf / --variant select other leaf sizes and classes for the build workload.
+
 Prints one JSON line (see the keys below). One GPU only: --gpus N > 1 exits with a message.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -36,41 +43,80 @@ def log(msg):
 VARIANT_NAMES = {"sh2": "ShockHash2", "sh1": "ShockHash", "simd": "SIMDShockHash"}
 
 
-def O_build(h, threads, variant="sh2", leaf=64):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    return O.OracleBuild(variant, leaf, h, 2000, threads=threads)
+def host_cpu():
+    """(CPU model, threads the baseline uses): the host's model name, and the threads this process may run on
+    (its CPU share on a shared host: OMP_NUM_THREADS when set, else the affinity mask)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+    return model, threads
 
 
-def cpu_baseline(n_sample, threads, variant="sh2", leaf=64):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    from shockhash_amd import workload
-    h = workload.fingerprints(n_sample, seed=7)
+def native_oracle():
+    """Compiles the CPU oracle with -march=native for this host (the prebuilt library targets x86-64-v3 so that it
+    runs anywhere) into a temporary directory; returns its path and a description, or the prebuilt library when
+    the compile fails."""
+    src = os.path.join(ROOT, "oracle", "shockhash_oracle.cpp")
+    out = os.path.join(tempfile.mkdtemp(prefix="shh_oracle_"), "liboracle_native.so")
     t0 = time.perf_counter()
-    ob = O.OracleBuild(variant, leaf, h, 2000, threads=threads)
-    dt = time.perf_counter() - t0
-    return h, ob, dt
+    r = subprocess.run(["g++", "-O3", "-march=native", "-std=c++20", "-fPIC", "-pthread", "-shared", "-o", out, src],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        log(f"native oracle build failed, using the x86-64-v3 build: {r.stderr[-300:]}")
+        return os.path.join(ROOT, "oracle", "_build", "liboracle.so"), "g++ -O3 -march=x86-64-v3"
+    log(f"oracle built with -march=native in {time.perf_counter() - t0:.1f} s")
+    return out, "g++ -O3 -march=native"
 
 
-def serving_mix(eng, args):
-    """Serving mix: args.serve_requests build requests, Poisson arrivals at args.serve_rate requests/s,
-    sizes log-uniform in [1e3, 1e6], fingerprints resident in HBM, each built as ShockHash2 leaf 64 /
-    bucket 2000 by the continuous-batching server. Latency = submission to completion."""
+def cpu_baseline(h, threads, variant, leaf, reps=3):
+    """The oracle's construction of the fingerprints h (the CPU baseline), reps times: (last build, median s,
+    all times)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    times, ob = [], None
+    for _ in range(reps):
+        ob = None
+        t0 = time.perf_counter()
+        ob = O.OracleBuild(variant, leaf, h, 2000, threads=threads)
+        times.append(time.perf_counter() - t0)
+    return ob, float(np.median(times)), times
+
+
+def serving_requests(eng, args):
+    """The serving mix's requests: sizes log-uniform in [1e3, 1e6] (seed 5), their offsets, and one device array
+    holding every request's fingerprints (one SplitMix64 stream, seed 2, generated in chunks straight into it)."""
     from shockhash_amd import workload
-    from shockhash_amd.engine import DeviceArray, Server
+    from shockhash_amd.engine import DeviceArray
     rng = np.random.default_rng(5)
     sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e6), args.serve_requests)).astype(np.int64)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, args.serve_requests))
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     total = int(offs[-1])
-    # one SplitMix64 stream (seed 2), generated in chunks straight into the device array
     d = DeviceArray(eng, 2 * total, np.uint64)
     chunk = 1 << 24
     for a in range(0, total, chunk):
         b = min(total, a + chunk)
         d.upload_at(2 * a, workload.splitmix64(2, 2 * (b - a), offset=2 * a))
-    log(f"serving mix: {args.serve_requests} requests, {total} keys resident, rate {args.serve_rate}/s, "
+    return sizes, offs, d
+
+
+def serving_mix(eng, args, mix, rate, alone_digests=None):
+    """Serving mix: args.serve_requests build requests, Poisson arrivals at `rate` requests/s,
+    sizes log-uniform in [1e3, 1e6], fingerprints resident in HBM, each built as ShockHash2 leaf 64 /
+    bucket 2000 by the continuous-batching server. Latency = submission to completion. alone_digests: digests of
+    every request built alone (filled on the first call), against which each result in the mix is compared."""
+    from shockhash_amd.engine import Server
+    sizes, offs, d = mix
+    total = int(offs[-1])
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, args.serve_requests))
+    log(f"serving mix: {args.serve_requests} requests, {total} keys resident, rate {rate}/s, "
         f"{args.serve_slots} slots, budget {args.serve_budget_gib} GiB")
     srv = Server(eng, "sh2", 64, 2000, slots=args.serve_slots, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)),
                  max_batch_keys=args.serve_batch_keys)
@@ -89,21 +135,21 @@ def serving_mix(eng, args):
     info = srv.info()
     lat = np.array([r[1] for r in results])
     queue = np.array([r[2] for r in results])
-    # identical alone and in the mix: rebuild a sample of the requests on their own
-    check = sorted(set([int(np.argmin(sizes)), int(np.argmax(sizes))] + list(rng.choice(len(sizes), 6, replace=False))))
-    same = True
-    for i in check:
-        alone = eng.build_dev(d, int(sizes[i]), "sh2", 64, 2000, copy_keys=False, offset=int(offs[i]))
-        same = same and bool(np.array_equal(alone.codes, results[i][0].codes)) and \
-            bool(np.array_equal(alone.size_acc, results[i][0].size_acc))
+    # identical alone and in the mix: every request is built on its own once (digests kept across rates)
+    if alone_digests is not None and not alone_digests:
+        for i in range(len(sizes)):
+            alone = eng.build_dev(d, int(sizes[i]), "sh2", 64, 2000, copy_keys=False, offset=int(offs[i]))
+            alone_digests.append(hash((alone.codes.tobytes(), alone.size_acc.tobytes())))
+    check = len(sizes) if alone_digests else 0
+    same = all(hash((r[0].codes.tobytes(), r[0].size_acc.tobytes())) == a for r, a in zip(results, alone_digests or []))
     srv.close()
-    d.free()
-    out = {"requests": args.serve_requests, "rate_per_s": args.serve_rate, "keys": total, "slots": args.serve_slots,
+    out = {"requests": args.serve_requests, "rate_per_s": rate, "keys": total, "slots": args.serve_slots,
            "batch_keys": args.serve_batch_keys,
            "hbm_budget_bytes": int(args.serve_budget_gib * (1 << 30)), "peak_reserved_bytes": info["peak_reserved_bytes"],
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "max_ms": float(lat.max()), "queue_p99_ms": float(np.percentile(queue, 99)),
-           "keys_per_s": total / wall, "wall_s": wall, "identical_alone_sample": len(check), "identical": same}
+           "keys_per_s": total / wall, "offered_keys_per_s": float(sizes.mean() * rate), "wall_s": wall,
+           "identical_alone_checked": check, "identical": same}
     log(f"serving: p50 {out['p50_ms']:.2f} ms, p99 {out['p99_ms']:.2f} ms, {out['keys_per_s'] / 1e6:.1f} M keys/s, "
         f"identical alone and in the mix: {same}")
     return out
@@ -136,25 +182,104 @@ def class_build(h, args, ob):
     return out
 
 
+def strings_workload(args, eng):
+    """ShockHash<30,false> class constructor over 1,000,000 distinct random ASCII strings of length 10..50 (seed
+    3), bucket 2000: strings packed and uploaded, SpookyV2 fingerprints on the device, sort, plan, search,
+    orientation, retrieval, host encoding. One step = one constructor call (its own wall clock)."""
+    from shockhash_amd import host, workload
+    from shockhash_amd.engine import plan_forest
+    from shockhash_amd.roofline import roofline
+    strings = workload.random_strings(args.n, seed=3)
+    ms, kms, f = [], [], None
+    for i in range(args.warmup + args.steps):
+        if f is not None:
+            f.close()
+        f = host.MPHF("sh1", 30, strings, 2000)
+        t = f.timings_ms
+        if i >= args.warmup:
+            ms.append(t["constructor"])
+            kms.append(t["search_kernel"])
+        log(f"{'warmup' if i < args.warmup else 'step'} {i}: constructor {t['constructor']:.1f} ms, "
+            f"search kernel {t['search_kernel']:.2f} ms")
+    q = f(strings)
+    bij = bool(int(q.max()) < len(strings) and len(np.unique(q)) == len(strings))
+    fp = eng.hash_strings(strings)
+    nodes = plan_forest("sh1", 30, workload.bucket_sizes(fp, 2000))
+    roof = roofline(nodes, f.codes, "sh1", float(np.mean(kms)))
+    ms_step = float(np.mean(ms))
+    return {"metric": "ShockHash<30> constructor keys/s over 1M strings", "value": args.n / (ms_step / 1e3),
+            "unit": "keys/s", "ms_per_step": ms_step, "search_kernel_ms": float(np.mean(kms)), "roofline": roof,
+            "queries_bijective": bij, "bits_per_key": f.bits / args.n,
+            "config": {"workload": f"ShockHash<30,false> class constructor, bucket 2000, {args.n} random strings of "
+                                   "length 10..50 (seed 3), SpookyV2 fingerprints on the device", "keys": args.n,
+                       "leaf_size": 30, "bucket_size": 2000, "variant": "ShockHash"}}
+
+
+def leaves_workload(args, eng):
+    """200,000 isolated ShockHash leaves of 40 keys (the .second of uniform fingerprints, seed 11), each searched
+    from seed 0 for its minimal seed (benchmark/bijections/ShockHash1.h:13-35) by the batched leaf search with the
+    keys resident in HBM. One step = the whole batch; value = hash evaluations per second."""
+    from shockhash_amd import workload
+    from shockhash_amd.engine import LEAF_DTYPE, DeviceArray
+    from shockhash_amd.roofline import INT32_PEAK_TOPS, SH1_LEAF_OPS
+    nl, m = args.n, 40
+    keys = workload.isolated_leaves(nl, m, seed=11)
+    tasks = np.zeros(nl, dtype=LEAF_DTYPE)
+    tasks["offset"], tasks["m"], tasks["start_seed"], tasks["max_trials"] = np.arange(nl) * m, m, 0, 1 << 40
+    d_keys = DeviceArray.from_host(eng, keys.reshape(-1))
+    d_tasks = DeviceArray.from_host(eng, tasks)
+    d_out = DeviceArray(eng, nl, np.uint64)
+    step_s = []
+    for i in range(args.warmup + args.steps):
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.sh1_leaf_search_dev(d_keys, d_tasks, nl, d_out)
+        eng.sync()
+        if i >= args.warmup:
+            step_s.append(time.perf_counter() - t0)
+        log(f"{'warmup' if i < args.warmup else 'step'} {i}: {1e3 * (time.perf_counter() - t0):.1f} ms")
+    seeds = d_out.download()
+    evals = float(m * (seeds.astype(np.float64) + 1).sum())
+    s_step = float(np.mean(step_s))
+    gold = os.path.join(ROOT, "tests", "golden", "config2_seeds.npz")
+    equal = None
+    if nl == 200_000 and os.path.exists(gold):
+        equal = bool(np.array_equal(seeds, np.load(gold)["seeds"]))
+    achieved = evals * SH1_LEAF_OPS / s_step / 1e12
+    return {"metric": "isolated ShockHash leaf hash evaluations/s (200k leaves of 40 keys)", "value": evals / s_step,
+            "unit": "evals/s", "ms_per_step": 1e3 * s_step,
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": INT32_PEAK_TOPS, "unit": "TOP/s",
+                         "frac": achieved / INT32_PEAK_TOPS, "traffic": None, "ops_per_eval": SH1_LEAF_OPS,
+                         "hash_evals": evals, "evals_per_s": evals / s_step},
+            "seeds_equal_oracle_fixture": equal,
+            "config": {"workload": f"{nl} isolated ShockHash leaves of {m} keys (seed 11), seeds from 0, batched leaf "
+                                   "search", "leaves": nl, "leaf_size": m}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--workload", choices=["build", "strings", "leaves"], default="build",
+                    help="build: the headline persistent build (default); strings: the ShockHash<30> constructor over "
+                         "1M strings; leaves: 200k isolated leaves of 40 keys")
+    ap.add_argument("--n", type=int, default=None, help="keys (build, strings) or leaves (leaves)")
     ap.add_argument("--variant", choices=sorted(VARIANT_NAMES), default="sh2",
                     help="class variant of the build (the headline workload is sh2 with leaf 64)")
     ap.add_argument("--leaf", type=int, default=64, help="leaf size")
     ap.add_argument("--plain-buckets", type=int, default=100, help="plain-form buckets (0: no plain-form run)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="CPU oracle sample size (0: the whole workload)")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: this process's CPU share)")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="CPU baseline runs (the median is reported)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split-seeds-per-lane", type=int, default=None)
     ap.add_argument("--leaf-seeds-per-lane", type=int, default=None)
     ap.add_argument("--waves-per-cu", type=int, default=None)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2048)
-    ap.add_argument("--serve-rate", type=float, default=800.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rates", type=str, default="800,1300,1800",
+                    help="Poisson arrival rates, requests/s (the first one is the headline p99)")
     ap.add_argument("--serve-slots", type=int, default=1,
                     help="concurrent builds; two or more overlap kernels that then stretch each other's tails")
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
@@ -174,6 +299,15 @@ def main():
     from shockhash_amd.roofline import roofline
 
     eng = Engine(0)
+    if args.workload != "build":
+        args.n = args.n or (1_000_000 if args.workload == "strings" else 200_000)
+        out = strings_workload(args, eng) if args.workload == "strings" else leaves_workload(args, eng)
+        out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                    "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic"})
+        print(json.dumps(out))
+        eng.close()
+        return
+    args.n = args.n or 10_000_000
     tune = {k: v for k, v in (("split_seeds_per_lane", args.split_seeds_per_lane),
                                 ("sh1_seeds_per_lane", args.leaf_seeds_per_lane),
                                 ("waves_per_cu", args.waves_per_cu)) if v is not None}
@@ -230,38 +364,49 @@ def main():
         plain_value = n_plain_keys / plain_search_s
         log(f"plain form: {args.plain_buckets} buckets ({n_plain_keys} keys) in {plain_s:.2f} s, codes equal: {plain_match}")
 
-    cpu = None
+    cpu, ob = None, None
     bit_exact = plain_match
     if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        lib_path, build_desc = native_oracle()
+        O.use_library(lib_path)
+        model, share = host_cpu()
+        threads = args.cpu_threads or share
         n_cpu = args.cpu_sample or args.n
         if n_cpu == args.n:
-            hs = h
-            t0 = time.perf_counter()
-            ob = O_build(hs, args.cpu_threads, V, L)
-            dt = time.perf_counter() - t0
-            bs = b  # the timed GPU build of the same keys
+            hs, bs = h, b  # the timed GPU build of the same keys
             sample = f"the whole workload ({args.n} keys, seed 1)"
         else:
-            hs, ob, dt = cpu_baseline(n_cpu, args.cpu_threads, V, L)
+            hs = workload.fingerprints(n_cpu, seed=7)
             bs = eng.build(hs, V, L, 2000)
             sample = f"{n_cpu} fingerprints (same generator, seed 7)"
+        ob, dt, times = cpu_baseline(hs, threads, V, L, reps=max(1, args.cpu_reps))
         bit_exact = bit_exact and bool(np.array_equal(bs.codes, ob.node_code)) and \
             bool(np.array_equal(bs.size_acc, ob.size_acc))
-        cpu = {"value": n_cpu / dt, "unit": "keys/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"{sample}, {VARIANT_NAMES[V]} leaf {L}, bucket 2000, oracle C++ restatement, "
-                         f"{args.cpu_threads} threads, {dt:.1f} s"}
-        log(f"cpu oracle: {cpu['value']:.0f} keys/s; GPU codes equal the oracle's: {bit_exact}")
+        cpu = {"value": n_cpu / dt, "unit": "keys/s", "cores": threads, "kind": "port",
+               "sample": f"{sample}, {VARIANT_NAMES[V]} leaf {L}, bucket 2000; the oracle's C++ restatement of the "
+                         f"reference construction ({build_desc}), {threads} threads (this process's CPU share) on "
+                         f"{model}; median of {len(times)} runs: {', '.join(f'{t:.2f}' for t in times)} s",
+               "cpu_model": model}
+        if n_cpu != args.n:
+            ob = None
+        log(f"cpu oracle: {cpu['value']:.0f} keys/s (median of {times}); GPU codes equal the oracle's: {bit_exact}")
 
     cls = None
     if args.no_class_build or not headline:
         log("class-constructor leg: skipped" + ("" if headline else " (ShockHash2 leaf 64 only)"))
     else:
-        cls = class_build(h, args, ob if (not args.no_cpu_baseline and (args.cpu_sample or args.n) == args.n) else None)
+        cls = class_build(h, args, ob)
         bit_exact = bit_exact and cls["queries_bijective"] and cls["descriptors_equal_oracle"] is not False
 
-    serving = None if args.no_serving or not headline else serving_mix(eng, args)
-    if serving is not None:
-        bit_exact = bit_exact and serving["identical"]
+    serving = None
+    if not args.no_serving and headline:
+        digests = []
+        mix = serving_requests(eng, args)
+        serving = [serving_mix(eng, args, mix, float(r), digests) for r in args.serve_rates.split(",")]
+        mix[2].free()
+        bit_exact = bit_exact and all(x["identical"] for x in serving)
 
     out = {
         "metric": "MPHF build keys/s at 10M keys (bit-exact) vs plain GPU form; p99 build latency",
@@ -287,8 +432,9 @@ def main():
         "search_kernel_ms": kms,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mphf_build_keys_per_s": cls["keys_per_s"] if cls else None,
         "class_build": cls,
-        "p99_build_latency_ms": serving["p99_ms"] if serving else None,
+        "p99_build_latency_ms": serving[0]["p99_ms"] if serving else None,
         "serving": serving,
     }
     print(json.dumps(out))

@@ -138,12 +138,12 @@ class MPHF:
 
     @property
     def timings_ms(self):
-        t = np.zeros(4, dtype=np.float64)
+        t = np.zeros(5, dtype=np.float64)
         load().shh_timings(self.h, t.ctypes.data_as(P))
         p = np.zeros(4, dtype=np.float64)
         load().shh_phase_timings(self.h, p.ctypes.data_as(P))
-        return {"sort": t[0], "tree": t[1], "search": t[2], "host_encode": t[3], "rice_ef": p[0],
-                "orientation": p[1], "retrieval": p[2], "constructor": p[3]}
+        return {"sort": t[0], "tree": t[1], "search": t[2], "host_encode": t[3], "search_kernel": t[4],
+                "rice_ef": p[0], "orientation": p[1], "retrieval": p[2], "constructor": p[3]}
 
     def close(self):
         if getattr(self, "h", None):

@@ -178,10 +178,13 @@ void shh_phase_timings(void *h, double *out4) {
     out4[0] = p[0]; out4[1] = p[1]; out4[2] = p[2];
     out4[3] = ((Handle *)h)->ctor_ms;
 }
-void shh_timings(void *h, double *out4) {
+// Sort + bucketing, tree setup, seed search (host clock), host phases after the search, and the persistent
+// search kernel alone (HIP events), in ms.
+void shh_timings(void *h, double *out5) {
     const auto &f = ((Handle *)h)->forest();
-    out4[0] = f.ms[0]; out4[1] = f.ms[1]; out4[2] = f.ms[2];
-    out4[3] = ((Handle *)h)->host_ms();
+    out5[0] = f.ms[0]; out5[1] = f.ms[1]; out5[2] = f.ms[2];
+    out5[3] = ((Handle *)h)->host_ms();
+    out5[4] = f.ms[3];
 }
 
 // Compile-time Golomb-Rice memo of a class (for comparison with the oracle's run-time formula).

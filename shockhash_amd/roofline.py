@@ -18,6 +18,13 @@ quarters or halves counts the top bits of a partial hash, other shapes compare t
 (SIMDShockHash) hash with one threshold per part boundary. Speculative seeds past the winner, pair tests
 of ShockHash2 leaves, partitions and scheduling are not counted: the fraction of peak is the share of the
 chip's VALU issue capacity spent on the evaluations the reference's algorithm needs.
+
+The price table below is the cost convention of this roofline and stays fixed: it is not re-priced when a
+kernel changes, so a kernel that needs fewer instructions for the same evaluations reports a higher fraction.
+It replaces a flat 32 ops per 64-bit-mix evaluation (16 for the 32-bit SIMDShockHash split hash), an estimate
+that put the measured 10M-key ShockHash2 build at 1.31 of peak: 32 ops per evaluation would need 2.41e10
+wave-level VALU instructions, and rocprofv3 counted 1.69e10 (SQ_INSTS_VALU) for the whole kernel, so the flat
+price overcounts the work. evals_per_s is reported beside the fraction.
 """
 import numpy as np
 

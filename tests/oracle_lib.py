@@ -59,14 +59,24 @@ _SIGS = {
 VARIANT = {"sh1": 0, "simd": 1, "sh2": 2}
 
 _lib = None
+_path = LIB_PATH
+
+
+def use_library(path):
+    """Loads the oracle from `path` (another build of oracle/shockhash_oracle.cpp) instead of the default build;
+    only before the first call."""
+    global _path
+    if _lib is not None and path != _path:
+        raise RuntimeError("the oracle library is already loaded")
+    _path = path
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle not built: {LIB_PATH} (run __graft_entry__.build())")
-        L = ctypes.CDLL(LIB_PATH)
+        if not os.path.exists(_path):
+            raise RuntimeError(f"oracle not built: {_path} (run __graft_entry__.build())")
+        L = ctypes.CDLL(_path)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
             f.restype = res
