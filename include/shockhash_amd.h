@@ -182,15 +182,7 @@ int shgpu_build_keys(const shgpu_build *b, uint64_t *keys);
 int shgpu_build_device_keys(const shgpu_build *b, const uint64_t **d_keys);
 /* Host-only: the forest shgpu_build_run would search for these bucket sizes (DFS pre-order). Call with
  * out = NULL to get the count. Needs no GPU. */
-/* One-bit retrieval structure of the leaf orientations, built on the device. Replaces the construction of the
- * reference's SimpleRibbon<1> retrieval (include/ShockHash.h:657-658, include/ShockHash2.h:491-497; the ribbon
- * library itself is not vendored). Layout: 3 * seg one-bit cells, seg = shgpu_retrieval_cells_per_third(n); key k
- * uses one cell in each third, from remix of k and the seed, and its bit is the XOR of its three cells
- * (shockhash::BitRetrieval answers queries on this layout). d_keys: n keys, d_bits: n bytes whose bit 0 is the
- * key's bit, both in device memory; keys must be distinct. Writes the seed, seg and shgpu_retrieval_words(n)
- * packed words (bit c of the structure is bit c % 64 of word c / 64) to host memory. Tries seeds 0..63;
- * SHGPU_ERR_SEARCH_LIMIT when none peels (duplicate keys). */
-uint64_t shgpu_retrieval_cells_per_third(uint64_t n);
+
 /* Retrieval input of leaves: replays each leaf's cuckoo insertions in leaf key order (the orientation that
  * fixes which of its two cells a key occupies) and writes the (key, bit) entries in the reference's order:
  * ShockHash / SIMDShockHash leaves cell by cell with bit = cell >= m/2 (include/ShockHash.h:425-432,
@@ -206,9 +198,23 @@ int shgpu_leaf_orientation(shgpu_engine *e, int variant, const uint64_t *d_keys,
  * (key i is d_data[d_offsets[i], d_offsets[i + 1])), d_hashes: 2n u64; all device memory. Runs on the engine's
  * stream (ordered before a following shgpu_build_run). */
 int shgpu_hash_strings(shgpu_engine *e, const uint8_t *d_data, const uint64_t *d_offsets, uint64_t n, uint64_t *d_hashes);
-uint64_t shgpu_retrieval_words(uint64_t n);
-int shgpu_retrieval_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
-                          uint64_t *seg, uint64_t *h_words, uint64_t words_capacity);
+/* One-bit retrieval structure of the leaf orientations, built on the device: a sharded ribbon. Replaces the
+ * construction of the reference's SimpleRibbon<1> retrieval (include/ShockHash.h:657-658,
+ * include/ShockHash2.h:491-497; the ribbon library itself is not vendored) at about 1.07 bits per key.
+ * Layout (shockhash::RibbonRetrieval answers queries on it): S = shgpu_ribbon_shards(n) shards; key k belongs to
+ * shard fastrange64(h, S) with h = remix(k + 0x632be59bd9b4e019). Shard s has n_s keys and
+ * m_s = n_s + ceil(n_s / 20) one-bit slots (at least 64; none when empty) at slot offset offsets[s] (offsets[S] is
+ * the total). With its seed byte d: g = remix(h + (d + 1) * 0x9E3779B97F4A7C15),
+ * start = fastrange64(g, m_s - 63), row = remix(g ^ 0xd6e8feb86659fd93) | 1, and the key's bit is the parity of
+ * row AND slots offsets[s] + start .. + 63 (bit j of row covers slot start + j; slot c is bit c % 64 of word c / 64).
+ * d_keys: n keys, d_bits: n bytes whose bit 0 is the key's bit, both in device memory. Writes offsets (S + 1),
+ * seeds (S) and the packed words (*words_used of them, including one word of padding, at most
+ * shgpu_ribbon_max_words(n)) to host memory. SHGPU_ERR_SEARCH_LIMIT when a shard has no consistent seed byte
+ * (repeated keys with different bits). */
+uint64_t shgpu_ribbon_shards(uint64_t n);
+uint64_t shgpu_ribbon_max_words(uint64_t n);
+int shgpu_ribbon_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint32_t *h_offsets,
+                       uint8_t *h_seeds, uint64_t *h_words, uint64_t words_capacity, uint64_t *words_used);
 
 int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc, uint64_t nbuckets, shgpu_node *out,
                       uint64_t capacity, uint64_t *count);

@@ -18,7 +18,8 @@
 #include "../include/shockhash/GolombMemo.h"
 #include "build_common.h"
 #include "search_sh2.h"
-#include "retrieval.h"
+#include "orientation.h"
+#include "ribbon.h"
 #include "sort.h"
 #include "strings.h"
 
@@ -184,7 +185,7 @@ struct shgpu_engine {
     shgpu_tuning tuning{4, 2, 0, 0, DEFAULT_MAX_TRIALS};  // batched per-node entry points (waves 0: 24 per CU)
     DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
     BuildBuffers main;                          // shgpu_build_run
-    DevBuf retrieval;                           // shgpu_retrieval_build working memory (~33 B per key)
+    DevBuf retrieval;                           // shgpu_ribbon_build working memory (~42 B per key)
 };
 
 struct shgpu_build {
@@ -1519,8 +1520,6 @@ int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *d, const uint64_t *
     return run_build(e, e->main, e->stream, 1, d, d_hashes, n, out);
 }
 
-uint64_t shgpu_retrieval_words(uint64_t n) { return (3 * shgpu_retrieval_cells_per_third(n) + 63) / 64; }
-
 int shgpu_leaf_orientation(shgpu_engine *e, int variant, const uint64_t *d_keys, const shgpu_orient_task *d_tasks,
                            uint64_t n, uint64_t *d_out_keys, uint8_t *d_out_bits) {
     if (!e || (n && (!d_keys || !d_tasks || !d_out_keys || !d_out_bits))) return fail(SHGPU_ERR_ARG, "NULL argument");
@@ -1541,13 +1540,14 @@ int shgpu_hash_strings(shgpu_engine *e, const uint8_t *d_data, const uint64_t *d
     return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
 }
 
-int shgpu_retrieval_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint64_t *seed,
-                          uint64_t *seg, uint64_t *h_words, uint64_t words_capacity) {
-    if (!e || !seed || !seg || !h_words || (n && (!d_keys || !d_bits))) return fail(SHGPU_ERR_ARG, "NULL argument");
+int shgpu_ribbon_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint32_t *h_offsets,
+                       uint8_t *h_seeds, uint64_t *h_words, uint64_t words_capacity, uint64_t *words_used) {
+    if (!e || !h_offsets || !h_seeds || !h_words || !words_used || (n && (!d_keys || !d_bits)))
+        return fail(SHGPU_ERR_ARG, "NULL argument");
     if (hipSetDevice(e->device) != hipSuccess) return fail(SHGPU_ERR_HIP, "hipSetDevice failed");
     std::string err;
-    const int rc = shgpu_retrieval_build_impl(e->stream, d_keys, d_bits, n, seed, seg, h_words, words_capacity, err,
-                                              &e->retrieval.p, &e->retrieval.bytes);
+    const int rc = shgpu_ribbon_build_impl(e->stream, d_keys, d_bits, n, h_offsets, h_seeds, h_words, words_capacity,
+                                           words_used, err, &e->retrieval.p, &e->retrieval.bytes);
     return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
 }
 

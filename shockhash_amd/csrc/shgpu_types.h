@@ -5,6 +5,13 @@
 
 #define SHGPU_NOT_FOUND (~(uint64_t)0)
 
+/* Layout constants of the sharded ribbon retrieval (shgpu_ribbon_build; queried by shockhash::RibbonRetrieval). */
+#define SHGPU_RIBBON_WIDTH 64u                           /* slots one key's row covers */
+#define SHGPU_RIBBON_SHARD_KEYS 2048u                    /* keys per shard on average */
+#define SHGPU_RIBBON_SALT_SHARD 0x632be59bd9b4e019ull
+#define SHGPU_RIBBON_SALT_ROW 0xd6e8feb86659fd93ull
+#define SHGPU_RIBBON_SEED_STEP 0x9E3779B97F4A7C15ull
+
 /* One leaf seed search: keys[offset .. offset+m), seeds start_seed, start_seed+1, ...
  * (at most max_trials of them). */
 typedef struct shgpu_leaf_task {

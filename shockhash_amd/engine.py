@@ -92,10 +92,10 @@ _SIGS = {
     "shgpu_build_run": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(P)]),
     "shgpu_hash_strings": (ctypes.c_int, [P, P, P, ctypes.c_uint64, P]),
     "shgpu_leaf_orientation": (ctypes.c_int, [P, ctypes.c_int, P, P, ctypes.c_uint64, P, P]),
-    "shgpu_retrieval_cells_per_third": (ctypes.c_uint64, [ctypes.c_uint64]),
-    "shgpu_retrieval_words": (ctypes.c_uint64, [ctypes.c_uint64]),
-    "shgpu_retrieval_build": (ctypes.c_int, [P, P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
-                                             ctypes.POINTER(ctypes.c_uint64), P, ctypes.c_uint64]),
+    "shgpu_ribbon_shards": (ctypes.c_uint64, [ctypes.c_uint64]),
+    "shgpu_ribbon_max_words": (ctypes.c_uint64, [ctypes.c_uint64]),
+    "shgpu_ribbon_build": (ctypes.c_int, [P, P, P, ctypes.c_uint64, P, P, P, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_uint64)]),
     "shgpu_build_free": (None, [P]),
     "shgpu_build_nbuckets": (ctypes.c_uint64, [P]),
     "shgpu_build_nnodes": (ctypes.c_uint64, [P]),
@@ -407,24 +407,27 @@ class Engine:
             d_out.free()
         return out
 
-    def retrieval_build(self, keys, bits):
-        """One-bit retrieval structure of distinct keys and their bits, built on the device
-        (shgpu_retrieval_build): (seed, cells per third, packed words)."""
+    def ribbon_build(self, keys, bits):
+        """One-bit retrieval structure of distinct keys and their bits, built on the device (shgpu_ribbon_build):
+        (slot offsets, seed bytes, packed slot words)."""
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         bits = np.ascontiguousarray(bits, dtype=np.uint8)
         n = len(keys)
         d_k = DeviceArray.from_host(self, keys if n else np.zeros(1, np.uint64))
         d_b = DeviceArray.from_host(self, bits if n else np.zeros(1, np.uint8))
         lib = load()
-        words = np.zeros(lib.shgpu_retrieval_words(n), dtype=np.uint64)
-        seed, seg = ctypes.c_uint64(), ctypes.c_uint64()
+        shards = lib.shgpu_ribbon_shards(n)
+        offsets = np.zeros(shards + 1, dtype=np.uint32)
+        seeds = np.zeros(shards, dtype=np.uint8)
+        words = np.zeros(lib.shgpu_ribbon_max_words(n), dtype=np.uint64)
+        used = ctypes.c_uint64()
         try:
-            _check(lib.shgpu_retrieval_build(self.h, d_k.ptr, d_b.ptr, n, ctypes.byref(seed), ctypes.byref(seg),
-                                             words.ctypes.data_as(P), len(words)))
+            _check(lib.shgpu_ribbon_build(self.h, d_k.ptr, d_b.ptr, n, offsets.ctypes.data_as(P), seeds.ctypes.data_as(P),
+                                          words.ctypes.data_as(P), len(words), ctypes.byref(used)))
         finally:
             d_k.free()
             d_b.free()
-        return seed.value, seg.value, words
+        return offsets, seeds, words[: used.value]
 
     def build(self, hashes, variant, leaf_size, bucket_size, **kw):
         hashes = np.ascontiguousarray(hashes, dtype=np.uint64).reshape(-1, 2)

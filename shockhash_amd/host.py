@@ -29,6 +29,8 @@ _SIGS = {
     "shh_keep_retrieval_input": (None, [ctypes.c_int]),
     "shh_memo": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P]),
     "shh_last_status": (ctypes.c_int, []),
+    "shh_save": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+    "shh_load": (P, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
 }
 VARIANTS = {"sh1": 0, "simd": 1, "sh2": 2}
 _lib = None
@@ -144,6 +146,23 @@ class MPHF:
         load().shh_phase_timings(self.h, p.ctypes.data_as(P))
         return {"sort": t[0], "tree": t[1], "search": t[2], "host_encode": t[3], "search_kernel": t[4],
                 "rice_ef": p[0], "orientation": p[1], "retrieval": p[2], "constructor": p[3]}
+
+    def save(self, path):
+        """Writes the function to `path` (the class template's save())."""
+        err = ctypes.create_string_buffer(512)
+        if load().shh_save(self.h, str(path).encode(), err, 512) != 0:
+            raise RuntimeError(f"save failed: {err.value.decode(errors='replace')}")
+
+    @classmethod
+    def load(cls, variant, leaf, path, strings=False):
+        """A function read from `path` (the class template's load()); answers queries only."""
+        err = ctypes.create_string_buffer(512)
+        h = load().shh_load(VARIANTS[variant], leaf, str(path).encode(), err, 512)
+        if not h:
+            raise RuntimeError(f"load failed: {err.value.decode(errors='replace')}")
+        obj = cls.__new__(cls)
+        obj.h, obj._strings, obj.n = h, strings, None
+        return obj
 
     def close(self):
         if getattr(self, "h", None):

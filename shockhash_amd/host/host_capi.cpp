@@ -2,6 +2,7 @@
 // (ShockHash, SIMDShockHash, ShockHash2), used by the Python tests and the benchmark.
 #include <cstdio>
 #include <chrono>
+#include <fstream>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -26,6 +27,7 @@ struct Handle {
     virtual size_t bits() const = 0;
     virtual double host_ms() const = 0;
     virtual const double *host_phases() const = 0;
+    virtual void save(std::ostream &os) const = 0;
     double ctor_ms = 0;  // wall time of the class constructor
 };
 
@@ -42,7 +44,15 @@ struct HandleT : Handle {
     size_t bits() const override { return obj->getBits(); }
     double host_ms() const override { return obj->hostEncodeMs(); }
     const double *host_phases() const override { return obj->hostPhaseMs(); }
+    void save(std::ostream &os) const override { obj->save(os); }
 };
+
+template <class T>
+Handle *make_loaded(std::istream &is) {
+    std::unique_ptr<T> o(new T());
+    o->load(is);
+    return new HandleT<T>(o.release());
+}
 
 template <class T>
 Handle *make_h(std::vector<hash128_t> &h, size_t bucket, size_t threads) {
@@ -140,6 +150,45 @@ void shh_free(void *h) { delete (Handle *)h; }
 
 // Status of the calling thread's last failed shh_build / shh_build_strings (SHGPU_* code, -1 otherwise).
 int shh_last_status(void) { return g_status; }
+
+// Writes the function to a file (the class's save()); 0 on success.
+int shh_save(void *h, const char *path, char *err, size_t errlen) {
+    try {
+        std::ofstream os(path, std::ios::binary);
+        ((Handle *)h)->save(os);
+        os.close();
+        if (!os) throw std::runtime_error("write failed");
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(err, errlen, e.what());
+        return 1;
+    }
+}
+
+// A function of the given class and leaf size read from a file (the class's load()); queries only.
+void *shh_load(int variant, int leaf, const char *path, char *err, size_t errlen) {
+    try {
+        std::ifstream is(path, std::ios::binary);
+        if (!is) throw std::runtime_error(std::string("cannot open ") + path);
+        if (variant == SHGPU_SHOCKHASH) {
+#define C(L) if (leaf == L) return make_loaded<ShockHash<L, false>>(is);
+            SHH_LEAVES_SH1(C)
+#undef C
+        } else if (variant == SHGPU_SIMD_SHOCKHASH) {
+#define C(L) if (leaf == L) return make_loaded<SIMDShockHash<L, false>>(is);
+            SHH_LEAVES_SIMD(C)
+#undef C
+        } else if (variant == SHGPU_SHOCKHASH2) {
+#define C(L) if (leaf == L) return make_loaded<ShockHash2<L>>(is);
+            SHH_LEAVES_SH2(C)
+#undef C
+        }
+        throw std::runtime_error("leaf size " + std::to_string(leaf) + " not instantiated in libshockhash_host");
+    } catch (const std::exception &e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
 
 void shh_query(void *h, const uint64_t *hashes, size_t n, uint64_t *out) {
     Handle *x = (Handle *)h;

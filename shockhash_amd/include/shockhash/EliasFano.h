@@ -4,6 +4,9 @@
 // library's own (sux is not part of the drop-in; query answers depend only on the stored values).
 #pragma once
 #include <cstdint>
+#include <istream>
+#include <ostream>
+#include <stdexcept>
 #include <vector>
 
 namespace shockhash {
@@ -64,6 +67,31 @@ class EliasFano {
     }
     uint64_t size() const { return n; }
     uint64_t bitCount() const { return 64 * (lower.size() + upper.size() + samples.size()) + 128; }
+    // Serialized form: n, l, then the lower, upper and sample arrays, each as a count and its words.
+    void save(std::ostream &os) const {
+        const uint64_t hdr[2] = {n, l};
+        os.write((const char *)hdr, 16);
+        for (const auto *v : {&lower, &upper, &samples}) {
+            const uint64_t c = v->size();
+            os.write((const char *)&c, 8);
+            os.write((const char *)v->data(), (std::streamsize)(c * 8));
+        }
+    }
+    void load(std::istream &is) {
+        uint64_t hdr[2] = {0, 0};
+        is.read((char *)hdr, 16);
+        if (!is || hdr[1] > 64) throw std::runtime_error("EliasFano: bad header");
+        n = hdr[0];
+        l = (unsigned)hdr[1];
+        for (auto *v : {&lower, &upper, &samples}) {
+            uint64_t c = 0;
+            is.read((char *)&c, 8);
+            if (!is || c > (1ull << 40)) throw std::runtime_error("EliasFano: bad array size");
+            v->assign(c, 0);
+            is.read((char *)v->data(), (std::streamsize)(c * 8));
+        }
+        if (!is) throw std::runtime_error("EliasFano: truncated stream");
+    }
 };
 
 // The pair of sequences (bucket_size_acc, bucket_pos_acc), include/ShockHash.h:654.
@@ -80,6 +108,14 @@ class DoubleEF {
     }
     uint64_t bitCountCumKeys() const { return cum_keys.bitCount(); }
     uint64_t bitCountPosition() const { return position.bitCount(); }
+    void save(std::ostream &os) const {
+        cum_keys.save(os);
+        position.save(os);
+    }
+    void load(std::istream &is) {
+        cum_keys.load(is);
+        position.load(is);
+    }
 };
 
 }  // namespace shockhash

@@ -166,12 +166,11 @@ class Engine {
 
   public:
     // The retrieval input of a forest (every leaf's orientation, in the reference's order) and its one-bit
-    // retrieval structure, both computed on the device (shgpu_leaf_orientation, shgpu_retrieval_build) from
-    // the build's device-resident leaf-ordered keys; the structure comes back, and the input too when `keep`
-    // is set.
+    // retrieval structure, both computed on the device (shgpu_leaf_orientation, shgpu_ribbon_build) from the build's
+    // device-resident leaf-ordered keys; the structure comes back, and the input too when `keep` is set.
     void orientation_retrieval(int variant, const Forest &f, bool keep, std::vector<uint64_t> &input_keys,
-                               std::vector<uint8_t> &input_bits, uint64_t &seed, uint64_t &seg,
-                               std::vector<uint64_t> &words, double *ms2 = nullptr) {
+                               std::vector<uint8_t> &input_bits, std::vector<uint32_t> &offsets,
+                               std::vector<uint8_t> &seeds, std::vector<uint64_t> &words, double *ms2 = nullptr) {
         std::lock_guard<std::mutex> lock(mtx);
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<shgpu_orient_task> tasks;
@@ -196,10 +195,15 @@ class Engine {
             rc = shgpu_leaf_orientation(e, variant, leaf_keys, (const shgpu_orient_task *)d_tasks, tasks.size(),
                                         (uint64_t *)d_out_keys, (uint8_t *)d_out_bits);
         const auto t1 = std::chrono::steady_clock::now();
-        words.assign(shgpu_retrieval_words(total), 0);
+        const uint64_t shards = shgpu_ribbon_shards(total);
+        offsets.assign(shards + 1, 0);
+        seeds.assign(shards, 0);
+        words.assign(shgpu_ribbon_max_words(total), 0);
+        uint64_t used = 0;
         if (rc == SHGPU_OK)
-            rc = shgpu_retrieval_build(e, (const uint64_t *)d_out_keys, (const uint8_t *)d_out_bits, total, &seed, &seg,
-                                       words.data(), words.size());
+            rc = shgpu_ribbon_build(e, (const uint64_t *)d_out_keys, (const uint8_t *)d_out_bits, total, offsets.data(),
+                                    seeds.data(), words.data(), words.size(), &used);
+        if (rc == SHGPU_OK) words.resize(used);
         input_keys.assign(keep ? total : 0, 0);
         input_bits.assign(keep ? total : 0, 0);
         if (rc == SHGPU_OK && keep && total) rc = shgpu_copy_to_host(e, input_keys.data(), d_out_keys, total * 8);

@@ -32,7 +32,7 @@ class GpuRecSplit {
     size_t keys_count = 0;
     RiceBitVector descriptors;
     DoubleEF ef;
-    BitRetrieval ribbon;
+    RibbonRetrieval ribbon;
     std::vector<uint64_t> ribbon_keys;  // retrieval input, kept for verification when gpu::keep_retrieval_input
     std::vector<uint8_t> ribbon_bits;
     gpu::Forest forest_meta;                                  // codes and nodes, kept for verification
@@ -62,11 +62,12 @@ class GpuRecSplit {
         descriptors = RiceBitVector(std::move(words));
         ef = DoubleEF(f.size_acc, pos_acc);
         phase_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        uint64_t rseed = 0, rseg = 0;
+        std::vector<uint32_t> roffsets;
+        std::vector<uint8_t> rseeds;
         std::vector<uint64_t> rwords;
-        gpu::Engine::instance().orientation_retrieval(VARIANT, f, gpu::keep_retrieval_input, ribbon_keys, ribbon_bits, rseed,
-                                                      rseg, rwords, phase_ms + 1);
-        ribbon = BitRetrieval(rseed, rseg, std::move(rwords));
+        gpu::Engine::instance().orientation_retrieval(VARIANT, f, gpu::keep_retrieval_input, ribbon_keys, ribbon_bits,
+                                                      roffsets, rseeds, rwords, phase_ms + 1);
+        ribbon = RibbonRetrieval(std::move(roffsets), std::move(rseeds), std::move(rwords));
         f.device.reset();  // frees the build's device keys
         forest_meta = std::move(f);
         host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -139,6 +140,37 @@ class GpuRecSplit {
         std::cout << "EF 2:   " << (double)ef.bitCountPosition() / keys_count << std::endl;
         std::cout << "trees:  " << (double)descriptors.getBits() / keys_count << std::endl;
         std::cout << "ribbon: " << (double)(8 * ribbon.sizeBytes()) / keys_count << std::endl;
+    }
+
+    // Serialized form of the whole function (not in the reference interface, whose only stream I/O is the
+    // descriptor bit vector's, include/shockhash/RiceBitVector.h:133-141): a header (magic, format version,
+    // variant, leaf size, bucket size, bucket count, key count), the descriptor words, the two Elias-Fano
+    // sequences and the retrieval structure, all little-endian. load() restores a function that answers every
+    // query as the saved one; it throws std::runtime_error on a stream of another class or leaf size.
+    static constexpr uint64_t FORMAT_MAGIC = 0x3146485048534853ull;  // "SHSHPHF1"
+    void save(std::ostream &os) const {
+        const uint64_t hdr[7] = {FORMAT_MAGIC, 1, (uint64_t)VARIANT, (uint64_t)LEAF_SIZE, bucket_size, nbuckets, keys_count};
+        os.write((const char *)hdr, sizeof(hdr));
+        descriptors.save(os);
+        ef.save(os);
+        ribbon.save(os);
+        if (!os) throw std::runtime_error("ShockHash save: stream error");
+    }
+    void load(std::istream &is) {
+        uint64_t hdr[7] = {0, 0, 0, 0, 0, 0, 0};
+        is.read((char *)hdr, sizeof(hdr));
+        if (!is || hdr[0] != FORMAT_MAGIC || hdr[1] != 1) throw std::runtime_error("ShockHash load: not a saved function");
+        if (hdr[2] != (uint64_t)VARIANT || hdr[3] != (uint64_t)LEAF_SIZE)
+            throw std::runtime_error("ShockHash load: the stream holds another class or leaf size");
+        bucket_size = hdr[4];
+        nbuckets = hdr[5];
+        keys_count = hdr[6];
+        descriptors.load(is);
+        ef.load(is);
+        ribbon.load(is);
+        ribbon_keys.clear();
+        ribbon_bits.clear();
+        forest_meta = gpu::Forest{};
     }
 
     // Verification accessors (not in the reference interface).

@@ -8,7 +8,7 @@
 #include <utility>
 #include <vector>
 
-#include "BitRetrieval.h"
+#include "RibbonRetrieval.h"
 #include "EliasFano.h"
 #include "GolombMemo.h"
 #include "GpuEngine.h"

@@ -8,6 +8,9 @@
 #pragma once
 #include <cstdint>
 #include <cstring>
+#include <istream>
+#include <ostream>
+#include <stdexcept>
 #include <vector>
 
 namespace shockhash {
@@ -73,6 +76,29 @@ class RiceBitVector {
     explicit RiceBitVector(std::vector<uint64_t> d) : data(std::move(d)) {}
     size_t getBits() const { return data.size() * 64; }
     const std::vector<uint64_t> &words() const { return data; }
+    // Serialized form: word count, then the words (little-endian u64).
+    void save(std::ostream &os) const {
+        const uint64_t n = data.size();
+        os.write((const char *)&n, 8);
+        os.write((const char *)data.data(), (std::streamsize)(n * 8));
+    }
+    void load(std::istream &is) {
+        uint64_t n = 0;
+        is.read((char *)&n, 8);
+        if (!is || n > (1ull << 40)) throw std::runtime_error("RiceBitVector: bad word count");
+        data.assign(n, 0);
+        is.read((char *)data.data(), (std::streamsize)(n * 8));
+        if (!is) throw std::runtime_error("RiceBitVector: truncated stream");
+    }
+    // Stream operators of the reference's RiceBitVector (RiceBitVector.h:133-141), in the form above.
+    friend std::ostream &operator<<(std::ostream &os, const RiceBitVector &rbv) {
+        rbv.save(os);
+        return os;
+    }
+    friend std::istream &operator>>(std::istream &is, RiceBitVector &rbv) {
+        rbv.load(is);
+        return is;
+    }
 
     // Sequential reader (Reader :198-263): fixed parts from bit_pos, unary parts from
     // bit_pos + unary_offset.

@@ -66,3 +66,21 @@ def test_string_keys_shockhash2():
     hashes = np.array([O.spooky128(k, 0) for k in keys], dtype=np.uint64)
     ob = O.OracleBuild("sh2", 24, hashes, 2000, threads=8)
     assert np.array_equal(q, ob.query(hashes))
+
+
+@pytest.mark.parametrize("variant,leaf", [("sh2", 64), ("sh1", 30), ("simd", 24), ("sh2", 100)])
+def test_save_load_round_trip(tmp_path, variant, leaf):
+    """save() then load() into a fresh object of the class: every query of the reloaded function equals the
+    original's (and the oracle's), and the file holds the function's bits."""
+    log(f"save/load {variant} L={leaf}")
+    h = workload.fingerprints(30000, seed=33)
+    f = host.MPHF(variant, leaf, h, 2000)
+    path = tmp_path / "f.bin"
+    f.save(path)
+    g = host.MPHF.load(variant, leaf, path)
+    q = f(h)
+    assert np.array_equal(g(h), q)
+    assert np.array_equal(q, O.OracleBuild(variant, leaf, h, 2000, threads=8).query(h))
+    assert path.stat().st_size * 8 <= f.bits + 1024
+    with pytest.raises(RuntimeError):
+        host.MPHF.load(variant, 16 if leaf != 16 else 8, path)

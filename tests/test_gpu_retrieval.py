@@ -1,16 +1,16 @@
-"""One-bit retrieval structure built on the device (shgpu_retrieval_build): every key of the set retrieves its
-bit. Queries are answered here from the documented layout with numpy (independent of the library's code);
-the class templates' queries over the same structure are checked against the oracle in test_gpu_classes.py and
-test_gpu_configs.py."""
+"""One-bit retrieval structure built on the device (shgpu_ribbon_build, a sharded ribbon): every key of the set
+retrieves its bit. Queries are answered here from the documented layout with numpy (independent of the library's
+code); the class templates' queries over the same structure are checked against the oracle in test_gpu_classes.py
+and test_gpu_configs.py."""
 import numpy as np
 import pytest
 
 from shockhash_amd import workload
-from shockhash_amd.engine import ERR_ARG, ERR_SEARCH_LIMIT, ShgpuError, load
+from shockhash_amd.engine import ERR_SEARCH_LIMIT, ShgpuError, load
 
 pytestmark = pytest.mark.gpu
 
-M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+SALT_SHARD, SALT_ROW, STEP = np.uint64(0x632BE59BD9B4E019), np.uint64(0xD6E8FEB86659FD93), np.uint64(0x9E3779B97F4A7C15)
 
 
 def log(msg):
@@ -26,61 +26,80 @@ def remix(z):
 
 
 def fastrange64(h, n):
-    """floor(h * n / 2^64) for n < 2^32."""
+    """floor(h * n / 2^64) for n < 2^32 (n may be an array)."""
     hi, lo = h >> np.uint64(32), h & np.uint64(0xFFFFFFFF)
-    n = np.uint64(n)
+    n = np.asarray(n, dtype=np.uint64)
     return (hi * n + ((lo * n) >> np.uint64(32))) >> np.uint64(32)
 
 
-def retrieve(keys, seed, seg, words):
+def parity(x):
+    for s in (32, 16, 8, 4, 2, 1):
+        x = x ^ (x >> np.uint64(s))
+    return x & np.uint64(1)
+
+
+def retrieve(keys, offsets, seeds, words):
     with np.errstate(over="ignore"):
-        h0 = remix(keys + np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.uint64(0x632BE59BD9B4E019))
-        h1 = remix(h0 ^ np.uint64(0xD6E8FEB86659FD93))
-        h2 = remix(h1 + np.uint64(0x8CB92BA72F3D8DD7))
-    seg = np.uint64(seg)
-    cells = [fastrange64(h0, seg), seg + fastrange64(h1, seg), np.uint64(2) * seg + fastrange64(h2, seg)]
-    out = np.zeros(len(keys), dtype=np.uint64)
-    for c in cells:
-        out ^= (words[c >> np.uint64(6)] >> (c & np.uint64(63))) & np.uint64(1)
-    return out.astype(np.uint8)
+        h = remix(keys + SALT_SHARD)
+        s = fastrange64(h, len(seeds))
+        base = offsets[s].astype(np.uint64)
+        m = offsets[s + 1].astype(np.uint64) - base
+        g = remix(h + (seeds[s].astype(np.uint64) + np.uint64(1)) * STEP)
+        p = base + fastrange64(g, m - np.uint64(63))
+        row = remix(g ^ SALT_ROW) | np.uint64(1)
+    w, b = p >> np.uint64(6), p & np.uint64(63)
+    lo = words[w] >> b
+    hi = np.where(b > 0, words[w + np.uint64(1)] << ((np.uint64(64) - b) & np.uint64(63)), np.uint64(0))
+    return parity(row & (lo | hi)).astype(np.uint8)
 
 
 @pytest.mark.parametrize("n", [0, 1, 2, 3, 1000, 100_000, 2_000_000])
 def test_every_key_retrieves_its_bit(engine, n):
-    log(f"retrieval n={n}")
+    log(f"ribbon n={n}")
     keys = workload.fingerprints(max(n, 1), seed=40)[:n, 1].copy()
     assert len(np.unique(keys)) == n
     bits = np.random.default_rng(41).integers(0, 2, n, dtype=np.uint8)
-    seed, seg, words = engine.retrieval_build(keys, bits)
+    offsets, seeds, words = engine.ribbon_build(keys, bits)
     lib = load()
-    assert seg == lib.shgpu_retrieval_cells_per_third(n)
-    assert len(words) == (3 * seg + 63) // 64
-    if 3 * seg % 64:
-        assert int(words[-1]) >> (3 * seg % 64) == 0  # cells past the last are zero
-    assert np.array_equal(retrieve(keys, seed, seg, words), bits)
+    S = lib.shgpu_ribbon_shards(n)
+    assert len(seeds) == S and len(offsets) == S + 1
+    m = np.diff(offsets.astype(np.int64))
+    counts = np.bincount(fastrange64(remix(keys + SALT_SHARD), S).astype(np.int64), minlength=S) if n else np.zeros(S, int)
+    want_m = np.where(counts == 0, 0, np.maximum(64, counts + (counts + 19) // 20))
+    assert np.array_equal(m, want_m)  # slots per shard follow the documented rule
+    assert len(words) == (int(offsets[-1]) + 63) // 64 + 1
+    if n:
+        assert np.array_equal(retrieve(keys, offsets, seeds, words), bits)
+    if n >= 100_000:
+        bits_per_key = (64 * len(words) + 32 * len(offsets) + 8 * len(seeds)) / n
+        log(f"  {bits_per_key:.4f} bits/key, seeds used up to {int(seeds.max())}")
+        assert bits_per_key < 1.08
 
 
 def test_all_zero_and_all_one_bits(engine):
     keys = workload.fingerprints(50_000, seed=42)[:, 1].copy()
     for b in (0, 1):
         bits = np.full(len(keys), b, dtype=np.uint8)
-        seed, seg, words = engine.retrieval_build(keys, bits)
-        assert np.array_equal(retrieve(keys, seed, seg, words), bits)
+        offsets, seeds, words = engine.ribbon_build(keys, bits)
+        assert np.array_equal(retrieve(keys, offsets, seeds, words), bits)
 
 
-def test_duplicate_keys_are_reported(engine):
+def test_repeated_key_with_different_bits_is_reported(engine):
+    """A key present twice with different bits makes its shard inconsistent for every seed byte."""
     keys = workload.fingerprints(1000, seed=43)[:, 1].copy()
     keys[7] = keys[3]
+    bits = np.zeros(1000, dtype=np.uint8)
+    bits[7] = 1
     with pytest.raises(ShgpuError) as ei:
-        engine.retrieval_build(keys, np.zeros(1000, dtype=np.uint8))
+        engine.ribbon_build(keys, bits)
     assert ei.value.status == ERR_SEARCH_LIMIT
 
 
-def test_key_repeated_past_the_cell_limit_is_reported(engine):
-    """A key repeated 300 times puts 300 keys in its cells, past the 255 that a cell's index sum can count:
-    reported as an argument error at once, not searched over every seed."""
+def test_repeated_key_with_equal_bits_retrieves_it(engine):
+    """A repeated key with the same bit is a redundant equation: the structure still returns every bit."""
     keys = workload.fingerprints(1000, seed=44)[:, 1].copy()
     keys[:300] = keys[500]
-    with pytest.raises(ShgpuError) as ei:
-        engine.retrieval_build(keys, np.zeros(1000, dtype=np.uint8))
-    assert ei.value.status == ERR_ARG
+    bits = np.random.default_rng(45).integers(0, 2, 1000, dtype=np.uint8)
+    bits[:300] = bits[500]
+    offsets, seeds, words = engine.ribbon_build(keys, bits)
+    assert np.array_equal(retrieve(keys, offsets, seeds, words), bits)
