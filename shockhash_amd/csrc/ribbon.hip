@@ -11,11 +11,12 @@
 //    row = remix(g ^ RB_SALT_ROW) | 1; the key's bit is the parity of row AND slots [start, start + 64) of its
 //    shard (bit j of row covers slot start + j).
 // Construction, per shard (one wave each, independent): Gaussian elimination on the fly, the standard ribbon
-// insertion (rows kept in the shard's band, in LDS): a row meets the stored row of its leading slot, is XORed
-// with it and shifted to its next leading bit until it reaches a free slot; a row that vanishes with bit 1 makes
-// the system inconsistent. Then back substitution from the last slot, free slots set to 0. A shard whose system
-// is inconsistent (a few percent of shards at 1.05 slots per key) is solved again with its next seed byte by the
-// same wave; repeated keys with different bits fail for every seed (SHGPU_ERR_SEARCH_LIMIT).
+// insertion with keys taken in order of their start slots: a row meets the stored row of its leading slot, is XORed
+// with it and shifted to its next leading bit until it reaches a free slot; a row that vanishes with bit 1 makes the
+// system inconsistent. In start order every row stays inside its own 64 slots, so the live band fits the lanes of
+// one wave (k_rb_solve). Then back substitution from the last slot, free slots set to 0. A shard whose system is
+// inconsistent (a few percent of shards at 1.05 slots per key) is solved again with its next seed byte by the same
+// wave; repeated keys with different bits fail for every seed (SHGPU_ERR_SEARCH_LIMIT).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -36,19 +37,22 @@ __device__ __forceinline__ uint32_t rb_slots_dev(uint32_t n) { return n == 0 ? 0
 
 #define RB_GRID_STRIDE(i, n) for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
 
-// Shard of every key (the sort key) and its index (the value).
-__global__ void k_rb_shard(const uint64_t *__restrict__ keys, uint64_t n, uint64_t nshards, uint64_t *shard, uint64_t *idx) {
+// Sort key (shard << 1 | bit) and value (the key's hash h = remix(key + RB_SALT_SHARD)) of every key: sorted, the
+// shard's keys lie together with their bits and hashes, read in order by the solve.
+__global__ void k_rb_shard(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ bits, uint64_t n, uint64_t nshards,
+                           uint64_t *skey, uint64_t *hval) {
     RB_GRID_STRIDE(i, n) {
-        shard[i] = __umul64hi(shgpu::remix(keys[i] + RB_SALT_SHARD), nshards);
-        idx[i] = i;
+        const uint64_t h = shgpu::remix(keys[i] + RB_SALT_SHARD);
+        skey[i] = (__umul64hi(h, nshards) << 1) | (uint64_t)(bits[i] & 1u);
+        hval[i] = h;
     }
 }
 
-// key_begin[s] = index of shard s's first key in the sorted order (key_begin[S] = n).
-__global__ void k_rb_bounds(const uint64_t *__restrict__ sorted_shard, uint64_t n, uint64_t nshards, uint32_t *key_begin) {
+// key_begin[s] = index of shard s's first key in the sorted order (key_begin[S] = n); sorted keys are shard << 1 | bit.
+__global__ void k_rb_bounds(const uint64_t *__restrict__ sorted_key, uint64_t n, uint64_t nshards, uint32_t *key_begin) {
     RB_GRID_STRIDE(i, n + 1) {
-        const uint64_t b = i < n ? sorted_shard[i] : nshards;
-        const uint64_t lo = i > 0 ? sorted_shard[i - 1] + 1 : 0;
+        const uint64_t b = i < n ? sorted_key[i] >> 1 : nshards;
+        const uint64_t lo = i > 0 ? (sorted_key[i - 1] >> 1) + 1 : 0;
         for (uint64_t s = lo; s <= b && s <= nshards; s++) key_begin[s] = (uint32_t)i;
     }
 }
@@ -96,136 +100,195 @@ __global__ void k_rb_zero(uint64_t *words, uint64_t nwords, uint32_t *counter) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
 }
 
-// One wave per shard (blocks of 64 threads, grid-stride over shards), rows and bits of the shard's band in dynamic
-// LDS (9 bytes per slot). A shard whose system is inconsistent with its seed byte is solved again with the next one,
-// in the same wave; the seed byte that works goes to seeds[]. Shards without one (repeated keys with different bits)
-// are counted in *nfailed and write nothing. Solved shards OR their slot bits into words (zeroed beforehand).
-// The elimination is sequential per shard and runs on wave-uniform values: the lanes first hash 64 keys at once
-// (start slot, row, bit), then the wave inserts them one by one (readlane), walking a 64-slot window of the band
-// held in the lanes.
-// Back substitution reads 64 rows and bits per LDS access into the lanes and walks them by readlane.
-__global__ void __launch_bounds__(64) k_rb_solve(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ bits,
-                                                 const uint64_t *__restrict__ order, const uint32_t *__restrict__ key_begin,
+// Start slot and coefficient row of a key with hash h = remix(key + RB_SALT_SHARD) under seed byte d (d1 = d + 1),
+// nb = m - 63 start values.
+__device__ __forceinline__ void rb_hash(uint64_t h, uint64_t d1, uint32_t nb, uint32_t &start, uint64_t &row) {
+    const uint64_t g = shgpu::remix(h + d1 * GOLDEN);
+    start = (uint32_t)__umul64hi(g, (uint64_t)nb);
+    row = shgpu::remix(g ^ RB_SALT_ROW) | 1ull;
+}
+
+// One wave per shard (blocks of 64 threads, grid-stride over shards). Per seed byte d:
+//  1. Counting sort of the shard's keys by start slot, stable in the shard's key order: a histogram of u16 counts in
+//     LDS (32-bit atomics on the packed pairs), an exclusive scan, and a scatter whose rank among equal starts of one
+//     batch of 64 keys is counted from the lower lanes. The sorted rows and (start | bit << 31) go to srow/smeta.
+//  2. Insertion in start order. A row with start s reduced against rows of keys with starts <= s stays inside slots
+//     [s, s + 63], so the live band is a 64-slot window: slot p sits in lane p % 64 (its row and right-hand bit in
+//     two VGPRs), an elimination step is a readlane, and the band is never read from memory. Slots that fall below the window are final and go to band[] packed as (row >> 1) | bit << 63
+//     (row bit 0 is the pivot; a free slot is 0).
+//  3. Back substitution from the last slot over band[], 64 slots per load; the solution bits are ORed into words.
+// A shard whose system is inconsistent is solved again with the next seed byte; the byte that works goes to seeds[],
+// shards without one (repeated keys with different bits) are counted in *nfailed and write nothing to words.
+__global__ void __launch_bounds__(64) k_rb_solve(const uint64_t *__restrict__ hs, const uint64_t *__restrict__ sk,
+                                                 const uint32_t *__restrict__ key_begin,
                                                  const uint32_t *__restrict__ off, uint32_t nshards, uint8_t *seeds,
-                                                 uint32_t *nfailed, uint64_t *words) {
-    extern __shared__ uint64_t rows[];
+                                                 uint32_t *nfailed, uint64_t *words, uint64_t *srow, uint32_t *smeta,
+                                                 uint64_t *band) {
+    extern __shared__ uint32_t hist32[];  // u16 count / cursor per start value, two per word
+    uint16_t *hist = (uint16_t *)hist32;
     const uint32_t lane = shgpu::lane_id();
     for (uint32_t sh = blockIdx.x; sh < nshards; sh += gridDim.x) {
         const uint32_t k0 = key_begin[sh], k1 = key_begin[sh + 1];
         const uint32_t base_slot = off[sh], m = off[sh + 1] - base_slot;
         if (k1 == k0) continue;
-        uint8_t *bb = (uint8_t *)(rows + m);
-        bool ok = false;
+        const uint32_t nb = m - RB_WIDTH + 1, nw = (nb + 1) / 2;
+        uint64_t *sband = band + base_slot;
+        bool ok = false, lost = false;
         uint32_t d = 0;
-        for (; d < RB_SEEDS && !ok; d++) {
+        for (; d < RB_SEEDS && !ok && !lost; d++) {
             const uint64_t d1 = (uint64_t)d + 1;
-            for (uint32_t i = lane; i < m; i += 64) {
-                rows[i] = 0;
-                bb[i] = 0;
-            }
-            shgpu::wave_sync();
-            ok = true;
-            for (uint32_t kb = k0; kb < k1 && ok; kb += 64) {
-                // lane l hashes key kb + l: start slot, row, bit
-                uint32_t start = 0, bit = 0;
-                uint64_t row = 0;
+            // 1. counting sort by start
+            for (uint32_t i = lane; i < nw; i += 64) hist32[i] = 0;
+            __syncthreads();
+            for (uint32_t kb = k0; kb < k1; kb += 64) {
                 if (kb + lane < k1) {
-                    const uint64_t i = order[kb + lane];
-                    const uint64_t g = shgpu::remix(shgpu::remix(keys[i] + RB_SALT_SHARD) + d1 * GOLDEN);
-                    start = (uint32_t)__umul64hi(g, (uint64_t)(m - RB_WIDTH + 1));
-                    row = shgpu::remix(g ^ RB_SALT_ROW) | 1ull;
-                    bit = bits[i] & 1u;
+                    uint32_t s;
+                    uint64_t row;
+                    rb_hash(hs[kb + lane], d1, nb, s, row);
+                    atomicAdd(&hist32[s >> 1], 1u << ((s & 1) * 16));
                 }
+            }
+            __syncthreads();
+            {
+                const uint32_t cw = (nw + 63) / 64, w0 = min(nw, lane * cw), w1 = min(nw, w0 + cw);
+                uint32_t sum = 0;
+                for (uint32_t w = w0; w < w1; w++) {
+                    const uint32_t v = hist32[w];
+                    sum += (v & 0xFFFFu) + (v >> 16);
+                }
+                uint32_t incl = sum;  // inclusive scan over the lanes
+                for (uint32_t o = 1; o < 64; o <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+                    if (lane >= o) incl += y;
+                }
+                uint32_t run = incl - sum;
+                for (uint32_t w = w0; w < w1; w++) {
+                    const uint32_t v = hist32[w], lo = v & 0xFFFFu;
+                    hist32[w] = run | ((run + lo) << 16);
+                    run += lo + (v >> 16);
+                }
+            }
+            __syncthreads();
+            for (uint32_t kb = k0; kb < k1; kb += 64) {
                 const uint32_t cnt = min(64u, k1 - kb);
-                // The band window [base, base + 64) sits in the lanes (lane l: row and bit of slot base + l), so an
-                // elimination step is a readlane, not a dependent LDS read. The next key's window is loaded while
-                // this key is inserted and patched with this key's stored row if it falls inside.
-                uint32_t nstart = (uint32_t)__builtin_amdgcn_readlane((int)start, 0);
-                uint64_t nwr = rows[nstart + lane];  // start + 63 < m
-                uint32_t nwb = bb[nstart + lane];
+                const bool act = lane < cnt;
+                uint32_t s = 0xFFFFFFFFu, b = 0;
+                uint64_t row = 0;
+                if (act) {
+                    rb_hash(hs[kb + lane], d1, nb, s, row);
+                    b = (uint32_t)sk[kb + lane] & 1u;
+                }
+                uint32_t below = 0, above = 0;
                 for (uint32_t j = 0; j < cnt; j++) {
-                    uint32_t sj = nstart, base = nstart;
-                    uint64_t wr = nwr;
-                    uint32_t wb = nwb;
-                    if (j + 1 < cnt) {
-                        nstart = (uint32_t)__builtin_amdgcn_readlane((int)start, (int)j + 1);
-                        nwr = rows[nstart + lane];
-                        nwb = bb[nstart + lane];
+                    const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)s, (int)j);
+                    if (sj == s) {
+                        below += j < lane;
+                        above += j > lane;
                     }
-                    uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)bit, (int)j);
-                    uint64_t r = shgpu::readlane64(row, j);
-                    // eliminate against the stored rows until the row reaches a free slot or vanishes
-                    bool stored = false;
-                    for (;;) {
-                        uint32_t o = sj - base;
-                        if (o >= 64) {  // the row left the window: load the one starting at its leading slot
-                            base = sj;
-                            o = 0;
-                            wr = sj + lane < m ? rows[sj + lane] : 0;
-                            wb = sj + lane < m ? bb[sj + lane] : 0u;
-                        }
-                        const uint64_t c = shgpu::readlane64(wr, o);
-                        if (c == 0) {
-                            stored = true;
-                            break;
-                        }
-                        r ^= c;
-                        b ^= (uint32_t)__builtin_amdgcn_readlane((int)wb, (int)o);
-                        if (r == 0) break;
-                        const uint32_t t = (uint32_t)__builtin_ctzll(r);
-                        r >>= t;
-                        sj += t;
-                    }
-                    if (stored) {
-                        if (lane == 0) {
-                            rows[sj] = r;
-                            bb[sj] = (uint8_t)b;
-                        }
-                        if (j + 1 < cnt && lane == sj - nstart) {  // the next key's window holds slot sj
-                            nwr = r;
-                            nwb = b;
-                        }
-                    } else if (b) {
-                        ok = false;  // a redundant row is fine, a contradicting one is not
-                        break;
+                }
+                const uint32_t cur = act ? hist[s] : 0u;
+                shgpu::wave_sync();  // every lane has read its cursor before the cursors move
+                if (act && above == 0) hist[s] = (uint16_t)(cur + below + 1);
+                const uint32_t pos = cur + below;
+                if (act) {
+                    if (pos < k1 - k0) {
+                        srow[k0 + pos] = row;
+                        smeta[k0 + pos] = s | (b << 31);
+                    } else {
+                        lost = true;  // cannot happen with a consistent histogram; never write past the shard
                     }
                 }
                 shgpu::wave_sync();
             }
+            lost = __ballot(lost) != 0;
+            __syncthreads();  // sorted rows visible to every lane
+            if (lost) break;
+            // 2. insertion in start order over a 64-slot window: lane p % 64 holds slot p's row (0 when free, bit 0
+            //    set when occupied) and right-hand bit
+            uint64_t wrow = 0;
+            uint32_t wbit = 0, s0 = 0;
+            ok = true;
+            // slots [s0, s1) leave the window: stored packed, their lanes cleared; slots past the window are free
+            auto advance = [&](uint32_t s1) {
+                const uint32_t adv = s1 - s0, rel = (lane - s0) & 63u;
+                if (rel < adv) {
+                    sband[s0 + rel] = (wrow >> 1) | ((uint64_t)wbit << 63);
+                    wrow = 0;
+                    wbit = 0;
+                }
+                for (uint32_t qb = (s0 + 64) & ~63u; qb < s1; qb += 64) {
+                    const uint32_t q = qb + lane;  // lane q % 64 writes slot q, as it does in the window
+                    if (q >= s0 + 64 && q < s1) sband[q] = 0;
+                }
+                s0 = s1;
+            };
+            for (uint32_t kb = k0; kb < k1 && ok; kb += 64) {
+                const uint32_t cnt = min(64u, k1 - kb);
+                const uint64_t erow = lane < cnt ? srow[kb + lane] : 0;
+                const uint32_t emeta = lane < cnt ? smeta[kb + lane] : 0u;
+                for (uint32_t j = 0; j < cnt; j++) {
+                    const uint32_t meta = (uint32_t)__builtin_amdgcn_readlane((int)emeta, (int)j);
+                    const uint32_t s1 = meta & 0x7FFFFFFFu;
+                    uint32_t b = meta >> 31;
+                    uint64_t r = shgpu::readlane64(erow, j);
+                    if (s1 != s0) advance(s1);
+                    uint32_t sj = s1;
+                    uint64_t c = shgpu::readlane64(wrow, sj & 63u);
+                    while (c != 0) {
+                        r ^= c;
+                        b ^= (uint32_t)__builtin_amdgcn_readlane((int)wbit, (int)(sj & 63u));
+                        if (r == 0) break;
+                        const uint32_t t = (uint32_t)__builtin_ctzll(r);
+                        r >>= t;
+                        sj += t;
+                        if (sj - s1 >= RB_WIDTH) break;  // never taken in start order; bounds the loop regardless
+                        c = shgpu::readlane64(wrow, sj & 63u);
+                    }
+                    if (r == 0) {
+                        if (b) {
+                            ok = false;  // a redundant row is fine, a contradicting one is not
+                            break;
+                        }
+                    } else if (sj - s1 >= RB_WIDTH) {
+                        ok = false;
+                        lost = true;
+                        break;
+                    } else if (lane == (sj & 63u)) {  // a free slot
+                        wrow = r;
+                        wbit = b;
+                    }
+                }
+            }
+            if (!ok) continue;  // lost ends the seed loop
+            advance(m);
+            __syncthreads();  // the band visible to every lane
         }
         if (!ok) {
             if (lane == 0) atomicAdd(nfailed, 1u);
             continue;
         }
         if (lane == 0) seeds[sh] = (uint8_t)(d - 1);
-        // Back substitution from the last slot: zw holds z[s + 1 .. s + 64] in bits 0..63. Rows and bits of slots
-        // [blk, blk + 64) are read into the lanes (lane l: slot blk + l) and walked downwards by readlane.
-        uint64_t zw = 0, acc = 0;
-        uint64_t word = (base_slot + m - 1) >> 6;
+        // 3. back substitution: acc holds z[s + 1 .. s + 64] in bits 0..63 before slot s; a slot's bit is
+        //    parity(v & (z[s + 1 .. s + 63] | bit 63)), v = (row >> 1) | rhs << 63
+        uint64_t acc = 0;
         for (int32_t top = (int32_t)m; top > 0; top -= 64) {
             const int32_t blk = top - 64 > 0 ? top - 64 : 0;
-            const int32_t mine = blk + (int32_t)lane;
-            const uint64_t rl = mine < top ? rows[mine] : 0;
-            const uint32_t bl = mine < top ? bb[mine] : 0u;
+            const uint64_t vl = blk + (int32_t)lane < top ? sband[blk + lane] : 0;
             for (int32_t s = top - 1; s >= blk; s--) {
-                const uint64_t c = shgpu::readlane64(rl, (uint32_t)(s - blk));
-                uint64_t z = 0;
-                if (c) {
-                    const uint32_t cb = (uint32_t)__builtin_amdgcn_readlane((int)bl, s - blk);
-                    z = (cb ^ (uint64_t)__builtin_popcountll((c >> 1) & zw)) & 1ull;
-                }
-                zw = (zw << 1) | z;
-                const uint64_t p = base_slot + (uint64_t)s;
-                if ((p >> 6) != word) {
-                    if (acc && lane == 0) atomicOr((unsigned long long *)&words[word], (unsigned long long)acc);
-                    acc = 0;
-                    word = p >> 6;
-                }
-                acc |= z << (p & 63);
+                const uint64_t v = shgpu::readlane64(vl, (uint32_t)(s - blk));
+                const uint64_t z = (uint64_t)__builtin_popcountll(v & (acc | (1ull << 63))) & 1ull;
+                acc = (acc << 1) | z;
+            }
+            // acc bits 0 .. top - blk - 1 are the slots blk .. top - 1
+            const uint32_t nbits = (uint32_t)(top - blk);
+            const uint64_t zb = nbits == 64 ? acc : acc & ((1ull << nbits) - 1);
+            const uint64_t g = (uint64_t)base_slot + (uint64_t)blk, w = g >> 6, sft = g & 63;
+            if (lane == 0 && zb) {
+                atomicOr((unsigned long long *)&words[w], (unsigned long long)(zb << sft));
+                if (sft && (zb >> (64 - sft))) atomicOr((unsigned long long *)&words[w + 1], (unsigned long long)(zb >> (64 - sft)));
             }
         }
-        if (acc && lane == 0) atomicOr((unsigned long long *)&words[word], (unsigned long long)acc);
-        shgpu::wave_sync();
+        __syncthreads();
     }
 }
 
@@ -252,8 +315,8 @@ int shgpu_ribbon_build_impl(hipStream_t st, const uint64_t *d_keys, const uint8_
     if (n >= (1ull << 32)) { err = "ribbon retrieval: more than 2^32 - 1 keys"; return SHGPU_ERR_UNSUPPORTED; }
     const uint64_t maxw = shgpu_ribbon_max_words(n);
     if (words_capacity < maxw) { err = "ribbon retrieval: output capacity below " + std::to_string(maxw) + " words"; return SHGPU_ERR_ARG; }
-    // one allocation: shard ids, indices, sorted shard ids, sorted indices (n u64 each), sort scratch, key_begin and
-    // offsets (S + 2 u32 each), seeds (S), counters, words
+    // one allocation: sort keys (shard << 1 | bit), hashes, both sorted (n u64 each), sort scratch, key_begin and
+    // offsets (S + 2 u32 each), seeds (S), counters, words, band (u64 per slot)
     size_t sort_bytes = 0;
     if (shgpu_sort_pairs_u64(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, (int)std::max<uint64_t>(n, 1), 64, st) !=
         hipSuccess) {
@@ -261,7 +324,7 @@ int shgpu_ribbon_build_impl(hipStream_t st, const uint64_t *d_keys, const uint8_
         return SHGPU_ERR_HIP;
     }
     const size_t parts[] = {(n + 1) * 8, (n + 1) * 8, (n + 1) * 8, (n + 1) * 8, sort_bytes + 16, (S + 2) * 4, (S + 2) * 4,
-                            S + 8, 64, maxw * 8};
+                            S + 8, 64, maxw * 8, maxw * 64 * 8};
     size_t bytes = 0;
     for (size_t b : parts) bytes += (b + 255) / 256 * 256;
     RbBuf own;
@@ -282,34 +345,39 @@ int shgpu_ribbon_build_impl(hipStream_t st, const uint64_t *d_keys, const uint8_
         if (!shgpu::device_malloc(&own.p, bytes)) { err = "ribbon retrieval: hipMalloc of " + std::to_string(bytes) + " bytes failed"; return SHGPU_ERR_NOMEM; }
         base = (uint8_t *)own.p;
     }
-    void *ptr[10];
+    void *ptr[11];
     size_t at = 0;
-    for (int i = 0; i < 10; i++) {
+    for (int i = 0; i < 11; i++) {
         ptr[i] = base + at;
         at += (parts[i] + 255) / 256 * 256;
     }
-    uint64_t *shard = (uint64_t *)ptr[0], *idx = (uint64_t *)ptr[1], *shard_sorted = (uint64_t *)ptr[2],
-             *order = (uint64_t *)ptr[3];
+    uint64_t *skey = (uint64_t *)ptr[0], *hval = (uint64_t *)ptr[1], *skey_sorted = (uint64_t *)ptr[2],
+             *h_sorted = (uint64_t *)ptr[3];
     void *sort_tmp = ptr[4];
     uint32_t *key_begin = (uint32_t *)ptr[5], *off = (uint32_t *)ptr[6];
     uint8_t *seeds = (uint8_t *)ptr[7];
     uint32_t *nfailed = (uint32_t *)ptr[8];
     uint64_t *words = (uint64_t *)ptr[9];
+    uint64_t *band = (uint64_t *)ptr[10];  // one packed row per slot (at most 64 * maxw slots)
+    // after the shard sort, the unsorted arrays are dead: the solve's start-sorted rows and starts take their place
+    uint64_t *srow = skey;
+    uint32_t *smeta = (uint32_t *)hval;
 #define RB_HIP(call)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
         if (e_ != hipSuccess) { err = std::string("ribbon retrieval: ") + hipGetErrorString(e_); return SHGPU_ERR_HIP; } \
     } while (0)
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256 + 1, 4096);
-    int bits = 1;
-    while (bits < 64 && (1ull << bits) < S) bits++;
+    int bits = 1;  // sort key bits: the shard id's and the retrieved bit
+    while (bits < 63 && (1ull << bits) < S) bits++;
+    bits++;
     if (n) {
-        k_rb_shard<<<grid, 256, 0, st>>>(d_keys, n, S, shard, idx);
+        k_rb_shard<<<grid, 256, 0, st>>>(d_keys, d_bits, n, S, skey, hval);
         RB_HIP(hipGetLastError());
         size_t sb = sort_bytes;
-        RB_HIP(shgpu_sort_pairs_u64(sort_tmp, sb, shard, shard_sorted, idx, order, (int)n, bits, st));
+        RB_HIP(shgpu_sort_pairs_u64(sort_tmp, sb, skey, skey_sorted, hval, h_sorted, (int)n, bits, st));
     }
-    k_rb_bounds<<<grid, 256, 0, st>>>(shard_sorted, n, S, key_begin);
+    k_rb_bounds<<<grid, 256, 0, st>>>(skey_sorted, n, S, key_begin);
     RB_HIP(hipGetLastError());
     k_rb_offsets<<<1, RB_SCAN_THREADS, 0, st>>>(key_begin, (uint32_t)S, off, seeds);
     RB_HIP(hipGetLastError());
@@ -319,16 +387,17 @@ int shgpu_ribbon_build_impl(hipStream_t st, const uint64_t *d_keys, const uint8_
     uint32_t max_slots = 0;
     RB_HIP(hipMemcpyAsync(&max_slots, off + S + 1, 4, hipMemcpyDeviceToHost, st));
     RB_HIP(hipStreamSynchronize(st));
-    const size_t lds = (size_t)max_slots * 9 + 16;
-    if (lds > RB_MAX_LDS) {
-        err = "ribbon retrieval: a shard holds " + std::to_string(max_slots) + " slots, more than fit in LDS (repeated keys?)";
+    if (max_slots > RB_MAX_SLOTS) {
+        err = "ribbon retrieval: a shard holds " + std::to_string(max_slots) + " slots, more than its u16 start counts allow (repeated keys?)";
         return SHGPU_ERR_SEARCH_LIMIT;
     }
+    const size_t lds = max_slots >= RB_WIDTH ? (size_t)(max_slots - RB_WIDTH + 2) / 2 * 4 + 16 : 16;
     const uint32_t total_slots = h_offsets[S];
     const uint64_t nwords = (uint64_t)(total_slots + 63) / 64 + 1;  // one word of padding for the 64-slot windows
-    // every shard in one launch; LDS bounds the waves per CU, so one block per shard is resident or queued
+    // every shard in one launch, one block of one wave per shard
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(S, 1u << 20));
-    k_rb_solve<<<blocks, 64, lds, st>>>(d_keys, d_bits, order, key_begin, off, (uint32_t)S, seeds, nfailed, words);
+    k_rb_solve<<<blocks, 64, lds, st>>>(h_sorted, skey_sorted, key_begin, off, (uint32_t)S, seeds, nfailed, words, srow,
+                                        smeta, band);
     RB_HIP(hipGetLastError());
     uint32_t nf = 0;
     RB_HIP(hipMemcpyAsync(&nf, nfailed, 4, hipMemcpyDeviceToHost, st));
