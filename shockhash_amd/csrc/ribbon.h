@@ -13,6 +13,7 @@ constexpr uint64_t RB_SHARD_KEYS = SHGPU_RIBBON_SHARD_KEYS;
 constexpr uint64_t RB_SALT_SHARD = SHGPU_RIBBON_SALT_SHARD;
 constexpr uint64_t RB_SALT_ROW = SHGPU_RIBBON_SALT_ROW;
 constexpr uint32_t RB_MAX_SLOTS = 65535;  // slots of one shard: its start counts and cursors are u16
+constexpr uint32_t RB_WAVES_PER_CU = 16;  // persistent solve waves per CU
 
 // Builds the structure for n keys and their bits (device arrays) on stream s; writes the S + 1 slot offsets, the S
 // seed bytes and the packed slot bits (*words_used words, at most shgpu_ribbon_max_words(n)) to host memory.

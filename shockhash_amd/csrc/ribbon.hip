@@ -95,10 +95,14 @@ __global__ void __launch_bounds__(RB_SCAN_THREADS) k_rb_offsets(const uint32_t *
     }
 }
 
-__global__ void k_rb_zero(uint64_t *words, uint64_t nwords, uint32_t *counter) {
+// Clears the slot words and the two counters (failed shards, shard queue).
+__global__ void k_rb_zero(uint64_t *words, uint64_t nwords, uint32_t *counters) {
     RB_GRID_STRIDE(i, nwords) words[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 2) counters[threadIdx.x] = 0;
 }
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, uint32_t k) { return k ? (x << k) | (x >> (64 - k)) : x; }
+__device__ __forceinline__ uint64_t rotr64(uint64_t x, uint32_t k) { return k ? (x >> k) | (x << (64 - k)) : x; }
 
 // Start slot and coefficient row of a key with hash h = remix(key + RB_SALT_SHARD) under seed byte d (d1 = d + 1),
 // nb = m - 63 start values.
@@ -108,26 +112,29 @@ __device__ __forceinline__ void rb_hash(uint64_t h, uint64_t d1, uint32_t nb, ui
     row = shgpu::remix(g ^ RB_SALT_ROW) | 1ull;
 }
 
-// One wave per shard (blocks of 64 threads, grid-stride over shards). Per seed byte d:
+// Persistent waves (blocks of 64 threads) take shards from a queue counter, one at a time. Per seed byte d:
 //  1. Counting sort of the shard's keys by start slot, stable in the shard's key order: a histogram of u16 counts in
 //     LDS (32-bit atomics on the packed pairs), an exclusive scan, and a scatter whose rank among equal starts of one
 //     batch of 64 keys is counted from the lower lanes. The sorted rows and (start | bit << 31) go to srow/smeta.
 //  2. Insertion in start order. A row with start s reduced against rows of keys with starts <= s stays inside slots
-//     [s, s + 63], so the live band is a 64-slot window: slot p sits in lane p % 64 (its row and right-hand bit in
-//     two VGPRs), an elimination step is a readlane, and the band is never read from memory. Slots that fall below the window are final and go to band[] packed as (row >> 1) | bit << 63
-//     (row bit 0 is the pivot; a free slot is 0).
+//     [s, s + 63], so the live band is a 64-slot window held in the lanes, transposed and kept in reduced row
+//     echelon form: a key's elimination is one ballot instead of a chain of dependent steps, and the band is never
+//     read from memory. Slots that fall below the window are final and go to band[] packed as
+//     (row >> 1) | bit << 63 (row relative to its pivot, whose bit 0 it drops; a free slot is 0).
 //  3. Back substitution from the last slot over band[], 64 slots per load; the solution bits are ORed into words.
 // A shard whose system is inconsistent is solved again with the next seed byte; the byte that works goes to seeds[],
 // shards without one (repeated keys with different bits) are counted in *nfailed and write nothing to words.
 __global__ void __launch_bounds__(64) k_rb_solve(const uint64_t *__restrict__ hs, const uint64_t *__restrict__ sk,
                                                  const uint32_t *__restrict__ key_begin,
                                                  const uint32_t *__restrict__ off, uint32_t nshards, uint8_t *seeds,
-                                                 uint32_t *nfailed, uint64_t *words, uint64_t *srow, uint32_t *smeta,
-                                                 uint64_t *band) {
+                                                 uint32_t *nfailed, uint32_t *queue, uint64_t *words, uint64_t *srow,
+                                                 uint32_t *smeta, uint64_t *band) {
     extern __shared__ uint32_t hist32[];  // u16 count / cursor per start value, two per word
     uint16_t *hist = (uint16_t *)hist32;
     const uint32_t lane = shgpu::lane_id();
-    for (uint32_t sh = blockIdx.x; sh < nshards; sh += gridDim.x) {
+    for (;;) {
+        const uint32_t sh = shgpu::wave_fetch_add(queue, 1u);
+        if (sh >= nshards) break;
         const uint32_t k0 = key_begin[sh], k1 = key_begin[sh + 1];
         const uint32_t base_slot = off[sh], m = off[sh + 1] - base_slot;
         if (k1 == k0) continue;
@@ -203,19 +210,32 @@ __global__ void __launch_bounds__(64) k_rb_solve(const uint64_t *__restrict__ hs
             lost = __ballot(lost) != 0;
             __syncthreads();  // sorted rows visible to every lane
             if (lost) break;
-            // 2. insertion in start order over a 64-slot window: lane p % 64 holds slot p's row (0 when free, bit 0
-            //    set when occupied) and right-hand bit
-            uint64_t wrow = 0;
-            uint32_t wbit = 0, s0 = 0;
+            // 2. insertion in start order over a 64-slot window kept in reduced row echelon form (every live row is 0
+            //    at every other pivot). Slot p is bit p % 64 of the window's masks; rows are rotated so that bit q % 64
+            //    stands for slot q, and held transposed: lane j holds column j (bit l: bit j of the row whose pivot
+            //    is at window bit l). A new row then meets each pivot at most once, and all at once: its reduction
+            //    is a ballot of the parities of (column AND covered pivots); it takes its lowest remaining slot as
+            //    pivot and is cleared from the other rows by one XOR per lane. P: pivot slots, W: right-hand bits.
+            uint64_t col = 0, P = 0, W = 0;
+            uint32_t s0 = 0;
+            const uint64_t lanebit = 1ull << lane;
             ok = true;
-            // slots [s0, s1) leave the window: stored packed, their lanes cleared; slots past the window are free
+            // slots [s0, s1) leave the window: their rows are stored relative to their pivot (packed, one store per
+            // lane), and cleared from the columns; slots past the window are free
             auto advance = [&](uint32_t s1) {
-                const uint32_t adv = s1 - s0, rel = (lane - s0) & 63u;
-                if (rel < adv) {
-                    sband[s0 + rel] = (wrow >> 1) | ((uint64_t)wbit << 63);
-                    wrow = 0;
-                    wbit = 0;
+                const uint32_t adv = s1 - s0, nout = min(adv, 64u);
+                uint64_t gone = 0, outv = 0;
+                for (uint32_t k = 0; k < nout; k++) {
+                    const uint32_t l = (s0 + k) & 63u;
+                    const uint64_t row = __ballot((col >> l) & 1ull);
+                    const uint64_t v = (rotr64(row, l) >> 1) | (((W >> l) & 1ull) << 63);
+                    if (lane == l) outv = v;
+                    gone |= 1ull << l;
                 }
+                if (gone & lanebit) sband[s0 + ((lane - s0) & 63u)] = outv;
+                col &= ~gone;
+                P &= ~gone;
+                W &= ~gone;
                 for (uint32_t qb = (s0 + 64) & ~63u; qb < s1; qb += 64) {
                     const uint32_t q = qb + lane;  // lane q % 64 writes slot q, as it does in the window
                     if (q >= s0 + 64 && q < s1) sband[q] = 0;
@@ -224,39 +244,30 @@ __global__ void __launch_bounds__(64) k_rb_solve(const uint64_t *__restrict__ hs
             };
             for (uint32_t kb = k0; kb < k1 && ok; kb += 64) {
                 const uint32_t cnt = min(64u, k1 - kb);
-                const uint64_t erow = lane < cnt ? srow[kb + lane] : 0;
                 const uint32_t emeta = lane < cnt ? smeta[kb + lane] : 0u;
+                const uint64_t erot = lane < cnt ? rotl64(srow[kb + lane], emeta & 63u) : 0;  // row in window bits
                 for (uint32_t j = 0; j < cnt; j++) {
                     const uint32_t meta = (uint32_t)__builtin_amdgcn_readlane((int)emeta, (int)j);
                     const uint32_t s1 = meta & 0x7FFFFFFFu;
                     uint32_t b = meta >> 31;
-                    uint64_t r = shgpu::readlane64(erow, j);
+                    const uint64_t r0 = shgpu::readlane64(erot, j);
                     if (s1 != s0) advance(s1);
-                    uint32_t sj = s1;
-                    uint64_t c = shgpu::readlane64(wrow, sj & 63u);
-                    while (c != 0) {
-                        r ^= c;
-                        b ^= (uint32_t)__builtin_amdgcn_readlane((int)wbit, (int)(sj & 63u));
-                        if (r == 0) break;
-                        const uint32_t t = (uint32_t)__builtin_ctzll(r);
-                        r >>= t;
-                        sj += t;
-                        if (sj - s1 >= RB_WIDTH) break;  // never taken in start order; bounds the loop regardless
-                        c = shgpu::readlane64(wrow, sj & 63u);
-                    }
+                    const uint64_t x = r0 & P;  // pivots the row covers
+                    const uint64_t r = r0 ^ __ballot(__builtin_popcountll(col & x) & 1);
+                    b ^= (uint32_t)__builtin_popcountll(x & W) & 1u;
                     if (r == 0) {
                         if (b) {
                             ok = false;  // a redundant row is fine, a contradicting one is not
                             break;
                         }
-                    } else if (sj - s1 >= RB_WIDTH) {
-                        ok = false;
-                        lost = true;
-                        break;
-                    } else if (lane == (sj & 63u)) {  // a free slot
-                        wrow = r;
-                        wbit = b;
+                        continue;
                     }
+                    const uint32_t pl = (s1 + (uint32_t)__builtin_ctzll(rotr64(r, s1 & 63u))) & 63u;  // new pivot
+                    const uint64_t has = shgpu::readlane64(col, pl);  // rows with a 1 at the new pivot
+                    if (r & lanebit) col ^= has | (1ull << pl);
+                    if (b) W ^= has;
+                    P |= 1ull << pl;
+                    W |= (uint64_t)b << pl;
                 }
             }
             if (!ok) continue;  // lost ends the seed loop
@@ -394,10 +405,14 @@ int shgpu_ribbon_build_impl(hipStream_t st, const uint64_t *d_keys, const uint8_
     const size_t lds = max_slots >= RB_WIDTH ? (size_t)(max_slots - RB_WIDTH + 2) / 2 * 4 + 16 : 16;
     const uint32_t total_slots = h_offsets[S];
     const uint64_t nwords = (uint64_t)(total_slots + 63) / 64 + 1;  // one word of padding for the 64-slot windows
-    // every shard in one launch, one block of one wave per shard
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(S, 1u << 20));
-    k_rb_solve<<<blocks, 64, lds, st>>>(h_sorted, skey_sorted, key_begin, off, (uint32_t)S, seeds, nfailed, words, srow,
-                                        smeta, band);
+    // Persistent waves, RB_WAVES_PER_CU per CU, take shards from the queue: every CU gets the same load even when the
+    // dispatcher would pack one-shard blocks unevenly, and a shard solved again runs while other shards go on.
+    int dev = 0, cus = 0;
+    RB_HIP(hipGetDevice(&dev));
+    RB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(S, (uint64_t)std::max(cus, 1) * RB_WAVES_PER_CU));
+    k_rb_solve<<<blocks, 64, lds, st>>>(h_sorted, skey_sorted, key_begin, off, (uint32_t)S, seeds, nfailed, nfailed + 1, words,
+                                        srow, smeta, band);
     RB_HIP(hipGetLastError());
     uint32_t nf = 0;
     RB_HIP(hipMemcpyAsync(&nf, nfailed, 4, hipMemcpyDeviceToHost, st));
