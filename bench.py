@@ -315,8 +315,12 @@ def main():
     headline = V == "sh2" and L == 64
     if tune:
         eng.set_variant_tuning(V, **tune)
+    vidx = {"sh1": 0, "simd": 1, "sh2": 2}[V]
+    sched_class = next((c for c in eng.schedule if c["variant"] == vidx and args.n <= c["max_keys"]),
+                       [c for c in eng.schedule if c["variant"] == vidx][-1])
     log(f"device {eng.arch} CUs {eng.cu_count} clock {eng.clock_khz / 1e6:.2f} GHz; "
-        f"{VARIANT_NAMES[V]} schedule {eng.variant_tuning(V)}")
+        f"{VARIANT_NAMES[V]} tuning {eng.variant_tuning(V)}; schedule from {eng.schedule_source}, "
+        f"class for {args.n} keys {sched_class}")
     h = workload.fingerprints(args.n, seed=1)
     d_h = eng.array(h.reshape(-1))  # fingerprints resident in HBM
     log(f"{args.n} fingerprints resident on the device")
@@ -424,7 +428,10 @@ def main():
         "config": {"workload": f"{VARIANT_NAMES[V]} leaf {L}, bucket 2000, "
                                f"{f'{args.n // 1_000_000}M' if args.n % 1_000_000 == 0 else args.n} uniform 128-bit fingerprints "
                                "(SplitMix64, seed 1), persistent device-queue build", "keys": args.n, "leaf_size": L,
-                   "bucket_size": 2000, "variant": VARIANT_NAMES[V]},
+                   "bucket_size": 2000, "variant": VARIANT_NAMES[V],
+                   "schedule": {"source": os.path.basename(eng.schedule_source),
+                                "waves_per_cu": tune.get("waves_per_cu") or sched_class["waves_per_cu"],
+                                "tuned_class": None if tune.get("waves_per_cu") else sched_class}},
         "plain_form": {"keys_per_s": plain_value, "buckets": args.plain_buckets, "keys": n_plain_keys,
                        "codes_equal_persistent": plain_match},
         "speedup_vs_plain": value / plain_value if plain_value else None,

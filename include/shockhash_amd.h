@@ -58,9 +58,9 @@ typedef struct shgpu_build shgpu_build;
 typedef struct shgpu_tuning {
     uint32_t sh1_seeds_per_lane;    /* 1, 2 or 4: ShockHash1 leaf window = 64 * this */
     uint32_t split_seeds_per_lane;  /* 1, 2 or 4: split window for aggregation nodes */
-    uint32_t waves_per_cu;          /* persistent grid = CUs * this; 0: by the batch's key count (4 below
-                                       350k keys, 8 below 1.5M, 12 below 3M, 16 below 5M, else 24; the
-                                       ShockHash and ShockHash2 default, 24 for SIMDShockHash) */
+    uint32_t waves_per_cu;          /* persistent grid = CUs * this; 0 (the default): waves per CU and both
+                                       seeds-per-lane values come from the engine's schedule class for the
+                                       batch's key count (shgpu_schedule_class) */
     uint32_t flags;                 /* SHGPU_TUNE_* bits, 0 by default */
     uint64_t max_trials;            /* seed bound per split node / ShockHash1 leaf */
 } shgpu_tuning;
@@ -96,6 +96,32 @@ int shgpu_get_variant_tuning(shgpu_engine *e, int variant, shgpu_tuning *out);
 int shgpu_set_variant_tuning(shgpu_engine *e, int variant, const shgpu_tuning *t);
 /* Synchronises the engine's stream. */
 int shgpu_sync(shgpu_engine *e);
+
+/* Build schedule by size class: a build of a variant whose tuning has waves_per_cu 0 takes its waves per CU and
+ * seeds per lane from the variant's first class with max_keys >= the batch's key count (the variant's largest
+ * class takes larger batches). shgpu_engine_create loads the table from the JSON file named by the environment
+ * variable SHGPU_SCHEDULE ("none": the built-in table), else from schedule_<arch>.json beside the library or in
+ * its parent directory, else uses the built-in hand-measured table. No reference counterpart: the reference's
+ * CPU build has no launch schedule. */
+typedef struct shgpu_schedule_class {
+    uint32_t variant;               /* SHGPU_SHOCKHASH / SHGPU_SIMD_SHOCKHASH / SHGPU_SHOCKHASH2 */
+    uint32_t waves_per_cu;          /* 1..32 */
+    uint32_t sh1_seeds_per_lane;    /* 1, 2 or 4 */
+    uint32_t split_seeds_per_lane;  /* 1, 2 or 4 */
+    uint64_t max_keys;              /* batches of at most this many keys */
+    double tuned_ms;                /* search kernel ms of the tuning sample under this class (0: not tuned) */
+    uint64_t sample_keys;           /* keys of that sample (0: not tuned) */
+} shgpu_schedule_class;
+/* Copies the engine's classes (sorted by variant, then max_keys); *count = their number. */
+int shgpu_schedule_get(shgpu_engine *e, shgpu_schedule_class *out, uint32_t capacity, uint32_t *count);
+/* Replaces the classes of every variant named in `classes`. */
+int shgpu_schedule_set(shgpu_engine *e, const shgpu_schedule_class *classes, uint32_t count);
+int shgpu_schedule_save(shgpu_engine *e, const char *path);
+int shgpu_schedule_load(shgpu_engine *e, const char *path);
+/* Where the engine's table came from: a file path or "built-in". */
+const char *shgpu_schedule_source(shgpu_engine *e);
+/* Parses the JSON form without a device (what shgpu_schedule_load reads). */
+int shgpu_schedule_parse(const char *json, shgpu_schedule_class *out, uint32_t capacity, uint32_t *count);
 
 /* Free and total bytes of the engine's device (hipMemGetInfo), e.g. to size a serving HBM budget. */
 int shgpu_device_memory(shgpu_engine *e, size_t *free_bytes, size_t *total_bytes);
@@ -159,11 +185,18 @@ typedef struct shgpu_build_desc {
 
 int shgpu_build_run(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
                     shgpu_build **out);
-/* Autotuner: builds the given sample with every candidate schedule (seeds per lane for leaf and
- * split windows, waves per CU), keeps the fastest for that variant and returns it. Results are
- * identical for every schedule; only the time differs. d_hashes: device fingerprints. */
+/* Autotuner for one explicit schedule: the sweep of shgpu_autotune_class (below); the winner becomes the
+ * variant's tuning (waves_per_cu set, so the size classes no longer apply to the variant) and is returned.
+ * Results are identical for every schedule; only the time differs. d_hashes: device fingerprints. */
 int shgpu_autotune(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
                    shgpu_tuning *best, double *best_ms);
+/* Autotunes one class: builds the sample (device fingerprints) under candidate schedules in two stages, waves
+ * per CU in {4, 8, 12, 16, 24} at the default seeds per lane, then every pair of leaf seeds per lane {1, 2, 4}
+ * (ShockHash1 leaves only) and split seeds per lane {1, 2, 4} at the fastest wave count, twice each (once when a
+ * build takes over a second); checks that every candidate gives the same codes (SHGPU_ERR_HIP otherwise), and
+ * stores the one with the shortest search kernel as the variant's class for batches up to max_keys keys. */
+int shgpu_autotune_class(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
+                         uint64_t max_keys, shgpu_schedule_class *chosen);
 /* Same with the fingerprints in host memory (staged to the device by the engine). */
 int shgpu_build_run_host(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *h_hashes, uint64_t n,
                          shgpu_build **out);

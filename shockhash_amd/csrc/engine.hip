@@ -10,9 +10,14 @@
 #include <thread>
 #include <unordered_map>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
+
+#include <dlfcn.h>
 
 #include "../../include/shockhash_amd.h"
 #include "../include/shockhash/GolombMemo.h"
@@ -20,6 +25,7 @@
 #include "search_sh2.h"
 #include "orientation.h"
 #include "ribbon.h"
+#include "schedule.h"
 #include "sort.h"
 #include "strings.h"
 
@@ -176,12 +182,11 @@ struct shgpu_engine {
     int cu_count = 0;
     int clock_khz = 0;
     char arch[64] = {0};
-    // Schedule per variant (ShockHash, SIMDShockHash, ShockHash2); the defaults are the measured best on
-    // MI355X for leaf 30 / 58 / 64 builds. waves_per_cu 0 selects the variant's rule (waves_for): by the
-    // batch's key count for ShockHash and ShockHash2, 24 for SIMDShockHash, whose leaves carry ~1000x the
-    // work per key.
+    // Schedule per variant (ShockHash, SIMDShockHash, ShockHash2). waves_per_cu 0 (the default) takes waves per
+    // CU and seeds per lane from the size class of `sched` for the batch's key count.
     shgpu_tuning tuning_v[3] = {{4, 2, 0, 0, DEFAULT_MAX_TRIALS}, {4, 2, 0, 0, DEFAULT_MAX_TRIALS},
                                 {4, 2, 0, 0, DEFAULT_MAX_TRIALS}};
+    shgpu::Schedule sched = shgpu::Schedule::builtin();
     shgpu_tuning tuning{4, 2, 0, 0, DEFAULT_MAX_TRIALS};  // batched per-node entry points (waves 0: 24 per CU)
     DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
     BuildBuffers main;                          // shgpu_build_run
@@ -709,25 +714,15 @@ int check_variant(int variant, uint32_t leaf) {
 }
 
 // Resident waves per CU of a persistent build: the schedule's value, or by the batch's key count when it is 0.
-// Measured on ShockHash2 leaf 64, bucket 2000, kernel ms at 4 / 8 / 12 / 16 / 24 waves per CU: 200k keys
-// 2.5 / 2.8 / 3.7 / 4.7 / 6.3, 500k 4.2 / 3.8 / - / - / -, 1M 6.6 / 5.1 / 5.3 / 5.8 / 7.6, 2M 12.6 / 8.7 /
-// 7.9 / 8.0 / 10.2, 4M - / - / 14.0 / 13.4 / 14.4, 6M - / - / 20.1 / 19.2 / 19.1, 10M - / 38.2 / 33.0 /
-// 30.9 / 29.7. ShockHash leaf 30 has the same optimum at every size (200k: 1.9 ms at 4 waves, 3.7 at 12;
-// 1M: 5.0 at 8, 8.1 at 24; 2M: 8.1 at 12; 4M: 14.0 at 16; 10M: 32.5 at 24, 38.4 at 8). Small builds are set
-// by their longest chain of dependent searches, and fewer resident
-// waves (fewer idle pollers) leave each searching wave more of its SIMD and of the memory system; large
-// ones need the waves to hide latency.
-uint32_t waves_for(const shgpu_tuning &t, int variant, uint64_t n_keys) {
-    if (t.waves_per_cu) return t.waves_per_cu;
-    if (variant == SHGPU_SIMD_SHOCKHASH) return 24u;
-    return n_keys < 350000 ? 4u : n_keys < 1500000 ? 8u : n_keys < 3000000 ? 12u : n_keys < 5000000 ? 16u : 24u;
-}
-
-int grid_for(const shgpu_engine *e, const shgpu_tuning &t, int variant, uint64_t n_keys) {
-    return std::max(1, e->cu_count * (int)waves_for(t, variant, n_keys) / WAVES_PER_BLOCK);
+// Persistent grid of a schedule: CUs x waves per CU (a tuning resolved by resolve_tuning names its count; the
+// batched per-node entry points default to 24). Small builds are set by their longest chain of dependent searches,
+// and fewer resident waves (fewer idle pollers) leave each searching wave more of its SIMD and of the memory
+// system; large ones need the waves to hide latency: the size classes of the schedule carry that choice.
+int grid_for(const shgpu_engine *e, const shgpu_tuning &t) {
+    return std::max(1, e->cu_count * (int)(t.waves_per_cu ? t.waves_per_cu : 24u) / WAVES_PER_BLOCK);
 }
 // Batched per-node entry points: the engine-wide schedule, 24 waves per CU unless it names a count.
-int grid_for(const shgpu_engine *e) { return grid_for(e, e->tuning, -1, ~0ull); }
+int grid_for(const shgpu_engine *e) { return grid_for(e, e->tuning); }
 
 hipStream_t pick(shgpu_engine *e, void *stream) { return stream ? (hipStream_t)stream : e->stream; }
 
@@ -786,10 +781,24 @@ int sh2_grid_cap(const Sh2Layout &l, int grid) {
 
 int ensure_sh2_scratch(shgpu_engine *e, size_t waves, const Sh2Layout &l) { return e->sh2_scratch.ensure(waves * l.wave_bytes); }
 
+static int load_default_schedule(shgpu_engine *e);  // defined with the schedule API below
+
+// A variant's schedule for a batch of n keys: its tuning, with waves per CU and seeds per lane from the size class
+// when the tuning leaves waves_per_cu at 0.
+shgpu_tuning resolve_tuning(const shgpu_engine *e, int variant, uint64_t n) {
+    shgpu_tuning t = e->tuning_v[variant];
+    if (t.waves_per_cu == 0) {
+        const shgpu_schedule_class &c = e->sched.lookup((uint32_t)variant, n);
+        t.waves_per_cu = c.waves_per_cu;
+        t.sh1_seeds_per_lane = c.sh1_seeds_per_lane;
+        t.split_seeds_per_lane = c.split_seeds_per_lane;
+    }
+    return t;
+}
+
 template <int VARIANT>
-int launch_build_v(shgpu_engine *e, bool wide, const BuildArgs &a, int grid, hipStream_t s) {
-    const hipError_t err = launch_build_variant<VARIANT>(e->tuning_v[VARIANT].sh1_seeds_per_lane,
-                                                         e->tuning_v[VARIANT].split_seeds_per_lane, wide, a, grid, s);
+int launch_build_v(const shgpu_tuning &t, bool wide, const BuildArgs &a, int grid, hipStream_t s) {
+    const hipError_t err = launch_build_variant<VARIANT>(t.sh1_seeds_per_lane, t.split_seeds_per_lane, wide, a, grid, s);
     if (err != hipSuccess) return fail(SHGPU_ERR_HIP, std::string("k_build launch: ") + hipGetErrorString(err));
     return SHGPU_OK;
 }
@@ -844,6 +853,10 @@ int shgpu_engine_create(int device, shgpu_engine **out) {
     e->cu_count = prop.multiProcessorCount;
     e->clock_khz = prop.clockRate;
     std::snprintf(e->arch, sizeof(e->arch), "%s", prop.gcnArchName);
+    if ((rc = load_default_schedule(e))) {
+        delete e;
+        return rc;
+    }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess || e->main.create_events()) {
         delete e;
         return fail(SHGPU_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
@@ -1155,7 +1168,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         if (err_ != hipSuccess) return fail_all(fail(SHGPU_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_))); \
     } while (0)
     auto t0 = clk::now();
-    const shgpu_tuning &tune = e->tuning_v[d->variant];
+    const shgpu_tuning tune = resolve_tuning(e, d->variant, N);
     BATCH_HIP(hipEventRecord(B.evA, s));
     std::vector<std::unique_ptr<shgpu_build>> bs(count);
     for (size_t r = 0; r < count; r++) {
@@ -1207,7 +1220,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     const Sh2Layout sh2l = d->variant == SHGPU_SHOCKHASH2
                                ? sh2_layout(d->leaf_size, 1, (tune.flags & SHGPU_TUNE_SH2_TIGHT_TABLES) != 0)
                                : Sh2Layout{};
-    const int grid = sh2_grid_cap(sh2l, std::max(1, grid_for(e, tune, d->variant, N) / (int)std::max(1u, grid_div)));
+    const int grid = sh2_grid_cap(sh2l, std::max(1, grid_for(e, tune) / (int)std::max(1u, grid_div)));
     const uint32_t nslots = (uint32_t)grid * WAVES_PER_BLOCK;
     std::vector<uint64_t> acc_all(NB);
     std::vector<uint64_t> codes;
@@ -1310,9 +1323,9 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             a.help = p.help;
             a.total = 0;
             BATCH_HIP(hipEventRecord(B.ev0, s));
-            if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(e, false, a, grid, s);
-            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_build_v<SHGPU_SIMD_SHOCKHASH>(e, false, a, grid, s);
-            else rc = launch_build_v<SHGPU_SHOCKHASH2>(e, sh2l.wide, a, grid, s);
+            if (d->variant == SHGPU_SHOCKHASH) rc = launch_build_v<SHGPU_SHOCKHASH>(tune, false, a, grid, s);
+            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_build_v<SHGPU_SIMD_SHOCKHASH>(tune, false, a, grid, s);
+            else rc = launch_build_v<SHGPU_SHOCKHASH2>(tune, sh2l.wide, a, grid, s);
             if (rc) return fail_all(rc);
             BATCH_HIP(hipEventRecord(B.ev1, s));
             timed_kernel = true;
@@ -1574,40 +1587,201 @@ int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc,
     return SHGPU_OK;
 }
 
-int shgpu_autotune(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
-                   shgpu_tuning *best, double *best_ms) {
-    if (!e || !desc || (n && !d_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
+namespace {
+
+// Builds the sample under candidate schedules in two stages: waves per CU in {4, 8, 12, 16, 24} at the default
+// seeds per lane (leaf 4, split 2), then every seeds-per-lane pair (leaf {1, 2, 4} for ShockHash1 leaves,
+// split {1, 2, 4}) at the fastest wave count. Each candidate runs twice (once when a build takes over a second) and
+// must give the first candidate's codes. The variant's tuning is restored afterwards. best: the candidate with the
+// shortest search kernel.
+static int sweep_schedules(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
+                           shgpu_tuning &best, double &best_ms) {
     int rc = check_variant(desc->variant, desc->leaf_size);
     if (rc) return rc;
-    const shgpu_tuning base = e->tuning_v[desc->variant];
+    const int v = desc->variant;
+    const shgpu_tuning base = e->tuning_v[v];
     shgpu_build_desc d = *desc;
     d.mode = SHGPU_MODE_PERSISTENT;
     d.copy_keys = 0;
-    const uint32_t leafS[3] = {1, 2, 4}, splitS[3] = {1, 2, 4}, waves[3] = {8, 16, 24};
-    const int nleaf = desc->variant == SHGPU_SHOCKHASH2 ? 1 : 3;  // ShockHash2 leaves are not windowed
-    shgpu_tuning winner = base;
-    double winner_ms = 1e30;
-    for (int li = 0; li < nleaf; li++)
-        for (uint32_t ss : splitS)
-            for (uint32_t wv : waves) {
-                shgpu_tuning t = base;
-                t.sh1_seeds_per_lane = desc->variant == SHGPU_SHOCKHASH2 ? base.sh1_seeds_per_lane : leafS[li];
-                t.split_seeds_per_lane = ss;
-                t.waves_per_cu = wv;
-                e->tuning_v[desc->variant] = t;
-                double ms = 1e30;
-                for (int rep = 0; rep < 2; rep++) {
-                    shgpu_build *b = nullptr;
-                    rc = shgpu_build_run(e, &d, d_hashes, n, &b);
-                    if (rc) { e->tuning_v[desc->variant] = base; return rc; }
-                    ms = std::min(ms, b->ms[3]);
-                    shgpu_build_free(b);
-                }
-                if (ms < winner_ms) { winner_ms = ms; winner = t; }
+    std::vector<uint64_t> ref;
+    bool have_ref = false;
+    int reps = 2;
+    best = base;
+    best_ms = 1e30;
+    auto run = [&](uint32_t waves, uint32_t leaf_s, uint32_t split_s) -> int {
+        shgpu_tuning t = base;
+        t.sh1_seeds_per_lane = leaf_s;
+        t.split_seeds_per_lane = split_s;
+        t.waves_per_cu = waves;
+        e->tuning_v[v] = t;
+        double ms = 1e30;
+        for (int rep = 0; rep < reps; rep++) {
+            shgpu_build *b = nullptr;
+            int rc2 = shgpu_build_run(e, &d, d_hashes, n, &b);
+            if (rc2) return rc2;
+            ms = std::min(ms, b->ms[3]);
+            const bool same = !have_ref || b->codes == ref;
+            if (!have_ref) {
+                ref = b->codes;
+                have_ref = true;
+                if (b->ms[3] > 1000.0) reps = 1;
             }
-    e->tuning_v[desc->variant] = winner;
-    if (best) *best = winner;
-    if (best_ms) *best_ms = winner_ms;
+            shgpu_build_free(b);
+            if (!same) return fail(SHGPU_ERR_HIP, "autotune: the schedule (" + std::to_string(waves) + " waves per CU) changed the codes");
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+        return SHGPU_OK;
+    };
+    const uint32_t waves[5] = {4, 8, 12, 16, 24}, seeds[3] = {1, 2, 4};
+    for (uint32_t wv : waves)
+        if ((rc = run(wv, 4, 2))) break;
+    const uint32_t bw = best.waves_per_cu;
+    const int nleaf = v == SHGPU_SHOCKHASH2 ? 1 : 3;  // ShockHash2 leaves are not windowed
+    for (int li = 0; li < nleaf && !rc; li++)
+        for (uint32_t ss : seeds) {
+            const uint32_t ls = nleaf == 1 ? 4 : seeds[li];
+            if (ls == 4 && ss == 2) continue;  // timed in the first stage
+            if ((rc = run(bw, ls, ss))) break;
+        }
+    e->tuning_v[v] = base;
+    return rc;
+}
+
+// Directory of the loaded library, "" when unknown.
+static std::string library_dir() {
+    Dl_info info;
+    if (dladdr((void *)&shgpu_engine_create, &info) && info.dli_fname) {
+        const std::string p = info.dli_fname;
+        const size_t k = p.rfind('/');
+        return k == std::string::npos ? std::string(".") : p.substr(0, k);
+    }
+    return "";
+}
+
+static bool read_text(const std::string &path, std::string &out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
+    return (bool)f || f.eof();
+}
+
+static int load_schedule_file(shgpu_engine *e, const std::string &path) {
+    std::string text, err;
+    if (!read_text(path, text)) return fail(SHGPU_ERR_ARG, "schedule: cannot read " + path);
+    std::vector<shgpu_schedule_class> cls;
+    int rc = shgpu::parse_schedule_json(text, cls, err);
+    if (!rc) rc = e->sched.replace(cls, err);
+    if (rc) return fail(rc, path + ": " + err);
+    e->sched.source = path;
+    return SHGPU_OK;
+}
+
+// SHGPU_SCHEDULE (a path, or "none" for the built-in table), else schedule_<arch>.json beside the library or in its
+// parent directory, else the built-in table.
+static int load_default_schedule(shgpu_engine *e) {
+    const char *env = std::getenv("SHGPU_SCHEDULE");
+    if (env && *env) return std::strcmp(env, "none") == 0 ? SHGPU_OK : load_schedule_file(e, env);
+    std::string arch = e->arch;
+    arch = arch.substr(0, arch.find(':'));
+    const std::string dir = library_dir();
+    if (dir.empty()) return SHGPU_OK;
+    for (const std::string &p : {dir + "/schedule_" + arch + ".json", dir + "/../schedule_" + arch + ".json"}) {
+        std::ifstream f(p);
+        if (f) return load_schedule_file(e, p);
+    }
+    return SHGPU_OK;
+}
+
+}  // namespace
+
+int shgpu_autotune(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
+                   shgpu_tuning *best, double *best_ms) {
+    if (!e || !desc || (n && !d_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
+    shgpu_tuning t;
+    double ms = 0;
+    int rc = sweep_schedules(e, desc, d_hashes, n, t, ms);
+    if (rc) return rc;
+    e->tuning_v[desc->variant] = t;
+    if (best) *best = t;
+    if (best_ms) *best_ms = ms;
+    return SHGPU_OK;
+}
+
+int shgpu_autotune_class(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *d_hashes, uint64_t n,
+                         uint64_t max_keys, shgpu_schedule_class *chosen) {
+    if (!e || !desc || (n && !d_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (max_keys == 0) return fail(SHGPU_ERR_ARG, "max_keys must be positive");
+    shgpu_tuning t;
+    double ms = 0;
+    int rc = sweep_schedules(e, desc, d_hashes, n, t, ms);
+    if (rc) return rc;
+    shgpu_schedule_class c{};
+    c.variant = (uint32_t)desc->variant;
+    c.waves_per_cu = t.waves_per_cu;
+    c.sh1_seeds_per_lane = t.sh1_seeds_per_lane;
+    c.split_seeds_per_lane = t.split_seeds_per_lane;
+    c.max_keys = max_keys;
+    c.tuned_ms = ms;
+    c.sample_keys = n;
+    e->sched.put(c);
+    if (chosen) *chosen = c;
+    return SHGPU_OK;
+}
+
+int shgpu_schedule_get(shgpu_engine *e, shgpu_schedule_class *out, uint32_t capacity, uint32_t *count) {
+    if (!e || !count) return fail(SHGPU_ERR_ARG, "NULL argument");
+    *count = (uint32_t)e->sched.classes.size();
+    if (out) {
+        if (capacity < *count) return fail(SHGPU_ERR_ARG, "capacity too small");
+        std::memcpy(out, e->sched.classes.data(), *count * sizeof(shgpu_schedule_class));
+    }
+    return SHGPU_OK;
+}
+
+int shgpu_schedule_set(shgpu_engine *e, const shgpu_schedule_class *classes, uint32_t count) {
+    if (!e || (count && !classes)) return fail(SHGPU_ERR_ARG, "NULL argument");
+    std::string err;
+    const int rc = e->sched.replace(std::vector<shgpu_schedule_class>(classes, classes + count), err);
+    if (rc) return fail(rc, err);
+    e->sched.source = "set";
+    return SHGPU_OK;
+}
+
+int shgpu_schedule_save(shgpu_engine *e, const char *path) {
+    if (!e || !path) return fail(SHGPU_ERR_ARG, "NULL argument");
+    std::string arch = e->arch;
+    arch = arch.substr(0, arch.find(':'));
+    std::ofstream f(path, std::ios::binary | std::ios::trunc);
+    if (!f) return fail(SHGPU_ERR_ARG, std::string("schedule: cannot write ") + path);
+    f << e->sched.to_json(arch);
+    f.close();
+    if (!f) return fail(SHGPU_ERR_ARG, std::string("schedule: write failed: ") + path);
+    return SHGPU_OK;
+}
+
+int shgpu_schedule_load(shgpu_engine *e, const char *path) {
+    if (!e || !path) return fail(SHGPU_ERR_ARG, "NULL argument");
+    return load_schedule_file(e, path);
+}
+
+const char *shgpu_schedule_source(shgpu_engine *e) { return e ? e->sched.source.c_str() : ""; }
+
+int shgpu_schedule_parse(const char *json, shgpu_schedule_class *out, uint32_t capacity, uint32_t *count) {
+    if (!json || !count) return fail(SHGPU_ERR_ARG, "NULL argument");
+    std::vector<shgpu_schedule_class> cls;
+    std::string err;
+    const int rc = shgpu::parse_schedule_json(json, cls, err);
+    if (rc) return fail(rc, err);
+    *count = (uint32_t)cls.size();
+    if (out) {
+        if (capacity < *count) return fail(SHGPU_ERR_ARG, "capacity too small");
+        std::memcpy(out, cls.data(), cls.size() * sizeof(shgpu_schedule_class));
+    }
     return SHGPU_OK;
 }
 
@@ -1764,7 +1938,7 @@ struct shgpu_server {
         return cfg.variant == SHGPU_SHOCKHASH2 ? cfg.slots * slot_waves() * sh2_layout(cfg.leaf_size).wave_bytes : 0;
     }
     size_t slot_waves() const {
-        const int grid = std::max(1, grid_for(e, e->tuning_v[cfg.variant], cfg.variant, ~0ull) / (int)cfg.slots);
+        const int grid = std::max(1, grid_for(e, resolve_tuning(e, cfg.variant, ~0ull)) / (int)cfg.slots);
         return (size_t)(cfg.variant == SHGPU_SHOCKHASH2 ? sh2_grid_cap(sh2_layout(cfg.leaf_size), grid) : grid) * WAVES_PER_BLOCK;
     }
     size_t allocated() const {

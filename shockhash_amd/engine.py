@@ -47,6 +47,16 @@ class Tuning(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("max_trials", ctypes.c_uint64)]
 
 
+class ScheduleClass(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("sh1_seeds_per_lane", ctypes.c_uint32),
+                ("split_seeds_per_lane", ctypes.c_uint32), ("max_keys", ctypes.c_uint64), ("tuned_ms", ctypes.c_double),
+                ("sample_keys", ctypes.c_uint64)]
+
+
+def _class_dict(c):
+    return {f: getattr(c, f) for f, _ in ScheduleClass._fields_}
+
+
 KEYS_HOST, KEYS_DEVICE, NODES_HOST = 1, 2, 4  # shgpu_build_desc.copy_keys bits
 
 
@@ -85,6 +95,14 @@ _SIGS = {
     "shgpu_set_variant_tuning": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(Tuning)]),
     "shgpu_autotune": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.POINTER(Tuning),
                                       ctypes.POINTER(ctypes.c_double)]),
+    "shgpu_autotune_class": (ctypes.c_int, [P, ctypes.POINTER(BuildDesc), P, ctypes.c_uint64, ctypes.c_uint64,
+                                            ctypes.POINTER(ScheduleClass)]),
+    "shgpu_schedule_get": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    "shgpu_schedule_set": (ctypes.c_int, [P, P, ctypes.c_uint32]),
+    "shgpu_schedule_save": (ctypes.c_int, [P, ctypes.c_char_p]),
+    "shgpu_schedule_load": (ctypes.c_int, [P, ctypes.c_char_p]),
+    "shgpu_schedule_source": (ctypes.c_char_p, [P]),
+    "shgpu_schedule_parse": (ctypes.c_int, [ctypes.c_char_p, P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     "shgpu_sh1_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_sh2_leaf_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
     "shgpu_split_search": (ctypes.c_int, [P, P, P, ctypes.c_uint32, P, P]),
@@ -311,6 +329,39 @@ class Engine:
         _check(load().shgpu_autotune(self.h, ctypes.byref(desc), d_hashes.ptr, n, ctypes.byref(t), ctypes.byref(ms)))
         return {f: getattr(t, f) for f, _ in Tuning._fields_}, ms.value
 
+    def autotune_class(self, d_hashes, n, variant, leaf_size, bucket_size, max_keys):
+        """Tunes the variant's size class for batches up to max_keys keys on the sample (shgpu_autotune_class)."""
+        desc = BuildDesc(VARIANTS[variant], leaf_size, bucket_size, 0, 0, 0)
+        c = ScheduleClass()
+        _check(load().shgpu_autotune_class(self.h, ctypes.byref(desc), d_hashes.ptr, n, max_keys, ctypes.byref(c)))
+        return _class_dict(c)
+
+    @property
+    def schedule(self):
+        """The size classes of the build schedule, sorted by variant and max_keys."""
+        lib, cnt = load(), ctypes.c_uint32()
+        _check(lib.shgpu_schedule_get(self.h, None, 0, ctypes.byref(cnt)))
+        arr = (ScheduleClass * max(1, cnt.value))()
+        _check(lib.shgpu_schedule_get(self.h, arr, len(arr), ctypes.byref(cnt)))
+        return [_class_dict(arr[i]) for i in range(cnt.value)]
+
+    def set_schedule(self, classes):
+        arr = (ScheduleClass * max(1, len(classes)))()
+        for i, c in enumerate(classes):
+            for k, v in c.items():
+                setattr(arr[i], k, v)
+        _check(load().shgpu_schedule_set(self.h, arr, len(classes)))
+
+    def save_schedule(self, path):
+        _check(load().shgpu_schedule_save(self.h, os.fsencode(path)))
+
+    def load_schedule(self, path):
+        _check(load().shgpu_schedule_load(self.h, os.fsencode(path)))
+
+    @property
+    def schedule_source(self):
+        return load().shgpu_schedule_source(self.h).decode()
+
     def sync(self):
         _check(load().shgpu_sync(self.h))
 
@@ -504,3 +555,13 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def parse_schedule(text):
+    """Size classes of a schedule JSON text (shgpu_schedule_parse; no device needed)."""
+    lib, cnt = load(), ctypes.c_uint32()
+    raw = text.encode() if isinstance(text, str) else text
+    _check(lib.shgpu_schedule_parse(raw, None, 0, ctypes.byref(cnt)))
+    arr = (ScheduleClass * max(1, cnt.value))()
+    _check(lib.shgpu_schedule_parse(raw, arr, len(arr), ctypes.byref(cnt)))
+    return [_class_dict(arr[i]) for i in range(cnt.value)]
