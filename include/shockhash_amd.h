@@ -177,6 +177,9 @@ typedef struct shgpu_build_desc {
                               SHGPU_KEYS_DEVICE: keep a device copy (shgpu_build_device_keys);
                               SHGPU_NODES_HOST: copy the device-planned nodes back with the codes (otherwise
                               shgpu_build_nodes plans them again on the host on first request); any OR */
+    int rotation_fitting;  /* ShockHash / SIMDShockHash ROTATION_FITTING (include/ShockHash.h:316-403,
+                              include/SIMDShockHash.hpp:419-506): leaves of exactly leaf_size keys get the code
+                              x * leaf_size + r of the rotation search; 0: off (ShockHash2 ignores it) */
 } shgpu_build_desc;
 
 #define SHGPU_KEYS_HOST 1

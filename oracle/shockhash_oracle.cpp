@@ -13,6 +13,7 @@
 //   pairing          the value tables drawn in PairingFunction.h's comments + round trips
 //   cuckoo predicates three independent predicates agree with exhaustive matching search
 //   leaf searches    brute force over seeds with an exhaustive orientation check (tiny m)
+//   rotation fitting brute force over (x, r) with perfect-matching search (m up to 12, and 33)
 //   split searches   brute force over seeds with plain per-part counts
 //   Rice encoding    decode round trip + hand-built streams
 // Parity unpinned (no vector source in the reference or the image): sux::remap16's 48-bit
@@ -191,6 +192,72 @@ static uint64_t sh1_leaf(const uint64_t *keys, uint32_t m, uint64_t x, int32_t *
         }
         x++;
     }
+}
+
+// Rotation fitting of a full leaf (m == LEAF_SIZE, ROTATION_FITTING): ShockHash.h:316-403 and
+// SIMDShockHash.hpp:419-506 (same search, vectorised over r). Keys with even mhc (group A) are hashed with seed
+// x & ~mask (mask 7 for leaves above 32 keys, GROUP_A_HF_MASK at :141), odd ones (group B) with seed x; for each
+// x in order and each rotation r in 0..m-1 in order, a candidate needs A's cells OR B's cells rotated by r to
+// cover all m cells, then the cuckoo insertions of every key in heap order (A in leaf order, then B in reverse
+// leaf order) with B's cells rotated by r must succeed. Returns x (absolute) and r; key_by_cell receives the
+// leaf-index of the key in each cell, and the caller's bit for cell i is (i' < m/2) with i' = i for A and
+// (i - r + m) % m for B (:380-388).
+static inline uint64_t rot_mask(uint32_t m) { return m > 32 ? 7ull : 0ull; }
+static inline uint64_t rotl_bits(uint64_t v, uint32_t r, uint32_t m) {  // ShockHash.h:297-299
+    const uint64_t all = m == 64 ? ~0ull : ((1ull << m) - 1);
+    return r == 0 ? v & all : ((v << r) | (v >> (m - r))) & all;
+}
+static void sh1_leaf_rot(const uint64_t *keys, uint32_t m, uint64_t x, uint64_t *x_out, uint32_t *r_out, int32_t *key_by_cell) {
+    const uint64_t all = m == 64 ? ~0ull : ((1ull << m) - 1), mask = rot_mask(m);
+    std::vector<int32_t> heap(m);  // heap position -> leaf index
+    uint32_t na = 0, ib = m - 1;
+    for (uint32_t i = 0; i < m; i++) {
+        if ((keys[i] & 1) == 0) heap[na++] = (int32_t)i;
+        else heap[ib--] = (int32_t)i;
+    }
+    std::vector<Cells> cache(m);
+    uint64_t a = 0, b = 0;
+    for (;; x++) {
+        if (a == 0 || (x & mask) == 0) {
+            a = 0;
+            for (uint32_t i = 0; i < na; i++) {
+                cache[i] = sh1_cells(keys[heap[i]], x & ~mask, m);
+                a |= (1ull << cache[i].c1) | (1ull << cache[i].c2);
+            }
+        }
+        b = 0;
+        for (uint32_t i = na; i < m; i++) {
+            cache[i] = sh1_cells(keys[heap[i]], x, m);
+            b |= (1ull << cache[i].c1) | (1ull << cache[i].c2);
+        }
+        for (uint32_t r = 0; r < m; r++) {
+            if ((a | rotl_bits(b, r, m)) != all) continue;
+            IndexCuckoo t(m, m);
+            uint32_t i = 0;
+            for (; i < m; i++) {
+                Cells c = cache[i];
+                if (keys[heap[i]] & 1) { c.c1 = (c.c1 + r) % m; c.c2 = (c.c2 + r) % m; }
+                if (!t.insert((int32_t)i, c.c1, c.c2)) break;
+            }
+            if (i == m) {
+                *x_out = x;
+                *r_out = r;
+                if (key_by_cell) for (uint32_t q = 0; q < m; q++) key_by_cell[q] = heap[t.cells[q]];
+                return;
+            }
+        }
+    }
+}
+// Cell of a key in a rotation-fitted leaf (ShockHash.h:251-265, SIMDShockHash.hpp:826-840).
+static uint64_t sh1_rot_query_cell(uint32_t m, uint64_t code, uint64_t start, uint64_t key, uint64_t bit) {
+    const uint32_t r = (uint32_t)(code % m);
+    const uint64_t x = code / m + start;
+    if ((key & 1) == 0) {
+        Cells c = sh1_cells(key, x & ~rot_mask(m), m);
+        return bit == 0 ? c.c1 : c.c2;
+    }
+    Cells c = sh1_cells(key, x, m);
+    return ((bit == 0 ? c.c1 : c.c2) + r) % m;
 }
 
 // ---------------------------------------------------------------- ShockHash2 leaf
@@ -513,6 +580,7 @@ struct NodeRec { uint32_t bucket, m; int32_t level; uint8_t kind; uint64_t code;
 
 struct Build {
     Variant v; Strategy st; size_t n = 0, nbuckets = 0;
+    bool rot = false;  // ROTATION_FITTING (ShockHash / SIMDShockHash): full leaves use sh1_leaf_rot
     std::vector<uint64_t> memo, size_acc, pos_acc, words;
     uint64_t bits = 0;
     std::vector<std::pair<uint64_t, uint8_t>> ribbon;
@@ -544,6 +612,18 @@ static void leaf(Ctx &c, uint64_t *keys, uint32_t m, int level) {
         uint8_t bits[128];
         sh2_retrieval(keys, m, code, bits);
         for (uint32_t k = 0; k < m; k++) c.ribbon.emplace_back(keys[k], bits[k]);
+    } else if (c.b.rot && m == c.b.st.leaf) {
+        const uint64_t start = c.b.v == SIMD ? START_SEED[19] : START_SEED[level];
+        int32_t by_cell[64];
+        uint64_t x = 0;
+        uint32_t r = 0;
+        sh1_leaf_rot(keys, m, start, &x, &r, by_cell);
+        for (uint32_t i = 0; i < m; i++) {
+            const uint64_t k = keys[by_cell[i]];
+            const uint32_t cell = (k & 1) ? (i - r + m) % m : i;
+            c.ribbon.emplace_back(k, cell < m / 2 ? 0 : 1);
+        }
+        code = (x - start) * m + r;  // ShockHash.h:401-402
     } else {
         uint64_t start = c.b.v == SIMD ? START_SEED[19] : START_SEED[level];
         int32_t by_cell[64];
@@ -634,8 +714,9 @@ static Build *build_buckets(Variant v, uint32_t L, const uint64_t *h, size_t n, 
 
 // hash_gen + compute_thread (ShockHash.h:557-688; ShockHash2.h:396-498; SIMDShockHash.hpp:619-760):
 // partition by .first (Sorter.hpp:15-36), contiguous bucket ranges per thread, tid-ordered merge.
-static Build *build(Variant v, uint32_t L, const uint64_t *h, size_t n, size_t bucket_size, int threads) {
+static Build *build(Variant v, uint32_t L, const uint64_t *h, size_t n, size_t bucket_size, int threads, bool rot = false) {
     Build *b = new Build(v, L);
+    b->rot = rot && v != SH2;
     std::vector<uint64_t> sorted;
     partition(b, h, n, bucket_size, sorted);
     if (threads < 1) threads = 1;
@@ -721,6 +802,7 @@ static uint64_t query(const Build &b, uint64_t first, uint64_t second) {
     auto it = b.retr.find(second);
     uint64_t bit = it == b.retr.end() ? 0 : it->second;
     if (b.v == SH2) return cum + sh2_query_cell((uint32_t)m, code, second, bit);
+    if (b.rot && m == st.leaf) return cum + sh1_rot_query_cell((uint32_t)m, code, b.v == SIMD ? START_SEED[19] : START_SEED[level], second, bit);
     uint64_t seed = code + (b.v == SIMD ? START_SEED[19] : START_SEED[level]);
     Cells cc = sh1_cells(second, seed, (uint32_t)m);
     return cum + (bit == 0 ? cc.c1 : cc.c2);
@@ -965,6 +1047,18 @@ void orc_golomb_memo(int variant, int leaf, uint64_t *out /* 3000 */) {
 void *orc_build(int variant, int leaf, const uint64_t *hashes, size_t n, size_t bucket_size, int threads) {
     try { return build((Variant)variant, (uint32_t)leaf, hashes, n, bucket_size, threads); }
     catch (...) { return nullptr; }
+}
+// ROTATION_FITTING = true builds of ShockHash / SIMDShockHash (ShockHash.h:316-403, SIMDShockHash.hpp:419-506).
+void *orc_build_rot(int variant, int leaf, const uint64_t *hashes, size_t n, size_t bucket_size, int threads) {
+    try { return build((Variant)variant, (uint32_t)leaf, hashes, n, bucket_size, threads, true); }
+    catch (...) { return nullptr; }
+}
+// One rotation-fitted leaf: absolute x, rotation r and the leaf index of the key in each cell.
+void orc_sh1_leaf_rot(const uint64_t *keys, uint32_t m, uint64_t start, uint64_t *x, uint32_t *r, int32_t *key_by_cell) {
+    sh1_leaf_rot(keys, m, start, x, r, key_by_cell);
+}
+uint64_t orc_sh1_rot_query_cell(uint32_t m, uint64_t code, uint64_t start, uint64_t key, uint64_t bit) {
+    return sh1_rot_query_cell(m, code, start, key, bit);
 }
 // Nodes of the chosen buckets only (orc_build_nnodes / orc_build_nodes read them; nothing else is built).
 void *orc_build_buckets(int variant, int leaf, const uint64_t *hashes, size_t n, size_t bucket_size, const uint32_t *sel,

@@ -50,8 +50,15 @@ struct BuildArgs {
     uint64_t sh2_wave_bytes;
     Sh2Caps sh2_caps;          // table capacities of leaves above 64 keys
     int split_hash;
-    uint32_t flags;  // shgpu_tuning.flags of the variant
+    uint32_t flags;     // shgpu_tuning.flags of the variant
+    uint32_t rot_leaf;  // ROTATION_FITTING: leaves of exactly this many keys are rotation-fitted (0: none)
 };
+
+// A ShockHash1 / SIMDShockHash leaf searched by rotation fitting: its window results are (seed offset << 6) | r
+// and its code is offset * m + r (include/ShockHash.h:401-402).
+__host__ __device__ __forceinline__ bool rotation_leaf(const BuildArgs &a, uint32_t kind, uint32_t m) {
+    return a.rot_leaf != 0 && kind == SHGPU_NODE_LEAF && m == a.rot_leaf && m >= 2;
+}
 
 // Queue layout: one FIFO over all nodes, or a split part [0, qsplit) and a leaf part [qsplit, total).
 __host__ __device__ __forceinline__ bool two_part_queue(uint32_t flags) { return (flags & SHGPU_TUNE_SPLITS_FIRST) != 0; }

@@ -35,6 +35,7 @@ template <int VARIANT>
 __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, uint64_t off, bool persistent) {
     const shgpu_node nd = a.nodes[nid];
     const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
+    if (VARIANT != SHGPU_SHOCKHASH2 && rotation_leaf(a, kind, m) && off != SHGPU_NOT_FOUND) off = (off >> 6) * m + (off & 63);
     if (kind != SHGPU_NODE_LEAF) {
         const SplitShapeDev sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
         if (off != SHGPU_NOT_FOUND) split_partition(a.keys + o, a.tmp + o, sh, uniform64(nd.start_seed) + off);
@@ -77,6 +78,9 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
     // The window count has 31 bits: a bound beyond WINDOW_CAP windows ends the search at the cap, and
     // the finish rule below uses the same limit, so a node that fails is finalised as not found.
     const uint64_t start = uniform64(nd.start_seed), lim = min(uniform64(nd.max_trials), WINDOW_CAP * W);
+    // rotation-fitted leaves report (seed offset << 6) | r: compare their seed offsets with the windows
+    const bool rot = VARIANT != SHGPU_SHOCKHASH2 && rotation_leaf(a, kind, m);
+    const uint32_t rsh = rot ? 6u : 0u;
     constexpr int SPLIT_HASH = VARIANT == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
     SplitShapeDev sh;
     if (kind != SHGPU_NODE_LEAF) sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
@@ -103,12 +107,13 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
         if (!exhausted && !found) {
             if (kind == SHGPU_NODE_LEAF) {
                 if constexpr (VARIANT != SHGPU_SHOCKHASH2)  // ShockHash2 leaves never come here
-                    r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND) : sh1_window_staged<SL>(kv, m, start, woff, L);
+                    r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND)
+                              : rot ? sh1_rot_window_staged<SL>(kv, m, start, woff, L) : sh1_window_staged<SL>(kv, m, start, woff, L);
             } else if (kind == SHGPU_NODE_AGGR_LOWER)
                 r = split_window<SS, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
             else
                 r = split_window<1, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
-            if (r != SHGPU_NOT_FOUND && r >= lim) r = SHGPU_NOT_FOUND;
+            if (r != SHGPU_NOT_FOUND && (r >> rsh) >= lim) r = SHGPU_NOT_FOUND;
             if (r != SHGPU_NOT_FOUND) {
                 // same value in every lane; waiting for each returned value orders the best seed before the
                 // found flag and the flag before the completion below (all resolve at the coherence point)
@@ -126,7 +131,7 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
             // completion, so without the flag none of them holds one
             const uint64_t b = (old & GD_FOUND) ? read_best(st) : SHGPU_NOT_FOUND;
             const uint64_t covered = handed * W;
-            if (b < covered || (b == SHGPU_NOT_FOUND && covered >= lim)) {
+            if ((b != SHGPU_NOT_FOUND && (b >> rsh) < covered) || (b == SHGPU_NOT_FOUND && covered >= lim)) {
                 if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, b, persistent);
             }
         }

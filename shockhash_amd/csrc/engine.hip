@@ -1236,6 +1236,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     a.tmp = B.tmp.as<uint64_t>();
     a.split_hash = d->variant == SHGPU_SIMD_SHOCKHASH ? SHGPU_SPLIT_SIMD : SHGPU_SPLIT_SUX;
     a.flags = tune.flags;
+    a.rot_leaf = d->rotation_fitting && d->variant != SHGPU_SHOCKHASH2 ? d->leaf_size : 0u;
     a.scratch = nullptr;
     a.sh2_caps = sh2l.caps;
     a.sh2_wave_bytes = sh2l.wave_bytes;

@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(BS) k_leaf_orientation(const uint64_t *__restr
         if (MAXM == 128 && m <= 64) continue;
         for (uint32_t c = 0; c < m; c++) cell[c] = CELL_EMPTY;
         bool ok = true;
-        if (!BIPARTITE) {
+        if (!BIPARTITE && tk.rotation == 0) {
             for (uint32_t k = 0; k < m && ok; k++) {
                 const uint64_t r = shgpu::remix(lk[k] + tk.seed);
                 const uint32_t c1 = (uint32_t)(((uint64_t)(uint32_t)r * (m / 2)) >> 32);
@@ -100,6 +100,38 @@ __global__ void __launch_bounds__(BS) k_leaf_orientation(const uint64_t *__restr
             for (uint32_t c = 0; c < m; c++) {
                 out_keys[tk.out_offset + c] = lk[cell[c]];
                 out_bits[tk.out_offset + c] = c < m / 2 ? 0 : 1;
+            }
+        } else if (!BIPARTITE) {
+            // Rotation-fitted leaf (ShockHash.h:316-403): insertion order A keys (even) in leaf order, then B keys
+            // (odd) in reverse leaf order; A cells of seed x & ~amask, B cells of seed x rotated by r. The c2s
+            // column maps insertion positions to leaf indices. Entries in cell order, bit = (cell' >= m/2) with
+            // cell' the cell before rotation (ShockHash.h:380-388).
+            const uint32_t rot = tk.rotation - 1;
+            const uint64_t amask = m > 32 ? 7ull : 0ull;
+            if (rot >= m) { atomicOr(error, 1u); continue; }
+            uint32_t na = 0, ib = m - 1;
+            for (uint32_t k = 0; k < m; k++) {
+                if ((lk[k] & 1) == 0) c2s[na++] = (uint8_t)k;
+                else c2s[ib--] = (uint8_t)k;
+            }
+            for (uint32_t h = 0; h < m && ok; h++) {
+                const uint64_t key = lk[c2s[h]];
+                const bool odd = (key & 1) != 0;
+                const uint64_t r = shgpu::remix(key + (odd ? tk.seed : (tk.seed & ~amask)));
+                uint32_t c1 = (uint32_t)(((uint64_t)(uint32_t)r * (m / 2)) >> 32);
+                uint32_t c2 = (uint32_t)(((r >> 32) * ((m + 1) / 2)) >> 32) + m / 2;
+                if (odd) {
+                    c1 = (c1 + rot) % m;
+                    c2 = (c2 + rot) % m;
+                }
+                ok = orient_insert(cell, xr, h, c1, c2);
+            }
+            if (!ok) { atomicOr(error, 2u); continue; }
+            for (uint32_t c = 0; c < m; c++) {
+                const uint64_t key = lk[c2s[cell[c]]];
+                const uint32_t own = (key & 1) ? (c + m - rot) % m : c;
+                out_keys[tk.out_offset + c] = key;
+                out_bits[tk.out_offset + c] = own < m / 2 ? 0 : 1;
             }
         } else {
             uint64_t s1, s2;

@@ -137,6 +137,77 @@ __device__ __forceinline__ uint64_t sh1_window_staged(uint64_t kv, uint32_t m, u
     return SHGPU_NOT_FOUND;
 }
 
+// ROTATION_FITTING full leaf (m == LEAF_SIZE): include/ShockHash.h:316-403, include/SIMDShockHash.hpp:419-506.
+// Keys with even value (group A) take cells of seed x & ~amask (amask 7 above 32 keys, GROUP_A_HF_MASK at
+// ShockHash.h:141), odd keys (group B) cells of seed x rotated by r (cell + r mod m). The reference's result is
+// the smallest x, and for it the smallest r, whose cells cover all m cells and admit the cuckoo insertions (a
+// pseudoforest). Lanes over seeds as in sh1_window_staged: full 64-bit coverage masks of A and B per seed, the
+// rotations r whose A | rotl(B, r) covers every cell collected in a mask, then candidates tested in (x, r)
+// order with the wave's pseudoforest test. Returns (seed offset << 6) | r of the first success in the window
+// (increasing in (x, r), so the window protocol's minimum is the reference's result), or SHGPU_NOT_FOUND.
+__device__ __forceinline__ uint64_t rot_mask_of(uint32_t m) { return m > 32 ? 7ull : 0ull; }
+__device__ __forceinline__ uint64_t rotl_cells(uint64_t v, uint32_t r, uint32_t m, uint64_t all) {
+    return r == 0 ? v : ((v << r) | (v >> (m - r))) & all;
+}
+
+template <int S>
+__device__ __forceinline__ uint64_t sh1_rot_window_staged(uint64_t kv, uint32_t m, uint64_t start, uint64_t woff, PfLds &L) {
+    const uint32_t lane = lane_id();
+    const uint32_t halfA = m / 2, halfB = (m + 1) / 2;
+    const uint64_t all = m == 64 ? ~0ull : ((1ull << m) - 1), amask = rot_mask_of(m);
+    uint64_t seed[S], ma[S], mb[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) { seed[s] = start + woff + (uint64_t)(s * 64 + lane); ma[s] = 0; mb[s] = 0; }
+    for (uint32_t k = 0; k < m; k++) {
+        const uint64_t key = L.stage[k];
+        const bool odd = (key & 1) != 0;  // wave-uniform
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const uint64_t r = remix(key + (odd ? seed[s] : (seed[s] & ~amask)));
+            const uint64_t bits = (1ull << __umulhi((uint32_t)r, halfA)) | (1ull << (__umulhi((uint32_t)(r >> 32), halfB) + halfA));
+            if (odd) mb[s] |= bits;
+            else ma[s] |= bits;
+        }
+    }
+    uint64_t rok[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        rok[s] = 0;
+        for (uint32_t r = 0; r < m; r++)
+            if ((ma[s] | rotl_cells(mb[s], r, m, all)) == all) rok[s] |= 1ull << r;
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        uint64_t cand = ballot(rok[s] != 0);
+        while (cand) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(cand);
+            cand &= cand - 1;
+            const uint64_t off = woff + (uint64_t)(s * 64 + l);
+            const uint64_t x = start + off;
+            uint64_t rs = readlane64(rok[s], l);
+            // this lane's key (lane < m) under seed x: its cells before rotation
+            uint32_t a = 0, b = 0;
+            const bool odd = (kv & 1) != 0;
+            if (lane < m) {
+                const uint64_t r = remix(kv + (odd ? x : (x & ~amask)));
+                a = __umulhi((uint32_t)r, halfA);
+                b = __umulhi((uint32_t)(r >> 32), halfB) + halfA;
+            }
+            while (rs) {
+                const uint32_t rot = (uint32_t)__builtin_ctzll(rs);
+                rs &= rs - 1;
+                uint32_t ra = a, rb = b;
+                if (odd) {
+                    ra = a + rot >= m ? a + rot - m : a + rot;
+                    rb = b + rot >= m ? b + rot - m : b + rot;
+                }
+                if (pseudoforest_wave(ra, rb, lane < m, m, L)) return (off << 6) | rot;
+            }
+        }
+    }
+    return SHGPU_NOT_FOUND;
+}
+
 // Sequential leaf search by one wave. Returns the absolute seed, or SHGPU_NOT_FOUND when no seed in
 // [start, start + max_trials) qualifies.
 template <int S>

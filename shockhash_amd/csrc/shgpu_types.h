@@ -29,6 +29,8 @@ typedef struct shgpu_orient_task {
     uint64_t seed;
     uint32_t key_offset;
     uint32_t m;
+    uint32_t rotation;  /* ShockHash1 / SIMDShockHash rotation-fitted leaf: r + 1 (seed = x); 0: plain leaf */
+    uint32_t pad;
 } shgpu_orient_task;
 
 /* Hash flavour of a split search. */

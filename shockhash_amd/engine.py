@@ -62,7 +62,8 @@ KEYS_HOST, KEYS_DEVICE, NODES_HOST = 1, 2, 4  # shgpu_build_desc.copy_keys bits
 
 class BuildDesc(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("leaf_size", ctypes.c_uint32), ("bucket_size", ctypes.c_uint64),
-                ("mode", ctypes.c_int), ("plain_bucket_limit", ctypes.c_uint32), ("copy_keys", ctypes.c_int)]
+                ("mode", ctypes.c_int), ("plain_bucket_limit", ctypes.c_uint32), ("copy_keys", ctypes.c_int),
+                ("rotation_fitting", ctypes.c_int)]
 
 
 LEAF_DTYPE = np.dtype([("offset", "<u4"), ("m", "<u4"), ("start_seed", "<u8"), ("max_trials", "<u8")])
@@ -429,10 +430,12 @@ class Engine:
         return d_out.download(), d_keys.download()
 
     def build_dev(self, d_hashes, n, variant, leaf_size, bucket_size, mode="persistent", plain_bucket_limit=0,
-                  copy_keys=True, offset=0):
-        """Builds over n fingerprints at `offset` (in fingerprints) of a device array of uint64 pairs."""
+                  copy_keys=True, offset=0, rotation=False):
+        """Builds over n fingerprints at `offset` (in fingerprints) of a device array of uint64 pairs; rotation:
+        ROTATION_FITTING for ShockHash / SIMDShockHash leaves of exactly leaf_size keys."""
         assert n == 0 or (offset + n) * 16 <= d_hashes.nbytes
-        desc = BuildDesc(VARIANTS[variant], leaf_size, bucket_size, MODES[mode], plain_bucket_limit, int(copy_keys))
+        desc = BuildDesc(VARIANTS[variant], leaf_size, bucket_size, MODES[mode], plain_bucket_limit, int(copy_keys),
+                         int(rotation))
         h = P()
         ptr = P((d_hashes.ptr.value or 0) + offset * 16)
         _check(load().shgpu_build_run(self.h, ctypes.byref(desc), ptr, n, ctypes.byref(h)))

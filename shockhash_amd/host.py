@@ -33,6 +33,7 @@ _SIGS = {
     "shh_load": (P, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
 }
 VARIANTS = {"sh1": 0, "simd": 1, "sh2": 2}
+ROTATION = 0x100  # ROTATION_FITTING = true instantiations (ShockHash / SIMDShockHash)
 _lib = None
 
 
@@ -79,21 +80,21 @@ class ConstructionError(RuntimeError):
 class MPHF:
     """A ShockHash / SIMDShockHash / ShockHash2 instance built by the C++ class template."""
 
-    def __init__(self, variant, leaf, keys, bucket_size, threads=1, keep_retrieval_input=False):
+    def __init__(self, variant, leaf, keys, bucket_size, threads=1, keep_retrieval_input=False, rotation=False):
         lib = load()
         err = ctypes.create_string_buffer(512)
+        vcode = VARIANTS[variant] | (ROTATION if rotation else 0)
         lib.shh_keep_retrieval_input(1 if keep_retrieval_input else 0)
         if isinstance(keys, (list, tuple)):
             data, offs = _pack_strings(keys)
             self._strings = True
-            h = lib.shh_build_strings(VARIANTS[variant], leaf, data, offs.ctypes.data_as(P), len(keys), bucket_size,
+            h = lib.shh_build_strings(vcode, leaf, data, offs.ctypes.data_as(P), len(keys), bucket_size,
                                       threads, err, 512)
             self.n = len(keys)
         else:
             arr = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, 2)
             self._strings = False
-            h = lib.shh_build(VARIANTS[variant], leaf, arr.ctypes.data_as(P), arr.shape[0], bucket_size, threads, err,
-                              512)
+            h = lib.shh_build(vcode, leaf, arr.ctypes.data_as(P), arr.shape[0], bucket_size, threads, err, 512)
             self.n = arr.shape[0]
         if not h:
             raise ConstructionError(lib.shh_last_status(), err.value.decode(errors="replace"))
@@ -154,10 +155,10 @@ class MPHF:
             raise RuntimeError(f"save failed: {err.value.decode(errors='replace')}")
 
     @classmethod
-    def load(cls, variant, leaf, path, strings=False):
+    def load(cls, variant, leaf, path, strings=False, rotation=False):
         """A function read from `path` (the class template's load()); answers queries only."""
         err = ctypes.create_string_buffer(512)
-        h = load().shh_load(VARIANTS[variant], leaf, str(path).encode(), err, 512)
+        h = load().shh_load(VARIANTS[variant] | (ROTATION if rotation else 0), leaf, str(path).encode(), err, 512)
         if not h:
             raise RuntimeError(f"load failed: {err.value.decode(errors='replace')}")
         obj = cls.__new__(cls)

@@ -66,10 +66,22 @@ Handle *make_s(const std::vector<std::string> &s, size_t bucket, size_t threads)
 #define SHH_LEAVES_SH1(X) X(2) X(4) X(8) X(16) X(24) X(30)
 #define SHH_LEAVES_SIMD(X) X(8) X(16) X(24) X(58)
 #define SHH_LEAVES_SH2(X) X(4) X(8) X(16) X(24) X(32) X(64) X(100) X(128)
+// ROTATION_FITTING instantiations (variant | SHH_ROTATION)
+#define SHH_LEAVES_SH1_ROT(X) X(8) X(16) X(30) X(40)
+#define SHH_LEAVES_SIMD_ROT(X) X(16) X(58)
+constexpr int SHH_ROTATION = 0x100;
 
 template <class Keys, class MakeSH1, class MakeSIMD, class MakeSH2>
 Handle *dispatch(int variant, int leaf, Keys &keys, size_t bucket, size_t threads) {
-    if (variant == SHGPU_SHOCKHASH) {
+    if (variant == (SHGPU_SHOCKHASH | SHH_ROTATION)) {
+#define C(L) if (leaf == L) return MakeSH1::template make<ShockHash<L, true>>(keys, bucket, threads);
+        SHH_LEAVES_SH1_ROT(C)
+#undef C
+    } else if (variant == (SHGPU_SIMD_SHOCKHASH | SHH_ROTATION)) {
+#define C(L) if (leaf == L) return MakeSIMD::template make<SIMDShockHash<L, true>>(keys, bucket, threads);
+        SHH_LEAVES_SIMD_ROT(C)
+#undef C
+    } else if (variant == SHGPU_SHOCKHASH) {
 #define C(L) if (leaf == L) return MakeSH1::template make<ShockHash<L, false>>(keys, bucket, threads);
         SHH_LEAVES_SH1(C)
 #undef C
@@ -170,7 +182,15 @@ void *shh_load(int variant, int leaf, const char *path, char *err, size_t errlen
     try {
         std::ifstream is(path, std::ios::binary);
         if (!is) throw std::runtime_error(std::string("cannot open ") + path);
-        if (variant == SHGPU_SHOCKHASH) {
+        if (variant == (SHGPU_SHOCKHASH | SHH_ROTATION)) {
+#define C(L) if (leaf == L) return make_loaded<ShockHash<L, true>>(is);
+            SHH_LEAVES_SH1_ROT(C)
+#undef C
+        } else if (variant == (SHGPU_SIMD_SHOCKHASH | SHH_ROTATION)) {
+#define C(L) if (leaf == L) return make_loaded<SIMDShockHash<L, true>>(is);
+            SHH_LEAVES_SIMD_ROT(C)
+#undef C
+        } else if (variant == SHGPU_SHOCKHASH) {
 #define C(L) if (leaf == L) return make_loaded<ShockHash<L, false>>(is);
             SHH_LEAVES_SH1(C)
 #undef C

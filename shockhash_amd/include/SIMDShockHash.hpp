@@ -13,25 +13,18 @@
 namespace shockhash {
 
 template <size_t LEAF_SIZE, bool ROTATION_FITTING = false>
-class SIMDShockHash : public detail::GpuRecSplit<SHGPU_SIMD_SHOCKHASH, LEAF_SIZE> {
+class SIMDShockHash : public detail::GpuRecSplit<SHGPU_SIMD_SHOCKHASH, LEAF_SIZE, ROTATION_FITTING> {
     static_assert(LEAF_SIZE <= MAX_LEAF_SIZE);
-    static void reject_rotation() {
-        if (ROTATION_FITTING)
-            throw gpu::Error(SHGPU_ERR_UNSUPPORTED, "ShockHash rotation fitting is not implemented by the GPU engine");
-    }
 
   public:
     SIMDShockHash() {}
     SIMDShockHash(std::span<const std::string> keys, const size_t bucket_size, size_t num_threads = 1) {
-        reject_rotation();
         this->construct_strings(keys, bucket_size, num_threads);
     }
     SIMDShockHash(std::span<hash128_t> keys, const size_t bucket_size, size_t num_threads = 1) {
-        reject_rotation();
         this->construct(keys.data(), keys.size(), bucket_size, num_threads);
     }
     SIMDShockHash(std::ifstream &input, const size_t bucket_size, size_t num_threads = 1) {
-        reject_rotation();
         std::vector<hash128_t> h;
         for (std::string key; std::getline(input, key);) h.push_back(spooky(key.c_str(), key.size(), 0));
         this->construct(h.data(), h.size(), bucket_size, num_threads);

@@ -12,23 +12,16 @@
 namespace shockhash {
 
 template <size_t LEAF_SIZE, bool ROTATION_FITTING = false>
-class ShockHash : public detail::GpuRecSplit<SHGPU_SHOCKHASH, LEAF_SIZE> {
+class ShockHash : public detail::GpuRecSplit<SHGPU_SHOCKHASH, LEAF_SIZE, ROTATION_FITTING> {
     static_assert(LEAF_SIZE <= MAX_LEAF_SIZE);
-    using Base = detail::GpuRecSplit<SHGPU_SHOCKHASH, LEAF_SIZE>;
-    static void reject_rotation() {
-        if (ROTATION_FITTING)
-            throw gpu::Error(SHGPU_ERR_UNSUPPORTED, "ShockHash rotation fitting is not implemented by the GPU engine");
-    }
 
   public:
     ShockHash() {}
     ShockHash(const std::vector<std::string> &keys, const size_t bucket_size, size_t num_threads = 1) {
-        reject_rotation();
         this->construct_strings(keys, bucket_size, num_threads);
     }
     // The reference sorts the caller's array in place; this engine leaves it unchanged.
     ShockHash(std::vector<hash128_t> &keys, const size_t bucket_size, size_t num_threads = 1) {
-        reject_rotation();
         this->construct(keys.data(), keys.size(), bucket_size, num_threads);
     }
     size_t operator()(const hash128_t &hash) { return this->query(hash); }

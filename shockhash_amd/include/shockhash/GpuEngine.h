@@ -81,10 +81,11 @@ class Engine {
 
     shgpu_engine *handle() { return e; }
 
-    Forest build(int variant, uint32_t leaf_size, const hash128_t *hashes, uint64_t n, uint64_t bucket_size) {
+    Forest build(int variant, uint32_t leaf_size, const hash128_t *hashes, uint64_t n, uint64_t bucket_size,
+                 bool rotation = false) {
         static_assert(sizeof(hash128_t) == 16, "hash128_t must be two packed 64-bit words");
         std::lock_guard<std::mutex> lock(mtx);
-        const shgpu_build_desc d = desc(variant, leaf_size, bucket_size);
+        const shgpu_build_desc d = desc(variant, leaf_size, bucket_size, rotation);
         shgpu_build *b = nullptr;
         check(shgpu_build_run_host(e, &d, reinterpret_cast<const uint64_t *>(hashes), n, &b), "shgpu_build_run");
         return collect(b, n);
@@ -93,7 +94,7 @@ class Engine {
     // String keys: packed and fingerprinted on the device (shgpu_hash_strings, SpookyHash V2 as
     // first_hash), then built from the device fingerprints.
     template <class Strings>
-    Forest build_strings(int variant, uint32_t leaf_size, const Strings &keys, uint64_t bucket_size) {
+    Forest build_strings(int variant, uint32_t leaf_size, const Strings &keys, uint64_t bucket_size, bool rotation = false) {
         std::lock_guard<std::mutex> lock(mtx);
         const uint64_t n = keys.size();
         std::vector<uint64_t> offsets(n + 1, 0);
@@ -108,7 +109,7 @@ class Engine {
         if (rc == SHGPU_OK) rc = shgpu_copy_to_device(e, d_off, offsets.data(), offsets.size() * 8);
         if (rc == SHGPU_OK) rc = shgpu_hash_strings(e, (const uint8_t *)d_data, (const uint64_t *)d_off, n, (uint64_t *)d_hashes);
         shgpu_build *b = nullptr;
-        const shgpu_build_desc d = desc(variant, leaf_size, bucket_size);
+        const shgpu_build_desc d = desc(variant, leaf_size, bucket_size, rotation);
         if (rc == SHGPU_OK) rc = shgpu_build_run(e, &d, (const uint64_t *)d_hashes, n, &b);
         const std::string msg = rc == SHGPU_OK ? "" : shgpu_last_error();
         if (d_data) shgpu_device_free(e, d_data);
@@ -139,8 +140,9 @@ class Engine {
     }
 
   private:
-    static shgpu_build_desc desc(int variant, uint32_t leaf_size, uint64_t bucket_size) {
+    static shgpu_build_desc desc(int variant, uint32_t leaf_size, uint64_t bucket_size, bool rotation) {
         shgpu_build_desc d{};
+        d.rotation_fitting = rotation ? 1 : 0;
         d.variant = variant;
         d.leaf_size = leaf_size;
         d.bucket_size = bucket_size;
@@ -167,10 +169,12 @@ class Engine {
   public:
     // The retrieval input of a forest (every leaf's orientation, in the reference's order) and its one-bit
     // retrieval structure, both computed on the device (shgpu_leaf_orientation, shgpu_ribbon_build) from the build's
-    // device-resident leaf-ordered keys; the structure comes back, and the input too when `keep` is set.
+    // device-resident leaf-ordered keys; the structure comes back, and the input too when `keep` is set. rot_leaf:
+    // the leaf size of a ROTATION_FITTING build (its full leaves' codes are x * leaf + r), else 0.
     void orientation_retrieval(int variant, const Forest &f, bool keep, std::vector<uint64_t> &input_keys,
                                std::vector<uint8_t> &input_bits, std::vector<uint32_t> &offsets,
-                               std::vector<uint8_t> &seeds, std::vector<uint64_t> &words, double *ms2 = nullptr) {
+                               std::vector<uint8_t> &seeds, std::vector<uint64_t> &words, double *ms2 = nullptr,
+                               uint32_t rot_leaf = 0) {
         std::lock_guard<std::mutex> lock(mtx);
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<shgpu_orient_task> tasks;
@@ -179,7 +183,10 @@ class Engine {
         for (size_t i = 0; i < f.nodes.size(); i++) {
             const shgpu_node &nd = f.nodes[i];
             if (nd.kind != SHGPU_NODE_LEAF) continue;
-            tasks.push_back({total, variant == SHGPU_SHOCKHASH2 ? f.codes[i] : f.codes[i] + nd.start_seed, nd.offset, nd.m});
+            if (variant != SHGPU_SHOCKHASH2 && rot_leaf != 0 && nd.m == rot_leaf && nd.m >= 2)  // ShockHash.h:401-402
+                tasks.push_back({total, f.codes[i] / nd.m + nd.start_seed, nd.offset, nd.m, (uint32_t)(f.codes[i] % nd.m) + 1, 0});
+            else
+                tasks.push_back({total, variant == SHGPU_SHOCKHASH2 ? f.codes[i] : f.codes[i] + nd.start_seed, nd.offset, nd.m, 0, 0});
             total += nd.m;
         }
         if (!f.device) throw Error(SHGPU_ERR_ARG, "leaf orientation: the forest holds no device build");

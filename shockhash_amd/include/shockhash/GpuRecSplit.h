@@ -15,10 +15,14 @@
 namespace shockhash {
 namespace detail {
 
-template <int VARIANT, size_t LEAF_SIZE>
+// ROT: ROTATION_FITTING of ShockHash / SIMDShockHash (full leaves carry x * LEAF_SIZE + r; include/ShockHash.h:
+// 251-265,316-403, include/SIMDShockHash.hpp:419-506,826-840).
+template <int VARIANT, size_t LEAF_SIZE, bool ROT = false>
 class GpuRecSplit {
   protected:
     static constexpr bool BIPARTITE = VARIANT == SHGPU_SHOCKHASH2;
+    static_assert(!(ROT && BIPARTITE), "rotation fitting applies to ShockHash and SIMDShockHash leaves");
+    static constexpr uint64_t GROUP_A_HF_MASK = LEAF_SIZE > 32 ? 7u : 0u;  // include/ShockHash.h:141
     static constexpr size_t _leaf = LEAF_SIZE;
     static constexpr size_t lower_aggr = Splitting<LEAF_SIZE, BIPARTITE>::lower_aggr;
     static constexpr size_t upper_aggr = Splitting<LEAF_SIZE, BIPARTITE>::upper_aggr;
@@ -40,14 +44,14 @@ class GpuRecSplit {
     double phase_ms[3] = {0, 0, 0};  // Golomb-Rice + Elias-Fano, leaf orientation (device), retrieval (device)
 
     void construct(const hash128_t *h, size_t n, size_t bsize, size_t num_threads) {
-        construct_forest(gpu::Engine::instance().build(VARIANT, (uint32_t)LEAF_SIZE, h, n, bsize), n, bsize, num_threads);
+        construct_forest(gpu::Engine::instance().build(VARIANT, (uint32_t)LEAF_SIZE, h, n, bsize, ROT), n, bsize, num_threads);
     }
 
     // String keys: fingerprinted on the device (first_hash, include/ShockHash.h:129,159-185).
     template <class Strings>
     void construct_strings(const Strings &keys, size_t bsize, size_t num_threads) {
-        construct_forest(gpu::Engine::instance().build_strings(VARIANT, (uint32_t)LEAF_SIZE, keys, bsize), keys.size(), bsize,
-                         num_threads);
+        construct_forest(gpu::Engine::instance().build_strings(VARIANT, (uint32_t)LEAF_SIZE, keys, bsize, ROT), keys.size(),
+                         bsize, num_threads);
     }
 
     // num_threads (the reference's host parallelism) has no host phase left to split: the per-key work
@@ -66,7 +70,7 @@ class GpuRecSplit {
         std::vector<uint8_t> rseeds;
         std::vector<uint64_t> rwords;
         gpu::Engine::instance().orientation_retrieval(VARIANT, f, gpu::keep_retrieval_input, ribbon_keys, ribbon_bits,
-                                                      roffsets, rseeds, rwords, phase_ms + 1);
+                                                      roffsets, rseeds, rwords, phase_ms + 1, ROT ? (uint32_t)LEAF_SIZE : 0u);
         ribbon = RibbonRetrieval(std::move(roffsets), std::move(rseeds), std::move(rwords));
         f.device.reset();  // frees the build's device keys
         forest_meta = std::move(f);
@@ -125,7 +129,17 @@ class GpuRecSplit {
         const uint64_t retrieved = ribbon.retrieve(hash.second);
         if (BIPARTITE) return cum_keys + sh2Cell((uint32_t)m, b, hash.second, retrieved);
         if (m <= 1) return cum_keys;
-        const uint64_t seed = b + (VARIANT == SHGPU_SIMD_SHOCKHASH ? start_seed[NUM_START_SEEDS - 1] : start_seed[level]);
+        const uint64_t leaf_seed = VARIANT == SHGPU_SIMD_SHOCKHASH ? start_seed[NUM_START_SEEDS - 1] : start_seed[level];
+        if (ROT && m == LEAF_SIZE) {  // include/ShockHash.h:251-265, include/SIMDShockHash.hpp:826-840
+            const uint64_t r = b % LEAF_SIZE, x = b / LEAF_SIZE + leaf_seed;
+            if ((hash.second & 1) == 0) {
+                const CandidateCells c = sh1CandidateCells(hash.second, x & ~GROUP_A_HF_MASK, (uint32_t)m);
+                return cum_keys + (retrieved == 0 ? c.cell1 : c.cell2);
+            }
+            const CandidateCells c = sh1CandidateCells(hash.second, x, (uint32_t)m);
+            return cum_keys + ((retrieved == 0 ? c.cell1 : c.cell2) + r) % LEAF_SIZE;
+        }
+        const uint64_t seed = b + leaf_seed;
         const CandidateCells c = sh1CandidateCells(hash.second, seed, (uint32_t)m);
         return cum_keys + (retrieved == 0 ? c.cell1 : c.cell2);
     }
@@ -144,12 +158,13 @@ class GpuRecSplit {
 
     // Serialized form of the whole function (not in the reference interface, whose only stream I/O is the
     // descriptor bit vector's, include/shockhash/RiceBitVector.h:133-141): a header (magic, format version,
-    // variant, leaf size, bucket size, bucket count, key count), the descriptor words, the two Elias-Fano
+    // variant (+ 0x100 for rotation fitting), leaf size, bucket size, bucket count, key count), the descriptor words, the two Elias-Fano
     // sequences and the retrieval structure, all little-endian. load() restores a function that answers every
     // query as the saved one; it throws std::runtime_error on a stream of another class or leaf size.
     static constexpr uint64_t FORMAT_MAGIC = 0x3146485048534853ull;  // "SHSHPHF1"
     void save(std::ostream &os) const {
-        const uint64_t hdr[7] = {FORMAT_MAGIC, 1, (uint64_t)VARIANT, (uint64_t)LEAF_SIZE, bucket_size, nbuckets, keys_count};
+        const uint64_t hdr[7] = {FORMAT_MAGIC, 1, (uint64_t)VARIANT | (ROT ? 0x100u : 0u), (uint64_t)LEAF_SIZE, bucket_size,
+                                 nbuckets, keys_count};
         os.write((const char *)hdr, sizeof(hdr));
         descriptors.save(os);
         ef.save(os);
@@ -160,7 +175,7 @@ class GpuRecSplit {
         uint64_t hdr[7] = {0, 0, 0, 0, 0, 0, 0};
         is.read((char *)hdr, sizeof(hdr));
         if (!is || hdr[0] != FORMAT_MAGIC || hdr[1] != 1) throw std::runtime_error("ShockHash load: not a saved function");
-        if (hdr[2] != (uint64_t)VARIANT || hdr[3] != (uint64_t)LEAF_SIZE)
+        if (hdr[2] != ((uint64_t)VARIANT | (ROT ? 0x100u : 0u)) || hdr[3] != (uint64_t)LEAF_SIZE)
             throw std::runtime_error("ShockHash load: the stream holds another class or leaf size");
         bucket_size = hdr[4];
         nbuckets = hdr[5];

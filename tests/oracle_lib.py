@@ -42,6 +42,9 @@ _SIGS = {
     "orc_split_ok": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, U32, U64]),
     "orc_golomb_memo": (None, [ctypes.c_int, ctypes.c_int, P]),
     "orc_build": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]),
+    "orc_build_rot": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]),
+    "orc_sh1_leaf_rot": (None, [P, U32, U64, ctypes.POINTER(U64), ctypes.POINTER(U32), P]),
+    "orc_sh1_rot_query_cell": (U64, [U32, U64, U64, U64, U64]),
     "orc_build_buckets": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, P, ctypes.c_size_t,
                               ctypes.c_int]),
     "orc_build_free": (None, [P]),
@@ -98,6 +101,16 @@ def sh1_leaf(keys, start=0, with_cells=False):
     cells = np.zeros(len(keys), dtype=np.int32)
     x = lib().orc_sh1_leaf(ptr(keys), len(keys), start, ptr(cells))
     return (x, cells) if with_cells else x
+
+
+def sh1_leaf_rot(keys, start=0):
+    """Rotation-fitted full leaf (ShockHash.h:316-403): (absolute x, rotation r, leaf index of the key in each
+    cell)."""
+    keys = u64(keys)
+    cells = np.zeros(len(keys), dtype=np.int32)
+    x, r = ctypes.c_uint64(), ctypes.c_uint32()
+    lib().orc_sh1_leaf_rot(ptr(keys), len(keys), start, ctypes.byref(x), ctypes.byref(r), ptr(cells))
+    return x.value, r.value, cells
 
 
 def sh1_leaf_batch(keys2d, start=0, threads=8):
@@ -180,11 +193,12 @@ def sh1_valid_batch(keys2d, seeds, threads=8):
 class OracleBuild:
     """Full RecSplit construction by the oracle: descriptors, EF inputs, ribbon input, node codes."""
 
-    def __init__(self, variant, leaf, hashes, bucket_size, threads=1):
+    def __init__(self, variant, leaf, hashes, bucket_size, threads=1, rotation=False):
         hashes = u64(hashes).reshape(-1, 2)
         self.n = hashes.shape[0]
         L = lib()
-        self.h = L.orc_build(VARIANT[variant], leaf, ptr(hashes), self.n, bucket_size, threads)
+        make = L.orc_build_rot if rotation else L.orc_build
+        self.h = make(VARIANT[variant], leaf, ptr(hashes), self.n, bucket_size, threads)
         if not self.h:
             raise RuntimeError("oracle build failed")
         self.nbuckets = L.orc_build_nbuckets(self.h)

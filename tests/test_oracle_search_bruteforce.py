@@ -183,3 +183,67 @@ def test_sh1_valid_batch():
         trial = np.where(seeds > s, np.uint64(s), seeds + np.uint64(1 << 40))
         ok = O.sh1_valid_batch(keys, trial)
         assert not ok[seeds > s].any()
+
+
+def rot_bits(v, r, m):
+    """Rotation of an m-bit mask by r (ShockHash.h:297-299)."""
+    allm = (1 << m) - 1
+    return ((v << r) | (v >> (m - r))) & allm if r else v & allm
+
+
+@pytest.mark.parametrize("m", [3, 4, 6, 8, 10, 12, 33])
+def test_sh1_rotation_fitting_minimal(m):
+    """ROTATION_FITTING leaves (ShockHash.h:316-403): the first (x, r) in order whose A cells (seed x & ~mask, mask 7
+    above 32 keys) and B cells (seed x, rotated by r) admit a perfect matching of keys to cells; the oracle's
+    placement puts every key in one of its own (rotated) cells."""
+    mask = 7 if m > 32 else 0
+    for trial in range(3 if m < 20 else 1):
+        keys = [int(k) for k in workload.fingerprints(m, seed=5000 + 10 * m + trial)[:, 1]]
+        start = [0, 12345, 2**40 + 7][trial]
+        x = start
+        found = None
+        while found is None:
+            ca = {i: cells_sh1(k, x & ~mask, m) for i, k in enumerate(keys) if k % 2 == 0}
+            cb = {i: cells_sh1(k, x, m) for i, k in enumerate(keys) if k % 2 == 1}
+            a = b = 0
+            for p in ca.values():
+                a |= (1 << p[0]) | (1 << p[1])
+            for p in cb.values():
+                b |= (1 << p[0]) | (1 << p[1])
+            for r in range(m):
+                if a | rot_bits(b, r, m) != (1 << m) - 1:
+                    continue
+                edges = [ca[i] if i in ca else tuple((c + r) % m for c in cb[i]) for i in range(m)]
+                if has_perfect_matching(edges, m):
+                    found = (x, r, edges)
+                    break
+            x += 1
+        gx, gr, by_cell = O.sh1_leaf_rot(np.array(keys, dtype=np.uint64), start)
+        assert (gx, gr) == found[:2]
+        assert sorted(by_cell) == list(range(m))
+        for cell, i in enumerate(by_cell):
+            assert cell in found[2][i]
+        # the query's cell of every key equals its placement, given the bit the construction stores
+        code = (gx - start) * m + gr
+        for cell, i in enumerate(by_cell):
+            k = keys[i]
+            own = (cell - gr) % m if k & 1 else cell
+            bit = 0 if own < m // 2 else 1
+            assert L.orc_sh1_rot_query_cell(m, code, start, k, bit) == cell
+
+
+@pytest.mark.parametrize("variant,leaf", [("sh1", 8), ("sh1", 16), ("simd", 12), ("sh1", 34)])
+def test_rotation_build_is_a_bijection(variant, leaf):
+    """A whole ROTATION_FITTING build (ShockHash.h:316-403; SIMDShockHash.hpp:419-506): full leaves carry codes
+    x * leaf + r, shorter leaves the plain search, and the oracle's queries map the keys onto 0..n-1."""
+    n = 3000 if leaf > 20 else 6000
+    h = workload.fingerprints(n, seed=77 + leaf)
+    b = O.OracleBuild(variant, leaf, h, 100 if leaf > 20 else 60, rotation=True)
+    q = b.query(h)
+    assert len(set(q.tolist())) == n and int(q.max()) == n - 1
+    full = (b.node_kind == 3) & (b.node_m == leaf)
+    assert full.any()
+    plain = O.OracleBuild(variant, leaf, h, 100 if leaf > 20 else 60)
+    # same tree shapes; rotation codes differ from the plain leaves' seeds only on full leaves
+    assert np.array_equal(b.node_m, plain.node_m) and np.array_equal(b.node_kind, plain.node_kind)
+    assert np.array_equal(b.node_code[~full], plain.node_code[~full])
