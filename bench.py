@@ -75,7 +75,7 @@ def native_oracle():
     return out, "g++ -O3 -march=native"
 
 
-def cpu_baseline(h, threads, variant, leaf, reps=3):
+def cpu_baseline(h, threads, variant, leaf, reps=3, rotation=False):
     """The oracle's construction of the fingerprints h (the CPU baseline), reps times: (last build, median s,
     all times)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -84,7 +84,7 @@ def cpu_baseline(h, threads, variant, leaf, reps=3):
     for _ in range(reps):
         ob = None
         t0 = time.perf_counter()
-        ob = O.OracleBuild(variant, leaf, h, 2000, threads=threads)
+        ob = O.OracleBuild(variant, leaf, h, 2000, threads=threads, rotation=rotation)
         times.append(time.perf_counter() - t0)
     return ob, float(np.median(times)), times
 
@@ -268,6 +268,8 @@ def main():
     ap.add_argument("--variant", choices=sorted(VARIANT_NAMES), default="sh2",
                     help="class variant of the build (the headline workload is sh2 with leaf 64)")
     ap.add_argument("--leaf", type=int, default=64, help="leaf size")
+    ap.add_argument("--rotation", action="store_true",
+                    help="ROTATION_FITTING build (ShockHash / SIMDShockHash: full leaves rotation-fitted)")
     ap.add_argument("--plain-buckets", type=int, default=100, help="plain-form buckets (0: no plain-form run)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="CPU oracle sample size (0: the whole workload)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: this process's CPU share)")
@@ -312,6 +314,8 @@ def main():
                                 ("sh1_seeds_per_lane", args.leaf_seeds_per_lane),
                                 ("waves_per_cu", args.waves_per_cu)) if v is not None}
     V, L = args.variant, args.leaf
+    if args.rotation and V == "sh2":
+        ap.error("--rotation applies to the sh1 and simd variants")
     headline = V == "sh2" and L == 64
     if tune:
         eng.set_variant_tuning(V, **tune)
@@ -336,13 +340,13 @@ def main():
         dst.free()
 
     for i in range(args.warmup):
-        b = eng.build_dev(d_h, args.n, V, L, 2000, copy_keys=False)
+        b = eng.build_dev(d_h, args.n, V, L, 2000, copy_keys=False, rotation=args.rotation)
         log(f"warmup {i}: timings {np.round(b.timings_ms, 2)}")
     step_s, kernel_ms = [], []
     for i in range(args.steps):
         eng.sync()
         t0 = time.perf_counter()
-        b = eng.build_dev(d_h, args.n, V, L, 2000, copy_keys=False)
+        b = eng.build_dev(d_h, args.n, V, L, 2000, copy_keys=False, rotation=args.rotation)
         eng.sync()
         step_s.append(time.perf_counter() - t0)
         kernel_ms.append(b.timings_ms[3])
@@ -351,14 +355,15 @@ def main():
     ms_per_step = 1e3 * float(np.mean(step_s))
     value = args.n / (ms_per_step / 1e3)
     kms = float(np.mean(kernel_ms))
-    roof = roofline(b.nodes, b.codes, V, kms, MEASURED_TRAFFIC_BYTES if headline and args.n == 10_000_000 else None)
+    roof = roofline(b.nodes, b.codes, V, kms, MEASURED_TRAFFIC_BYTES if headline and args.n == 10_000_000 else None,
+                    rotation_leaf=L if args.rotation else 0)
 
     # plain form: one launch per node over the first buckets, same device input
     plain_value, n_plain_keys, plain_match = None, 0, True
     if args.plain_buckets:
         t0 = time.perf_counter()
         bp = eng.build_dev(d_h, args.n, V, L, 2000, mode="plain", plain_bucket_limit=args.plain_buckets,
-                           copy_keys=False)
+                           copy_keys=False, rotation=args.rotation)
         plain_s = time.perf_counter() - t0
         n_plain_keys = int(b.size_acc[args.plain_buckets])
         nn = len(bp.codes)
@@ -383,9 +388,9 @@ def main():
             sample = f"the whole workload ({args.n} keys, seed 1)"
         else:
             hs = workload.fingerprints(n_cpu, seed=7)
-            bs = eng.build(hs, V, L, 2000)
+            bs = eng.build(hs, V, L, 2000, rotation=args.rotation)
             sample = f"{n_cpu} fingerprints (same generator, seed 7)"
-        ob, dt, times = cpu_baseline(hs, threads, V, L, reps=max(1, args.cpu_reps))
+        ob, dt, times = cpu_baseline(hs, threads, V, L, reps=max(1, args.cpu_reps), rotation=args.rotation)
         bit_exact = bit_exact and bool(np.array_equal(bs.codes, ob.node_code)) and \
             bool(np.array_equal(bs.size_acc, ob.size_acc))
         cpu = {"value": n_cpu / dt, "unit": "keys/s", "cores": threads, "kind": "port",
@@ -425,7 +430,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"{VARIANT_NAMES[V]} leaf {L}, bucket 2000, "
+        "config": {"workload": f"{VARIANT_NAMES[V]}{'<ROTATION_FITTING>' if args.rotation else ''} leaf {L}, bucket 2000, "
                                f"{f'{args.n // 1_000_000}M' if args.n % 1_000_000 == 0 else args.n} uniform 128-bit fingerprints "
                                "(SplitMix64, seed 1), persistent device-queue build", "keys": args.n, "leaf_size": L,
                    "bucket_size": 2000, "variant": VARIANT_NAMES[V],

@@ -7,6 +7,9 @@ as the reference's sequential loops make them (every key for every seed up to th
 exit, include/ShockHash.h:414-418,457-459):
 
 * split node, ShockHash1 leaf: m * (code + 1);
+* ROTATION_FITTING leaf (m = leaf size, code = x * m + r): per x the odd keys (group B, ~m/2) are hashed, the even
+  ones (group A) only when x & mask is 0 (mask 7 above 32 keys, 0 below; include/ShockHash.h:340-352), so
+  m/2 * (x + 1) + m/2 * ceil((x + 1) / (mask + 1)); the m rotation checks per x are bit operations, not counted;
 * ShockHash2 leaf, m >= 10: m * (s* + 1) half-seed evaluations, where (c_j, c_i) = unpairTriangular(code),
   (a, b) = unpairElegant(c_j) and s* = max(a, b) is the last half-seed the candidate search needs
   (include/shockhash/ShockHash2-internal.h:435-470); m < 10: m * (c_j + 1).
@@ -74,8 +77,9 @@ def unpair_elegant(z):
     return np.where(r < f, r, f), np.where(r < f, f, r - f)
 
 
-def evaluations(nodes, codes, variant):
-    """Per-node hash evaluations a sequential search makes to reach the stored codes (float64)."""
+def evaluations(nodes, codes, variant, rotation_leaf=0):
+    """Per-node hash evaluations a sequential search makes to reach the stored codes (float64); rotation_leaf: the
+    leaf size of a ROTATION_FITTING build (0: none)."""
     m = nodes["m"].astype(np.float64)
     kind = nodes["kind"]
     codes = codes.astype(np.float64)
@@ -89,12 +93,17 @@ def evaluations(nodes, codes, variant):
         ev[leaf] = np.where(ml >= 10, ml * (s_star + 1), ml * (cj + 1))
     else:
         ev[leaf] = np.where(m[leaf] >= 2, m[leaf] * (codes[leaf] + 1), 0.0)
+        if rotation_leaf:
+            rot = leaf & (nodes["m"] == rotation_leaf)
+            x1 = np.floor(codes[rot] / rotation_leaf) + 1
+            per = 8.0 if rotation_leaf > 32 else 1.0
+            ev[rot] = rotation_leaf / 2 * x1 + rotation_leaf / 2 * np.ceil(x1 / per)
     return ev
 
 
-def valu_ops(nodes, codes, variant):
+def valu_ops(nodes, codes, variant, rotation_leaf=0):
     """(evaluations, VALU ops, evaluations in leaves) of a build."""
-    ev = evaluations(nodes, codes, variant)
+    ev = evaluations(nodes, codes, variant, rotation_leaf)
     modes = split_modes(nodes)
     flavour = "simd" if variant == "simd" else "sux"
     price = np.array([SPLIT_OPS[flavour][md] if md else 0.0 for md in modes])
@@ -103,9 +112,9 @@ def valu_ops(nodes, codes, variant):
     return float(ev.sum()), float((ev * price).sum()), float(ev[leaf].sum())
 
 
-def roofline(nodes, codes, variant, kernel_ms, traffic_bytes=None):
+def roofline(nodes, codes, variant, kernel_ms, traffic_bytes=None, rotation_leaf=0):
     """The bench's roofline record for one search kernel launch of kernel_ms milliseconds."""
-    evals, ops, evals_leaf = valu_ops(nodes, codes, variant)
+    evals, ops, evals_leaf = valu_ops(nodes, codes, variant, rotation_leaf)
     achieved = ops / (kernel_ms / 1e3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": INT32_PEAK_TOPS, "unit": "TOP/s",
             "frac": achieved / INT32_PEAK_TOPS, "traffic": traffic_bytes,
