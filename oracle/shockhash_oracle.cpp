@@ -26,7 +26,9 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <string>
 #include <stdexcept>
 #include <thread>
 #include <unordered_map>
@@ -903,6 +905,198 @@ static void spooky128(const uint8_t *p, size_t len, uint64_t &o1, uint64_t &o2) 
 
 // ================================================================ C ABI (ctypes)
 using namespace orc;
+
+// ---------------------------------------------------------------- ShockHash2Flat (include/ShockHash2Flat.h)
+// k-perfect bucketing of the keys' 64-bit hashes (mhc) into N / k buckets of exactly k keys, ShockHash2's leaf
+// search inside each bucket, and a ShockHash2<k> fallback over the keys the thresholds bump. Restated from
+// ShockHash2Flat.h:37-302 for keys given by their mhc (the reference computes mhc = MurmurHash64(key) of string
+// keys at :70-75; that hash is an un-vendored dependency and stays outside this restatement).
+struct Flat {
+    uint32_t k = 0, tbits = 0, trange = 0, seed_bits = 0;
+    uint64_t max_seed = 0;
+    size_t N = 0, nbuckets = 0, layers = 2;
+    std::vector<uint32_t> mapping;               // THRESHOLD_MAPPING
+    std::vector<uint32_t> thr;                   // compacted threshold per bucket
+    std::vector<uint64_t> seed;                  // stored seed per bucket (0: in seed_fb)
+    std::map<size_t, uint64_t> seed_fb;          // seedsFallback
+    std::vector<size_t> layer_bases;             // layerBases
+    Build *fallback = nullptr;                   // fallbackPhf: ShockHash2<k> over the decimal strings of mhc
+    std::vector<size_t> free_pos;                // freePositions (select1(i + 1) - i of freePositionsBv)
+    std::unordered_map<uint64_t, uint8_t> retr;  // the ribbon's answers for the keys
+    size_t bumped = 0;                           // keys handed to the fallback
+    ~Flat() { delete fallback; }
+};
+
+static uint32_t floor_log2(uint64_t x) { uint32_t r = 0; while (x >>= 1) r++; return r; }
+
+// _fill_mapping<THRESHOLD_RANGE> (ShockHash2Flat.h:17-35).
+static std::vector<uint32_t> flat_mapping(uint32_t range) {
+    const uint64_t one_third = 0xFFFFFFFFull / 3;
+    std::vector<uint32_t> a(range, 0);
+    if (range == 1) return a;
+    if (range == 2) { a[0] = 0; a[1] = 0xFFFFFFFFu; return a; }
+    a[0] = 0;
+    a[1] = (uint32_t)one_third;
+    const uint64_t ir = range - 3;
+    for (uint64_t i = 0; i < ir; i++) a[2 + i] = (uint32_t)(2 * one_third + one_third * i / ir);
+    a[range - 1] = 0xFFFFFFFFu;
+    return a;
+}
+
+// compact_threshold (:191-199).
+static uint32_t flat_compact(const Flat &f, uint32_t t) {
+    for (uint32_t i = 0; i < f.trange; i++)
+        if (t <= f.mapping[i]) return i;
+    return f.mapping.back();
+}
+
+struct FlatKey { uint64_t mhc; uint32_t bucket, threshold; };  // KeyInfo (ShockHash2FlatBase.h)
+
+static inline uint64_t flat_fallback_query(const Flat &f, uint64_t mhc) {
+    const std::string s = std::to_string(mhc);
+    uint64_t h0 = 0, h1 = 0;
+    spooky128((const uint8_t *)s.data(), s.size(), h0, h1);
+    return query(*f.fallback, h1, h0);  // sux spooky() = {h1, h0} = {first, second}
+}
+
+// evaluateKPerfect (:270-299): (bucket, stored seed).
+static std::pair<size_t, uint64_t> flat_evaluate(const Flat &f, uint64_t mhc) {
+    for (size_t layer = 0; layer < f.layers; layer++) {
+        if (layer != 0) mhc = remix(mhc);
+        const size_t base = f.layer_bases[layer], size = f.layer_bases[layer + 1] - base;
+        const uint32_t bucket = fastrange32((uint32_t)mhc, (uint32_t)size);
+        const uint32_t t = (uint32_t)(mhc >> 32);
+        if (t <= f.mapping[f.thr[base + bucket]]) return {base + bucket, f.seed[base + bucket]};
+    }
+    const size_t phf = flat_fallback_query(f, mhc);
+    const size_t bucket = f.free_pos[phf];
+    if (bucket >= f.nbuckets) return {bucket - f.nbuckets + (size_t)f.k * f.nbuckets, 1};
+    return {bucket, f.seed[bucket]};
+}
+
+// Constructor (:64-172) from the keys' mhc values.
+static Flat *flat_build(uint32_t k, const uint64_t *mhcs, size_t n, int threads) {
+    Flat *f = new Flat();
+    f->k = k;
+    f->tbits = floor_log2(k) - 1;
+    f->trange = 1u << f->tbits;
+    f->seed_bits = (uint32_t)std::ceil(0.442 * k - 0.2);
+    f->max_seed = 1ull << f->seed_bits;
+    f->mapping = flat_mapping(f->trange);
+    f->N = n;
+    f->nbuckets = n / k;
+    const size_t keys_in_end = n - f->nbuckets * k;
+    size_t buckets_this_layer = (size_t)std::ceil(0.9 * f->nbuckets);
+    std::vector<FlatKey> hashes(n);
+    for (size_t i = 0; i < n; i++)
+        hashes[i] = {mhcs[i], fastrange32((uint32_t)mhcs[i], (uint32_t)buckets_this_layer), (uint32_t)(mhcs[i] >> 32)};
+    std::vector<FlatKey> all = hashes;
+    f->layer_bases.push_back(0);
+    f->thr.assign(f->nbuckets, 0);
+    f->seed.assign(f->nbuckets, 0);
+    if (buckets_this_layer == 0) f->layers = 0;
+    for (size_t layer = 0; layer < f->layers; layer++) {
+        const size_t layer_base = f->layer_bases.back();
+        if (layer != 0) {
+            buckets_this_layer = (size_t)(0.9 * (double)(hashes.size() / k));
+            buckets_this_layer = std::min(buckets_this_layer, f->nbuckets - layer_base);
+            if (buckets_this_layer == 0) { f->layers = 1; break; }
+            for (auto &h : hashes) {  // rehash
+                h.mhc = remix(h.mhc);
+                h.bucket = fastrange32((uint32_t)h.mhc, (uint32_t)buckets_this_layer);
+                h.threshold = (uint32_t)(h.mhc >> 32);
+            }
+        }
+        f->layer_bases.push_back(layer_base + buckets_this_layer);
+        // sort_keyInfo (src/Sorter.cpp:10-12): by bucket, then threshold
+        std::stable_sort(hashes.begin(), hashes.end(), [](const FlatKey &a, const FlatKey &b) {
+            return ((uint64_t)a.bucket << 32 | a.threshold) < ((uint64_t)b.bucket << 32 | b.threshold);
+        });
+        std::vector<FlatKey> bumped;
+        // flushBucket (:201-228)
+        auto flush = [&](size_t start, size_t end, size_t bidx) {
+            const size_t size = end - start, b = layer_base + bidx;
+            if (size <= k) {
+                f->thr[b] = f->trange - 1;
+                for (size_t q = size; q < k; q++) f->free_pos.push_back(b);
+                return;
+            }
+            const uint32_t last = flat_compact(*f, hashes[start + k - 1].threshold);
+            const uint32_t first_bumped = flat_compact(*f, hashes[start + k].threshold);
+            if (first_bumped == last && last == 0) throw std::runtime_error("ShockHash2Flat: threshold below 0");
+            const uint32_t t = first_bumped == last ? last - 1 : last;
+            f->thr[b] = t;
+            const uint32_t uncompressed = f->mapping[t];
+            for (size_t l = 0; l < size; l++)
+                if (hashes[start + l].threshold > uncompressed) {
+                    bumped.push_back(hashes[start + l]);
+                    if (l < k) f->free_pos.push_back(b);
+                }
+        };
+        size_t bucket_start = 0, prev = 0;
+        for (size_t i = 0; i < hashes.size(); i++) {
+            while (hashes[i].bucket != prev) { flush(bucket_start, i, prev); prev++; bucket_start = i; }
+        }
+        while (prev < buckets_this_layer) { flush(bucket_start, hashes.size(), prev); prev++; bucket_start = hashes.size(); }
+        hashes = bumped;
+    }
+    // fallback PHF over the bumped keys (:124-129)
+    f->bumped = hashes.size();
+    std::vector<uint64_t> fh(2 * hashes.size());
+    for (size_t i = 0; i < hashes.size(); i++) {
+        const std::string s = std::to_string(hashes[i].mhc);
+        uint64_t h0 = 0, h1 = 0;
+        spooky128((const uint8_t *)s.data(), s.size(), h0, h1);
+        fh[2 * i] = h1;
+        fh[2 * i + 1] = h0;
+    }
+    f->fallback = build(SH2, k, fh.data(), hashes.size(), 2000, threads);
+    // free positions of the buckets no layer handled and of the last, partial bucket (:130-146)
+    const size_t additional = hashes.size() - f->free_pos.size(), handled = f->layer_bases.back();
+    {
+        size_t i = 0;
+        for (; i < additional - keys_in_end; i++) f->free_pos.push_back(handled + i / k);
+        for (; i < additional; i++) f->free_pos.push_back(f->nbuckets + i);
+    }
+    // base case in every bucket (:148-169): keys in input order, ShockHash2 leaf search and its retrieval bits
+    std::vector<std::vector<uint64_t>> contents(f->nbuckets);
+    for (const FlatKey &key : all) {
+        const size_t b = flat_evaluate(*f, key.mhc).first;
+        if (b < f->nbuckets) contents[b].push_back(key.mhc);
+    }
+    for (auto &c : contents)
+        if (c.size() != k) throw std::runtime_error("ShockHash2Flat: a bucket does not hold k keys");
+    std::vector<uint64_t> seeds(f->nbuckets);
+    std::vector<std::vector<uint8_t>> bits(f->nbuckets, std::vector<uint8_t>(k));
+    std::vector<std::thread> th;
+    const int nt = std::max(1, threads);
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t]() {
+            for (size_t b = t; b < f->nbuckets; b += nt) {
+                seeds[b] = sh2_find_seed(contents[b].data(), k);
+                sh2_retrieval(contents[b].data(), k, seeds[b], bits[b].data());
+            }
+        });
+    for (auto &x : th) x.join();
+    for (size_t b = 0; b < f->nbuckets; b++) {
+        for (uint32_t q = 0; q < k; q++) f->retr[contents[b][q]] = bits[b][q];
+        uint64_t sd = seeds[b];
+        if (sd >= f->max_seed || sd == 0) { f->seed_fb[b] = sd; sd = 0; }
+        f->seed[b] = sd;
+    }
+    return f;
+}
+
+// operator()(uint64_t) (:254-267).
+static uint64_t flat_query(const Flat &f, uint64_t mhc) {
+    auto [bucket, sd] = flat_evaluate(f, mhc);
+    if (bucket >= f.nbuckets) return bucket;
+    if (sd == 0) sd = f.seed_fb.at(bucket);
+    auto it = f.retr.find(mhc);
+    const uint64_t bit = it == f.retr.end() ? 0 : it->second;
+    return bucket * f.k + sh2_query_cell(f.k, sd, mhc, bit);
+}
+
 extern "C" {
 
 uint64_t orc_remix(uint64_t z) { return remix(z); }
@@ -1092,4 +1286,26 @@ void orc_build_query(void *b, const uint64_t *hashes, size_t n, uint64_t *out) {
     for (size_t i = 0; i < n; i++) out[i] = query(*x, hashes[2 * i], hashes[2 * i + 1]);
 }
 
+
+// ShockHash2Flat<k> built from the keys' mhc values (ShockHash2Flat.h:64-172), queried by mhc (:254-267).
+void *orc_flat_build(uint32_t k, const uint64_t *mhcs, size_t n, int threads) {
+    try { return flat_build(k, mhcs, n, threads); }
+    catch (...) { return nullptr; }
+}
+void orc_flat_free(void *f) { delete (Flat *)f; }
+void orc_flat_query(void *f, const uint64_t *mhcs, size_t n, uint64_t *out) {
+    for (size_t i = 0; i < n; i++) out[i] = flat_query(*(Flat *)f, mhcs[i]);
+}
+// nbuckets, layers, layer bases (3 entries), keys handed to the fallback, seeds kept in the fallback map
+void orc_flat_info(void *fp, uint64_t *out7) {
+    const Flat &f = *(Flat *)fp;
+    out7[0] = f.nbuckets; out7[1] = f.layers;
+    for (int i = 0; i < 3; i++) out7[2 + i] = i < (int)f.layer_bases.size() ? f.layer_bases[i] : 0;
+    out7[5] = f.bumped; out7[6] = f.seed_fb.size();
+}
+// (bucket, stored seed) of each key (evaluateKPerfect) and THRESHOLD_MAPPING of k
+void orc_flat_evaluate(void *f, const uint64_t *mhcs, size_t n, uint64_t *bucket, uint64_t *seed) {
+    for (size_t i = 0; i < n; i++) { auto r = flat_evaluate(*(Flat *)f, mhcs[i]); bucket[i] = r.first; seed[i] = r.second; }
+}
+void orc_flat_mapping(uint32_t range, uint32_t *out) { auto m = flat_mapping(range); std::memcpy(out, m.data(), m.size() * 4); }
 }  // extern "C"

@@ -31,6 +31,12 @@ _SIGS = {
     "shh_last_status": (ctypes.c_int, []),
     "shh_save": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
     "shh_load": (P, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+    "shh_flat_build": (P, [ctypes.c_int, P, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]),
+    "shh_flat_free": (None, [P]),
+    "shh_flat_query": (ctypes.c_int, [P, P, ctypes.c_size_t, P, ctypes.c_char_p, ctypes.c_size_t]),
+    "shh_flat_info": (None, [P, P]),
+    "shh_flat_nbuckets": (ctypes.c_size_t, [P]),
+    "shh_flat_codes": (None, [P, P]),
 }
 VARIANTS = {"sh1": 0, "simd": 1, "sh2": 2}
 ROTATION = 0x100  # ROTATION_FITTING = true instantiations (ShockHash / SIMDShockHash)
@@ -171,3 +177,47 @@ class MPHF:
             self.h = None
 
     __del__ = close
+
+
+class FlatMPHF:
+    """A ShockHash2Flat<k> built by the C++ class template from 64-bit key hashes (mhc)."""
+
+    def __init__(self, k, mhcs):
+        lib = load()
+        err = ctypes.create_string_buffer(512)
+        arr = np.ascontiguousarray(mhcs, dtype=np.uint64)
+        self.n = len(arr)
+        h = lib.shh_flat_build(k, arr.ctypes.data_as(P), len(arr), err, 512)
+        if not h:
+            raise ConstructionError(lib.shh_last_status(), err.value.decode(errors="replace"))
+        self.h = h
+
+    def __call__(self, mhcs):
+        arr = np.ascontiguousarray(mhcs, dtype=np.uint64)
+        out = np.zeros(len(arr), dtype=np.uint64)
+        err = ctypes.create_string_buffer(512)
+        if load().shh_flat_query(self.h, arr.ctypes.data_as(P), len(arr), out.ctypes.data_as(P), err, 512) != 0:
+            raise RuntimeError(err.value.decode(errors="replace"))
+        return out
+
+    @property
+    def info(self):
+        v = np.zeros(5, dtype=np.float64)
+        load().shh_flat_info(self.h, v.ctypes.data_as(P))
+        return {"nbuckets": int(v[0]), "layers": int(v[1]), "seed_fallbacks": int(v[2]), "bits": int(v[3]),
+                "constructor_ms": float(v[4])}
+
+    @property
+    def codes(self):
+        lib = load()
+        c = np.zeros(lib.shh_flat_nbuckets(self.h), dtype=np.uint64)
+        lib.shh_flat_codes(self.h, c.ctypes.data_as(P))
+        return c
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().shh_flat_free(self.h)
+            self.h = None
+
+    __del__ = close
+

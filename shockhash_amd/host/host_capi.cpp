@@ -11,6 +11,7 @@
 #include "../include/SIMDShockHash.hpp"
 #include "../include/ShockHash.h"
 #include "../include/ShockHash2.h"
+#include "../include/ShockHash2Flat.h"
 
 using namespace shockhash;
 
@@ -121,6 +122,28 @@ void set_status(const std::exception &e) {
     const auto *ge = dynamic_cast<const gpu::Error *>(&e);
     g_status = ge ? ge->status : -1;
 }
+
+// ShockHash2Flat<k> over 64-bit key hashes (the mhc values the reference computes from string keys).
+#define SHH_FLAT_K(X) X(16) X(32) X(64) X(100) X(128)
+struct FlatHandle {
+    virtual ~FlatHandle() = default;
+    virtual size_t query(uint64_t mhc) const = 0;
+    virtual size_t bits() const = 0;
+    virtual void info(uint64_t *out) const = 0;
+    virtual const std::vector<uint64_t> &codes() const = 0;
+    double ctor_ms = 0;
+};
+template <size_t K>
+struct FlatHandleT : FlatHandle {
+    ShockHash2Flat<K> f;
+    explicit FlatHandleT(std::span<const uint64_t> m) : f(m) {}
+    size_t query(uint64_t mhc) const override { return f(mhc); }
+    size_t bits() const override { return f.getBits(); }
+    void info(uint64_t *out) const override {
+        out[0] = f.bucketCount(); out[1] = f.layerCount(); out[2] = f.fallbackSeedCount(); out[3] = f.getBits();
+    }
+    const std::vector<uint64_t> &codes() const override { return f.baseCaseCodes(); }
+};
 
 }  // namespace
 
@@ -254,6 +277,48 @@ void shh_timings(void *h, double *out5) {
     out5[0] = f.ms[0]; out5[1] = f.ms[1]; out5[2] = f.ms[2];
     out5[3] = ((Handle *)h)->host_ms();
     out5[4] = f.ms[3];
+}
+
+// ShockHash2Flat<k> over 64-bit key hashes (the mhc values the reference computes from string keys).
+void *shh_flat_build(int k, const uint64_t *mhcs, size_t n, char *err, size_t errlen) {
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        FlatHandle *h = nullptr;
+        std::span<const uint64_t> m(mhcs, n);
+#define C(K) if (k == K) h = new FlatHandleT<K>(m);
+        SHH_FLAT_K(C)
+#undef C
+        if (!h) throw gpu::Error(SHGPU_ERR_UNSUPPORTED, "ShockHash2Flat k " + std::to_string(k) + " not instantiated in libshockhash_host");
+        h->ctor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return h;
+    } catch (const std::exception &e) {
+        set_err(err, errlen, e.what());
+        set_status(e);
+        return nullptr;
+    }
+}
+void shh_flat_free(void *h) { delete (FlatHandle *)h; }
+// out[i] = the function's value of mhcs[i]; keys outside the set may raise inside (they are not queried by tests)
+int shh_flat_query(void *h, const uint64_t *mhcs, size_t n, uint64_t *out, char *err, size_t errlen) {
+    try {
+        for (size_t i = 0; i < n; i++) out[i] = ((FlatHandle *)h)->query(mhcs[i]);
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(err, errlen, e.what());
+        return 1;
+    }
+}
+// nbuckets, layers, base-case seeds kept in the fallback map, getBits(), constructor ms
+void shh_flat_info(void *h, double *out5) {
+    uint64_t v[4];
+    ((FlatHandle *)h)->info(v);
+    for (int i = 0; i < 4; i++) out5[i] = (double)v[i];
+    out5[4] = ((FlatHandle *)h)->ctor_ms;
+}
+size_t shh_flat_nbuckets(void *h) { return ((FlatHandle *)h)->codes().size(); }
+void shh_flat_codes(void *h, uint64_t *out) {
+    const auto &c = ((FlatHandle *)h)->codes();
+    std::memcpy(out, c.data(), c.size() * 8);
 }
 
 // Compile-time Golomb-Rice memo of a class (for comparison with the oracle's run-time formula).

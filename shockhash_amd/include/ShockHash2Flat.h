@@ -1,0 +1,400 @@
+// shockhash::ShockHash2Flat<k> backed by the MI355X engine: the k-perfect "flat" ShockHash2 variant
+// (include/ShockHash2Flat.h:37-302). Keys are hashed to 64-bit values (mhc), bucketed into N / k buckets of
+// exactly k keys by two threshold layers that bump overflowing keys, the bumped keys placed through a ShockHash2<k>
+// fallback over their decimal strings and a free-position map, and each bucket solved by ShockHash2's leaf search
+// (BijectionsShockHash2<k, true, QuadSplitCandidateFinderList>). The per-bucket searches, their orientation and
+// the retrieval structure run on the GPU (one batched leaf search for all buckets); the bucketing runs on the host.
+#pragma once
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <iostream>
+#include <map>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "ShockHash2.h"
+
+namespace shockhash {
+
+namespace flat_detail {
+
+// MurmurHash64A (Austin Appleby, public domain). The reference hashes string keys with the un-vendored
+// bytehamster::util::MurmurHash64 (ShockHash2Flat.h:70-75, 233-235); this is the published 64-bit MurmurHash2 with
+// seed 0, so string keys are placed as by that function only if it is this one.
+inline uint64_t murmurHash64A(const void *key, size_t len, uint64_t seed = 0) {
+    const uint64_t m = 0xc6a4a7935bd1e995ull;
+    const int r = 47;
+    uint64_t h = seed ^ (len * m);
+    const unsigned char *p = (const unsigned char *)key;
+    const size_t blocks = len / 8;
+    for (size_t i = 0; i < blocks; i++) {
+        uint64_t k;
+        std::memcpy(&k, p + 8 * i, 8);
+        k *= m;
+        k ^= k >> r;
+        k *= m;
+        h ^= k;
+        h *= m;
+    }
+    const unsigned char *t = p + 8 * blocks;
+    const size_t rest = len & 7;
+    if (rest) {
+        for (size_t i = rest; i-- > 0;) h ^= (uint64_t)t[i] << (8 * i);
+        h *= m;
+    }
+    h ^= h >> r;
+    h *= m;
+    h ^= h >> r;
+    return h;
+}
+
+constexpr size_t floorLog2(size_t x) { return x <= 1 ? 0 : 1 + floorLog2(x >> 1); }
+
+// THRESHOLD_MAPPING (_fill_mapping, ShockHash2Flat.h:17-35): compacted threshold index -> 32-bit threshold.
+template <size_t RANGE>
+constexpr std::array<uint32_t, RANGE> fillMapping() {
+    constexpr uint64_t ONE_THIRD = 0xFFFFFFFFull / 3;
+    std::array<uint32_t, RANGE> a{};
+    if constexpr (RANGE == 2) {
+        a[0] = 0;
+        a[1] = 0xFFFFFFFFu;
+    } else if constexpr (RANGE > 2) {
+        a[0] = 0;
+        a[1] = (uint32_t)ONE_THIRD;
+        constexpr uint64_t ir = RANGE - 3;
+        for (uint64_t i = 0; i < ir; i++) a[2 + i] = (uint32_t)(2 * ONE_THIRD + ONE_THIRD * i / ir);
+        a[RANGE - 1] = 0xFFFFFFFFu;
+    }
+    return a;
+}
+
+// Unsigned integers of a fixed bit width, packed (bytehamster::util::IntVector's role).
+class PackedInts {
+    std::vector<uint64_t> w;
+    size_t width = 1, n = 0;
+
+  public:
+    void reset(size_t count, size_t bits) {
+        n = count;
+        width = bits;
+        w.assign((count * bits + 63) / 64 + 1, 0);
+    }
+    uint64_t get(size_t i) const {
+        const size_t b = i * width, q = b >> 6, o = b & 63;
+        uint64_t v = w[q] >> o;
+        if (o + width > 64) v |= w[q + 1] << (64 - o);
+        return width == 64 ? v : v & ((1ull << width) - 1);
+    }
+    void set(size_t i, uint64_t v) {
+        const size_t b = i * width, q = b >> 6, o = b & 63;
+        const uint64_t mask = width == 64 ? ~0ull : (1ull << width) - 1;
+        v &= mask;
+        w[q] = (w[q] & ~(mask << o)) | (v << o);
+        if (o + width > 64) {
+            const size_t hi = o + width - 64;
+            w[q + 1] = (w[q + 1] & ~((1ull << hi) - 1)) | (v >> (64 - o));
+        }
+    }
+    size_t dataSizeBytes() const { return w.size() * 8; }
+};
+
+// freePositions as a bit vector with a one at i + freePositions[i] and select1 by sampled positions (the reference
+// uses pasta::FlatRankSelect on the same bit vector, ShockHash2Flat.h:139-146, 290-291): select1(j + 1) - j is
+// freePositions[j].
+class FreePositions {
+    std::vector<uint64_t> bits;
+    std::vector<uint64_t> sampleWord, sampleRank;  // for ones j = 0, SAMPLE, 2 SAMPLE, ...: its word, ones before it
+    size_t nbits = 0, ones = 0;
+    static constexpr size_t SAMPLE = 256;
+
+  public:
+    void build(const std::vector<size_t> &fp) {
+        ones = fp.size();
+        nbits = fp.empty() ? 0 : fp.size() + fp.back() + 1;
+        bits.assign((nbits + 63) / 64 + 1, 0);
+        for (size_t i = 0; i < fp.size(); i++) {
+            const size_t p = i + fp[i];
+            bits[p >> 6] |= 1ull << (p & 63);
+        }
+        sampleWord.clear();
+        sampleRank.clear();
+        size_t rank = 0;
+        for (size_t q = 0; q < bits.size(); q++) {
+            const size_t c = (size_t)__builtin_popcountll(bits[q]);
+            while (sampleWord.size() * SAMPLE < rank + c) {  // the next sampled one lies in word q
+                sampleWord.push_back(q);
+                sampleRank.push_back(rank);
+            }
+            rank += c;
+        }
+    }
+    // select1(j + 1) - j, that is freePositions[j].
+    size_t at(size_t j) const {
+        if (j >= ones) throw std::out_of_range("ShockHash2Flat: free position index");
+        size_t q = sampleWord[j / SAMPLE], rank = sampleRank[j / SAMPLE];
+        for (;; q++) {
+            uint64_t w = bits[q];
+            const size_t c = (size_t)__builtin_popcountll(w);
+            if (rank + c > j) {
+                for (size_t t = rank; t < j; t++) w &= w - 1;
+                return q * 64 + (size_t)__builtin_ctzll(w) - j;
+            }
+            rank += c;
+        }
+    }
+    size_t sizeBits() const { return nbits; }
+    size_t selectBytes() const { return sampleWord.size() * 16; }
+};
+
+}  // namespace flat_detail
+
+template <size_t k>
+class ShockHash2Flat {
+    static_assert(k >= 10 && k <= (size_t)MAX_LEAF_SIZE2,
+                  "the base case runs the QuadSplit candidate search, k >= 10 (leaf sizes up to 128)");
+    static constexpr double OVERLOAD_FACTOR = 0.9;
+    static constexpr size_t THRESHOLD_BITS = flat_detail::floorLog2(k) - 1;
+    static constexpr size_t THRESHOLD_RANGE = 1ul << THRESHOLD_BITS;
+    static constexpr std::array<uint32_t, THRESHOLD_RANGE> THRESHOLD_MAPPING = flat_detail::fillMapping<THRESHOLD_RANGE>();
+    static size_t seedBits() { return (size_t)std::ceil(0.442 * k - 0.2); }  // SEED_BITS, evaluated as the reference
+    static constexpr size_t SEED_FALLBACK_INDICATOR = 0;
+
+    struct KeyInfo {  // ShockHash2FlatBase.h
+        uint64_t mhc;
+        uint32_t bucket, threshold;
+    };
+
+    flat_detail::PackedInts thresholdsAndSeeds;
+    std::map<size_t, size_t> seedsFallback;
+    std::vector<size_t> layerBases;
+    ShockHash2<k> fallbackPhf;
+    bool haveFallback = false;
+    size_t N = 0, nbuckets = 0, layers = 2, maxSeed = 0;
+    std::vector<size_t> freePositionsList;
+    flat_detail::FreePositions freePositions;
+    RibbonRetrieval ribbon;
+    std::vector<uint64_t> retrievalKeys;  // retrieval input, kept when gpu::keep_retrieval_input
+    std::vector<uint8_t> retrievalBits;
+    std::vector<uint64_t> bucketCodes;  // base-case seed per bucket, for verification
+
+    void setThreshold(size_t b, size_t v) {
+        uint64_t x = thresholdsAndSeeds.get(b);
+        x &= ~((1ull << THRESHOLD_BITS) - 1);
+        thresholdsAndSeeds.set(b, x | v);
+    }
+    void setSeed(size_t b, size_t v) {
+        uint64_t x = thresholdsAndSeeds.get(b);
+        x &= (1ull << THRESHOLD_BITS) - 1;
+        thresholdsAndSeeds.set(b, x | ((uint64_t)v << THRESHOLD_BITS));
+    }
+    std::pair<size_t, size_t> getThresholdAndSeed(size_t b) const {
+        const uint64_t x = thresholdsAndSeeds.get(b);
+        return {x & (THRESHOLD_RANGE - 1), x >> THRESHOLD_BITS};
+    }
+    static uint32_t compactThreshold(uint32_t t) {
+        for (size_t i = 0; i < THRESHOLD_RANGE; i++)
+            if (t <= THRESHOLD_MAPPING[i]) return (uint32_t)i;
+        return THRESHOLD_MAPPING.back();
+    }
+
+    // flushBucket (ShockHash2Flat.h:201-228)
+    void flushBucket(size_t layerBase, size_t bucketStart, size_t i, size_t bucketIdx, const std::vector<KeyInfo> &hashes,
+                     std::vector<KeyInfo> &bumped) {
+        const size_t size = i - bucketStart, b = layerBase + bucketIdx;
+        if (size <= k) {
+            setThreshold(b, THRESHOLD_RANGE - 1);
+            for (size_t q = size; q < k; q++) freePositionsList.push_back(b);
+            return;
+        }
+        const uint32_t last = compactThreshold(hashes[bucketStart + k - 1].threshold);
+        const uint32_t firstBumped = compactThreshold(hashes[bucketStart + k].threshold);
+        if (firstBumped == last && last == 0) throw std::runtime_error("ShockHash2Flat: threshold index below 0");
+        const uint32_t t = firstBumped == last ? last - 1 : last;
+        setThreshold(b, t);
+        const uint32_t uncompressed = THRESHOLD_MAPPING[t];
+        for (size_t l = 0; l < size; l++)
+            if (hashes[bucketStart + l].threshold > uncompressed) {
+                bumped.push_back(hashes[bucketStart + l]);
+                if (l < k) freePositionsList.push_back(b);
+            }
+    }
+
+    size_t fallbackQuery(uint64_t mhc) const { return fallbackPhf(std::to_string(mhc)); }
+
+    void construct(std::vector<uint64_t> &&mhcs) {
+        N = mhcs.size();
+        nbuckets = N / k;
+        maxSeed = 1ull << seedBits();
+        const size_t keysInEndBucket = N - nbuckets * k;
+        size_t bucketsThisLayer = (size_t)std::ceil(OVERLOAD_FACTOR * nbuckets);
+        std::vector<KeyInfo> hashes(N);
+        for (size_t i = 0; i < N; i++)
+            hashes[i] = {mhcs[i], fastrange32((uint32_t)mhcs[i], (uint32_t)bucketsThisLayer), (uint32_t)(mhcs[i] >> 32)};
+        layerBases.assign(1, 0);
+        thresholdsAndSeeds.reset(nbuckets, THRESHOLD_BITS + seedBits());
+        if (bucketsThisLayer == 0) layers = 0;
+        std::vector<uint64_t> sortKey;
+        for (size_t layer = 0; layer < layers; layer++) {
+            const size_t layerBase = layerBases.back();
+            if (layer != 0) {
+                bucketsThisLayer = (size_t)(OVERLOAD_FACTOR * (double)(hashes.size() / k));
+                bucketsThisLayer = std::min(bucketsThisLayer, nbuckets - layerBase);
+                if (bucketsThisLayer == 0) {
+                    layers = 1;
+                    break;
+                }
+                for (auto &h : hashes) {  // rehash
+                    h.mhc = remix(h.mhc);
+                    h.bucket = fastrange32((uint32_t)h.mhc, (uint32_t)bucketsThisLayer);
+                    h.threshold = (uint32_t)(h.mhc >> 32);
+                }
+            }
+            layerBases.push_back(layerBase + bucketsThisLayer);
+            // sort_keyInfo (src/Sorter.cpp:10-12): by bucket, then threshold; counting sort by bucket, then each
+            // bucket's few keys by threshold
+            {
+                std::vector<size_t> start(bucketsThisLayer + 1, 0);
+                for (const auto &h : hashes) start[h.bucket + 1]++;
+                for (size_t b = 0; b < bucketsThisLayer; b++) start[b + 1] += start[b];
+                std::vector<KeyInfo> sorted(hashes.size());
+                std::vector<size_t> pos(start.begin(), start.end() - 1);
+                for (const auto &h : hashes) sorted[pos[h.bucket]++] = h;
+                for (size_t b = 0; b < bucketsThisLayer; b++)
+                    std::stable_sort(sorted.begin() + start[b], sorted.begin() + start[b + 1],
+                                     [](const KeyInfo &a, const KeyInfo &c) { return a.threshold < c.threshold; });
+                hashes.swap(sorted);
+            }
+            std::vector<KeyInfo> bumped;
+            size_t bucketStart = 0, previousBucket = 0;
+            for (size_t i = 0; i < hashes.size(); i++) {
+                while (hashes[i].bucket != previousBucket) {
+                    flushBucket(layerBase, bucketStart, i, previousBucket, hashes, bumped);
+                    previousBucket++;
+                    bucketStart = i;
+                }
+            }
+            while (previousBucket < bucketsThisLayer) {
+                flushBucket(layerBase, bucketStart, hashes.size(), previousBucket, hashes, bumped);
+                previousBucket++;
+                bucketStart = hashes.size();
+            }
+            hashes.swap(bumped);
+        }
+        // fallback PHF over the bumped keys' decimal strings (ShockHash2Flat.h:124-129)
+        if (!hashes.empty()) {
+            std::vector<std::string> content;
+            content.reserve(hashes.size());
+            for (const auto &h : hashes) content.push_back(std::to_string(h.mhc));
+            fallbackPhf = ShockHash2<k>(std::span<const std::string>(content.data(), content.size()), 2000, 1);
+            haveFallback = true;
+        }
+        const size_t additional = hashes.size() - freePositionsList.size(), handled = layerBases.back();
+        {
+            size_t i = 0;
+            for (; i < additional - keysInEndBucket; i++) freePositionsList.push_back(handled + i / k);
+            for (; i < additional; i++) freePositionsList.push_back(nbuckets + i);
+        }
+        freePositions.build(freePositionsList);
+        std::vector<size_t>().swap(freePositionsList);
+        // bucket contents in input order (ShockHash2Flat.h:148-158)
+        std::vector<size_t> bucketOf(N);
+        std::vector<size_t> fill(nbuckets + 1, 0);
+        for (size_t i = 0; i < N; i++) {
+            bucketOf[i] = evaluateKPerfect(mhcs[i]).first;
+            if (bucketOf[i] < nbuckets) fill[bucketOf[i] + 1]++;
+        }
+        for (size_t b = 0; b < nbuckets; b++)
+            if (fill[b + 1] != k) throw std::runtime_error("ShockHash2Flat: a bucket does not hold k keys");
+        for (size_t b = 0; b < nbuckets; b++) fill[b + 1] += fill[b];
+        std::vector<uint64_t> bucketKeys(nbuckets * k);
+        for (size_t i = 0; i < N; i++)
+            if (bucketOf[i] < nbuckets) bucketKeys[fill[bucketOf[i]]++] = mhcs[i];
+        // base case of every bucket, its orientation and the retrieval structure, on the device
+        std::vector<uint32_t> roffsets;
+        std::vector<uint8_t> rseeds;
+        std::vector<uint64_t> rwords;
+        gpu::Engine::instance().sh2_leaves_retrieval(bucketKeys, (uint32_t)k, bucketCodes, gpu::keep_retrieval_input,
+                                                     retrievalKeys, retrievalBits, roffsets, rseeds, rwords);
+        ribbon = RibbonRetrieval(std::move(roffsets), std::move(rseeds), std::move(rwords));
+        for (size_t b = 0; b < nbuckets; b++) {
+            size_t seed = bucketCodes[b];
+            if (seed >= maxSeed || seed == SEED_FALLBACK_INDICATOR) {
+                seedsFallback.insert({b, seed});
+                seed = SEED_FALLBACK_INDICATOR;
+            }
+            setSeed(b, seed);
+        }
+    }
+
+  public:
+    ShockHash2Flat() = default;
+    ShockHash2Flat(std::span<const std::string> keys, size_t ignore, size_t ignore2) : ShockHash2Flat(keys) {
+        (void)ignore;
+        (void)ignore2;
+    }
+    explicit ShockHash2Flat(std::span<const std::string> keys) {
+        std::vector<uint64_t> mhcs(keys.size());
+        for (size_t i = 0; i < keys.size(); i++) mhcs[i] = flat_detail::murmurHash64A(keys[i].data(), keys[i].size());
+        construct(std::move(mhcs));
+    }
+    // Keys given by their 64-bit hashes (not in the reference interface, which hashes string keys itself).
+    explicit ShockHash2Flat(std::span<const uint64_t> mhcs) { construct(std::vector<uint64_t>(mhcs.begin(), mhcs.end())); }
+
+    size_t operator()(const std::string &key) const { return operator()(flat_detail::murmurHash64A(key.data(), key.size())); }
+    size_t operator()(const hash128_t &hash) const { return operator()(hash.second); }
+    // ShockHash2Flat.h:254-267
+    size_t operator()(uint64_t hash) const {
+        auto [bucket, seed] = evaluateKPerfect(hash);
+        if (bucket >= nbuckets) return bucket;  // N that are not multiples of k
+        if (seed == SEED_FALLBACK_INDICATOR) seed = seedsFallback.at(bucket);
+        const uint64_t retrieved = ribbon.retrieve(hash);
+        return bucket * k + sh2Cell((uint32_t)k, seed, hash, retrieved);
+    }
+
+    // (bucket, stored seed) of a key (ShockHash2Flat.h:270-299).
+    std::pair<size_t, size_t> evaluateKPerfect(uint64_t mhc) const {
+        for (size_t layer = 0; layer < layers; layer++) {
+            if (layer != 0) mhc = remix(mhc);
+            const size_t base = layerBases[layer], layerSize = layerBases[layer + 1] - base;
+            const uint32_t bucket = fastrange32((uint32_t)mhc, (uint32_t)layerSize);
+            const uint32_t threshold = (uint32_t)(mhc >> 32);
+            auto [storedThreshold, storedSeed] = getThresholdAndSeed(base + bucket);
+            if (threshold <= THRESHOLD_MAPPING[storedThreshold]) return {base + bucket, storedSeed};
+        }
+        if (!haveFallback) throw std::out_of_range("ShockHash2Flat: key not in the set");
+        const size_t phf = fallbackQuery(mhc);
+        const size_t bucket = freePositions.at(phf);
+        if (bucket >= nbuckets) return {bucket - nbuckets + k * nbuckets, 1};  // last, partial bucket
+        return {bucket, getThresholdAndSeed(bucket).second};
+    }
+
+    size_t size() const { return N; }
+    [[nodiscard]] size_t getBits() const {
+        return 8 * sizeof(*this) + (haveFallback ? fallbackPhf.getBits() : 0) + freePositions.sizeBits() +
+               8 * freePositions.selectBytes() + 8 * ribbon.sizeBytes() + 8 * thresholdsAndSeeds.dataSizeBytes() +
+               64 * seedsFallback.size();
+    }
+    void printBits() const {
+        std::cout << "Thresholds: " << 1.0f * THRESHOLD_BITS / k << std::endl;
+        std::cout << "Fallback PHF keys: " << (freePositions.sizeBits() > 0 ? freePositions.sizeBits() - N / k : 0) << std::endl;
+        std::cout << "PHF: " << (haveFallback ? 1.0f * fallbackPhf.getBits() / N : 0.0f) << std::endl;
+        std::cout << "Base case seeds: " << 1.0f * seedBits() / k << std::endl;
+        std::cout << "Base case seeds overflow: " << 1.0f * seedsFallback.size() * 64 / N << std::endl;
+        std::cout << "Ribbon: " << 8.0f * ribbon.sizeBytes() / N << std::endl;
+    }
+
+    // Verification accessors (not in the reference interface).
+    const std::vector<uint64_t> &baseCaseCodes() const { return bucketCodes; }
+    const std::vector<uint64_t> &ribbonKeys() const { return retrievalKeys; }
+    const std::vector<uint8_t> &ribbonBits() const { return retrievalBits; }
+    size_t bucketCount() const { return nbuckets; }
+    size_t layerCount() const { return layers; }
+    size_t fallbackSeedCount() const { return seedsFallback.size(); }
+};
+
+}  // namespace shockhash

@@ -176,7 +176,6 @@ class Engine {
                                std::vector<uint8_t> &seeds, std::vector<uint64_t> &words, double *ms2 = nullptr,
                                uint32_t rot_leaf = 0) {
         std::lock_guard<std::mutex> lock(mtx);
-        const auto t0 = std::chrono::steady_clock::now();
         std::vector<shgpu_orient_task> tasks;
         tasks.reserve(f.nodes.size());
         uint64_t total = 0;
@@ -191,15 +190,54 @@ class Engine {
         }
         if (!f.device) throw Error(SHGPU_ERR_ARG, "leaf orientation: the forest holds no device build");
         const uint64_t *leaf_keys = nullptr;
-        int rc = shgpu_build_device_keys(f.device.get(), &leaf_keys);
-        if (rc == SHGPU_OK) rc = ensure(tasks_buf, tasks.size() * sizeof(shgpu_orient_task) + 8);
+        check(shgpu_build_device_keys(f.device.get(), &leaf_keys), "shgpu_build_device_keys");
+        retrieval_from_device(variant, leaf_keys, tasks, total, keep, input_keys, input_bits, offsets, seeds, words, ms2);
+    }
+
+    // ShockHash2 leaves of k keys each, keys[i * k .. (i + 1) * k) (ShockHash2Flat's buckets): their codes
+    // (BijectionsShockHash2::findSeed, one batched search on the device) and the retrieval structure of their
+    // orientations (constructRetrieval, ShockHash2-internal.h:548-572), built as for a forest.
+    void sh2_leaves_retrieval(const std::vector<uint64_t> &keys, uint32_t k, std::vector<uint64_t> &codes, bool keep,
+                              std::vector<uint64_t> &input_keys, std::vector<uint8_t> &input_bits,
+                              std::vector<uint32_t> &offsets, std::vector<uint8_t> &seeds, std::vector<uint64_t> &words) {
+        std::lock_guard<std::mutex> lock(mtx);
+        const uint64_t nb = k ? keys.size() / k : 0;
+        std::vector<shgpu_leaf_task> lt(nb);
+        for (uint64_t i = 0; i < nb; i++) lt[i] = {(uint32_t)(i * k), k, 0, 0};
+        codes.assign(nb, SHGPU_NOT_FOUND);
+        int rc = ensure(leaf_keys_buf, keys.size() * 8 + 8);
+        if (rc == SHGPU_OK) rc = ensure(leaf_task_buf, nb * sizeof(shgpu_leaf_task) + 8);
+        if (rc == SHGPU_OK) rc = ensure(leaf_code_buf, nb * 8 + 8);
+        if (rc == SHGPU_OK && !keys.empty()) rc = shgpu_copy_to_device(e, leaf_keys_buf.p, keys.data(), keys.size() * 8);
+        if (rc == SHGPU_OK && nb) rc = shgpu_copy_to_device(e, leaf_task_buf.p, lt.data(), nb * sizeof(shgpu_leaf_task));
+        if (rc == SHGPU_OK && nb)
+            rc = shgpu_sh2_leaf_search(e, (const uint64_t *)leaf_keys_buf.p, (const shgpu_leaf_task *)leaf_task_buf.p,
+                                       (uint32_t)nb, (uint64_t *)leaf_code_buf.p, nullptr);
+        if (rc == SHGPU_OK && nb) rc = shgpu_copy_to_host(e, codes.data(), leaf_code_buf.p, nb * 8);
+        check(rc, "shgpu_sh2_leaf_search (buckets)");
+        for (uint64_t c : codes)
+            if (c == SHGPU_NOT_FOUND) throw Error(SHGPU_ERR_SEARCH_LIMIT, "ShockHash2 leaf search exhausted its tables (duplicate keys?)");
+        std::vector<shgpu_orient_task> tasks(nb);
+        for (uint64_t i = 0; i < nb; i++) tasks[i] = {i * k, codes[i], (uint32_t)(i * k), k, 0, 0};
+        retrieval_from_device(SHGPU_SHOCKHASH2, (const uint64_t *)leaf_keys_buf.p, tasks, nb * k, keep, input_keys,
+                              input_bits, offsets, seeds, words, nullptr);
+    }
+
+  private:
+    // Orientation replay of `tasks` over device keys, then the ribbon over the replay's output (caller holds mtx).
+    void retrieval_from_device(int variant, const uint64_t *d_keys, const std::vector<shgpu_orient_task> &tasks,
+                               uint64_t total, bool keep, std::vector<uint64_t> &input_keys,
+                               std::vector<uint8_t> &input_bits, std::vector<uint32_t> &offsets,
+                               std::vector<uint8_t> &seeds, std::vector<uint64_t> &words, double *ms2) {
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = ensure(tasks_buf, tasks.size() * sizeof(shgpu_orient_task) + 8);
         if (rc == SHGPU_OK) rc = ensure(out_keys_buf, total * 8 + 8);
         if (rc == SHGPU_OK) rc = ensure(out_bits_buf, total + 8);
         void *d_tasks = tasks_buf.p, *d_out_keys = out_keys_buf.p, *d_out_bits = out_bits_buf.p;
         if (rc == SHGPU_OK && !tasks.empty())
             rc = shgpu_copy_to_device(e, d_tasks, tasks.data(), tasks.size() * sizeof(shgpu_orient_task));
         if (rc == SHGPU_OK)
-            rc = shgpu_leaf_orientation(e, variant, leaf_keys, (const shgpu_orient_task *)d_tasks, tasks.size(),
+            rc = shgpu_leaf_orientation(e, variant, d_keys, (const shgpu_orient_task *)d_tasks, tasks.size(),
                                         (uint64_t *)d_out_keys, (uint8_t *)d_out_bits);
         const auto t1 = std::chrono::steady_clock::now();
         const uint64_t shards = shgpu_ribbon_shards(total);

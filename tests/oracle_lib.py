@@ -43,6 +43,12 @@ _SIGS = {
     "orc_golomb_memo": (None, [ctypes.c_int, ctypes.c_int, P]),
     "orc_build": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]),
     "orc_build_rot": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]),
+    "orc_flat_build": (P, [U32, P, ctypes.c_size_t, ctypes.c_int]),
+    "orc_flat_free": (None, [P]),
+    "orc_flat_query": (None, [P, P, ctypes.c_size_t, P]),
+    "orc_flat_info": (None, [P, P]),
+    "orc_flat_evaluate": (None, [P, P, ctypes.c_size_t, P, P]),
+    "orc_flat_mapping": (None, [U32, P]),
     "orc_sh1_leaf_rot": (None, [P, U32, U64, ctypes.POINTER(U64), ctypes.POINTER(U32), P]),
     "orc_sh1_rot_query_cell": (U64, [U32, U64, U64, U64, U64]),
     "orc_build_buckets": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, P, ctypes.c_size_t,
@@ -256,3 +262,44 @@ class OracleBucketBuild:
                               ptr(self.node_code))
         finally:
             L.orc_build_free(h)
+
+
+class OracleFlat:
+    """ShockHash2Flat<k> by the oracle (include/ShockHash2Flat.h:37-302) from the keys' 64-bit hashes."""
+
+    def __init__(self, k, mhcs, threads=8):
+        mhcs = u64(mhcs)
+        self.k, self.n = k, len(mhcs)
+        self.h = lib().orc_flat_build(k, ptr(mhcs), len(mhcs), threads)
+        if not self.h:
+            raise RuntimeError("oracle flat build failed")
+        info = np.zeros(7, dtype=np.uint64)
+        lib().orc_flat_info(self.h, ptr(info))
+        self.nbuckets, self.layers = int(info[0]), int(info[1])
+        self.layer_bases = [int(x) for x in info[2:5]]
+        self.bumped, self.seed_fallbacks = int(info[5]), int(info[6])
+
+    def query(self, mhcs):
+        mhcs = u64(mhcs)
+        out = np.zeros(len(mhcs), dtype=np.uint64)
+        lib().orc_flat_query(self.h, ptr(mhcs), len(mhcs), ptr(out))
+        return out
+
+    def evaluate(self, mhcs):
+        mhcs = u64(mhcs)
+        b = np.zeros(len(mhcs), dtype=np.uint64)
+        s = np.zeros(len(mhcs), dtype=np.uint64)
+        lib().orc_flat_evaluate(self.h, ptr(mhcs), len(mhcs), ptr(b), ptr(s))
+        return b, s
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_flat_free(self.h)
+            self.h = None
+
+
+def flat_mapping(trange):
+    out = np.zeros(trange, dtype=np.uint32)
+    lib().orc_flat_mapping(trange, ptr(out))
+    return out
+

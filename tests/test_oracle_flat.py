@@ -1,0 +1,61 @@
+"""The oracle's ShockHash2Flat restatement (include/ShockHash2Flat.h:37-302) on the CPU: its threshold mapping,
+and the invariants its construction must meet whatever a slip in the bucketing would break (every bucket holds
+exactly k keys, which the build checks; the queries are injective, and a bijection onto [0, N) when N is a multiple
+of k; a key's bucket is the first layer whose threshold admits it)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from shockhash_amd import workload
+
+ONE_THIRD = 0xFFFFFFFF // 3
+
+
+def test_threshold_mapping_shapes():
+    """_fill_mapping (ShockHash2Flat.h:17-35): 0 and one third at the bottom, an interpolation from two thirds, all
+    thresholds at the top."""
+    assert list(O.flat_mapping(2)) == [0, 0xFFFFFFFF]
+    assert list(O.flat_mapping(3)) == [0, ONE_THIRD, 0xFFFFFFFF]
+    m = O.flat_mapping(32)
+    assert m[0] == 0 and m[1] == ONE_THIRD and m[2] == 2 * ONE_THIRD and m[31] == 0xFFFFFFFF
+    assert m[30] == 2 * ONE_THIRD + ONE_THIRD * 28 // 29
+    assert all(int(a) < int(b) for a, b in zip(m, m[1:]))
+
+
+@pytest.mark.parametrize("k,n", [(16, 16 * 400), (32, 32 * 500), (64, 64 * 1000), (100, 100 * 300)])
+def test_flat_is_a_bijection_for_multiples_of_k(k, n):
+    mh = workload.fingerprints(n, seed=100 + k)[:, 1]
+    f = O.OracleFlat(k, mh)
+    q = f.query(mh)
+    assert len(np.unique(q)) == n and int(q.max()) == n - 1
+    assert f.layers == 2 and f.nbuckets == n // k and f.bumped > 0
+
+
+@pytest.mark.parametrize("k,n", [(16, 5003), (64, 100037), (100, 100037), (32, 31), (16, 7)])
+def test_flat_ragged_sizes_are_injective(k, n):
+    """N not a multiple of k: the reference places the last partial bucket's keys past the buckets that no layer
+    handled (ShockHash2Flat.h:130-137), so the values stay distinct but can exceed N - 1."""
+    mh = workload.fingerprints(n, seed=200 + k)[:, 1]
+    f = O.OracleFlat(k, mh)
+    q = f.query(mh)
+    assert len(np.unique(q)) == n
+    full = f.nbuckets * k
+    assert int((q < full).sum()) == full  # every bucket position is taken by exactly one key
+
+
+def test_flat_layers_decide_first():
+    """evaluateKPerfect (ShockHash2Flat.h:270-299): a key sits in its layer-0 bucket or, bumped there, in its
+    layer-1 bucket (hash remix(mhc)), unless the fallback placed it; the fallback places exactly the keys both
+    layers bumped, each into a free position of a bucket, and every bucket ends with k keys."""
+    k, n = 64, 64 * 800
+    mh = workload.fingerprints(n, seed=7)[:, 1]
+    f = O.OracleFlat(k, mh)
+    bucket, _ = f.evaluate(mh)
+    b1, b2 = f.layer_bases[1], f.layer_bases[2]
+    lo32 = np.uint64(0xFFFFFFFF)
+    layer0 = (mh & lo32) * np.uint64(b1) >> np.uint64(32)
+    layer1 = np.uint64(b1) + (((workload.mix13(mh) & lo32) * np.uint64(b2 - b1)) >> np.uint64(32))
+    elsewhere = (bucket != layer0) & (bucket != layer1)
+    assert elsewhere.sum() <= f.bumped and elsewhere.sum() >= 0.95 * f.bumped
+    counts = np.bincount(bucket[bucket < f.nbuckets].astype(np.int64), minlength=f.nbuckets)
+    assert (counts == k).all()
