@@ -12,6 +12,19 @@
 namespace shgpu {
 
 constexpr uint32_t EMPTY_SLOT = 0xFFFFFFFFu;
+
+// Queue control words of a persistent build (BuildArgs::ctrl, written by the forest plan). Counters that
+// many waves update at once sit in 128-byte lines of their own, the read-mostly words share one line.
+constexpr uint32_t CTRL_LINE = 32;
+constexpr uint32_t C_HEAD = 0;                   // queue head (over both parts of a two-part queue)
+constexpr uint32_t C_SPLIT_TAIL = 1 * CTRL_LINE;  // split (or FIFO) tail
+constexpr uint32_t C_FINISHED = 2 * CTRL_LINE;    // nodes finished
+constexpr uint32_t C_LEAF_TAIL = 3 * CTRL_LINE;   // two-part queue: leaf tail
+constexpr uint32_t C_STARTED = 4 * CTRL_LINE;     // windowed nodes started
+constexpr uint32_t C_ERROR = 5 * CTRL_LINE;       // set when a wait is declared stuck
+constexpr uint32_t C_NODES = C_ERROR + 1;         // nodes of the forest
+constexpr uint32_t C_WINDOWED = C_ERROR + 2;      // windowed nodes of the forest
+constexpr uint32_t CTRL_WORDS = 6 * CTRL_LINE;
 constexpr int BLOCK = 256;
 constexpr int WAVES_PER_BLOCK = BLOCK / 64;
 
@@ -32,7 +45,9 @@ struct NodeState {
     unsigned long long gd;    // GD_FOUND | windows handed out << 32 | windows completed
     unsigned long long best;  // smallest qualifying seed offset found so far (SHGPU_NOT_FOUND: none)
     uint32_t finalized;       // number of waves that claimed finalisation (the first one does it)
-    uint32_t pad[3];
+    uint32_t help_cap;        // waves that may help the node's owner at once (the plan's)
+    uint32_t helpers;         // waves helping it now
+    uint32_t pad;
 };
 
 struct BuildArgs {
@@ -41,10 +56,10 @@ struct BuildArgs {
     const shgpu_node *nodes;
     uint64_t *codes;
     uint32_t *queue;
-    uint32_t *ctrl;  // [0] queue head, [1] split tail, [2] error flag, [3] nodes finished, [5] leaf tail, [6] nodes
+    uint32_t *ctrl;  // queue control words (C_*)
     NodeState *state;
     uint32_t *help;  // per wave slot: node whose windows the slot's wave searches, or EMPTY_SLOT
-    uint32_t total;  // plain form: nodes (the persistent kernel reads ctrl[6])
+    uint32_t total;  // plain form: nodes (the persistent kernel reads ctrl[C_NODES])
     uint32_t nslots;
     uint8_t *scratch;          // ShockHash2 leaves: sh2_wave_bytes per wave slot
     uint64_t sh2_wave_bytes;

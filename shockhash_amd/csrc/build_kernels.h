@@ -48,14 +48,14 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
             for (uint32_t c = 0; c < nc; c++) {
                 const shgpu_node cn = a.nodes[child];
                 const bool to_leaf_part = two_part_queue(a.flags) && uniform(cn.kind) == SHGPU_NODE_LEAF;
-                const uint32_t pos = wave_fetch_add(&a.ctrl[to_leaf_part ? 5 : 1], 1u);
+                const uint32_t pos = wave_fetch_add(&a.ctrl[to_leaf_part ? C_LEAF_TAIL : C_SPLIT_TAIL], 1u);
                 __hip_atomic_store(&a.queue[pos], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 child = uniform(child + cn.subtree);
             }
         }
     }
     a.codes[nid] = off;  // read by the host after the kernel ends
-    if (persistent) wave_fetch_add(&a.ctrl[3], 1u);
+    if (persistent) wave_fetch_add(&a.ctrl[C_FINISHED], 1u);
 }
 
 // Searches windows of a split node or ShockHash1 leaf until its smallest qualifying seed is known.
@@ -99,8 +99,12 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
             staged = true;
         }
     }
+    // A split node's children wait for it: its windows run at the highest issue priority, so leaf searches on
+    // the same SIMD do not slow it down (they rise with age, see age_priority).
+    const bool split = kind != SHGPU_NODE_LEAF;
     for (;;) {
-        age_priority(t_start);
+        if (split) __builtin_amdgcn_s_setprio(3);
+        else age_priority(t_start);
         const uint64_t woff = w * W;
         const bool exhausted = woff >= lim;
         uint64_t r = SHGPU_NOT_FOUND;
@@ -144,7 +148,9 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
     }
 }
 
-// Joins one node another wave is searching in windows, if any. Returns true when it did some work.
+// Joins one node another wave is searching in windows, if any. Returns true when it did some work. A node
+// takes at most help_cap helpers at once: a wave joins after raising the node's helper count below the cap
+// (waves that scan the registry at the same moment would otherwise all join a newly registered node).
 template <int VARIANT, int SL, int SS>
 __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLds &L) {
     const uint32_t lane = lane_id();
@@ -154,14 +160,24 @@ __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLd
         if (i >= a.nslots) i -= a.nslots;
         const uint32_t v = base + lane < a.nslots ? __hip_atomic_load(&a.help[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                                   : EMPTY_SLOT;
-        // a registered node whose best seed is already known needs no help
-        const bool open = v != EMPTY_SLOT &&
-                          !(__hip_atomic_load(&a.state[v].gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & GD_FOUND);
+        // a registered node whose best seed is already known needs no help, nor one with its helpers complete
+        bool open = false;
+        if (v != EMPTY_SLOT) {
+            const NodeState *st = &a.state[v];
+            open = !(__hip_atomic_load(&st->gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & GD_FOUND) &&
+                   __hip_atomic_load(&st->helpers, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < st->help_cap;
+        }
         const uint64_t hit = ballot(open);
         if (hit) {
             const uint32_t nid = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));
+            NodeState *st = &a.state[nid];
+            if (wave_fetch_add(&st->helpers, 1u) >= uniform(st->help_cap)) {
+                wave_fetch_add(&st->helpers, 0xFFFFFFFFu);  // full meanwhile: leave the count as it was
+                return false;
+            }
             __threadfence();
             work_windows<VARIANT, SL, SS>(a, nid, L, true, task_clock());
+            wave_fetch_add(&st->helpers, 0xFFFFFFFFu);
             return true;
         }
     }
@@ -188,11 +204,11 @@ __device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint3
             const uint64_t code = sh2_leaf_search_any<WIDE>(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.sh2_caps,
                                                             a.flags, t_start);
             a.codes[nid] = code;
-            wave_fetch_add(&a.ctrl[3], 1u);
+            wave_fetch_add(&a.ctrl[C_FINISHED], 1u);
         }
         return;
     }
-    wave_fetch_add(&a.ctrl[8], 1u);  // windowed nodes started
+    wave_fetch_add(&a.ctrl[C_STARTED], 1u);  // windowed nodes started
     __hip_atomic_store(&a.help[slot], nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     work_windows<VARIANT, SL, SS>(a, nid, L.pf, true, t_start);
     __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -213,34 +229,34 @@ __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
     const uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
     BuildLds<VARIANT, WIDE> &L = lds[w];
     uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)slot * a.sh2_wave_bytes : nullptr;
-    const uint32_t total = uniform(a.ctrl[6]);  // written by the forest plan
+    const uint32_t total = uniform(a.ctrl[C_NODES]);  // written by the forest plan
     for (;;) {
-        const uint32_t idx = grab(&a.ctrl[0]);
+        const uint32_t idx = grab(&a.ctrl[C_HEAD]);
         if (idx >= total) break;
         uint32_t nid = EMPTY_SLOT;
         for (uint32_t spin = 0;; spin++) {
             nid = uniform(__hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
             if (nid != EMPTY_SLOT) break;
             if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
-            if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
+            if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[C_ERROR], 1u); break; }
             __builtin_amdgcn_s_sleep(4);
         }
         if (nid == EMPTY_SLOT) return;
         __threadfence();
         run_node<VARIANT, SL, SS, WIDE>(a, nid, slot, L, ws);
     }
-    // Queue exhausted: help until every node is finished. Once every windowed node has started (ctrl[8]
-    // reaches ctrl[7]) no node will register for help any more, and a wave that finds none open leaves
+    // Queue exhausted: help until every node is finished. Once every windowed node has started (C_STARTED
+    // reaches C_WINDOWED) no node will register for help any more, and a wave that finds none open leaves
     // instead of polling beside the waves still searching (checked every 8th poll: ~6000 waves polling one
     // word load a single L2 channel).
-    const uint32_t windowed = uniform(a.ctrl[7]);
+    const uint32_t windowed = uniform(a.ctrl[C_WINDOWED]);
     for (uint32_t spin = 0;; spin++) {
-        if (uniform(__hip_atomic_load(&a.ctrl[3], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= total) break;
+        if (uniform(__hip_atomic_load(&a.ctrl[C_FINISHED], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= total) break;
         if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
         if ((spin & 7) == 0 &&
-            uniform(__hip_atomic_load(&a.ctrl[8], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= windowed)
+            uniform(__hip_atomic_load(&a.ctrl[C_STARTED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= windowed)
             break;
-        if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[2], 1u); break; }
+        if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[C_ERROR], 1u); break; }
         __builtin_amdgcn_s_sleep(8);
     }
 }

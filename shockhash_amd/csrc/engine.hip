@@ -359,6 +359,8 @@ struct PlanArgs {
     const uint32_t *cnt_all;         // nodes of a tree of m keys, m <= MAX_ENCODABLE_BUCKET
     const uint32_t *cnt_split;       // non-leaf nodes of that tree
     const uint32_t *bound_lg;        // log2 of the default seed bound of a node of m keys (seed_bound_log2)
+    const float *seed_rate;          // -ln(1 - p) for the probability p that a seed suits a node of m keys
+    uint32_t win_leaf, win_lower;    // seeds per window of leaves and of lower aggregation nodes
     uint4 *bucket;                   // per size_acc entry: first node, split roots before, leaf roots before
     uint32_t *summary;
     uint32_t capacity;               // node capacity of nodes / codes / queue / states
@@ -465,16 +467,14 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_scan(PlanArgs p) {
         p.summary[SUM_LEAF_ROOTS] = total.w;
         p.summary[SUM_OVERFLOW] = overflow ? 1u : 0u;
         p.summary[SUM_QSPLIT] = qsplit;
-        // head, split tail, error, finished, unused, leaf tail, nodes, windowed nodes, windowed nodes started
-        p.ctrl[0] = 0;
-        p.ctrl[1] = p.two_part ? total.z : total.z + total.w;
-        p.ctrl[2] = 0;
-        p.ctrl[3] = 0;
-        p.ctrl[4] = 0;
-        p.ctrl[5] = qsplit + (p.two_part ? total.w : 0u);
-        p.ctrl[6] = n_total;
-        p.ctrl[7] = p.variant == SHGPU_SHOCKHASH2 ? (overflow ? 0u : total.y) : n_total;
-        p.ctrl[8] = 0;
+        p.ctrl[C_HEAD] = 0;
+        p.ctrl[C_SPLIT_TAIL] = p.two_part ? total.z : total.z + total.w;
+        p.ctrl[C_ERROR] = 0;
+        p.ctrl[C_FINISHED] = 0;
+        p.ctrl[C_LEAF_TAIL] = qsplit + (p.two_part ? total.w : 0u);
+        p.ctrl[C_NODES] = n_total;
+        p.ctrl[C_WINDOWED] = p.variant == SHGPU_SHOCKHASH2 ? (overflow ? 0u : total.y) : n_total;
+        p.ctrl[C_STARTED] = 0;
     }
     for (uint32_t i = t; i < p.nhelp; i += PLAN_THREADS) p.help[i] = EMPTY_SLOT;
 }
@@ -528,6 +528,20 @@ __device__ __forceinline__ uint32_t plan_node(const PlanArgs &p, uint32_t offset
     return n;
 }
 
+// Waves that may help a node's owner at once. With q the chance that one window of W seeds holds a suitable
+// seed, the i-th of several windows in flight is needed with probability (1 - q)^(i - 1); helpers join only
+// while that is at least HELP_USEFUL for the window each would take, so nodes that one window most likely
+// settles (a root split of 2000 keys: q ~ 0.7) are not swarmed by waves searching windows past the winner,
+// while long searches (leaves, lower aggregations) take many helpers.
+constexpr float HELP_USEFUL = 0.5f;
+__device__ __forceinline__ uint32_t plan_help_cap(const PlanArgs &p, const shgpu_node &nd) {
+    const uint32_t W = nd.kind == SHGPU_NODE_LEAF ? p.win_leaf : nd.kind == SHGPU_NODE_AGGR_LOWER ? p.win_lower : 64u;
+    const float per_window = p.seed_rate[nd.m] * (float)W;  // -ln(1 - q)
+    if (!(per_window > 0.f)) return 0xFFFFFFFFu;
+    const float extra = -__logf(HELP_USEFUL) / per_window;
+    return extra >= 1e6f ? 1000000u : (uint32_t)extra;
+}
+
 // One thread per bucket: the bucket's tree in DFS pre-order from its first node index, the node states,
 // and the queue slots of its node range (its root's slot gets the root, other slots EMPTY_SLOT; every
 // slot of [0, total) belongs to exactly one bucket's range, and the root slots are exactly those the
@@ -561,14 +575,16 @@ __global__ void __launch_bounds__(256) k_plan_emit(PlanArgs p) {
         s.gd = 0;
         s.best = SHGPU_NOT_FOUND;
         s.finalized = 0;
-        s.pad[0] = s.pad[1] = s.pad[2] = 0;
+        s.help_cap = plan_help_cap(p, nd);
+        s.helpers = 0;
+        s.pad = 0;
         p.state[idx] = s;
         const bool root_region = p.two_part ? (idx < split_roots || (idx >= qsplit && idx < qsplit + leaf_roots))
                                                 : idx < split_roots + leaf_roots;
         if (!root_region) p.queue[idx] = EMPTY_SLOT;
         idx++;
         for (uint32_t c = nc; c-- > 0;) {  // reversed, so the first child is emitted next
-            if (sp == PLAN_STACK) { atomicExch(&p.ctrl[2], 1u); return; }
+            if (sp == PLAN_STACK) { atomicExch(&p.ctrl[C_ERROR], 1u); return; }
             st_off[sp] = co[c];
             st_m[sp] = cm[c];
             st_level[sp++] = level + 1;
@@ -597,6 +613,18 @@ struct Strategy {
 // forever, include/SIMDShockHash.hpp:131,171). The schedule's max_trials caps it further. Returned as log2
 // per node size, evaluated with libm at run time (a bound, not part of the stored format).
 constexpr uint32_t SEED_BOUND_SLACK = 24;
+
+// Estimated probability that a seed splits a node of m > leaf keys into the strategy's parts (the multinomial
+// estimate of include/ShockHash.h:79-127, include/ShockHash2.h:63-115).
+double split_success_probability(const Strategy &st, uint32_t m) {
+    uint32_t fanout, unit;
+    if (m > st.upper) { unit = st.upper * (uint16_t(m / 2 + st.upper - 1) / st.upper); fanout = 2; }
+    else if (m > st.lower) { unit = st.lower; fanout = uint16_t(m + st.lower - 1) / st.lower; }
+    else { unit = st.leaf; fanout = uint16_t(m + st.leaf - 1) / st.leaf; }
+    double sqrt_prod = std::sqrt((double)(m - (fanout - 1) * unit));
+    for (uint32_t i = 0; i + 1 < fanout; i++) sqrt_prod *= std::sqrt((double)unit);
+    return std::sqrt((double)m) / (std::pow(2 * M_PI, (fanout - 1.) / 2) * sqrt_prod);
+}
 std::vector<uint32_t> seed_bound_log2(const Strategy &st) {
     std::vector<uint32_t> lg(MAX_ENCODABLE_BUCKET + 1, 0);
     for (uint32_t m = 0; m <= MAX_ENCODABLE_BUCKET; m++) {
@@ -605,19 +633,33 @@ std::vector<uint32_t> seed_bound_log2(const Strategy &st) {
             k = st.variant == SHGPU_SHOCKHASH2 ? shockhash::bij_memo2[std::min<uint32_t>(m, shockhash::MAX_LEAF_SIZE2)]
                                                : shockhash::bij_memo[std::min<uint32_t>(m, shockhash::MAX_LEAF_SIZE)];
         } else {
-            uint32_t fanout, unit;
-            if (m > st.upper) { unit = st.upper * (uint16_t(m / 2 + st.upper - 1) / st.upper); fanout = 2; }
-            else if (m > st.lower) { unit = st.lower; fanout = uint16_t(m + st.lower - 1) / st.lower; }
-            else { unit = st.leaf; fanout = uint16_t(m + st.leaf - 1) / st.leaf; }
-            double sqrt_prod = std::sqrt((double)(m - (fanout - 1) * unit));
-            for (uint32_t i = 0; i + 1 < fanout; i++) sqrt_prod *= std::sqrt((double)unit);
-            const double p = std::sqrt((double)m) / (std::pow(2 * M_PI, (fanout - 1.) / 2) * sqrt_prod);
+            const double p = split_success_probability(st, m);
             const double len = std::ceil(std::log2(-std::log((std::sqrt(5) + 1) / 2) / std::log1p(-p)));
             k = len > 0 ? (uint32_t)std::min(len, 40.0) : 0u;
         }
         lg[m] = std::min<uint32_t>(63, k + SEED_BOUND_SLACK);
     }
     return lg;
+}
+
+// Per node size m: -ln(1 - p) as the bits of a float, p the probability that a seed suits a node of m keys:
+// a split node's from the multinomial estimate its Golomb parameter comes from, a leaf's from that parameter k
+// (codes are about geometric with mean 2^k / ln 2). It sizes how many waves help a node's search (plan_help_cap).
+std::vector<uint32_t> seed_rates(const Strategy &st) {
+    std::vector<uint32_t> out(MAX_ENCODABLE_BUCKET + 1, 0);
+    for (uint32_t m = 2; m <= MAX_ENCODABLE_BUCKET; m++) {
+        double p;
+        if (m <= st.leaf) {
+            const uint32_t k = st.variant == SHGPU_SHOCKHASH2 ? shockhash::bij_memo2[std::min<uint32_t>(m, shockhash::MAX_LEAF_SIZE2)]
+                                                               : shockhash::bij_memo[std::min<uint32_t>(m, shockhash::MAX_LEAF_SIZE)];
+            p = std::min(1.0, std::log(2.0) / std::ldexp(1.0, (int)k));
+        } else {
+            p = split_success_probability(st, m);
+        }
+        const float r = p >= 1.0 ? 1e30f : (float)-std::log1p(-p);
+        std::memcpy(&out[m], &r, 4);
+    }
+    return out;
 }
 
 // Per node size: min(cap, 2^seed_bound_log2), the max_trials of every node of that size.
@@ -1225,7 +1267,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
     std::vector<uint64_t> acc_all(NB);
     std::vector<uint64_t> codes;
     std::vector<uint64_t> node_begin(count, 0), node_end(count, 0);
-    uint32_t ctrl_out[4] = {0, 0, 0, 0};
+    uint32_t ctrl_error = 0;
     uint32_t sort_rerun = 0;  // a run of keys sharing the sorted bits was too long: sort on all 64 bits
     const uint64_t *codes_pinned = nullptr;  // persistent form: codes arrive in pinned memory (n_codes of them)
     const shgpu_node *nodes_pinned = nullptr;  // ... and with SHGPU_NODES_HOST the device-planned nodes
@@ -1250,9 +1292,11 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             std::vector<uint32_t> all, split;
             if ((rc = subtree_counts(st, all, split))) return fail_all(rc);
             const std::vector<uint32_t> lg = seed_bound_log2(st);
+            const std::vector<uint32_t> rate = seed_rates(st);
             B.plan_counts = all;
             B.plan_counts.insert(B.plan_counts.end(), split.begin(), split.end());
             B.plan_counts.insert(B.plan_counts.end(), lg.begin(), lg.end());
+            B.plan_counts.insert(B.plan_counts.end(), rate.begin(), rate.end());
             if ((rc = B.plan_tab.ensure(B.plan_counts.size() * 4))) return fail_all(rc);
             BATCH_HIP(hipMemcpyAsync(B.plan_tab.p, B.plan_counts.data(), B.plan_counts.size() * 4, hipMemcpyHostToDevice, s));
             B.plan_variant = d->variant;
@@ -1271,7 +1315,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         std::vector<PlanSet> sets(count);
         for (size_t r = 0; r < count; r++) sets[r] = PlanSet{base[r], (uint32_t)bbase[r], (uint32_t)nbk[r]};
         if ((rc = B.plan_sets.ensure(count * sizeof(PlanSet))) || (rc = B.plan_bucket.ensure(NB * sizeof(uint4))) ||
-            (rc = B.plan_summary.ensure((SUM_SETS + 3 * count) * 4)) || (rc = B.ctrl.ensure(64)) ||
+            (rc = B.plan_summary.ensure((SUM_SETS + 3 * count) * 4)) || (rc = B.ctrl.ensure(CTRL_WORDS * 4)) ||
             (rc = B.help.ensure((size_t)nslots * 4 + 64)))
             return fail_all(rc);
         BATCH_HIP(hipMemcpyAsync(B.plan_sets.p, sets.data(), count * sizeof(PlanSet), hipMemcpyHostToDevice, s));
@@ -1293,6 +1337,9 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             p.cnt_all = B.plan_tab.as<uint32_t>();
             p.cnt_split = B.plan_tab.as<uint32_t>() + MAX_ENCODABLE_BUCKET + 1;
             p.bound_lg = B.plan_tab.as<uint32_t>() + 2 * (MAX_ENCODABLE_BUCKET + 1);
+            p.seed_rate = (const float *)(B.plan_tab.as<uint32_t>() + 3 * (MAX_ENCODABLE_BUCKET + 1));
+            p.win_leaf = 64u * tune.sh1_seeds_per_lane;
+            p.win_lower = 64u * tune.split_seeds_per_lane;
             p.bucket = B.plan_bucket.as<uint4>();
             p.summary = B.plan_summary.as<uint32_t>();
             p.capacity = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
@@ -1332,7 +1379,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             timed_kernel = true;
             BATCH_HIP(hipMemcpyAsync(summary.data(), p.summary, summary.size() * 4, hipMemcpyDeviceToHost, s));
             BATCH_HIP(hipMemcpyAsync(acc_all.data(), B.size_acc.p, NB * 8, hipMemcpyDeviceToHost, s));
-            BATCH_HIP(hipMemcpyAsync(ctrl_out, a.ctrl, sizeof(ctrl_out), hipMemcpyDeviceToHost, s));
+            BATCH_HIP(hipMemcpyAsync(&ctrl_error, a.ctrl + C_ERROR, 4, hipMemcpyDeviceToHost, s));
             BATCH_HIP(hipMemcpyAsync(&sort_rerun, B.sort_flag.p, 4, hipMemcpyDeviceToHost, s));
             BATCH_HIP(hipStreamSynchronize(s));
             if (sort_rerun) return run_batch(e, B, s, grid_div, d, in, count, out, status, true);
@@ -1341,7 +1388,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             cap = (uint64_t)summary[SUM_TOTAL] + 1024;  // exact size, then run the plan and the search again
         }
         const uint32_t total = summary[SUM_TOTAL];
-        if (sh2l.wide && total && !ctrl_out[2] &&
+        if (sh2l.wide && total && !ctrl_error &&
             (rc = sh2_retry_overflowed(e, B, s, a, total, d->leaf_size, tune.flags)))
             return fail_all(rc);
         if ((rc = B.codes_host.ensure((size_t)total * 8 + 8))) return fail_all(rc);
@@ -1358,7 +1405,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             node_end[r] = summary[SUM_SETS + 3 * r + 1];
             if (summary[SUM_SETS + 3 * r + 2]) status[r] = fail(SHGPU_ERR_BUCKET_TOO_LARGE, "a bucket holds more than 2999 keys");
         }
-        if (total && ctrl_out[2]) return fail_all(fail(SHGPU_ERR_HIP, "persistent build: a queue slot was never filled (inconsistent forest)"));
+        if (total && ctrl_error) return fail_all(fail(SHGPU_ERR_HIP, "persistent build: a queue slot was never filled (inconsistent forest)"));
         if (tune.flags & SHGPU_TUNE_CHECK_PLAN) {  // the device plan against the host plan, node by node
             std::vector<shgpu_node> dev(total);
             if (total) BATCH_HIP(hipMemcpyAsync(dev.data(), a.nodes, (size_t)total * sizeof(shgpu_node), hipMemcpyDeviceToHost, s));
@@ -1414,7 +1461,7 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
         const uint32_t total = (uint32_t)nodes.size();
         if ((rc = check_forest(nodes, roots))) return fail_all(rc);
         if ((rc = B.nodes.ensure((size_t)total * sizeof(shgpu_node) + 64)) || (rc = B.codes.ensure((size_t)total * 8 + 64)) ||
-            (rc = B.ctrl.ensure(64)))
+            (rc = B.ctrl.ensure(CTRL_WORDS * 4)))
             return fail_all(rc);
         if (total) BATCH_HIP(hipMemcpyAsync(B.nodes.p, nodes.data(), (size_t)total * sizeof(shgpu_node), hipMemcpyHostToDevice, s));
         t2 = clk::now();
