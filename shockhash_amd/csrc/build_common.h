@@ -47,7 +47,7 @@ struct NodeState {
     uint32_t finalized;       // number of waves that claimed finalisation (the first one does it)
     uint32_t help_cap;        // waves that may help the node's owner at once (the plan's)
     uint32_t helpers;         // waves helping it now
-    uint32_t pad;
+    uint32_t help_all;        // windows completed after which any number of waves may help
 };
 
 struct BuildArgs {
@@ -79,12 +79,13 @@ __host__ __device__ __forceinline__ bool rotation_leaf(const BuildArgs &a, uint3
 __host__ __device__ __forceinline__ bool two_part_queue(uint32_t flags) { return (flags & SHGPU_TUNE_SPLITS_FIRST) != 0; }
 
 // Persistent build of one variant (k_build, grid x BLOCK threads) with sl / ss seeds per lane for leaf / split
-// windows (1, 2 or 4; ShockHash2 leaves ignore sl), and the plain form's launch for node nid (k_node). wide:
+// windows (1, 2 or 4; ShockHash2 leaves ignore sl), and the plain form's launch for node nid whose keys are in
+// key buffer buf (k_node: 0 = keys, 1 = tmp; the buffer alternates with the node's depth). wide:
 // ShockHash2 leaves of more than 64 keys occur (the kernels then carry the two-keys-per-lane search).
 template <int VARIANT>
 hipError_t launch_build_variant(uint32_t sl, uint32_t ss, bool wide, const BuildArgs &a, int grid, hipStream_t s);
 template <int VARIANT>
-hipError_t launch_node_variant(bool wide, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s);
+hipError_t launch_node_variant(bool wide, const BuildArgs &a, uint32_t nid, uint32_t buf, uint32_t waves, hipStream_t s);
 
 // Batched per-node searches (batch.hip); every call returns the launch's status.
 hipError_t launch_sh1_batch(uint32_t seeds_per_lane, int grid, hipStream_t s, const uint64_t *keys,

@@ -16,6 +16,16 @@ __device__ __forceinline__ unsigned long long wave_fetch_add64(unsigned long lon
     return uniform64(old);
 }
 
+// Keys live in two buffers of the same indexing, a.keys and a.tmp: a split node partitions its keys from its
+// buffer into the other one, where its children find them, so no partition is copied back. Roots start in
+// a.keys; a leaf found in a.tmp copies its keys to a.keys when it is finished, so the leaf-ordered keys end up
+// in a.keys. Queue and help-registry entries carry a node id and its buffer (bit 31).
+constexpr uint32_t ENTRY_BUF = 1u << 31;
+__device__ __forceinline__ uint64_t *key_buf(const BuildArgs &a, uint32_t buf) { return buf ? a.tmp : a.keys; }
+__device__ __forceinline__ void copy_keys(const uint64_t *src, uint64_t *dst, uint32_t m) {
+    for (uint32_t i = lane_id(); i < m; i += 64) dst[i] = src[i];
+}
+
 // The value that decides a node's code is read with a read-modify-write (atomicOr 0), which resolves at
 // the coherence point.
 __device__ __forceinline__ uint64_t read_best(NodeState *st) {
@@ -32,13 +42,23 @@ __device__ __forceinline__ uint32_t window_seeds(uint32_t kind) {
 // persistent form) publishes the children. An exhausted search still publishes the children so that
 // the forest drains; the host then reports the error.
 template <int VARIANT>
-__device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, uint64_t off, bool persistent) {
+__device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, uint32_t buf, uint64_t off, bool persistent) {
     const shgpu_node nd = a.nodes[nid];
     const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
     if (VARIANT != SHGPU_SHOCKHASH2 && rotation_leaf(a, kind, m) && off != SHGPU_NOT_FOUND) off = (off >> 6) * m + (off & 63);
+    const uint64_t *src = key_buf(a, buf) + o;
     if (kind != SHGPU_NODE_LEAF) {
         const SplitShapeDev sh = make_split_shape(m, uniform(nd.unit), kind, a.split_hash);
-        if (off != SHGPU_NOT_FOUND) split_partition(a.keys + o, a.tmp + o, sh, uniform64(nd.start_seed) + off);
+        uint64_t *dst = key_buf(a, buf ^ 1u) + o;
+        // an exhausted search leaves its children the node's keys as they are
+        if (off != SHGPU_NOT_FOUND) split_partition_to(src, dst, sh, uniform64(nd.start_seed) + off);
+        else copy_keys(src, dst, m);
+        if (!buf) {
+            // a part of one key is no node: its key is in leaf order already and goes to a.keys now
+            const uint32_t p = lane_id(), unit = uniform(nd.unit);
+            const uint32_t size = p + 1 < sh.fanout ? unit : m - (sh.fanout - 1) * unit;
+            if (p < sh.fanout && size == 1) a.keys[o + p * unit] = dst[p * unit];
+        }
         if (persistent) {
             __threadfence();  // children's keys visible device-wide before they are queued
             const uint32_t nc = uniform(nd.n_children);
@@ -49,10 +69,12 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
                 const shgpu_node cn = a.nodes[child];
                 const bool to_leaf_part = two_part_queue(a.flags) && uniform(cn.kind) == SHGPU_NODE_LEAF;
                 const uint32_t pos = wave_fetch_add(&a.ctrl[to_leaf_part ? C_LEAF_TAIL : C_SPLIT_TAIL], 1u);
-                __hip_atomic_store(&a.queue[pos], child, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&a.queue[pos], child | (buf ? 0u : ENTRY_BUF), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 child = uniform(child + cn.subtree);
             }
         }
+    } else if (buf) {
+        copy_keys(src, a.keys + o, m);
     }
     a.codes[nid] = off;  // read by the host after the kernel ends
     if (persistent) wave_fetch_add(&a.ctrl[C_FINISHED], 1u);
@@ -70,8 +92,10 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
 // been searched, so the result is the sequential minimum.
 // Keys are staged in the wave's LDS once per call (leaves, and split nodes of up to STAGE_KEYS keys).
 template <int VARIANT, int SL, int SS>
-__device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, PfLds &L, bool persistent, uint64_t t_start) {
+__device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, uint32_t buf, PfLds &L, bool persistent,
+                                             uint64_t t_start) {
     const shgpu_node nd = a.nodes[nid];
+    const uint64_t *keys = key_buf(a, buf) + uniform(nd.offset);
     const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
     NodeState *st = &a.state[nid];
     const uint64_t W = window_seeds<SL, SS>(kind);
@@ -93,9 +117,9 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
     if (!found) {
         if (kind == SHGPU_NODE_LEAF) {
             if constexpr (VARIANT != SHGPU_SHOCKHASH2)
-                if (m >= 2) kv = sh1_stage(a.keys + o, m, L);
+                if (m >= 2) kv = sh1_stage(keys, m, L);
         } else if (m <= STAGE_KEYS) {
-            split_stage(a.keys + o, m, L.stage);
+            split_stage(keys, m, L.stage);
             staged = true;
         }
     }
@@ -114,9 +138,9 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
                     r = m < 2 ? (woff == 0 ? 0 : SHGPU_NOT_FOUND)
                               : rot ? sh1_rot_window_staged<SL>(kv, m, start, woff, L) : sh1_window_staged<SL>(kv, m, start, woff, L);
             } else if (kind == SHGPU_NODE_AGGR_LOWER)
-                r = split_window<SS, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
+                r = split_window<SS, SPLIT_HASH>(keys, sh, start, woff, L.stage, staged);
             else
-                r = split_window<1, SPLIT_HASH>(a.keys + o, sh, start, woff, L.stage, staged);
+                r = split_window<1, SPLIT_HASH>(keys, sh, start, woff, L.stage, staged);
             if (r != SHGPU_NOT_FOUND && (r >> rsh) >= lim) r = SHGPU_NOT_FOUND;
             if (r != SHGPU_NOT_FOUND) {
                 // same value in every lane; waiting for each returned value orders the best seed before the
@@ -136,7 +160,7 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, P
             const uint64_t b = (old & GD_FOUND) ? read_best(st) : SHGPU_NOT_FOUND;
             const uint64_t covered = handed * W;
             if ((b != SHGPU_NOT_FOUND && (b >> rsh) < covered) || (b == SHGPU_NOT_FOUND && covered >= lim)) {
-                if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, b, persistent);
+                if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, buf, b, persistent);
             }
         }
         if (!more) {
@@ -161,22 +185,27 @@ __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLd
         const uint32_t v = base + lane < a.nslots ? __hip_atomic_load(&a.help[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                                   : EMPTY_SLOT;
         // a registered node whose best seed is already known needs no help, nor one with its helpers complete
+        // (unless it has searched so long that every wave may help)
         bool open = false;
         if (v != EMPTY_SLOT) {
-            const NodeState *st = &a.state[v];
-            open = !(__hip_atomic_load(&st->gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & GD_FOUND) &&
-                   __hip_atomic_load(&st->helpers, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < st->help_cap;
+            const NodeState *st = &a.state[v & ~ENTRY_BUF];
+            const unsigned long long gd = __hip_atomic_load(&st->gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            open = !(gd & GD_FOUND) &&
+                   (__hip_atomic_load(&st->helpers, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < st->help_cap ||
+                    gd_completed(gd) >= st->help_all);
         }
         const uint64_t hit = ballot(open);
         if (hit) {
-            const uint32_t nid = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));
+            const uint32_t entry = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));
+            const uint32_t nid = entry & ~ENTRY_BUF;
             NodeState *st = &a.state[nid];
-            if (wave_fetch_add(&st->helpers, 1u) >= uniform(st->help_cap)) {
+            if (wave_fetch_add(&st->helpers, 1u) >= uniform(st->help_cap) &&
+                gd_completed(uniform64(__hip_atomic_load(&st->gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) < uniform(st->help_all)) {
                 wave_fetch_add(&st->helpers, 0xFFFFFFFFu);  // full meanwhile: leave the count as it was
                 return false;
             }
             __threadfence();
-            work_windows<VARIANT, SL, SS>(a, nid, L, true, task_clock());
+            work_windows<VARIANT, SL, SS>(a, nid, entry >> 31, L, true, task_clock());
             wave_fetch_add(&st->helpers, 0xFFFFFFFFu);
             return true;
         }
@@ -194,23 +223,25 @@ union BuildLds {
 // One node taken from the queue: ShockHash2 leaves are searched by this wave alone; splits and
 // ShockHash1 leaves are searched in windows that idle waves may join (help registry).
 template <int VARIANT, int SL, int SS, bool WIDE>
-__device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t nid, uint32_t slot, BuildLds<VARIANT, WIDE> &L,
+__device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t entry, uint32_t slot, BuildLds<VARIANT, WIDE> &L,
                                          uint8_t *ws) {
+    const uint32_t nid = entry & ~ENTRY_BUF, buf = entry >> 31;
     const shgpu_node nd = a.nodes[nid];
     const uint32_t kind = uniform(nd.kind);
     const uint64_t t_start = task_clock();
     if (VARIANT == SHGPU_SHOCKHASH2 && kind == SHGPU_NODE_LEAF) {
         if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            const uint64_t code = sh2_leaf_search_any<WIDE>(a.keys + uniform(nd.offset), uniform(nd.m), ws, L.sh2, a.sh2_caps,
-                                                            a.flags, t_start);
+            const uint32_t o = uniform(nd.offset), m = uniform(nd.m);
+            const uint64_t code = sh2_leaf_search_any<WIDE>(key_buf(a, buf) + o, m, ws, L.sh2, a.sh2_caps, a.flags, t_start);
+            if (buf) copy_keys(a.tmp + o, a.keys + o, m);
             a.codes[nid] = code;
             wave_fetch_add(&a.ctrl[C_FINISHED], 1u);
         }
         return;
     }
     wave_fetch_add(&a.ctrl[C_STARTED], 1u);  // windowed nodes started
-    __hip_atomic_store(&a.help[slot], nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    work_windows<VARIANT, SL, SS>(a, nid, L.pf, true, t_start);
+    __hip_atomic_store(&a.help[slot], entry, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    work_windows<VARIANT, SL, SS>(a, nid, buf, L.pf, true, t_start);
     __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -264,18 +295,21 @@ __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
 // Plain form: one launch per node (the host walks the forest in DFS order on one stream). Every
 // wave of the launch joins the node's windowed search; a ShockHash2 leaf is searched by wave 0.
 template <int VARIANT, int SL, int SS, bool WIDE>
-__global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid) {
+__global__ void __launch_bounds__(64) k_node(BuildArgs a, uint32_t nid, uint32_t buf) {
     __shared__ BuildLds<VARIANT, WIDE> L;
     const shgpu_node nd = a.nodes[nid];
     if (VARIANT == SHGPU_SHOCKHASH2 && uniform(nd.kind) == SHGPU_NODE_LEAF) {
         if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            if (blockIdx.x == 0)
-                a.codes[nid] = sh2_leaf_search_any<WIDE>(a.keys + uniform(nd.offset), uniform(nd.m), a.scratch, L.sh2,
-                                                         a.sh2_caps, a.flags, task_clock());
+            if (blockIdx.x == 0) {
+                const uint32_t o = uniform(nd.offset), m = uniform(nd.m);
+                a.codes[nid] = sh2_leaf_search_any<WIDE>(key_buf(a, buf) + o, m, a.scratch, L.sh2, a.sh2_caps, a.flags,
+                                                         task_clock());
+                if (buf) copy_keys(a.tmp + o, a.keys + o, m);
+            }
         }
         return;
     }
-    work_windows<VARIANT, SL, SS>(a, nid, L.pf, false, task_clock());
+    work_windows<VARIANT, SL, SS>(a, nid, buf, L.pf, false, task_clock());
 }
 
 
@@ -307,15 +341,15 @@ hipError_t launch_build_variant(uint32_t sl, uint32_t ss, bool wide, const Build
 }
 
 template <int VARIANT>
-hipError_t launch_node_variant(bool wide, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
+hipError_t launch_node_variant(bool wide, const BuildArgs &a, uint32_t nid, uint32_t buf, uint32_t waves, hipStream_t s) {
     constexpr int SL = VARIANT == SHGPU_SHOCKHASH2 ? 1 : 4;
     if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
         if (wide) {
-            k_node<VARIANT, SL, 4, true><<<waves, 64, 0, s>>>(a, nid);
+            k_node<VARIANT, SL, 4, true><<<waves, 64, 0, s>>>(a, nid, buf);
             return hipGetLastError();
         }
     }
-    k_node<VARIANT, SL, 4, false><<<waves, 64, 0, s>>>(a, nid);
+    k_node<VARIANT, SL, 4, false><<<waves, 64, 0, s>>>(a, nid, buf);
     return hipGetLastError();
 }
 

@@ -4,5 +4,5 @@
 
 namespace shgpu {
 template hipError_t launch_build_variant<SHGPU_SIMD_SHOCKHASH>(uint32_t, uint32_t, bool, const BuildArgs &, int, hipStream_t);
-template hipError_t launch_node_variant<SHGPU_SIMD_SHOCKHASH>(bool, const BuildArgs &, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_node_variant<SHGPU_SIMD_SHOCKHASH>(bool, const BuildArgs &, uint32_t, uint32_t, uint32_t, hipStream_t);
 }  // namespace shgpu

@@ -528,18 +528,25 @@ __device__ __forceinline__ uint32_t plan_node(const PlanArgs &p, uint32_t offset
     return n;
 }
 
-// Waves that may help a node's owner at once. With q the chance that one window of W seeds holds a suitable
-// seed, the i-th of several windows in flight is needed with probability (1 - q)^(i - 1); helpers join only
-// while that is at least HELP_USEFUL for the window each would take, so nodes that one window most likely
-// settles (a root split of 2000 keys: q ~ 0.7) are not swarmed by waves searching windows past the winner,
-// while long searches (leaves, lower aggregations) take many helpers.
-constexpr float HELP_USEFUL = 0.5f;
-__device__ __forceinline__ uint32_t plan_help_cap(const PlanArgs &p, const shgpu_node &nd) {
+// Waves that may help a node's owner at once (cap), and the completed windows after which any number may
+// (all). With q the chance that one window of W seeds holds a suitable seed, the i-th of several windows in
+// flight is needed with probability (1 - q)^(i - 1); helpers join only while that is at least HELP_USEFUL for
+// the window each would take, so nodes that one window most likely settles (a root split of 2000 keys:
+// q ~ 0.7) are not swarmed by waves searching windows past the winner, while long searches (leaves, lower
+// aggregations) take many helpers. A search still running after HELP_ALL_AFTER / q windows (a chance of
+// e^-8 for a search that can succeed; one over keys that repeat never does) takes every idle wave.
+constexpr float HELP_USEFUL = 0.5f, HELP_ALL_AFTER = 8.f;
+__device__ __forceinline__ void plan_help_limits(const PlanArgs &p, const shgpu_node &nd, uint32_t &cap, uint32_t &all) {
     const uint32_t W = nd.kind == SHGPU_NODE_LEAF ? p.win_leaf : nd.kind == SHGPU_NODE_AGGR_LOWER ? p.win_lower : 64u;
     const float per_window = p.seed_rate[nd.m] * (float)W;  // -ln(1 - q)
-    if (!(per_window > 0.f)) return 0xFFFFFFFFu;
+    cap = 0xFFFFFFFFu;
+    all = 0;
+    if (!(per_window > 0.f)) return;
     const float extra = -__logf(HELP_USEFUL) / per_window;
-    return extra >= 1e6f ? 1000000u : (uint32_t)extra;
+    const float q = -expm1f(-per_window);
+    const float after = HELP_ALL_AFTER / q;
+    cap = extra >= 1e6f ? 1000000u : (uint32_t)extra;
+    all = after >= 2e9f ? 2000000000u : (uint32_t)after;
 }
 
 // One thread per bucket: the bucket's tree in DFS pre-order from its first node index, the node states,
@@ -575,9 +582,8 @@ __global__ void __launch_bounds__(256) k_plan_emit(PlanArgs p) {
         s.gd = 0;
         s.best = SHGPU_NOT_FOUND;
         s.finalized = 0;
-        s.help_cap = plan_help_cap(p, nd);
+        plan_help_limits(p, nd, s.help_cap, s.help_all);
         s.helpers = 0;
-        s.pad = 0;
         p.state[idx] = s;
         const bool root_region = p.two_part ? (idx < split_roots || (idx >= qsplit && idx < qsplit + leaf_roots))
                                                 : idx < split_roots + leaf_roots;
@@ -846,8 +852,8 @@ int launch_build_v(const shgpu_tuning &t, bool wide, const BuildArgs &a, int gri
 }
 
 template <int VARIANT>
-int launch_node_v(shgpu_engine *, bool wide, const BuildArgs &a, uint32_t nid, uint32_t waves, hipStream_t s) {
-    const hipError_t err = launch_node_variant<VARIANT>(wide, a, nid, waves, s);
+int launch_node_v(shgpu_engine *, bool wide, const BuildArgs &a, uint32_t nid, uint32_t buf, uint32_t waves, hipStream_t s) {
+    const hipError_t err = launch_node_variant<VARIANT>(wide, a, nid, buf, waves, s);
     if (err != hipSuccess) return fail(SHGPU_ERR_HIP, std::string("k_node launch: ") + hipGetErrorString(err));
     return SHGPU_OK;
 }
@@ -1484,12 +1490,18 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             BATCH_HIP(hipGetLastError());
         }
         const uint32_t waves = (uint32_t)e->cu_count;  // one wave per CU joins every node's windows
+        // key buffer of every node: roots in keys, each level partitions into the other buffer
+        std::vector<uint8_t> buf(total, 0);
+        for (uint32_t nid = 0; nid < total; nid++) {
+            uint32_t c = nid + 1;
+            for (uint32_t k = 0; k < nodes[nid].n_children; k++, c += nodes[c].subtree) buf[c] = buf[nid] ^ 1;
+        }
         for (uint32_t nid = 0; nid < total; nid++) {
             const bool sh2_leaf = d->variant == SHGPU_SHOCKHASH2 && nodes[nid].kind == SHGPU_NODE_LEAF;
             const uint32_t wv = sh2_leaf ? 1u : waves;
-            if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, false, a, nid, wv, s);
-            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, false, a, nid, wv, s);
-            else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, sh2l.wide, a, nid, wv, s);
+            if (d->variant == SHGPU_SHOCKHASH) rc = launch_node_v<SHGPU_SHOCKHASH>(e, false, a, nid, buf[nid], wv, s);
+            else if (d->variant == SHGPU_SIMD_SHOCKHASH) rc = launch_node_v<SHGPU_SIMD_SHOCKHASH>(e, false, a, nid, buf[nid], wv, s);
+            else rc = launch_node_v<SHGPU_SHOCKHASH2>(e, sh2l.wide, a, nid, buf[nid], wv, s);
             if (rc) return fail_all(rc);
         }
         if (sh2l.wide && total) {

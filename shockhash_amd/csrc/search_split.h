@@ -206,15 +206,15 @@ __device__ __forceinline__ uint64_t split_search(const uint64_t *keys, const Spl
     return SHGPU_NOT_FOUND;
 }
 
-// Stable counting-sort partition of keys[0..m) by part for seed x into tmp, then copied back
-// (ShockHash.h:464-469,495-499,528-532; SIMDShockHash.hpp:234-252,337-354): part p starts at p*unit.
-__device__ __forceinline__ void split_partition(uint64_t *keys, uint64_t *tmp, const SplitShapeDev &sh, uint64_t x) {
+// Stable counting-sort partition of src[0..m) by part for seed x into dst (ShockHash.h:464-469,495-499,528-532;
+// SIMDShockHash.hpp:234-252,337-354): part p starts at p*unit.
+__device__ __forceinline__ void split_partition_to(const uint64_t *src, uint64_t *dst, const SplitShapeDev &sh, uint64_t x) {
     const uint32_t lane = lane_id();
     const uint32_t m = sh.m;
     uint32_t base[4] = {0, sh.unit, 2 * sh.unit, 3 * sh.unit};
     for (uint32_t chunk = 0; chunk < m; chunk += 64) {
         const bool act = chunk + lane < m;
-        const uint64_t key = act ? keys[chunk + lane] : 0;
+        const uint64_t key = act ? src[chunk + lane] : 0;
         const uint64_t v = split_value(key + x, sh.simd);
         uint32_t part = 0;
 #pragma unroll
@@ -226,10 +226,15 @@ __device__ __forceinline__ void split_partition(uint64_t *keys, uint64_t *tmp, c
             if (act && part == p) pos = base[p] + lanes_below(b);
             base[p] += (uint32_t)__builtin_popcountll(b);
         }
-        if (act) tmp[pos] = key;
+        if (act) dst[pos] = key;
     }
+}
+
+// The same partition in place: into tmp, then copied back.
+__device__ __forceinline__ void split_partition(uint64_t *keys, uint64_t *tmp, const SplitShapeDev &sh, uint64_t x) {
+    split_partition_to(keys, tmp, sh, x);
     __threadfence_block();
-    for (uint32_t i = lane; i < m; i += 64) keys[i] = tmp[i];
+    for (uint32_t i = lane_id(); i < sh.m; i += 64) keys[i] = tmp[i];
 }
 
 }  // namespace shgpu
