@@ -21,6 +21,8 @@ __device__ __forceinline__ unsigned long long wave_fetch_add64(unsigned long lon
 // a.keys; a leaf found in a.tmp copies its keys to a.keys when it is finished, so the leaf-ordered keys end up
 // in a.keys. Queue and help-registry entries carry a node id and its buffer (bit 31).
 constexpr uint32_t ENTRY_BUF = 1u << 31;
+// Queue slot of a child that its parent's finalising wave runs itself (continuation): taken and passed over.
+constexpr uint32_t SKIP_SLOT = 0xFFFFFFFEu;
 __device__ __forceinline__ uint64_t *key_buf(const BuildArgs &a, uint32_t buf) { return buf ? a.tmp : a.keys; }
 __device__ __forceinline__ void copy_keys(const uint64_t *src, uint64_t *dst, uint32_t m) {
     for (uint32_t i = lane_id(); i < m; i += 64) dst[i] = src[i];
@@ -40,9 +42,11 @@ __device__ __forceinline__ uint32_t window_seeds(uint32_t kind) {
 
 // Stores the node's code; for a split node also partitions its keys by the found seed and (in the
 // persistent form) publishes the children. An exhausted search still publishes the children so that
-// the forest drains; the host then reports the error.
+// the forest drains; the host then reports the error. Returns the queue entry of the child the calling wave
+// runs next itself (its first, largest child; its queue slot gets SKIP_SLOT), or EMPTY_SLOT.
 template <int VARIANT>
-__device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, uint32_t buf, uint64_t off, bool persistent) {
+__device__ __forceinline__ uint32_t finalize_node(const BuildArgs &a, uint32_t nid, uint32_t buf, uint64_t off, bool persistent) {
+    uint32_t cont = EMPTY_SLOT;
     const shgpu_node nd = a.nodes[nid];
     const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
     if (VARIANT != SHGPU_SHOCKHASH2 && rotation_leaf(a, kind, m) && off != SHGPU_NOT_FOUND) off = (off >> 6) * m + (off & 63);
@@ -69,7 +73,10 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
                 const shgpu_node cn = a.nodes[child];
                 const bool to_leaf_part = two_part_queue(a.flags) && uniform(cn.kind) == SHGPU_NODE_LEAF;
                 const uint32_t pos = wave_fetch_add(&a.ctrl[to_leaf_part ? C_LEAF_TAIL : C_SPLIT_TAIL], 1u);
-                __hip_atomic_store(&a.queue[pos], child | (buf ? 0u : ENTRY_BUF), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t entry = child | (buf ? 0u : ENTRY_BUF);
+                const bool keep = c == 0;
+                if (keep) cont = entry;
+                __hip_atomic_store(&a.queue[pos], keep ? SKIP_SLOT : entry, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 child = uniform(child + cn.subtree);
             }
         }
@@ -78,6 +85,7 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
     }
     a.codes[nid] = off;  // read by the host after the kernel ends
     if (persistent) wave_fetch_add(&a.ctrl[C_FINISHED], 1u);
+    return cont;
 }
 
 // Searches windows of a split node or ShockHash1 leaf until its smallest qualifying seed is known.
@@ -91,9 +99,11 @@ __device__ __forceinline__ void finalize_node(const BuildArgs &a, uint32_t nid, 
 // outstanding window sees the counts equal again and decides then). Every window below the winner has
 // been searched, so the result is the sequential minimum.
 // Keys are staged in the wave's LDS once per call (leaves, and split nodes of up to STAGE_KEYS keys).
+// Returns the continuation of finalize_node when this wave finalised the node, else EMPTY_SLOT.
 template <int VARIANT, int SL, int SS>
-__device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, uint32_t buf, PfLds &L, bool persistent,
-                                             uint64_t t_start) {
+__device__ __forceinline__ uint32_t work_windows(const BuildArgs &a, uint32_t nid, uint32_t buf, PfLds &L, bool persistent,
+                                                 uint64_t t_start) {
+    uint32_t cont = EMPTY_SLOT;
     const shgpu_node nd = a.nodes[nid];
     const uint64_t *keys = key_buf(a, buf) + uniform(nd.offset);
     const uint32_t o = uniform(nd.offset), m = uniform(nd.m), kind = uniform(nd.kind);
@@ -160,23 +170,24 @@ __device__ __forceinline__ void work_windows(const BuildArgs &a, uint32_t nid, u
             const uint64_t b = (old & GD_FOUND) ? read_best(st) : SHGPU_NOT_FOUND;
             const uint64_t covered = handed * W;
             if ((b != SHGPU_NOT_FOUND && (b >> rsh) < covered) || (b == SHGPU_NOT_FOUND && covered >= lim)) {
-                if (wave_fetch_add(&st->finalized, 1u) == 0) finalize_node<VARIANT>(a, nid, buf, b, persistent);
+                if (wave_fetch_add(&st->finalized, 1u) == 0) cont = finalize_node<VARIANT>(a, nid, buf, b, persistent);
             }
         }
         if (!more) {
             reset_priority();
-            return;
+            return cont;
         }
         w = handed;  // the window this operation handed out
         found = (old & GD_FOUND) != 0;
     }
 }
 
-// Joins one node another wave is searching in windows, if any. Returns true when it did some work. A node
-// takes at most help_cap helpers at once: a wave joins after raising the node's helper count below the cap
-// (waves that scan the registry at the same moment would otherwise all join a newly registered node).
-template <int VARIANT, int SL, int SS>
-__device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLds &L) {
+// A node another wave is searching in windows that this wave may join: its queue entry, with this wave counted
+// among the node's helpers, or EMPTY_SLOT. A node takes at most help_cap helpers at once (unless it has searched
+// so long that every wave may help): a wave joins after raising the node's helper count below the cap (waves
+// that scan the registry at the same moment would otherwise all join a newly registered node), and a node whose
+// best seed is already known needs no help.
+__device__ __forceinline__ uint32_t find_help(const BuildArgs &a, uint32_t slot) {
     const uint32_t lane = lane_id();
     const uint32_t first = (slot * 97u) % a.nslots;
     for (uint32_t base = 0; base < a.nslots; base += 64) {
@@ -184,8 +195,6 @@ __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLd
         if (i >= a.nslots) i -= a.nslots;
         const uint32_t v = base + lane < a.nslots ? __hip_atomic_load(&a.help[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                                   : EMPTY_SLOT;
-        // a registered node whose best seed is already known needs no help, nor one with its helpers complete
-        // (unless it has searched so long that every wave may help)
         bool open = false;
         if (v != EMPTY_SLOT) {
             const NodeState *st = &a.state[v & ~ENTRY_BUF];
@@ -197,20 +206,16 @@ __device__ __forceinline__ bool help_one(const BuildArgs &a, uint32_t slot, PfLd
         const uint64_t hit = ballot(open);
         if (hit) {
             const uint32_t entry = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)__builtin_ctzll(hit));
-            const uint32_t nid = entry & ~ENTRY_BUF;
-            NodeState *st = &a.state[nid];
+            NodeState *st = &a.state[entry & ~ENTRY_BUF];
             if (wave_fetch_add(&st->helpers, 1u) >= uniform(st->help_cap) &&
                 gd_completed(uniform64(__hip_atomic_load(&st->gd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) < uniform(st->help_all)) {
                 wave_fetch_add(&st->helpers, 0xFFFFFFFFu);  // full meanwhile: leave the count as it was
-                return false;
+                return EMPTY_SLOT;
             }
-            __threadfence();
-            work_windows<VARIANT, SL, SS>(a, nid, entry >> 31, L, true, task_clock());
-            wave_fetch_add(&st->helpers, 0xFFFFFFFFu);
-            return true;
+            return entry;
         }
     }
-    return false;
+    return EMPTY_SLOT;
 }
 
 // A wave runs either a windowed search (pf) or a ShockHash2 leaf (sh2), never both at once.
@@ -220,39 +225,17 @@ union BuildLds {
     typename std::conditional<VARIANT == SHGPU_SHOCKHASH2, Sh2LdsAny<WIDE>, char>::type sh2;
 };
 
-// One node taken from the queue: ShockHash2 leaves are searched by this wave alone; splits and
-// ShockHash1 leaves are searched in windows that idle waves may join (help registry).
-template <int VARIANT, int SL, int SS, bool WIDE>
-__device__ __forceinline__ void run_node(const BuildArgs &a, uint32_t entry, uint32_t slot, BuildLds<VARIANT, WIDE> &L,
-                                         uint8_t *ws) {
-    const uint32_t nid = entry & ~ENTRY_BUF, buf = entry >> 31;
-    const shgpu_node nd = a.nodes[nid];
-    const uint32_t kind = uniform(nd.kind);
-    const uint64_t t_start = task_clock();
-    if (VARIANT == SHGPU_SHOCKHASH2 && kind == SHGPU_NODE_LEAF) {
-        if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
-            const uint32_t o = uniform(nd.offset), m = uniform(nd.m);
-            const uint64_t code = sh2_leaf_search_any<WIDE>(key_buf(a, buf) + o, m, ws, L.sh2, a.sh2_caps, a.flags, t_start);
-            if (buf) copy_keys(a.tmp + o, a.keys + o, m);
-            a.codes[nid] = code;
-            wave_fetch_add(&a.ctrl[C_FINISHED], 1u);
-        }
-        return;
-    }
-    wave_fetch_add(&a.ctrl[C_STARTED], 1u);  // windowed nodes started
-    __hip_atomic_store(&a.help[slot], entry, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    work_windows<VARIANT, SL, SS>(a, nid, buf, L.pf, true, t_start);
-    __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 constexpr uint32_t SPIN_LIMIT = 1u << 27;  // sleeps before a wait is declared stuck (well over ten seconds)
 
-// Persistent construction kernel. Every wave takes queue slots in order; a finished split node
-// publishes its children. With SHGPU_TUNE_SPLITS_FIRST the queue has a split part (slots [0, n_split))
-// and a leaf part (slots [n_split, total)) and children go to the part of their kind, so all split
-// nodes are taken before any leaf; otherwise it is one FIFO. While a slot is not yet published (its
-// parent is still being split) the wave helps windowed searches. When the queue is exhausted, waves
-// keep helping until every node is finished.
+// Persistent construction kernel. Every wave takes queue slots in order; a finished split node publishes its
+// children, except the first (largest) one, which the wave that finished the node runs next itself. With
+// SHGPU_TUNE_SPLITS_FIRST the queue has a split part (slots [0, n_split)) and a leaf part (slots [n_split,
+// total)) and children go to the part of their kind, so all split nodes are taken before any leaf; otherwise it
+// is one FIFO. While a wave's slot is not yet published (its parent is still being split) the wave helps
+// windowed searches. When every slot has been taken, waves keep helping until every node is finished.
+// ShockHash2 leaves are searched by the wave that takes them; split nodes and ShockHash1 leaves by their owner
+// in windows that idle waves may join (help registry). The loop has one place that searches windows and one
+// that searches a ShockHash2 leaf, so each is compiled once.
 template <int VARIANT, int SL, int SS, bool WIDE>
 __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
     __shared__ BuildLds<VARIANT, WIDE> lds[WAVES_PER_BLOCK];
@@ -261,34 +244,69 @@ __global__ void __launch_bounds__(BLOCK, 6) k_build(BuildArgs a) {
     BuildLds<VARIANT, WIDE> &L = lds[w];
     uint8_t *ws = VARIANT == SHGPU_SHOCKHASH2 ? a.scratch + (size_t)slot * a.sh2_wave_bytes : nullptr;
     const uint32_t total = uniform(a.ctrl[C_NODES]);  // written by the forest plan
-    for (;;) {
-        const uint32_t idx = grab(&a.ctrl[C_HEAD]);
-        if (idx >= total) break;
-        uint32_t nid = EMPTY_SLOT;
-        for (uint32_t spin = 0;; spin++) {
-            nid = uniform(__hip_atomic_load(&a.queue[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-            if (nid != EMPTY_SLOT) break;
-            if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
-            if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[C_ERROR], 1u); break; }
-            __builtin_amdgcn_s_sleep(4);
-        }
-        if (nid == EMPTY_SLOT) return;
-        __threadfence();
-        run_node<VARIANT, SL, SS, WIDE>(a, nid, slot, L, ws);
-    }
-    // Queue exhausted: help until every node is finished. Once every windowed node has started (C_STARTED
-    // reaches C_WINDOWED) no node will register for help any more, and a wave that finds none open leaves
-    // instead of polling beside the waves still searching (checked every 8th poll: ~6000 waves polling one
-    // word load a single L2 channel).
     const uint32_t windowed = uniform(a.ctrl[C_WINDOWED]);
-    for (uint32_t spin = 0;; spin++) {
-        if (uniform(__hip_atomic_load(&a.ctrl[C_FINISHED], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= total) break;
-        if (help_one<VARIANT, SL, SS>(a, slot, L.pf)) { spin = 0; continue; }
-        if ((spin & 7) == 0 &&
-            uniform(__hip_atomic_load(&a.ctrl[C_STARTED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= windowed)
-            break;
-        if (spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[C_ERROR], 1u); break; }
-        __builtin_amdgcn_s_sleep(8);
+    uint32_t ticket = EMPTY_SLOT;  // queue slot this wave holds, not yet published
+    uint32_t cont = EMPTY_SLOT;    // child this wave runs next
+    bool taken_all = false;        // every queue slot has been handed out
+    for (uint32_t spin = 0;;) {
+        uint32_t entry = cont;
+        bool own = true;
+        cont = EMPTY_SLOT;
+        if (entry == EMPTY_SLOT) {
+            if (!taken_all && ticket == EMPTY_SLOT) {
+                ticket = grab(&a.ctrl[C_HEAD]);
+                if (ticket >= total) { taken_all = true; ticket = EMPTY_SLOT; }
+            }
+            if (ticket != EMPTY_SLOT) {
+                const uint32_t v = uniform(__hip_atomic_load(&a.queue[ticket], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                if (v == SKIP_SLOT) { ticket = EMPTY_SLOT; spin = 0; continue; }  // a child its parent's wave runs
+                if (v != EMPTY_SLOT) { entry = v; ticket = EMPTY_SLOT; }
+            } else if (uniform(__hip_atomic_load(&a.ctrl[C_FINISHED], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >= total) {
+                break;
+            }
+            if (entry == EMPTY_SLOT) {
+                entry = find_help(a, slot);
+                own = false;
+                if (entry == EMPTY_SLOT) {
+                    // Every slot taken and every windowed node started (C_STARTED reaches C_WINDOWED): no node
+                    // will register for help any more, and a wave that finds none open leaves instead of polling
+                    // beside the waves still searching (checked every 8th poll: ~6000 waves polling one word load
+                    // a single L2 channel).
+                    if (taken_all && (spin & 7) == 0 &&
+                        uniform(__hip_atomic_load(&a.ctrl[C_STARTED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= windowed)
+                        break;
+                    if (++spin >= SPIN_LIMIT) { atomicExch(&a.ctrl[C_ERROR], 1u); break; }
+                    if (taken_all) __builtin_amdgcn_s_sleep(8);
+                    else __builtin_amdgcn_s_sleep(4);
+                    continue;
+                }
+            }
+            spin = 0;
+        }
+        __threadfence();
+        // the same in every lane: keep the compiler's control flow scalar
+        entry = uniform(entry);
+        own = uniform(own ? 1u : 0u) != 0;
+        const uint32_t nid = entry & ~ENTRY_BUF, buf = entry >> 31;
+        const shgpu_node nd = a.nodes[nid];
+        const uint64_t t_start = task_clock();
+        if (VARIANT == SHGPU_SHOCKHASH2 && uniform(nd.kind) == SHGPU_NODE_LEAF) {
+            if constexpr (VARIANT == SHGPU_SHOCKHASH2) {
+                const uint32_t o = uniform(nd.offset), m = uniform(nd.m);
+                const uint64_t code = sh2_leaf_search_any<WIDE>(key_buf(a, buf) + o, m, ws, L.sh2, a.sh2_caps, a.flags, t_start);
+                if (buf) copy_keys(a.tmp + o, a.keys + o, m);
+                a.codes[nid] = code;
+                wave_fetch_add(&a.ctrl[C_FINISHED], 1u);
+            }
+            continue;
+        }
+        if (own) {
+            wave_fetch_add(&a.ctrl[C_STARTED], 1u);  // windowed nodes started
+            __hip_atomic_store(&a.help[slot], entry, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        cont = work_windows<VARIANT, SL, SS>(a, nid, buf, L.pf, true, t_start);
+        if (own) __hip_atomic_store(&a.help[slot], EMPTY_SLOT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else wave_fetch_add(&a.state[nid].helpers, 0xFFFFFFFFu);
     }
 }
 

@@ -1318,6 +1318,8 @@ static int run_batch(shgpu_engine *e, BuildBuffers &B, hipStream_t s, uint32_t g
             cap += nbk[r] * mx;
         }
         cap = std::max<uint64_t>(cap, B.plan_capacity);
+        // queue entries carry a node id below 2^31 - 2 (bit 31 names the key buffer, 0xFFFFFFFE and up are marks)
+        if (cap >= 0x7FFFFFFEull) return fail_all(fail(SHGPU_ERR_ARG, "forest too large for one build"));
         std::vector<PlanSet> sets(count);
         for (size_t r = 0; r < count; r++) sets[r] = PlanSet{base[r], (uint32_t)bbase[r], (uint32_t)nbk[r]};
         if ((rc = B.plan_sets.ensure(count * sizeof(PlanSet))) || (rc = B.plan_bucket.ensure(NB * sizeof(uint4))) ||
