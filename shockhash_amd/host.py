@@ -37,6 +37,7 @@ _SIGS = {
     "shh_flat_info": (None, [P, P]),
     "shh_flat_nbuckets": (ctypes.c_size_t, [P]),
     "shh_flat_codes": (None, [P, P]),
+    "shh_flat_phases": (None, [P, P]),
 }
 VARIANTS = {"sh1": 0, "simd": 1, "sh2": 2}
 ROTATION = 0x100  # ROTATION_FITTING = true instantiations (ShockHash / SIMDShockHash)
@@ -206,6 +207,14 @@ class FlatMPHF:
         load().shh_flat_info(self.h, v.ctypes.data_as(P))
         return {"nbuckets": int(v[0]), "layers": int(v[1]), "seed_fallbacks": int(v[2]), "bits": int(v[3]),
                 "constructor_ms": float(v[4])}
+
+    @property
+    def phases_ms(self):
+        """Construction phases: threshold layers and bumping, fallback function, placement of every key, device
+        base-case searches with orientation and retrieval."""
+        v = np.zeros(4, dtype=np.float64)
+        load().shh_flat_phases(self.h, v.ctypes.data_as(P))
+        return {"layers": float(v[0]), "fallback": float(v[1]), "placement": float(v[2]), "device": float(v[3])}
 
     @property
     def codes(self):

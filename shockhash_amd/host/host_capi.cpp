@@ -131,6 +131,7 @@ struct FlatHandle {
     virtual size_t bits() const = 0;
     virtual void info(uint64_t *out) const = 0;
     virtual const std::vector<uint64_t> &codes() const = 0;
+    virtual const double *phases() const = 0;
     double ctor_ms = 0;
 };
 template <size_t K>
@@ -143,6 +144,7 @@ struct FlatHandleT : FlatHandle {
         out[0] = f.bucketCount(); out[1] = f.layerCount(); out[2] = f.fallbackSeedCount(); out[3] = f.getBits();
     }
     const std::vector<uint64_t> &codes() const override { return f.baseCaseCodes(); }
+    const double *phases() const override { return f.phaseTimings(); }
 };
 
 }  // namespace
@@ -314,6 +316,12 @@ void shh_flat_info(void *h, double *out5) {
     ((FlatHandle *)h)->info(v);
     for (int i = 0; i < 4; i++) out5[i] = (double)v[i];
     out5[4] = ((FlatHandle *)h)->ctor_ms;
+}
+// construction phases in ms: threshold layers and bumping, fallback function, placement of every key, device
+// base-case searches + orientation + retrieval
+void shh_flat_phases(void *h, double *out4) {
+    const double *p = ((FlatHandle *)h)->phases();
+    for (int i = 0; i < 4; i++) out4[i] = p[i];
 }
 size_t shh_flat_nbuckets(void *h) { return ((FlatHandle *)h)->codes().size(); }
 void shh_flat_codes(void *h, uint64_t *out) {

@@ -6,6 +6,7 @@
 // the retrieval structure run on the GPU (one batched leaf search for all buckets); the bucketing runs on the host.
 #pragma once
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -181,6 +182,9 @@ class ShockHash2Flat {
     std::vector<uint64_t> retrievalKeys;  // retrieval input, kept when gpu::keep_retrieval_input
     std::vector<uint8_t> retrievalBits;
     std::vector<uint64_t> bucketCodes;  // base-case seed per bucket, for verification
+    // construction phases (ms): threshold layers and bumping, fallback function, placement of every key (free
+    // positions, bucket of each key), base-case searches + orientation + retrieval on the device
+    double phaseMs[4] = {0, 0, 0, 0};
 
     void setThreshold(size_t b, size_t v) {
         uint64_t x = thresholdsAndSeeds.get(b);
@@ -227,6 +231,9 @@ class ShockHash2Flat {
     size_t fallbackQuery(uint64_t mhc) const { return fallbackPhf(std::to_string(mhc)); }
 
     void construct(std::vector<uint64_t> &&mhcs) {
+        using clk = std::chrono::steady_clock;
+        const auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        auto t0 = clk::now();
         N = mhcs.size();
         nbuckets = N / k;
         maxSeed = 1ull << seedBits();
@@ -286,6 +293,8 @@ class ShockHash2Flat {
             hashes.swap(bumped);
         }
         // fallback PHF over the bumped keys' decimal strings (ShockHash2Flat.h:124-129)
+        auto t1 = clk::now();
+        phaseMs[0] = ms(t0, t1);
         if (!hashes.empty()) {
             std::vector<std::string> content;
             content.reserve(hashes.size());
@@ -293,6 +302,8 @@ class ShockHash2Flat {
             fallbackPhf = ShockHash2<k>(std::span<const std::string>(content.data(), content.size()), 2000, 1);
             haveFallback = true;
         }
+        auto t2 = clk::now();
+        phaseMs[1] = ms(t1, t2);
         const size_t additional = hashes.size() - freePositionsList.size(), handled = layerBases.back();
         {
             size_t i = 0;
@@ -318,9 +329,12 @@ class ShockHash2Flat {
         std::vector<uint32_t> roffsets;
         std::vector<uint8_t> rseeds;
         std::vector<uint64_t> rwords;
+        auto t3 = clk::now();
+        phaseMs[2] = ms(t2, t3);
         gpu::Engine::instance().sh2_leaves_retrieval(bucketKeys, (uint32_t)k, bucketCodes, gpu::keep_retrieval_input,
                                                      retrievalKeys, retrievalBits, roffsets, rseeds, rwords);
         ribbon = RibbonRetrieval(std::move(roffsets), std::move(rseeds), std::move(rwords));
+        phaseMs[3] = ms(t3, clk::now());
         for (size_t b = 0; b < nbuckets; b++) {
             size_t seed = bucketCodes[b];
             if (seed >= maxSeed || seed == SEED_FALLBACK_INDICATOR) {
@@ -390,6 +404,8 @@ class ShockHash2Flat {
 
     // Verification accessors (not in the reference interface).
     const std::vector<uint64_t> &baseCaseCodes() const { return bucketCodes; }
+    // construction phases in ms: threshold layers, fallback function, placement, device searches and retrieval
+    const double *phaseTimings() const { return phaseMs; }
     const std::vector<uint64_t> &ribbonKeys() const { return retrievalKeys; }
     const std::vector<uint8_t> &ribbonBits() const { return retrievalBits; }
     size_t bucketCount() const { return nbuckets; }
