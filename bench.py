@@ -11,10 +11,10 @@ device search, orientation and retrieval, host encoding; every key queried, desc
 with the oracle's), and a serving mix of 2048 Poisson-arriving requests on the continuous-batching
 server (p50/p99 latency).
 
-Two more workloads print lines of their own (not the headline): --workload strings times the ShockHash<30>
+Three more workloads print lines of their own (not the headline): --workload strings times the ShockHash<30>
 class constructor over 1,000,000 random strings (device fingerprinting, build, orientation, retrieval, host
 encoding), --workload leaves times 200,000 isolated ShockHash leaves of 40 keys searched from seed 0 (hash
-evaluations per second). --leaf / --variant select other leaf sizes and classes for the build workload.
+evaluations per second), --workload flat times the ShockHash2Flat<k> constructor over 10,000,000 key hashes. --leaf / --variant select other leaf sizes and classes for the build workload.
 
 Prints one JSON line (see the keys below). One GPU only: --gpus N > 1 exits with a message.
 """
@@ -215,6 +215,82 @@ def strings_workload(args, eng):
                        "leaf_size": 30, "bucket_size": 2000, "variant": "ShockHash"}}
 
 
+def flat_workload(args, eng):
+    """ShockHash2Flat<k> (k = --leaf, 64 by default) over n (--n, 10,000,000 by default) uniform 64-bit key hashes (the
+    .second of SplitMix64 fingerprints, seed 1), built by the class template from host memory: the threshold layers,
+    every key's placement and every bucket's base case (ShockHash2's leaf search, one batched launch), orientation and
+    retrieval on the device; thresholds, the fallback function over the bumped keys and their placement on the host.
+    One step = one construction (its own wall clock). All n queries must form a bijection onto [0, n) (n a multiple of
+    k); on a sample (--cpu-sample, 1,000,000 by default) the function equals the oracle's restatement key for key, and
+    the oracle's construction of that sample is the CPU baseline. The roofline is the batched base-case search's."""
+    from shockhash_amd import host, workload
+    from shockhash_amd.roofline import INT32_PEAK_TOPS, KIND_LEAF, SH2_HALF_SEED_OPS, evaluations
+    k, n = args.leaf, args.n
+    mh = workload.fingerprints(n, seed=1)[:, 1].copy()
+    ms, search_ms, f = [], [], None
+    for i in range(args.warmup + args.steps):
+        if f is not None:
+            f.close()
+        f = host.FlatMPHF(k, mh)
+        ph = f.phases_ms
+        if i >= args.warmup:
+            ms.append(f.info["constructor_ms"])
+            search_ms.append(ph["search"])
+        log(f"{'warmup' if i < args.warmup else 'step'} {i}: constructor {f.info['constructor_ms']:.1f} ms, phases "
+            + ", ".join(f"{a} {b:.1f}" for a, b in ph.items()))
+    q = f(mh)
+    bij = bool(len(np.unique(q)) == n and int(q.max()) == n - 1) if n % k == 0 else bool(len(np.unique(q)) == n)
+    codes = f.codes
+    nodes = np.zeros(len(codes), dtype=[("m", "<u2"), ("kind", "u1")])
+    nodes["m"], nodes["kind"] = k, KIND_LEAF
+    evals = float(evaluations(nodes, codes, "sh2").sum())
+    s_search = float(np.median(search_ms)) / 1e3
+    achieved = evals * SH2_HALF_SEED_OPS / s_search / 1e12
+    info = f.info
+    f.close()
+    ms_step = float(np.median(ms))
+    out = {"metric": f"ShockHash2Flat<{k}> constructor keys/s", "value": n / (ms_step / 1e3), "unit": "keys/s",
+           "ms_per_step": ms_step, "phases_ms": ph, "base_case_search_ms": 1e3 * s_search,
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": INT32_PEAK_TOPS, "unit": "TOP/s",
+                        "frac": achieved / INT32_PEAK_TOPS, "traffic": None, "ops_per_eval": SH2_HALF_SEED_OPS,
+                        "hash_evals": evals, "evals_per_s": evals / s_search,
+                        "kernel": "k_sh2_batch (every bucket's base case, one launch)"},
+           "queries_bijective": bij, "bits_per_key": info["bits"] / n, "buckets": info["nbuckets"],
+           "config": {"workload": f"ShockHash2Flat<{k}> class constructor over {n} uniform 64-bit key hashes (SplitMix64 "
+                                  "fingerprints' .second, seed 1) in host memory", "keys": n, "k": k}}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        lib_path, build_desc = native_oracle()
+        O.use_library(lib_path)
+        model, share = host_cpu()
+        threads = args.cpu_threads or share
+        ns = min(n, args.cpu_sample or 1_000_000)
+        ms_ = workload.fingerprints(ns, seed=7)[:, 1].copy()
+        times, of = [], None
+        for _ in range(max(1, args.cpu_reps)):
+            of = None
+            t0 = time.perf_counter()
+            of = O.OracleFlat(k, ms_, threads=threads)
+            times.append(time.perf_counter() - t0)
+        fs = host.FlatMPHF(k, ms_)
+        equal = bool(np.array_equal(fs(ms_), of.query(ms_)))
+        fs.close()
+        dt = float(np.median(times))
+        out["bit_exact_sample"] = equal
+        out["cpu_baseline"] = {"value": ns / dt, "unit": "keys/s", "cores": threads, "kind": "port",
+                               "sample": f"{ns} key hashes (same generator, seed 7), ShockHash2Flat<{k}>; the oracle's C++ "
+                                         f"restatement of the reference construction ({build_desc}), {threads} threads "
+                                         f"(this process's CPU share) on {model}; median of {len(times)} runs: "
+                                         f"{', '.join(f'{t:.2f}' for t in times)} s; its values equal the GPU "
+                                         f"construction's on every key: {equal}",
+                               "cpu_model": model}
+        log(f"flat CPU baseline: {ns / dt / 1e6:.2f} M keys/s on {ns} keys, values equal {equal}")
+    log(f"flat: {ms_step:.1f} ms per construction ({n / ms_step / 1e3:.0f} M keys/s), base-case search {1e3 * s_search:.2f} ms, "
+        f"roofline {out['roofline']['frac']:.3f}, bijective {bij}")
+    return out
+
+
 def leaves_workload(args, eng):
     """200,000 isolated ShockHash leaves of 40 keys (the .second of uniform fingerprints, seed 11), each searched
     from seed 0 for its minimal seed (benchmark/bijections/ShockHash1.h:13-35) by the batched leaf search with the
@@ -261,10 +337,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["build", "strings", "leaves"], default="build",
+    ap.add_argument("--workload", choices=["build", "strings", "leaves", "flat"], default="build",
                     help="build: the headline persistent build (default); strings: the ShockHash<30> constructor over "
-                         "1M strings; leaves: 200k isolated leaves of 40 keys")
-    ap.add_argument("--n", type=int, default=None, help="keys (build, strings) or leaves (leaves)")
+                         "1M strings; leaves: 200k isolated leaves of 40 keys; flat: the ShockHash2Flat<k> constructor "
+                         "(k = --leaf) over 10M key hashes")
+    ap.add_argument("--n", type=int, default=None, help="keys (build, strings, flat) or leaves (leaves)")
     ap.add_argument("--variant", choices=sorted(VARIANT_NAMES), default="sh2",
                     help="class variant of the build (the headline workload is sh2 with leaf 64)")
     ap.add_argument("--leaf", type=int, default=64, help="leaf size")
@@ -302,8 +379,9 @@ def main():
 
     eng = Engine(0)
     if args.workload != "build":
-        args.n = args.n or (1_000_000 if args.workload == "strings" else 200_000)
-        out = strings_workload(args, eng) if args.workload == "strings" else leaves_workload(args, eng)
+        args.n = args.n or {"strings": 1_000_000, "leaves": 200_000, "flat": 10_000_000}[args.workload]
+        run = {"strings": strings_workload, "leaves": leaves_workload, "flat": flat_workload}[args.workload]
+        out = run(args, eng)
         out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                     "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic"})
         print(json.dumps(out))

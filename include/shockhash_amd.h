@@ -252,6 +252,28 @@ uint64_t shgpu_ribbon_max_words(uint64_t n);
 int shgpu_ribbon_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d_bits, uint64_t n, uint32_t *h_offsets,
                        uint8_t *h_seeds, uint64_t *h_words, uint64_t words_capacity, uint64_t *words_used);
 
+/* ShockHash2Flat's bucketing on the device, for n (< 2^31) keys' 64-bit hashes d_mhcs (device memory) and buckets of
+ * k keys; mapping holds the `range` (2..64) threshold values of THRESHOLD_MAPPING, the last 2^32 - 1. It replaces the
+ * host loops of ShockHash2Flat::ShockHash2Flat (include/ShockHash2Flat.h:70-124: the threshold layers with
+ * bumping, flushBucket :204-233; :148-157 with evaluateKPerfect :280-302: every key's bucket).
+ * shgpu_flat_layers: per bucket of the layers (the first layer_bases[*layers] of n / k buckets) its threshold index
+ * (thresholds) and its free positions, k minus the keys it keeps (free_counts); the layer bases (*layers + 1 of at
+ * most 3 entries); the keys bumped from every layer, as their last layer's hash (bumped_mhcs) and input index
+ * (bumped_index), *n_bumped of them (capacity n each), in the reference's order. All outputs are host memory. The
+ * caller places the bumped keys (the reference's fallback function over their decimal strings and free positions).
+ * SHGPU_ERR_ARG when a bucket cannot take a threshold (the reference throws). */
+int shgpu_flat_layers(shgpu_engine *e, const uint64_t *d_mhcs, uint64_t n, uint32_t k, const uint32_t *mapping,
+                      uint32_t range, uint8_t *thresholds, uint32_t *free_counts, uint64_t *layer_bases, uint32_t *layers,
+                      uint64_t *bumped_mhcs, uint64_t *bumped_index, uint64_t *n_bumped);
+/* shgpu_flat_bucket_keys: every key's bucket, the first layer whose stored threshold admits it, else
+ * fb_bucket[j] for the key of input index fb_index[j] (values from (n / k) * k on, below (n / k) * k + n, name the
+ * last, partial bucket, as evaluateKPerfect returns them); each of the n / k buckets must receive exactly k keys (SHGPU_ERR_ARG
+ * otherwise), and bucket b's keys, in input order, go to d_bucket_keys[b * k .. (b + 1) * k) (device memory).
+ * layer_bases, thresholds, fb_index and fb_bucket are host memory as shgpu_flat_layers returns them. */
+int shgpu_flat_bucket_keys(shgpu_engine *e, const uint64_t *d_mhcs, uint64_t n, uint32_t k, const uint32_t *mapping,
+                           uint32_t range, const uint64_t *layer_bases, uint32_t layers, const uint8_t *thresholds,
+                           const uint64_t *fb_index, const uint64_t *fb_bucket, uint64_t n_fb, uint64_t *d_bucket_keys);
+
 int shgpu_plan_forest(int variant, uint32_t leaf_size, const uint64_t *size_acc, uint64_t nbuckets, shgpu_node *out,
                       uint64_t capacity, uint64_t *count);
 /* Milliseconds spent in the build's phases: [0] sort + bucketing, [1] tree setup, [2] seed search

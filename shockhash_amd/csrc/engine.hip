@@ -25,6 +25,7 @@
 #include "search_sh2.h"
 #include "orientation.h"
 #include "ribbon.h"
+#include "flat.h"
 #include "schedule.h"
 #include "sort.h"
 #include "strings.h"
@@ -191,6 +192,7 @@ struct shgpu_engine {
     DevBuf counter, sh2_scratch, staging, tmp;  // batched per-node entry points
     BuildBuffers main;                          // shgpu_build_run
     DevBuf retrieval;                           // shgpu_ribbon_build working memory (~42 B per key)
+    DevBuf flat;                                // ShockHash2Flat bucketing working memory (~70 B per key)
 };
 
 struct shgpu_build {
@@ -1623,6 +1625,32 @@ int shgpu_ribbon_build(shgpu_engine *e, const uint64_t *d_keys, const uint8_t *d
     std::string err;
     const int rc = shgpu_ribbon_build_impl(e->stream, d_keys, d_bits, n, h_offsets, h_seeds, h_words, words_capacity,
                                            words_used, err, &e->retrieval.p, &e->retrieval.bytes);
+    return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
+}
+
+int shgpu_flat_layers(shgpu_engine *e, const uint64_t *d_mhcs, uint64_t n, uint32_t k, const uint32_t *mapping,
+                      uint32_t range, uint8_t *thresholds, uint32_t *free_counts, uint64_t *layer_bases, uint32_t *layers,
+                      uint64_t *bumped_mhcs, uint64_t *bumped_index, uint64_t *n_bumped) {
+    if (!e || !mapping || !layer_bases || !layers || !n_bumped || (n && (!d_mhcs || !bumped_mhcs || !bumped_index)) ||
+        (n >= k && (!thresholds || !free_counts)))
+        return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(SHGPU_ERR_HIP, "hipSetDevice failed");
+    std::string err;
+    const int rc = shgpu_flat_layers_impl(e->stream, d_mhcs, n, k, mapping, range, thresholds, free_counts, layer_bases,
+                                          layers, bumped_mhcs, bumped_index, n_bumped, err, &e->flat.p, &e->flat.bytes);
+    return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
+}
+
+int shgpu_flat_bucket_keys(shgpu_engine *e, const uint64_t *d_mhcs, uint64_t n, uint32_t k, const uint32_t *mapping,
+                           uint32_t range, const uint64_t *layer_bases, uint32_t layers, const uint8_t *thresholds,
+                           const uint64_t *fb_index, const uint64_t *fb_bucket, uint64_t n_fb, uint64_t *d_bucket_keys) {
+    if (!e || !mapping || !layer_bases || (n && !d_mhcs) || (n_fb && (!fb_index || !fb_bucket)) ||
+        (n >= k && (!thresholds || !d_bucket_keys)))
+        return fail(SHGPU_ERR_ARG, "NULL argument");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(SHGPU_ERR_HIP, "hipSetDevice failed");
+    std::string err;
+    const int rc = shgpu_flat_bucket_keys_impl(e->stream, d_mhcs, n, k, mapping, range, layer_bases, layers, thresholds,
+                                               fb_index, fb_bucket, n_fb, d_bucket_keys, err, &e->flat.p, &e->flat.bytes);
     return rc == SHGPU_OK ? SHGPU_OK : fail(rc, err);
 }
 

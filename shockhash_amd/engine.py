@@ -113,6 +113,9 @@ _SIGS = {
     "shgpu_leaf_orientation": (ctypes.c_int, [P, ctypes.c_int, P, P, ctypes.c_uint64, P, P]),
     "shgpu_ribbon_shards": (ctypes.c_uint64, [ctypes.c_uint64]),
     "shgpu_ribbon_max_words": (ctypes.c_uint64, [ctypes.c_uint64]),
+    "shgpu_flat_layers": (ctypes.c_int, [P, P, ctypes.c_uint64, ctypes.c_uint32, P, ctypes.c_uint32, P, P, P, P, P, P, P]),
+    "shgpu_flat_bucket_keys": (ctypes.c_int, [P, P, ctypes.c_uint64, ctypes.c_uint32, P, ctypes.c_uint32, P, ctypes.c_uint32,
+                                              P, P, P, ctypes.c_uint64, P]),
     "shgpu_ribbon_build": (ctypes.c_int, [P, P, P, ctypes.c_uint64, P, P, P, ctypes.c_uint64,
                                           ctypes.POINTER(ctypes.c_uint64)]),
     "shgpu_build_free": (None, [P]),

@@ -211,10 +211,11 @@ class FlatMPHF:
     @property
     def phases_ms(self):
         """Construction phases: threshold layers and bumping, fallback function, placement of every key, device
-        base-case searches with orientation and retrieval."""
-        v = np.zeros(4, dtype=np.float64)
+        base-case searches with orientation and retrieval, and of that the batched base-case search alone."""
+        v = np.zeros(5, dtype=np.float64)
         load().shh_flat_phases(self.h, v.ctypes.data_as(P))
-        return {"layers": float(v[0]), "fallback": float(v[1]), "placement": float(v[2]), "device": float(v[3])}
+        return {"layers": float(v[0]), "fallback": float(v[1]), "placement": float(v[2]), "device": float(v[3]),
+                "search": float(v[4])}
 
     @property
     def codes(self):

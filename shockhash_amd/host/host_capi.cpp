@@ -318,10 +318,10 @@ void shh_flat_info(void *h, double *out5) {
     out5[4] = ((FlatHandle *)h)->ctor_ms;
 }
 // construction phases in ms: threshold layers and bumping, fallback function, placement of every key, device
-// base-case searches + orientation + retrieval
-void shh_flat_phases(void *h, double *out4) {
+// base-case searches + orientation + retrieval, and the batched base-case search alone
+void shh_flat_phases(void *h, double *out5) {
     const double *p = ((FlatHandle *)h)->phases();
-    for (int i = 0; i < 4; i++) out4[i] = p[i];
+    for (int i = 0; i < 5; i++) out5[i] = p[i];
 }
 size_t shh_flat_nbuckets(void *h) { return ((FlatHandle *)h)->codes().size(); }
 void shh_flat_codes(void *h, uint64_t *out) {

@@ -6,6 +6,9 @@
 // the retrieval structure run on the GPU (one batched leaf search for all buckets); the bucketing runs on the host.
 #pragma once
 #include <array>
+#include <algorithm>
+#include <exception>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -25,7 +28,7 @@ namespace shockhash {
 namespace flat_detail {
 
 // MurmurHash64A (Austin Appleby, public domain). The reference hashes string keys with the un-vendored
-// bytehamster::util::MurmurHash64 (ShockHash2Flat.h:70-75, 233-235); this is the published 64-bit MurmurHash2 with
+// bytehamster::util::MurmurHash64 (ShockHash2Flat.h:75, 258-260); this is the published 64-bit MurmurHash2 with
 // seed 0, so string keys are placed as by that function only if it is this one.
 inline uint64_t murmurHash64A(const void *key, size_t len, uint64_t seed = 0) {
     const uint64_t m = 0xc6a4a7935bd1e995ull;
@@ -165,10 +168,6 @@ class ShockHash2Flat {
     static size_t seedBits() { return (size_t)std::ceil(0.442 * k - 0.2); }  // SEED_BITS, evaluated as the reference
     static constexpr size_t SEED_FALLBACK_INDICATOR = 0;
 
-    struct KeyInfo {  // ShockHash2FlatBase.h
-        uint64_t mhc;
-        uint32_t bucket, threshold;
-    };
 
     flat_detail::PackedInts thresholdsAndSeeds;
     std::map<size_t, size_t> seedsFallback;
@@ -183,8 +182,9 @@ class ShockHash2Flat {
     std::vector<uint8_t> retrievalBits;
     std::vector<uint64_t> bucketCodes;  // base-case seed per bucket, for verification
     // construction phases (ms): threshold layers and bumping, fallback function, placement of every key (free
-    // positions, bucket of each key), base-case searches + orientation + retrieval on the device
-    double phaseMs[4] = {0, 0, 0, 0};
+    // positions, bucket of each key), base-case searches + orientation + retrieval on the device, and of that the
+    // batched base-case search alone
+    double phaseMs[5] = {0, 0, 0, 0, 0};
 
     void setThreshold(size_t b, size_t v) {
         uint64_t x = thresholdsAndSeeds.get(b);
@@ -200,37 +200,37 @@ class ShockHash2Flat {
         const uint64_t x = thresholdsAndSeeds.get(b);
         return {x & (THRESHOLD_RANGE - 1), x >> THRESHOLD_BITS};
     }
-    static uint32_t compactThreshold(uint32_t t) {
-        for (size_t i = 0; i < THRESHOLD_RANGE; i++)
-            if (t <= THRESHOLD_MAPPING[i]) return (uint32_t)i;
-        return THRESHOLD_MAPPING.back();
-    }
-
-    // flushBucket (ShockHash2Flat.h:201-228)
-    void flushBucket(size_t layerBase, size_t bucketStart, size_t i, size_t bucketIdx, const std::vector<KeyInfo> &hashes,
-                     std::vector<KeyInfo> &bumped) {
-        const size_t size = i - bucketStart, b = layerBase + bucketIdx;
-        if (size <= k) {
-            setThreshold(b, THRESHOLD_RANGE - 1);
-            for (size_t q = size; q < k; q++) freePositionsList.push_back(b);
+    // body(i) for i in [0, n) on up to 16 host threads (the queries it runs are read-only); rethrows the first
+    // exception a thread caught.
+    template <class F>
+    static void parallelFor(size_t n, F body) {
+        const size_t nt = std::max<size_t>(1, std::min<size_t>({16, (size_t)std::thread::hardware_concurrency(), n / 4096}));
+        if (nt <= 1) {
+            for (size_t i = 0; i < n; i++) body(i);
             return;
         }
-        const uint32_t last = compactThreshold(hashes[bucketStart + k - 1].threshold);
-        const uint32_t firstBumped = compactThreshold(hashes[bucketStart + k].threshold);
-        if (firstBumped == last && last == 0) throw std::runtime_error("ShockHash2Flat: threshold index below 0");
-        const uint32_t t = firstBumped == last ? last - 1 : last;
-        setThreshold(b, t);
-        const uint32_t uncompressed = THRESHOLD_MAPPING[t];
-        for (size_t l = 0; l < size; l++)
-            if (hashes[bucketStart + l].threshold > uncompressed) {
-                bumped.push_back(hashes[bucketStart + l]);
-                if (l < k) freePositionsList.push_back(b);
-            }
+        std::vector<std::thread> threads;
+        std::vector<std::exception_ptr> errors(nt);
+        for (size_t t = 0; t < nt; t++)
+            threads.emplace_back([&, t] {
+                try {
+                    for (size_t i = n * t / nt; i < n * (t + 1) / nt; i++) body(i);
+                } catch (...) {
+                    errors[t] = std::current_exception();
+                }
+            });
+        for (auto &th : threads) th.join();
+        for (auto &e : errors)
+            if (e) std::rethrow_exception(e);
     }
 
     size_t fallbackQuery(uint64_t mhc) const { return fallbackPhf(std::to_string(mhc)); }
 
-    void construct(std::vector<uint64_t> &&mhcs) {
+    // The reference's construction (ShockHash2Flat.h:66-171) with its per-key loops on the device: the threshold
+    // layers (shgpu_flat_layers), the placement of every key and the bucket key lists (shgpu_flat_bucket_keys), the
+    // base-case searches, orientation and retrieval. The host keeps what is per bucket or per bumped key: thresholds
+    // and free positions, the fallback function over the bumped keys' decimal strings, and their placement.
+    void construct(std::span<const uint64_t> mhcs) {
         using clk = std::chrono::steady_clock;
         const auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
         auto t0 = clk::now();
@@ -238,73 +238,31 @@ class ShockHash2Flat {
         nbuckets = N / k;
         maxSeed = 1ull << seedBits();
         const size_t keysInEndBucket = N - nbuckets * k;
-        size_t bucketsThisLayer = (size_t)std::ceil(OVERLOAD_FACTOR * nbuckets);
-        std::vector<KeyInfo> hashes(N);
-        for (size_t i = 0; i < N; i++)
-            hashes[i] = {mhcs[i], fastrange32((uint32_t)mhcs[i], (uint32_t)bucketsThisLayer), (uint32_t)(mhcs[i] >> 32)};
-        layerBases.assign(1, 0);
         thresholdsAndSeeds.reset(nbuckets, THRESHOLD_BITS + seedBits());
-        if (bucketsThisLayer == 0) layers = 0;
-        std::vector<uint64_t> sortKey;
-        for (size_t layer = 0; layer < layers; layer++) {
-            const size_t layerBase = layerBases.back();
-            if (layer != 0) {
-                bucketsThisLayer = (size_t)(OVERLOAD_FACTOR * (double)(hashes.size() / k));
-                bucketsThisLayer = std::min(bucketsThisLayer, nbuckets - layerBase);
-                if (bucketsThisLayer == 0) {
-                    layers = 1;
-                    break;
-                }
-                for (auto &h : hashes) {  // rehash
-                    h.mhc = remix(h.mhc);
-                    h.bucket = fastrange32((uint32_t)h.mhc, (uint32_t)bucketsThisLayer);
-                    h.threshold = (uint32_t)(h.mhc >> 32);
-                }
-            }
-            layerBases.push_back(layerBase + bucketsThisLayer);
-            // sort_keyInfo (src/Sorter.cpp:10-12): by bucket, then threshold; counting sort by bucket, then each
-            // bucket's few keys by threshold
-            {
-                std::vector<size_t> start(bucketsThisLayer + 1, 0);
-                for (const auto &h : hashes) start[h.bucket + 1]++;
-                for (size_t b = 0; b < bucketsThisLayer; b++) start[b + 1] += start[b];
-                std::vector<KeyInfo> sorted(hashes.size());
-                std::vector<size_t> pos(start.begin(), start.end() - 1);
-                for (const auto &h : hashes) sorted[pos[h.bucket]++] = h;
-                for (size_t b = 0; b < bucketsThisLayer; b++)
-                    std::stable_sort(sorted.begin() + start[b], sorted.begin() + start[b + 1],
-                                     [](const KeyInfo &a, const KeyInfo &c) { return a.threshold < c.threshold; });
-                hashes.swap(sorted);
-            }
-            std::vector<KeyInfo> bumped;
-            size_t bucketStart = 0, previousBucket = 0;
-            for (size_t i = 0; i < hashes.size(); i++) {
-                while (hashes[i].bucket != previousBucket) {
-                    flushBucket(layerBase, bucketStart, i, previousBucket, hashes, bumped);
-                    previousBucket++;
-                    bucketStart = i;
-                }
-            }
-            while (previousBucket < bucketsThisLayer) {
-                flushBucket(layerBase, bucketStart, hashes.size(), previousBucket, hashes, bumped);
-                previousBucket++;
-                bucketStart = hashes.size();
-            }
-            hashes.swap(bumped);
+        gpu::Engine &eng = gpu::Engine::instance();
+        std::vector<uint8_t> thresholds;
+        std::vector<uint32_t> freeCounts;
+        std::vector<uint64_t> bases, bumpedMhc, bumpedIndex;
+        eng.flat_layers(mhcs.data(), N, (uint32_t)k, THRESHOLD_MAPPING.data(), (uint32_t)THRESHOLD_RANGE, thresholds, freeCounts,
+                        bases, bumpedMhc, bumpedIndex);
+        layers = bases.size() - 1;
+        layerBases.assign(bases.begin(), bases.end());
+        for (size_t b = 0; b < layerBases.back(); b++) {  // flushBucket's thresholds and free positions, in bucket order
+            setThreshold(b, thresholds[b]);
+            for (uint32_t q = 0; q < freeCounts[b]; q++) freePositionsList.push_back(b);
         }
-        // fallback PHF over the bumped keys' decimal strings (ShockHash2Flat.h:124-129)
         auto t1 = clk::now();
         phaseMs[0] = ms(t0, t1);
-        if (!hashes.empty()) {
+        if (!bumpedMhc.empty()) {
             std::vector<std::string> content;
-            content.reserve(hashes.size());
-            for (const auto &h : hashes) content.push_back(std::to_string(h.mhc));
+            content.reserve(bumpedMhc.size());
+            for (uint64_t h : bumpedMhc) content.push_back(std::to_string(h));
             fallbackPhf = ShockHash2<k>(std::span<const std::string>(content.data(), content.size()), 2000, 1);
             haveFallback = true;
         }
         auto t2 = clk::now();
         phaseMs[1] = ms(t1, t2);
-        const size_t additional = hashes.size() - freePositionsList.size(), handled = layerBases.back();
+        const size_t additional = bumpedMhc.size() - freePositionsList.size(), handled = layerBases.back();
         {
             size_t i = 0;
             for (; i < additional - keysInEndBucket; i++) freePositionsList.push_back(handled + i / k);
@@ -312,27 +270,20 @@ class ShockHash2Flat {
         }
         freePositions.build(freePositionsList);
         std::vector<size_t>().swap(freePositionsList);
-        // bucket contents in input order (ShockHash2Flat.h:148-158)
-        std::vector<size_t> bucketOf(N);
-        std::vector<size_t> fill(nbuckets + 1, 0);
-        for (size_t i = 0; i < N; i++) {
-            bucketOf[i] = evaluateKPerfect(mhcs[i]).first;
-            if (bucketOf[i] < nbuckets) fill[bucketOf[i] + 1]++;
-        }
-        for (size_t b = 0; b < nbuckets; b++)
-            if (fill[b + 1] != k) throw std::runtime_error("ShockHash2Flat: a bucket does not hold k keys");
-        for (size_t b = 0; b < nbuckets; b++) fill[b + 1] += fill[b];
-        std::vector<uint64_t> bucketKeys(nbuckets * k);
-        for (size_t i = 0; i < N; i++)
-            if (bucketOf[i] < nbuckets) bucketKeys[fill[bucketOf[i]]++] = mhcs[i];
-        // base case of every bucket, its orientation and the retrieval structure, on the device
+        // the bumped keys' buckets (evaluateKPerfect's fallback branch, ShockHash2Flat.h:293-301)
+        std::vector<uint64_t> fbBucket(bumpedMhc.size());
+        parallelFor(bumpedMhc.size(), [&](size_t j) {
+            const size_t bucket = freePositions.at(fallbackQuery(bumpedMhc[j]));
+            fbBucket[j] = bucket >= nbuckets ? bucket - nbuckets + k * nbuckets : bucket;
+        });
         std::vector<uint32_t> roffsets;
         std::vector<uint8_t> rseeds;
         std::vector<uint64_t> rwords;
         auto t3 = clk::now();
         phaseMs[2] = ms(t2, t3);
-        gpu::Engine::instance().sh2_leaves_retrieval(bucketKeys, (uint32_t)k, bucketCodes, gpu::keep_retrieval_input,
-                                                     retrievalKeys, retrievalBits, roffsets, rseeds, rwords);
+        eng.flat_leaves_retrieval(N, (uint32_t)k, THRESHOLD_MAPPING.data(), (uint32_t)THRESHOLD_RANGE, layerBases, thresholds,
+                                  bumpedIndex, fbBucket, bucketCodes, gpu::keep_retrieval_input, retrievalKeys, retrievalBits,
+                                  roffsets, rseeds, rwords, &phaseMs[4]);
         ribbon = RibbonRetrieval(std::move(roffsets), std::move(rseeds), std::move(rwords));
         phaseMs[3] = ms(t3, clk::now());
         for (size_t b = 0; b < nbuckets; b++) {
@@ -354,14 +305,14 @@ class ShockHash2Flat {
     explicit ShockHash2Flat(std::span<const std::string> keys) {
         std::vector<uint64_t> mhcs(keys.size());
         for (size_t i = 0; i < keys.size(); i++) mhcs[i] = flat_detail::murmurHash64A(keys[i].data(), keys[i].size());
-        construct(std::move(mhcs));
+        construct(mhcs);
     }
     // Keys given by their 64-bit hashes (not in the reference interface, which hashes string keys itself).
-    explicit ShockHash2Flat(std::span<const uint64_t> mhcs) { construct(std::vector<uint64_t>(mhcs.begin(), mhcs.end())); }
+    explicit ShockHash2Flat(std::span<const uint64_t> mhcs) { construct(mhcs); }
 
     size_t operator()(const std::string &key) const { return operator()(flat_detail::murmurHash64A(key.data(), key.size())); }
     size_t operator()(const hash128_t &hash) const { return operator()(hash.second); }
-    // ShockHash2Flat.h:254-267
+    // ShockHash2Flat.h:258-278
     size_t operator()(uint64_t hash) const {
         auto [bucket, seed] = evaluateKPerfect(hash);
         if (bucket >= nbuckets) return bucket;  // N that are not multiples of k
@@ -370,7 +321,7 @@ class ShockHash2Flat {
         return bucket * k + sh2Cell((uint32_t)k, seed, hash, retrieved);
     }
 
-    // (bucket, stored seed) of a key (ShockHash2Flat.h:270-299).
+    // (bucket, stored seed) of a key (ShockHash2Flat.h:280-302).
     std::pair<size_t, size_t> evaluateKPerfect(uint64_t mhc) const {
         for (size_t layer = 0; layer < layers; layer++) {
             if (layer != 0) mhc = remix(mhc);
@@ -404,7 +355,8 @@ class ShockHash2Flat {
 
     // Verification accessors (not in the reference interface).
     const std::vector<uint64_t> &baseCaseCodes() const { return bucketCodes; }
-    // construction phases in ms: threshold layers, fallback function, placement, device searches and retrieval
+    // construction phases in ms: threshold layers, fallback function, placement, device searches and retrieval, and
+    // the batched base-case search alone
     const double *phaseTimings() const { return phaseMs; }
     const std::vector<uint64_t> &ribbonKeys() const { return retrievalKeys; }
     const std::vector<uint8_t> &ribbonBits() const { return retrievalBits; }

@@ -54,6 +54,7 @@ class Engine {
     };
     Buf tasks_buf, out_keys_buf, out_bits_buf;
     Buf leaf_keys_buf, leaf_task_buf, leaf_code_buf;  // one ShockHash2 leaf at a time (sh2LeafCode)
+    Buf flat_mhc_buf;                                 // ShockHash2Flat: the keys' hashes while a construction runs
     int ensure(Buf &b, size_t bytes) {
         if (b.bytes >= bytes) return SHGPU_OK;
         if (b.p) shgpu_device_free(e, b.p);
@@ -67,7 +68,7 @@ class Engine {
   public:
     explicit Engine(int device = 0) { check(shgpu_engine_create(device, &e), "shgpu_engine_create"); }
     ~Engine() {
-        for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf, &leaf_keys_buf, &leaf_task_buf, &leaf_code_buf})
+        for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf, &leaf_keys_buf, &leaf_task_buf, &leaf_code_buf, &flat_mhc_buf})
             if (b->p) shgpu_device_free(e, b->p);
         shgpu_engine_destroy(e);
     }
@@ -194,6 +195,51 @@ class Engine {
         retrieval_from_device(variant, leaf_keys, tasks, total, keep, input_keys, input_bits, offsets, seeds, words, ms2);
     }
 
+    // ShockHash2Flat's threshold layers (shgpu_flat_layers) over the hashes mhcs, which stay on the device for
+    // flat_leaves_retrieval. thresholds and free_counts get nbuckets = n / k entries, bumped_* up to n.
+    void flat_layers(const uint64_t *mhcs, uint64_t n, uint32_t k, const uint32_t *mapping, uint32_t range,
+                     std::vector<uint8_t> &thresholds, std::vector<uint32_t> &free_counts, std::vector<uint64_t> &layer_bases,
+                     std::vector<uint64_t> &bumped_mhcs, std::vector<uint64_t> &bumped_index) {
+        std::lock_guard<std::mutex> lock(mtx);
+        const uint64_t nb = k ? n / k : 0;
+        thresholds.assign(nb + 1, 0);
+        free_counts.assign(nb + 1, 0);
+        // room for every key to be bumped, left uninitialised (a zero-filled vector of n words costs more than the
+        // layers themselves); the bumped keys, a few percent, are copied out
+        std::unique_ptr<uint64_t[]> bm(new uint64_t[n + 1]), bi(new uint64_t[n + 1]);
+        uint64_t bases[3] = {0, 0, 0}, nbumped = 0;
+        uint32_t layers = 0;
+        int rc = ensure(flat_mhc_buf, n * 8 + 8);
+        if (rc == SHGPU_OK && n) rc = shgpu_copy_to_device(e, flat_mhc_buf.p, mhcs, n * 8);
+        if (rc == SHGPU_OK)
+            rc = shgpu_flat_layers(e, (const uint64_t *)flat_mhc_buf.p, n, k, mapping, range, thresholds.data(), free_counts.data(),
+                                   bases, &layers, bm.get(), bi.get(), &nbumped);
+        check(rc, "shgpu_flat_layers");
+        layer_bases.assign(bases, bases + layers + 1);
+        bumped_mhcs.assign(bm.get(), bm.get() + nbumped);
+        bumped_index.assign(bi.get(), bi.get() + nbumped);
+    }
+
+    // After flat_layers: every key's bucket (shgpu_flat_bucket_keys; fb_index / fb_bucket place the bumped keys), then
+    // every bucket's ShockHash2 leaf code and the retrieval structure, as sh2_leaves_retrieval, without the keys
+    // leaving the device.
+    void flat_leaves_retrieval(uint64_t n, uint32_t k, const uint32_t *mapping, uint32_t range,
+                               const std::vector<uint64_t> &layer_bases, const std::vector<uint8_t> &thresholds,
+                               const std::vector<uint64_t> &fb_index, const std::vector<uint64_t> &fb_bucket,
+                               std::vector<uint64_t> &codes, bool keep, std::vector<uint64_t> &input_keys,
+                               std::vector<uint8_t> &input_bits, std::vector<uint32_t> &offsets, std::vector<uint8_t> &seeds,
+                               std::vector<uint64_t> &words, double *search_ms = nullptr) {
+        std::lock_guard<std::mutex> lock(mtx);
+        const uint64_t nb = k ? n / k : 0;
+        int rc = ensure(leaf_keys_buf, nb * k * 8 + 8);
+        if (rc == SHGPU_OK)
+            rc = shgpu_flat_bucket_keys(e, (const uint64_t *)flat_mhc_buf.p, n, k, mapping, range, layer_bases.data(),
+                                        (uint32_t)layer_bases.size() - 1, thresholds.data(), fb_index.data(), fb_bucket.data(),
+                                        fb_index.size(), (uint64_t *)leaf_keys_buf.p);
+        check(rc, "shgpu_flat_bucket_keys");
+        leaves_retrieval_device(nb, k, codes, keep, input_keys, input_bits, offsets, seeds, words, search_ms);
+    }
+
     // ShockHash2 leaves of k keys each, keys[i * k .. (i + 1) * k) (ShockHash2Flat's buckets): their codes
     // (BijectionsShockHash2::findSeed, one batched search on the device) and the retrieval structure of their
     // orientations (constructRetrieval, ShockHash2-internal.h:548-572), built as for a forest.
@@ -202,18 +248,32 @@ class Engine {
                               std::vector<uint32_t> &offsets, std::vector<uint8_t> &seeds, std::vector<uint64_t> &words) {
         std::lock_guard<std::mutex> lock(mtx);
         const uint64_t nb = k ? keys.size() / k : 0;
+        int rc = ensure(leaf_keys_buf, keys.size() * 8 + 8);
+        if (rc == SHGPU_OK && !keys.empty()) rc = shgpu_copy_to_device(e, leaf_keys_buf.p, keys.data(), keys.size() * 8);
+        check(rc, "ShockHash2 leaf keys upload");
+        leaves_retrieval_device(nb, k, codes, keep, input_keys, input_bits, offsets, seeds, words);
+    }
+
+  private:
+    // The nb leaves of k keys in leaf_keys_buf: codes, orientation and retrieval (caller holds mtx); search_ms: the
+    // batched leaf search alone (host clock around the call and its copy of the codes).
+    void leaves_retrieval_device(uint64_t nb, uint32_t k, std::vector<uint64_t> &codes, bool keep,
+                                 std::vector<uint64_t> &input_keys, std::vector<uint8_t> &input_bits,
+                                 std::vector<uint32_t> &offsets, std::vector<uint8_t> &seeds, std::vector<uint64_t> &words,
+                                 double *search_ms = nullptr) {
         std::vector<shgpu_leaf_task> lt(nb);
         for (uint64_t i = 0; i < nb; i++) lt[i] = {(uint32_t)(i * k), k, 0, 0};
         codes.assign(nb, SHGPU_NOT_FOUND);
-        int rc = ensure(leaf_keys_buf, keys.size() * 8 + 8);
-        if (rc == SHGPU_OK) rc = ensure(leaf_task_buf, nb * sizeof(shgpu_leaf_task) + 8);
+        int rc = ensure(leaf_task_buf, nb * sizeof(shgpu_leaf_task) + 8);
         if (rc == SHGPU_OK) rc = ensure(leaf_code_buf, nb * 8 + 8);
-        if (rc == SHGPU_OK && !keys.empty()) rc = shgpu_copy_to_device(e, leaf_keys_buf.p, keys.data(), keys.size() * 8);
         if (rc == SHGPU_OK && nb) rc = shgpu_copy_to_device(e, leaf_task_buf.p, lt.data(), nb * sizeof(shgpu_leaf_task));
+        if (rc == SHGPU_OK) rc = shgpu_sync(e);
+        const auto t0 = std::chrono::steady_clock::now();
         if (rc == SHGPU_OK && nb)
             rc = shgpu_sh2_leaf_search(e, (const uint64_t *)leaf_keys_buf.p, (const shgpu_leaf_task *)leaf_task_buf.p,
                                        (uint32_t)nb, (uint64_t *)leaf_code_buf.p, nullptr);
         if (rc == SHGPU_OK && nb) rc = shgpu_copy_to_host(e, codes.data(), leaf_code_buf.p, nb * 8);
+        if (search_ms) *search_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         check(rc, "shgpu_sh2_leaf_search (buckets)");
         for (uint64_t c : codes)
             if (c == SHGPU_NOT_FOUND) throw Error(SHGPU_ERR_SEARCH_LIMIT, "ShockHash2 leaf search exhausted its tables (duplicate keys?)");
@@ -223,7 +283,6 @@ class Engine {
                               input_bits, offsets, seeds, words, nullptr);
     }
 
-  private:
     // Orientation replay of `tasks` over device keys, then the ribbon over the replay's output (caller holds mtx).
     void retrieval_from_device(int variant, const uint64_t *d_keys, const std::vector<shgpu_orient_task> &tasks,
                                uint64_t total, bool keep, std::vector<uint64_t> &input_keys,

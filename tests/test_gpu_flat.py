@@ -14,7 +14,8 @@ def log(msg):
     print(msg, flush=True)
 
 
-@pytest.mark.parametrize("k,n", [(16, 16 * 600), (32, 20011), (64, 64 * 1500), (100, 100037), (128, 128 * 120)])
+@pytest.mark.parametrize("k,n", [(16, 16 * 600), (32, 20011), (64, 64 * 1500), (100, 100037), (128, 128 * 120),
+                                 (16, 7), (16, 17), (32, 31), (64, 64 * 2 + 5), (64, 64 * 11)])
 def test_flat_matches_oracle(k, n):
     log(f"flat k={k} n={n}")
     mh = workload.fingerprints(n, seed=300 + k)[:, 1].copy()
@@ -30,6 +31,7 @@ def test_flat_matches_oracle(k, n):
     assert len(bad) == 0, f"{len(bad)} keys differ, e.g. {q[bad[:5]]} vs {want[bad[:5]]}"
     assert len(np.unique(q)) == n
     assert len(f.codes) == of.nbuckets
+    log(f"  phases {f.phases_ms}")
     log(f"  ok {info}")
 
 
