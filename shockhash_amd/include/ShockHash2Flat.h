@@ -254,9 +254,8 @@ class ShockHash2Flat {
         auto t1 = clk::now();
         phaseMs[0] = ms(t0, t1);
         if (!bumpedMhc.empty()) {
-            std::vector<std::string> content;
-            content.reserve(bumpedMhc.size());
-            for (uint64_t h : bumpedMhc) content.push_back(std::to_string(h));
+            std::vector<std::string> content(bumpedMhc.size());
+            parallelFor(bumpedMhc.size(), [&](size_t j) { content[j] = std::to_string(bumpedMhc[j]); });
             fallbackPhf = ShockHash2<k>(std::span<const std::string>(content.data(), content.size()), 2000, 1);
             haveFallback = true;
         }
