@@ -128,6 +128,10 @@ int shgpu_device_memory(shgpu_engine *e, size_t *free_bytes, size_t *total_bytes
 /* Device memory helpers (ordered on the engine's stream; the copies are synchronous). */
 int shgpu_device_alloc(shgpu_engine *e, size_t bytes, void **d_out);
 int shgpu_device_free(shgpu_engine *e, void *d);
+/* Page-locked host memory (hipHostMalloc), from which shgpu_copy_to_device reads by DMA without staging; freed
+ * with shgpu_host_free. */
+int shgpu_host_alloc(shgpu_engine *e, size_t bytes, void **h_out);
+int shgpu_host_free(shgpu_engine *e, void *h);
 int shgpu_copy_to_device(shgpu_engine *e, void *d_dst, const void *h_src, size_t bytes);
 int shgpu_copy_to_host(shgpu_engine *e, void *h_dst, const void *d_src, size_t bytes);
 /* Device-to-device copy by a plain streaming kernel (k_copy_u64); bytes must be a multiple of 8. Used as

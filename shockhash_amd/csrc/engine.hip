@@ -1907,6 +1907,26 @@ int shgpu_device_free(shgpu_engine *e, void *d) {
     return SHGPU_OK;
 }
 
+int shgpu_host_alloc(shgpu_engine *e, size_t bytes, void **h_out) {
+    if (!e || !h_out) return fail(SHGPU_ERR_ARG, "NULL argument");
+    *h_out = nullptr;
+    HIP_TRY(hipSetDevice(e->device));
+    if (hipHostMalloc(h_out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *h_out = nullptr;
+        (void)hipGetLastError();
+        return fail(SHGPU_ERR_NOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return SHGPU_OK;
+}
+
+int shgpu_host_free(shgpu_engine *e, void *h) {
+    if (!e) return fail(SHGPU_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (h) HIP_TRY(hipHostFree(h));
+    return SHGPU_OK;
+}
+
 int shgpu_copy_to_device(shgpu_engine *e, void *d_dst, const void *h_src, size_t bytes) {
     if (!e || (bytes && (!d_dst || !h_src))) return fail(SHGPU_ERR_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(e->device));

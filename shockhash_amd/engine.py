@@ -129,6 +129,8 @@ _SIGS = {
     "shgpu_device_memory": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "shgpu_device_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
     "shgpu_device_free": (ctypes.c_int, [P, P]),
+    "shgpu_host_alloc": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
+    "shgpu_host_free": (ctypes.c_int, [P, P]),
     "shgpu_copy_to_device": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
     "shgpu_copy_to_host": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),
     "shgpu_device_copy": (ctypes.c_int, [P, P, P, ctypes.c_size_t]),

@@ -7,8 +7,6 @@
 #pragma once
 #include <array>
 #include <algorithm>
-#include <exception>
-#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -22,6 +20,7 @@
 #include <vector>
 
 #include "ShockHash2.h"
+#include "shockhash/Parallel.h"
 
 namespace shockhash {
 
@@ -200,30 +199,6 @@ class ShockHash2Flat {
         const uint64_t x = thresholdsAndSeeds.get(b);
         return {x & (THRESHOLD_RANGE - 1), x >> THRESHOLD_BITS};
     }
-    // body(i) for i in [0, n) on up to 16 host threads (the queries it runs are read-only); rethrows the first
-    // exception a thread caught.
-    template <class F>
-    static void parallelFor(size_t n, F body) {
-        const size_t nt = std::max<size_t>(1, std::min<size_t>({16, (size_t)std::thread::hardware_concurrency(), n / 4096}));
-        if (nt <= 1) {
-            for (size_t i = 0; i < n; i++) body(i);
-            return;
-        }
-        std::vector<std::thread> threads;
-        std::vector<std::exception_ptr> errors(nt);
-        for (size_t t = 0; t < nt; t++)
-            threads.emplace_back([&, t] {
-                try {
-                    for (size_t i = n * t / nt; i < n * (t + 1) / nt; i++) body(i);
-                } catch (...) {
-                    errors[t] = std::current_exception();
-                }
-            });
-        for (auto &th : threads) th.join();
-        for (auto &e : errors)
-            if (e) std::rethrow_exception(e);
-    }
-
     size_t fallbackQuery(uint64_t mhc) const { return fallbackPhf(std::to_string(mhc)); }
 
     // The reference's construction (ShockHash2Flat.h:66-171) with its per-key loops on the device: the threshold

@@ -13,6 +13,7 @@
 
 #include "../../../include/shockhash_amd.h"
 #include "Hashing.h"
+#include "Parallel.h"
 
 namespace shockhash {
 namespace gpu {
@@ -55,6 +56,8 @@ class Engine {
     Buf tasks_buf, out_keys_buf, out_bits_buf;
     Buf leaf_keys_buf, leaf_task_buf, leaf_code_buf;  // one ShockHash2 leaf at a time (sh2LeafCode)
     Buf flat_mhc_buf;                                 // ShockHash2Flat: the keys' hashes while a construction runs
+    Buf str_data_buf, str_off_buf, str_hash_buf;      // string keys: bytes, offsets and fingerprints on the device
+    Buf str_data_host, str_off_host;                  // ... and the packed bytes and offsets in page-locked memory
     int ensure(Buf &b, size_t bytes) {
         if (b.bytes >= bytes) return SHGPU_OK;
         if (b.p) shgpu_device_free(e, b.p);
@@ -64,12 +67,24 @@ class Engine {
         if (rc == SHGPU_OK) b.bytes = bytes;
         return rc;
     }
+    int ensure_host(Buf &b, size_t bytes) {
+        if (b.bytes >= bytes) return SHGPU_OK;
+        if (b.p) shgpu_host_free(e, b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        const int rc = shgpu_host_alloc(e, bytes, &b.p);
+        if (rc == SHGPU_OK) b.bytes = bytes;
+        return rc;
+    }
 
   public:
     explicit Engine(int device = 0) { check(shgpu_engine_create(device, &e), "shgpu_engine_create"); }
     ~Engine() {
-        for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf, &leaf_keys_buf, &leaf_task_buf, &leaf_code_buf, &flat_mhc_buf})
+        for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf, &leaf_keys_buf, &leaf_task_buf, &leaf_code_buf, &flat_mhc_buf,
+                       &str_data_buf, &str_off_buf, &str_hash_buf})
             if (b->p) shgpu_device_free(e, b->p);
+        for (Buf *b : {&str_data_host, &str_off_host})
+            if (b->p) shgpu_host_free(e, b->p);
         shgpu_engine_destroy(e);
     }
     Engine(const Engine &) = delete;
@@ -92,31 +107,50 @@ class Engine {
         return collect(b, n);
     }
 
-    // String keys: packed and fingerprinted on the device (shgpu_hash_strings, SpookyHash V2 as
-    // first_hash), then built from the device fingerprints.
+    // String keys: packed on host threads into page-locked memory, copied by DMA and fingerprinted on the device
+    // (shgpu_hash_strings, SpookyHash V2 as first_hash), then built from the device fingerprints. The packed and
+    // device buffers are kept between constructions.
     template <class Strings>
     Forest build_strings(int variant, uint32_t leaf_size, const Strings &keys, uint64_t bucket_size, bool rotation = false) {
         std::lock_guard<std::mutex> lock(mtx);
         const uint64_t n = keys.size();
-        std::vector<uint64_t> offsets(n + 1, 0);
-        for (uint64_t i = 0; i < n; i++) offsets[i + 1] = offsets[i] + keys[i].size();
-        std::vector<uint8_t> data(offsets[n] + 8);
-        for (uint64_t i = 0; i < n; i++) std::memcpy(data.data() + offsets[i], keys[i].data(), keys[i].size());
-        void *d_data = nullptr, *d_off = nullptr, *d_hashes = nullptr;
-        int rc = shgpu_device_alloc(e, data.size(), &d_data);
-        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, offsets.size() * 8, &d_off);
-        if (rc == SHGPU_OK) rc = shgpu_device_alloc(e, n * 16 + 16, &d_hashes);
-        if (rc == SHGPU_OK) rc = shgpu_copy_to_device(e, d_data, data.data(), data.size());
-        if (rc == SHGPU_OK) rc = shgpu_copy_to_device(e, d_off, offsets.data(), offsets.size() * 8);
-        if (rc == SHGPU_OK) rc = shgpu_hash_strings(e, (const uint8_t *)d_data, (const uint64_t *)d_off, n, (uint64_t *)d_hashes);
+        const size_t nt = hostThreads(n);
+        std::vector<uint64_t> chunk(nt + 1, 0);  // bytes before each thread's range
+        parallelRanges(n, nt, [&](size_t t, size_t lo, size_t hi) {
+            uint64_t b = 0;
+            for (size_t i = lo; i < hi; i++) b += keys[i].size();
+            chunk[t + 1] = b;
+        });
+        for (size_t t = 0; t < nt; t++) chunk[t + 1] += chunk[t];
+        const uint64_t total = chunk[nt];
+        // the hashing kernel may read up to 8 bytes past the last key
+        int rc = ensure_host(str_data_host, total + 8);
+        if (rc == SHGPU_OK) rc = ensure_host(str_off_host, (n + 1) * 8);
+        check(rc, "shgpu_host_alloc (string keys)");
+        uint8_t *data = (uint8_t *)str_data_host.p;
+        uint64_t *off = (uint64_t *)str_off_host.p;
+        parallelRanges(n, nt, [&](size_t t, size_t lo, size_t hi) {
+            uint64_t o = chunk[t];
+            for (size_t i = lo; i < hi; i++) {
+                off[i] = o;
+                std::memcpy(data + o, keys[i].data(), keys[i].size());
+                o += keys[i].size();
+            }
+        });
+        off[n] = total;
+        std::memset(data + total, 0, 8);
+        rc = ensure(str_data_buf, total + 8);
+        if (rc == SHGPU_OK) rc = ensure(str_off_buf, (n + 1) * 8);
+        if (rc == SHGPU_OK) rc = ensure(str_hash_buf, n * 16 + 16);
+        if (rc == SHGPU_OK) rc = shgpu_copy_to_device(e, str_data_buf.p, data, total + 8);
+        if (rc == SHGPU_OK) rc = shgpu_copy_to_device(e, str_off_buf.p, off, (n + 1) * 8);
+        if (rc == SHGPU_OK)
+            rc = shgpu_hash_strings(e, (const uint8_t *)str_data_buf.p, (const uint64_t *)str_off_buf.p, n, (uint64_t *)str_hash_buf.p);
         shgpu_build *b = nullptr;
         const shgpu_build_desc d = desc(variant, leaf_size, bucket_size, rotation);
-        if (rc == SHGPU_OK) rc = shgpu_build_run(e, &d, (const uint64_t *)d_hashes, n, &b);
-        const std::string msg = rc == SHGPU_OK ? "" : shgpu_last_error();
-        if (d_data) shgpu_device_free(e, d_data);
-        if (d_off) shgpu_device_free(e, d_off);
-        if (d_hashes) shgpu_device_free(e, d_hashes);
-        if (rc != SHGPU_OK) throw Error(rc, std::string("shgpu_build_run (strings): ") + shgpu_status_string(rc) + ": " + msg);
+        if (rc == SHGPU_OK) rc = shgpu_build_run(e, &d, (const uint64_t *)str_hash_buf.p, n, &b);
+        if (rc != SHGPU_OK)
+            throw Error(rc, std::string("shgpu_build_run (strings): ") + shgpu_status_string(rc) + ": " + shgpu_last_error());
         return collect(b, n);
     }
 

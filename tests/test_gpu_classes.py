@@ -68,6 +68,25 @@ def test_string_keys_shockhash2():
     assert np.array_equal(q, ob.query(hashes))
 
 
+def test_string_keys_buffers_kept_between_builds():
+    """String constructions one after another reuse (and grow) the engine's page-locked and device buffers: a
+    small set, a larger one, then smaller ones again (one with empty and long keys, one of a single key), each
+    equal to the oracle's build over its SpookyV2 fingerprints."""
+    rng = np.random.default_rng(41)
+    sets = [workload.random_strings(3000, seed=42), workload.random_strings(60000, seed=43),
+            [b""] + [rng.integers(0, 256, size=int(L), dtype=np.uint8).tobytes() for L in rng.integers(1, 700, 2000)],
+            [b"only"], workload.random_strings(5000, seed=44)]
+    for keys in sets:
+        keys = list(dict.fromkeys(keys))  # distinct keys
+        log(f"strings sh1 n={len(keys)}")
+        f = host.MPHF("sh1", 30, keys, 2000)
+        hashes = np.array([O.spooky128(k, 0) for k in keys], dtype=np.uint64).reshape(-1, 2)
+        ob = O.OracleBuild("sh1", 30, hashes, 2000, threads=8)
+        assert np.array_equal(f.codes, ob.node_code)
+        assert np.array_equal(f(keys), ob.query(hashes))
+        f.close()
+
+
 @pytest.mark.parametrize("variant,leaf", [("sh2", 64), ("sh1", 30), ("simd", 24), ("sh2", 100)])
 def test_save_load_round_trip(tmp_path, variant, leaf):
     """save() then load() into a fresh object of the class: every query of the reloaded function equals the

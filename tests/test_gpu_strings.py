@@ -23,3 +23,26 @@ def test_random_ascii_keys(engine):
 
 def test_empty_batch(engine):
     assert engine.hash_strings([]).shape == (0, 2)
+
+
+def test_page_locked_host_buffer_round_trip(engine):
+    """shgpu_host_alloc memory is a valid source and target of the engine's copies (the class templates pack
+    string keys into it)."""
+    import ctypes
+    from shockhash_amd import engine as eng
+    lib = eng.load()
+    p = ctypes.c_void_p()
+    n = 1 << 20
+    assert lib.shgpu_host_alloc(engine.h, n * 8, ctypes.byref(p)) == 0 and p.value
+    try:
+        src = np.random.default_rng(52).integers(0, 2**63, size=n, dtype=np.uint64)
+        ctypes.memmove(p.value, src.ctypes.data, n * 8)
+        d = eng.DeviceArray(engine, n, np.uint64)
+        assert lib.shgpu_copy_to_device(engine.h, d.ptr, p.value, n * 8) == 0
+        assert np.array_equal(d.download(), src)
+        back = np.frombuffer((ctypes.c_uint64 * n).from_address(p.value), dtype=np.uint64)
+        assert lib.shgpu_copy_to_host(engine.h, p.value, d.ptr, 8 * 1000) == 0
+        assert np.array_equal(back[:1000], src[:1000])
+        d.free()
+    finally:
+        assert lib.shgpu_host_free(engine.h, p.value) == 0
