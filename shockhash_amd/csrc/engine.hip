@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -176,6 +177,50 @@ int noalloc_scratch_of(int device, void **out) {
 }
 }  // namespace
 
+// Copies from pageable host memory of at least UPLOAD_STAGED_MIN bytes go through page-locked chunks: the runtime's
+// own path for pageable memory moved ~29 GB/s (160 MB of fingerprints in 5.5 ms), the DMA engine reads page-locked
+// memory at the link's rate. UPLOAD_THREADS host threads fill the chunks, two buffers each, while earlier chunks'
+// DMAs run.
+constexpr size_t UPLOAD_CHUNK = size_t(4) << 20;
+constexpr int UPLOAD_THREADS = 8;
+constexpr size_t UPLOAD_STAGED_MIN = size_t(16) << 20;
+
+struct Uploader {
+    void *pin[UPLOAD_THREADS][2] = {};
+    hipEvent_t ev[UPLOAD_THREADS][2] = {};
+    bool ready = false, failed = false;
+    // Allocates the buffers and events on first use (true when they are there).
+    bool init(hipStream_t st) {
+        if (ready || failed) return ready;
+        for (int t = 0; t < UPLOAD_THREADS; t++)
+            for (int k = 0; k < 2; k++) {
+                if (hipHostMalloc(&pin[t][k], UPLOAD_CHUNK, hipHostMallocDefault) != hipSuccess) pin[t][k] = nullptr;
+                if (hipEventCreateWithFlags(&ev[t][k], hipEventDisableTiming) != hipSuccess) ev[t][k] = nullptr;
+                // recorded once, so that the first wait on every buffer returns at once
+                if (!pin[t][k] || !ev[t][k] || hipEventRecord(ev[t][k], st) != hipSuccess) failed = true;
+            }
+        if (failed) {
+            (void)hipGetLastError();
+            release();
+            failed = true;
+            return false;
+        }
+        ready = true;
+        return true;
+    }
+    void release() {
+        for (int t = 0; t < UPLOAD_THREADS; t++)
+            for (int k = 0; k < 2; k++) {
+                if (ev[t][k]) (void)hipEventSynchronize(ev[t][k]);
+                if (pin[t][k]) (void)hipHostFree(pin[t][k]);
+                if (ev[t][k]) (void)hipEventDestroy(ev[t][k]);
+                pin[t][k] = nullptr;
+                ev[t][k] = nullptr;
+            }
+        ready = false;
+    }
+};
+
 struct shgpu_engine {
     int device = 0;
     void *noalloc_scratch = nullptr;  // this device's copy of `scratch` (device address)
@@ -193,6 +238,7 @@ struct shgpu_engine {
     BuildBuffers main;                          // shgpu_build_run
     DevBuf retrieval;                           // shgpu_ribbon_build working memory (~42 B per key)
     DevBuf flat;                                // ShockHash2Flat bucketing working memory (~70 B per key)
+    Uploader up;                                // page-locked chunks of large uploads from pageable memory
 };
 
 struct shgpu_build {
@@ -919,6 +965,7 @@ void shgpu_engine_destroy(shgpu_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->stream);
+    e->up.release();
     (void)hipStreamDestroy(e->stream);
     e->main.destroy_events();
     delete e;
@@ -1875,13 +1922,63 @@ int shgpu_schedule_parse(const char *json, shgpu_schedule_class *out, uint32_t c
     return SHGPU_OK;
 }
 
+// Host memory the runtime does not know as page-locked (or registered).
+static bool pageable(const void *p) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return attr.type != hipMemoryTypeHost && attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged;
+}
+
+// Queues the copy of `bytes` host bytes to device memory on the engine's stream; the source may be reused once this
+// returns. Large pageable sources go through the uploader: thread t copies chunks t, t + UPLOAD_THREADS, ... into
+// its two page-locked buffers in turn, waiting for the event behind a buffer's previous DMA before refilling it,
+// and queues each chunk's DMA and a new event (the runtime orders the threads' operations on the one stream).
+static int upload(shgpu_engine *e, void *dst, const void *src, size_t bytes) {
+    if (bytes < UPLOAD_STAGED_MIN || !pageable(src) || !e->up.init(e->stream)) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream));
+        return SHGPU_OK;
+    }
+    const size_t nchunks = (bytes + UPLOAD_CHUNK - 1) / UPLOAD_CHUNK;
+    std::atomic<int> err{0};
+    std::vector<std::thread> threads;
+    threads.reserve(UPLOAD_THREADS);
+    for (int t = 0; t < UPLOAD_THREADS; t++)
+        threads.emplace_back([&, t] {
+            if (hipSetDevice(e->device) != hipSuccess) { err = 1; return; }
+            size_t i = 0;
+            for (size_t c = (size_t)t; c < nchunks && !err.load(); c += UPLOAD_THREADS, i++) {
+                const int k = (int)(i & 1);
+                const size_t off = c * UPLOAD_CHUNK, len = std::min(UPLOAD_CHUNK, bytes - off);
+                if (hipEventSynchronize(e->up.ev[t][k]) != hipSuccess) { err = 1; return; }
+                std::memcpy(e->up.pin[t][k], (const char *)src + off, len);
+                if (hipMemcpyAsync((char *)dst + off, e->up.pin[t][k], len, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+                    hipEventRecord(e->up.ev[t][k], e->stream) != hipSuccess) {
+                    err = 1;
+                    return;
+                }
+            }
+        });
+    for (auto &th : threads) th.join();
+    if (err.load()) {
+        (void)hipGetLastError();
+        return fail(SHGPU_ERR_HIP, "staged upload from pageable memory failed");
+    }
+    return SHGPU_OK;
+}
+
 int shgpu_build_run_host(shgpu_engine *e, const shgpu_build_desc *desc, const uint64_t *h_hashes, uint64_t n,
                          shgpu_build **out) {
     if (!e || !desc || !out || (n && !h_hashes)) return fail(SHGPU_ERR_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(e->device));
     int rc = e->staging.ensure(n * 16 + 16);
     if (rc) return rc;
-    if (n) HIP_TRY(hipMemcpyAsync(e->staging.p, h_hashes, n * 16, hipMemcpyHostToDevice, e->stream));
+    if (n) {
+        rc = upload(e, e->staging.p, h_hashes, n * 16);
+        if (rc) return rc;
+    }
     return shgpu_build_run(e, desc, e->staging.as<uint64_t>(), n, out);
 }
 
@@ -1930,7 +2027,10 @@ int shgpu_host_free(shgpu_engine *e, void *h) {
 int shgpu_copy_to_device(shgpu_engine *e, void *d_dst, const void *h_src, size_t bytes) {
     if (!e || (bytes && (!d_dst || !h_src))) return fail(SHGPU_ERR_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(e->device));
-    if (bytes) HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, e->stream));
+    if (bytes) {
+        const int rc = upload(e, d_dst, h_src, bytes);
+        if (rc) return rc;
+    }
     HIP_TRY(hipStreamSynchronize(e->stream));
     return SHGPU_OK;
 }
