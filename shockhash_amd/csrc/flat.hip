@@ -149,10 +149,14 @@ __global__ void k_flat_fallback(const uint64_t *__restrict__ fb_index, const uin
     }
 }
 
-// err bit 4 when a key has no bucket or a bucket value beyond the partial bucket's (limit).
-__global__ void k_flat_check_set(const uint64_t *__restrict__ bkey, uint64_t n, uint64_t limit, uint32_t *err) {
-    for (uint64_t i = gtid(); i < n; i += gstride())
-        if (bkey[i] >= limit) atomicOr(err, 4u);
+// err bit 4 when a key has no bucket or a bucket value that names neither a full bucket (below nbuckets) nor a place
+// in the partial bucket ([nbuckets * k, limit)).
+__global__ void k_flat_check_set(const uint64_t *__restrict__ bkey, uint64_t n, uint64_t nbuckets, uint32_t k, uint64_t limit,
+                                 uint32_t *err) {
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint64_t b = bkey[i];
+        if (b >= limit || (b >= nbuckets && b < nbuckets * k)) atomicOr(err, 4u);
+    }
 }
 
 // Sorted by bucket: position j < nbuckets * k must hold bucket j / k (every bucket exactly k keys, err bit 8
@@ -211,9 +215,11 @@ uint32_t grid_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<
     } while (0)
 
 int check_map(const uint32_t *mapping, uint32_t range, FlatMap &map, std::string &err) {
-    if (!mapping || range < 2 || range > FLAT_MAX_RANGE || mapping[range - 1] != 0xFFFFFFFFu) {
+    bool ok = mapping && range >= 2 && range <= FLAT_MAX_RANGE && mapping[range - 1] == 0xFFFFFFFFu;
+    for (uint32_t i = 1; ok && i < range; i++) ok = mapping[i - 1] <= mapping[i];  // compact_threshold searches it
+    if (!ok) {
         err = "ShockHash2Flat bucketing: threshold mapping must have 2.." + std::to_string(FLAT_MAX_RANGE) +
-              " values ending at 2^32 - 1";
+              " non-decreasing values ending at 2^32 - 1";
         return SHGPU_ERR_ARG;
     }
     std::memset(&map, 0, sizeof map);
@@ -378,7 +384,7 @@ int shgpu_flat_bucket_keys_impl(hipStream_t st, const uint64_t *d_mhcs, uint64_t
     FL_HIP(hipGetLastError());
     // the partial bucket's keys take the values k * nbuckets + i, i below the number of free positions (< n)
     const uint64_t limit = nbuckets * k + n;
-    k_flat_check_set<<<g, 256, 0, st>>>(bkey, n, limit, errw);
+    k_flat_check_set<<<g, 256, 0, st>>>(bkey, n, nbuckets, k, limit, errw);
     FL_HIP(hipGetLastError());
     uint32_t e = 0;
     FL_HIP(hipMemcpyAsync(&e, errw, 4, hipMemcpyDeviceToHost, st));

@@ -37,6 +37,7 @@ _SIGS = {
     "shh_flat_info": (None, [P, P]),
     "shh_flat_nbuckets": (ctypes.c_size_t, [P]),
     "shh_flat_codes": (None, [P, P]),
+    "shh_murmur64a": (ctypes.c_uint64, [P, ctypes.c_size_t, ctypes.c_uint64]),
     "shh_flat_phases": (None, [P, P]),
 }
 VARIANTS = {"sh1": 0, "simd": 1, "sh2": 2}

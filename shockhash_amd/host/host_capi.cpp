@@ -329,6 +329,10 @@ void shh_flat_codes(void *h, uint64_t *out) {
     std::memcpy(out, c.data(), c.size() * 8);
 }
 
+// ShockHash2Flat's string hash (MurmurHash64A with a seed; the class uses seed 0), for checks against its
+// published verification value.
+uint64_t shh_murmur64a(const void *key, size_t len, uint64_t seed) { return flat_detail::murmurHash64A(key, len, seed); }
+
 // Compile-time Golomb-Rice memo of a class (for comparison with the oracle's run-time formula).
 int shh_memo(int variant, int leaf, uint64_t *out) {
     if (variant == SHGPU_SHOCKHASH || variant == SHGPU_SIMD_SHOCKHASH) {

@@ -278,7 +278,7 @@ class ShockHash2Flat {
     }
     explicit ShockHash2Flat(std::span<const std::string> keys) {
         std::vector<uint64_t> mhcs(keys.size());
-        for (size_t i = 0; i < keys.size(); i++) mhcs[i] = flat_detail::murmurHash64A(keys[i].data(), keys[i].size());
+        parallelFor(keys.size(), [&](size_t i) { mhcs[i] = flat_detail::murmurHash64A(keys[i].data(), keys[i].size()); });
         construct(mhcs);
     }
     // Keys given by their 64-bit hashes (not in the reference interface, which hashes string keys itself).

@@ -48,3 +48,26 @@ def test_flat_base_case_codes_equal_leaf_search():
         keys = mh[bucket == b]  # input order, as the reference fills its buckets
         assert len(keys) == k
         assert codes[b] == O.sh2_find_seed(keys)
+
+
+def test_flat_layers_rejects_bad_mappings(engine):
+    """shgpu_flat_layers checks the threshold mapping before any work: 2..64 non-decreasing values ending at
+    2^32 - 1 (the reference's mapping has that shape, include/ShockHash2Flat.h:17-35)."""
+    import ctypes
+    from shockhash_amd import engine as eng
+    lib = eng.load()
+    n, k = 64 * 10, 64
+    d = eng.DeviceArray.from_host(engine, workload.fingerprints(n, seed=5)[:, 1].copy())
+    thr, free = np.zeros(n, np.uint8), np.zeros(n, np.uint32)
+    bases, layers, nb = np.zeros(4, np.uint64), ctypes.c_uint32(), ctypes.c_uint64()
+    bm, bi = np.zeros(n, np.uint64), np.zeros(n, np.uint64)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+
+    def run(mapping):
+        m = np.asarray(mapping, dtype=np.uint32)
+        return lib.shgpu_flat_layers(engine.h, d.ptr, n, k, P(m), len(m), P(thr), P(free), P(bases), ctypes.byref(layers),
+                                     P(bm), P(bi), ctypes.byref(nb))
+    assert run([(i + 1) * (1 << 28) - 1 for i in range(16)]) == 0
+    for bad in ([0xFFFFFFFF], [5, 0xFFFFFFFE], [9, 3, 0xFFFFFFFF], [0] * 64 + [0xFFFFFFFF]):
+        assert run(bad) == eng.ERR_ARG, bad
+    d.free()
