@@ -185,13 +185,32 @@ constexpr size_t UPLOAD_CHUNK = size_t(4) << 20;
 constexpr int UPLOAD_THREADS = 8;
 constexpr size_t UPLOAD_STAGED_MIN = size_t(16) << 20;
 
+// The uploader's worker threads are started once per engine and wait for jobs; worker t copies chunks t,
+// t + UPLOAD_THREADS, ... of a job into its two page-locked buffers in turn, waiting for the event behind a buffer's
+// previous DMA before refilling it, and queues each chunk's DMA and a new event on the engine's stream (the runtime
+// orders the threads' operations on the one stream).
 struct Uploader {
     void *pin[UPLOAD_THREADS][2] = {};
     hipEvent_t ev[UPLOAD_THREADS][2] = {};
     bool ready = false, failed = false;
-    // Allocates the buffers and events on first use (true when they are there).
-    bool init(hipStream_t st) {
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_job, cv_done;
+    uint64_t gen = 0;  // job number
+    int done = 0;      // workers finished with the current job
+    bool stop = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    const char *src = nullptr;
+    char *dst = nullptr;
+    size_t bytes = 0;
+    std::atomic<int> err{0};
+
+    // Allocates the buffers and events and starts the workers on first use (true when they are there).
+    bool init(int dev, hipStream_t st) {
         if (ready || failed) return ready;
+        device = dev;
+        stream = st;
         for (int t = 0; t < UPLOAD_THREADS; t++)
             for (int k = 0; k < 2; k++) {
                 if (hipHostMalloc(&pin[t][k], UPLOAD_CHUNK, hipHostMallocDefault) != hipSuccess) pin[t][k] = nullptr;
@@ -205,10 +224,61 @@ struct Uploader {
             failed = true;
             return false;
         }
+        for (int t = 0; t < UPLOAD_THREADS; t++) workers.emplace_back([this, t] { work(t); });
         ready = true;
         return true;
     }
+
+    void work(int t) {
+        if (hipSetDevice(device) != hipSuccess) err = 1;
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_job.wait(lk, [&] { return stop || gen != seen; });
+                if (stop) return;
+                seen = gen;
+            }
+            const size_t nchunks = (bytes + UPLOAD_CHUNK - 1) / UPLOAD_CHUNK;
+            size_t i = 0;
+            for (size_t c = (size_t)t; c < nchunks && !err.load(); c += UPLOAD_THREADS, i++) {
+                const int k = (int)(i & 1);
+                const size_t off = c * UPLOAD_CHUNK, len = std::min(UPLOAD_CHUNK, bytes - off);
+                if (hipEventSynchronize(ev[t][k]) != hipSuccess) { err = 1; break; }
+                std::memcpy(pin[t][k], src + off, len);
+                if (hipMemcpyAsync(dst + off, pin[t][k], len, hipMemcpyHostToDevice, stream) != hipSuccess ||
+                    hipEventRecord(ev[t][k], stream) != hipSuccess) {
+                    err = 1;
+                    break;
+                }
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            if (++done == UPLOAD_THREADS) cv_done.notify_one();
+        }
+    }
+
+    // Queues the copy; returns once every chunk has been copied out of `s` and its DMA queued (0), or 1 on failure.
+    int copy(void *d, const void *s, size_t n) {
+        std::unique_lock<std::mutex> lk(mu);
+        src = (const char *)s;
+        dst = (char *)d;
+        bytes = n;
+        err = 0;
+        done = 0;
+        gen++;
+        cv_job.notify_all();
+        cv_done.wait(lk, [&] { return done == UPLOAD_THREADS; });
+        return err.load();
+    }
+
     void release() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv_job.notify_all();
+        for (auto &w : workers) w.join();
+        workers.clear();
         for (int t = 0; t < UPLOAD_THREADS; t++)
             for (int k = 0; k < 2; k++) {
                 if (ev[t][k]) (void)hipEventSynchronize(ev[t][k]);
@@ -1933,36 +2003,13 @@ static bool pageable(const void *p) {
 }
 
 // Queues the copy of `bytes` host bytes to device memory on the engine's stream; the source may be reused once this
-// returns. Large pageable sources go through the uploader: thread t copies chunks t, t + UPLOAD_THREADS, ... into
-// its two page-locked buffers in turn, waiting for the event behind a buffer's previous DMA before refilling it,
-// and queues each chunk's DMA and a new event (the runtime orders the threads' operations on the one stream).
+// returns. Large pageable sources go through the uploader's page-locked chunks.
 static int upload(shgpu_engine *e, void *dst, const void *src, size_t bytes) {
-    if (bytes < UPLOAD_STAGED_MIN || !pageable(src) || !e->up.init(e->stream)) {
+    if (bytes < UPLOAD_STAGED_MIN || !pageable(src) || !e->up.init(e->device, e->stream)) {
         HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream));
         return SHGPU_OK;
     }
-    const size_t nchunks = (bytes + UPLOAD_CHUNK - 1) / UPLOAD_CHUNK;
-    std::atomic<int> err{0};
-    std::vector<std::thread> threads;
-    threads.reserve(UPLOAD_THREADS);
-    for (int t = 0; t < UPLOAD_THREADS; t++)
-        threads.emplace_back([&, t] {
-            if (hipSetDevice(e->device) != hipSuccess) { err = 1; return; }
-            size_t i = 0;
-            for (size_t c = (size_t)t; c < nchunks && !err.load(); c += UPLOAD_THREADS, i++) {
-                const int k = (int)(i & 1);
-                const size_t off = c * UPLOAD_CHUNK, len = std::min(UPLOAD_CHUNK, bytes - off);
-                if (hipEventSynchronize(e->up.ev[t][k]) != hipSuccess) { err = 1; return; }
-                std::memcpy(e->up.pin[t][k], (const char *)src + off, len);
-                if (hipMemcpyAsync((char *)dst + off, e->up.pin[t][k], len, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
-                    hipEventRecord(e->up.ev[t][k], e->stream) != hipSuccess) {
-                    err = 1;
-                    return;
-                }
-            }
-        });
-    for (auto &th : threads) th.join();
-    if (err.load()) {
+    if (e->up.copy(dst, src, bytes)) {
         (void)hipGetLastError();
         return fail(SHGPU_ERR_HIP, "staged upload from pageable memory failed");
     }
