@@ -22,8 +22,9 @@
 //    in VGPRs (lane l of block b holds entry 64 b + l), so candidate extraction and the isolated-vertex
 //    filter are ballots over registers. Keys are staged in LDS split by parity and read back as
 //    broadcasts. A leaf that needs more half-seeds or candidates (at m = 64 the median is ~100
-//    half-seeds and ~40 candidates; ~0.14% of leaves need more than 128 candidates) reports
-//    SH2_OVERFLOW and the first j whose pairs are not all tested, and is continued by
+//    half-seeds and ~40 candidates; ~0.15% of leaves need more than 128 candidates, none of 20,000
+//    simulated leaves more than 153 candidates or 221 half-seeds) reports SH2_OVERFLOW and the first j whose
+//    pairs are not all tested, and is continued by
 //  * the general path (sh2_leaf_search_general), whose tables live in the wave's global scratch area
 //    and are bounded only by SH2_MAX_HALF_SEEDS / SH2_MAX_CANDIDATES. It rebuilds the half-seed and
 //    candidate tables and tests pairs from (resume j, 0) on (the pairs before it already failed).
@@ -47,7 +48,7 @@ namespace shgpu {
 constexpr uint32_t SH2_MAX_HALF_SEEDS = 1024;  // half-seed bound of leaves up to 64 keys (~2^7 needed at m = 64)
 constexpr uint32_t SH2_MAX_CANDIDATES = 4096;
 constexpr uint32_t SH2_HS_BLOCKS = 4;    // register path: half-seeds [0, 256)
-constexpr uint32_t SH2_CAND_BLOCKS = 2;  // register path: candidates [0, 128)
+constexpr uint32_t SH2_CAND_BLOCKS = 3;  // register path: candidates [0, 192)
 constexpr uint64_t SH2_OVERFLOW = SHGPU_NOT_FOUND - 1;
 constexpr uint32_t SH2_MAX_LEAF = 128;   // MAX_LEAF_SIZE2 (include/ShockHash2.h:29)
 
@@ -341,7 +342,17 @@ struct Regs2 {
         r1 = c && b == 1 ? v : r1;
     }
 };
-static_assert(SH2_HS_BLOCKS == 4 && SH2_CAND_BLOCKS == 2, "register tables are Regs4 / Regs2");
+template <class T>
+struct Regs3 {
+    T r0 = 0, r1 = 0, r2 = 0;
+    __device__ __forceinline__ T at(uint32_t b) const { return b == 0 ? r0 : b == 1 ? r1 : r2; }
+    __device__ __forceinline__ void set_if(bool c, uint32_t b, T v) {
+        r0 = c && b == 0 ? v : r0;
+        r1 = c && b == 1 ? v : r1;
+        r2 = c && b == 2 ? v : r2;
+    }
+};
+static_assert(SH2_HS_BLOCKS == 4 && SH2_CAND_BLOCKS == 3, "register tables are Regs4 / Regs3");
 
 // OR of 1 << fastrange64(remix(key + s), H) over n staged keys (broadcast LDS reads, two keys per read).
 __device__ __forceinline__ uint32_t sh2_half_mask(const uint64_t *stage, uint32_t n, uint64_t s, uint32_t H) {
@@ -381,7 +392,7 @@ __device__ __forceinline__ uint64_t sh2_leaf_search_regs(const uint64_t *keys, u
     wave_sync();
 
     Regs4<uint32_t> hsA, hsB;
-    Regs2<uint64_t> isoR;
+    Regs3<uint64_t> isoR;
     uint32_t n_blocks = 0, s_next = 0, n_cand = 0, jq = 1, qlen = 0;
     // Overflow happens while candidate jq is made available: the pairs of every j < jq are queued, and the
     // queued pairs not yet tested start at the first queue entry's j.
