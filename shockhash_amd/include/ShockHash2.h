@@ -4,6 +4,7 @@
 #pragma once
 #include <span>
 #include <string>
+#include <string_view>
 #include <vector>
 
 #include "shockhash/GpuRecSplit.h"
@@ -17,6 +18,11 @@ class ShockHash2 : public detail::GpuRecSplit<SHGPU_SHOCKHASH2, LEAF_SIZE> {
   public:
     ShockHash2() {}
     ShockHash2(std::span<const std::string> keys, const size_t bucket_size, size_t num_threads = 1) {
+        this->construct_strings(keys, bucket_size, num_threads);
+    }
+    // String keys given as views (not in the reference interface): the same function as from std::string keys,
+    // for callers that hold the keys' bytes elsewhere and need not allocate a std::string per key.
+    ShockHash2(std::span<const std::string_view> keys, const size_t bucket_size, size_t num_threads = 1) {
         this->construct_strings(keys, bucket_size, num_threads);
     }
     ShockHash2(std::span<hash128_t> keys, const size_t bucket_size, size_t num_threads = 1) {

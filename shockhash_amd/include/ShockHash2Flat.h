@@ -6,6 +6,7 @@
 // the retrieval structure run on the GPU (one batched leaf search for all buckets); the bucketing runs on the host.
 #pragma once
 #include <array>
+#include <charconv>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -16,6 +17,7 @@
 #include <span>
 #include <stdexcept>
 #include <string>
+#include <string_view>
 #include <utility>
 #include <vector>
 
@@ -229,9 +231,17 @@ class ShockHash2Flat {
         auto t1 = clk::now();
         phaseMs[0] = ms(t0, t1);
         if (!bumpedMhc.empty()) {
-            std::vector<std::string> content(bumpedMhc.size());
-            parallelFor(bumpedMhc.size(), [&](size_t j) { content[j] = std::to_string(bumpedMhc[j]); });
-            fallbackPhf = ShockHash2<k>(std::span<const std::string>(content.data(), content.size()), 2000, 1);
+            // the bumped keys' decimal forms (std::to_string's digits) side by side in one buffer: a std::string per
+            // key would be a heap block (20 digits exceed the short-string buffer), allocated on the worker threads
+            // and freed on this one, which took longer than the fallback's construction
+            std::vector<char> digits(bumpedMhc.size() * 20);
+            std::vector<std::string_view> content(bumpedMhc.size());
+            parallelFor(bumpedMhc.size(), [&](size_t j) {
+                char *p = digits.data() + 20 * j;
+                const auto r = std::to_chars(p, p + 20, bumpedMhc[j]);
+                content[j] = std::string_view(p, (size_t)(r.ptr - p));
+            });
+            fallbackPhf = ShockHash2<k>(std::span<const std::string_view>(content.data(), content.size()), 2000, 1);
             haveFallback = true;
         }
         auto t2 = clk::now();
