@@ -201,7 +201,14 @@ class ShockHash2Flat {
         const uint64_t x = thresholdsAndSeeds.get(b);
         return {x & (THRESHOLD_RANGE - 1), x >> THRESHOLD_BITS};
     }
-    size_t fallbackQuery(uint64_t mhc) const { return fallbackPhf(std::to_string(mhc)); }
+    // The fallback function of a key's decimal form (std::to_string's digits), built in a buffer the thread keeps.
+    size_t fallbackQuery(uint64_t mhc) const {
+        thread_local std::string s;
+        char buf[20];
+        const auto r = std::to_chars(buf, buf + sizeof buf, mhc);
+        s.assign(buf, r.ptr);
+        return fallbackPhf(s);
+    }
 
     // The reference's construction (ShockHash2Flat.h:66-171) with its per-key loops on the device: the threshold
     // layers (shgpu_flat_layers), the placement of every key and the bucket key lists (shgpu_flat_bucket_keys), the
