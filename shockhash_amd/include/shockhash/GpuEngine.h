@@ -58,6 +58,7 @@ class Engine {
     Buf flat_mhc_buf;                                 // ShockHash2Flat: the keys' hashes while a construction runs
     Buf str_data_buf, str_off_buf, str_hash_buf;      // string keys: bytes, offsets and fingerprints on the device
     Buf str_data_host, str_off_host;                  // ... and the packed bytes and offsets in page-locked memory
+    Buf orient_host;                                  // the orientation pass's task list, in page-locked memory
     int ensure(Buf &b, size_t bytes) {
         if (b.bytes >= bytes) return SHGPU_OK;
         if (b.p) shgpu_device_free(e, b.p);
@@ -83,7 +84,7 @@ class Engine {
         for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf, &leaf_keys_buf, &leaf_task_buf, &leaf_code_buf, &flat_mhc_buf,
                        &str_data_buf, &str_off_buf, &str_hash_buf})
             if (b->p) shgpu_device_free(e, b->p);
-        for (Buf *b : {&str_data_host, &str_off_host})
+        for (Buf *b : {&str_data_host, &str_off_host, &orient_host})
             if (b->p) shgpu_host_free(e, b->p);
         shgpu_engine_destroy(e);
     }
@@ -211,22 +212,23 @@ class Engine {
                                std::vector<uint8_t> &seeds, std::vector<uint64_t> &words, double *ms2 = nullptr,
                                uint32_t rot_leaf = 0) {
         std::lock_guard<std::mutex> lock(mtx);
-        std::vector<shgpu_orient_task> tasks;
-        tasks.reserve(f.nodes.size());
+        check(ensure_host(orient_host, (f.nodes.size() + 1) * sizeof(shgpu_orient_task)), "shgpu_host_alloc (orientation tasks)");
+        shgpu_orient_task *tasks = (shgpu_orient_task *)orient_host.p;
+        size_t ntasks = 0;
         uint64_t total = 0;
         for (size_t i = 0; i < f.nodes.size(); i++) {
             const shgpu_node &nd = f.nodes[i];
             if (nd.kind != SHGPU_NODE_LEAF) continue;
             if (variant != SHGPU_SHOCKHASH2 && rot_leaf != 0 && nd.m == rot_leaf && nd.m >= 2)  // ShockHash.h:401-402
-                tasks.push_back({total, f.codes[i] / nd.m + nd.start_seed, nd.offset, nd.m, (uint32_t)(f.codes[i] % nd.m) + 1, 0});
+                tasks[ntasks++] = {total, f.codes[i] / nd.m + nd.start_seed, nd.offset, nd.m, (uint32_t)(f.codes[i] % nd.m) + 1, 0};
             else
-                tasks.push_back({total, variant == SHGPU_SHOCKHASH2 ? f.codes[i] : f.codes[i] + nd.start_seed, nd.offset, nd.m, 0, 0});
+                tasks[ntasks++] = {total, variant == SHGPU_SHOCKHASH2 ? f.codes[i] : f.codes[i] + nd.start_seed, nd.offset, nd.m, 0, 0};
             total += nd.m;
         }
         if (!f.device) throw Error(SHGPU_ERR_ARG, "leaf orientation: the forest holds no device build");
         const uint64_t *leaf_keys = nullptr;
         check(shgpu_build_device_keys(f.device.get(), &leaf_keys), "shgpu_build_device_keys");
-        retrieval_from_device(variant, leaf_keys, tasks, total, keep, input_keys, input_bits, offsets, seeds, words, ms2);
+        retrieval_from_device(variant, leaf_keys, tasks, ntasks, total, keep, input_keys, input_bits, offsets, seeds, words, ms2);
     }
 
     // ShockHash2Flat's threshold layers (shgpu_flat_layers) over the hashes mhcs, which stay on the device for
@@ -311,26 +313,26 @@ class Engine {
         check(rc, "shgpu_sh2_leaf_search (buckets)");
         for (uint64_t c : codes)
             if (c == SHGPU_NOT_FOUND) throw Error(SHGPU_ERR_SEARCH_LIMIT, "ShockHash2 leaf search exhausted its tables (duplicate keys?)");
-        std::vector<shgpu_orient_task> tasks(nb);
+        check(ensure_host(orient_host, (nb + 1) * sizeof(shgpu_orient_task)), "shgpu_host_alloc (orientation tasks)");
+        shgpu_orient_task *tasks = (shgpu_orient_task *)orient_host.p;
         for (uint64_t i = 0; i < nb; i++) tasks[i] = {i * k, codes[i], (uint32_t)(i * k), k, 0, 0};
-        retrieval_from_device(SHGPU_SHOCKHASH2, (const uint64_t *)leaf_keys_buf.p, tasks, nb * k, keep, input_keys,
+        retrieval_from_device(SHGPU_SHOCKHASH2, (const uint64_t *)leaf_keys_buf.p, tasks, nb, nb * k, keep, input_keys,
                               input_bits, offsets, seeds, words, nullptr);
     }
 
     // Orientation replay of `tasks` over device keys, then the ribbon over the replay's output (caller holds mtx).
-    void retrieval_from_device(int variant, const uint64_t *d_keys, const std::vector<shgpu_orient_task> &tasks,
+    void retrieval_from_device(int variant, const uint64_t *d_keys, const shgpu_orient_task *tasks, size_t ntasks,
                                uint64_t total, bool keep, std::vector<uint64_t> &input_keys,
                                std::vector<uint8_t> &input_bits, std::vector<uint32_t> &offsets,
                                std::vector<uint8_t> &seeds, std::vector<uint64_t> &words, double *ms2) {
         const auto t0 = std::chrono::steady_clock::now();
-        int rc = ensure(tasks_buf, tasks.size() * sizeof(shgpu_orient_task) + 8);
+        int rc = ensure(tasks_buf, ntasks * sizeof(shgpu_orient_task) + 8);
         if (rc == SHGPU_OK) rc = ensure(out_keys_buf, total * 8 + 8);
         if (rc == SHGPU_OK) rc = ensure(out_bits_buf, total + 8);
         void *d_tasks = tasks_buf.p, *d_out_keys = out_keys_buf.p, *d_out_bits = out_bits_buf.p;
-        if (rc == SHGPU_OK && !tasks.empty())
-            rc = shgpu_copy_to_device(e, d_tasks, tasks.data(), tasks.size() * sizeof(shgpu_orient_task));
+        if (rc == SHGPU_OK && ntasks) rc = shgpu_copy_to_device(e, d_tasks, tasks, ntasks * sizeof(shgpu_orient_task));
         if (rc == SHGPU_OK)
-            rc = shgpu_leaf_orientation(e, variant, d_keys, (const shgpu_orient_task *)d_tasks, tasks.size(),
+            rc = shgpu_leaf_orientation(e, variant, d_keys, (const shgpu_orient_task *)d_tasks, ntasks,
                                         (uint64_t *)d_out_keys, (uint8_t *)d_out_bits);
         const auto t1 = std::chrono::steady_clock::now();
         const uint64_t shards = shgpu_ribbon_shards(total);
