@@ -205,6 +205,7 @@ struct Uploader {
     char *dst = nullptr;
     size_t bytes = 0;
     std::atomic<int> err{0};
+    std::atomic<int> setup_failed{0};  // a worker could not select the engine's device
 
     // Allocates the buffers and events and starts the workers on first use (true when they are there).
     bool init(int dev, hipStream_t st) {
@@ -224,13 +225,19 @@ struct Uploader {
             failed = true;
             return false;
         }
-        for (int t = 0; t < UPLOAD_THREADS; t++) workers.emplace_back([this, t] { work(t); });
+        try {
+            for (int t = 0; t < UPLOAD_THREADS; t++) workers.emplace_back([this, t] { work(t); });
+        } catch (...) {  // no thread: copies take the runtime's own path
+            release();
+            failed = true;
+            return false;
+        }
         ready = true;
         return true;
     }
 
     void work(int t) {
-        if (hipSetDevice(device) != hipSuccess) err = 1;
+        if (hipSetDevice(device) != hipSuccess) setup_failed = 1;
         uint64_t seen = 0;
         for (;;) {
             {
@@ -239,7 +246,7 @@ struct Uploader {
                 if (stop) return;
                 seen = gen;
             }
-            const size_t nchunks = (bytes + UPLOAD_CHUNK - 1) / UPLOAD_CHUNK;
+            const size_t nchunks = setup_failed.load() ? 0 : (bytes + UPLOAD_CHUNK - 1) / UPLOAD_CHUNK;
             size_t i = 0;
             for (size_t c = (size_t)t; c < nchunks && !err.load(); c += UPLOAD_THREADS, i++) {
                 const int k = (int)(i & 1);
@@ -268,7 +275,7 @@ struct Uploader {
         gen++;
         cv_job.notify_all();
         cv_done.wait(lk, [&] { return done == UPLOAD_THREADS; });
-        return err.load();
+        return err.load() || setup_failed.load();
     }
 
     void release() {
@@ -2010,8 +2017,10 @@ static int upload(shgpu_engine *e, void *dst, const void *src, size_t bytes) {
         return SHGPU_OK;
     }
     if (e->up.copy(dst, src, bytes)) {
+        // the whole range again by the runtime's own path (chunks already queued carry the same bytes); a real
+        // error (a bad destination) fails here too
         (void)hipGetLastError();
-        return fail(SHGPU_ERR_HIP, "staged upload from pageable memory failed");
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream));
     }
     return SHGPU_OK;
 }
