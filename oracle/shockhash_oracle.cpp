@@ -1307,5 +1307,11 @@ void orc_flat_info(void *fp, uint64_t *out7) {
 void orc_flat_evaluate(void *f, const uint64_t *mhcs, size_t n, uint64_t *bucket, uint64_t *seed) {
     for (size_t i = 0; i < n; i++) { auto r = flat_evaluate(*(Flat *)f, mhcs[i]); bucket[i] = r.first; seed[i] = r.second; }
 }
+// The compacted threshold kept by every bucket of the layers (setThreshold's values), layer_bases.back() entries.
+void orc_flat_thresholds(void *fp, uint32_t *out) {
+    const Flat &f = *(Flat *)fp;
+    const size_t handled = f.layer_bases.empty() ? 0 : f.layer_bases.back();
+    for (size_t b = 0; b < handled; b++) out[b] = f.thr[b];
+}
 void orc_flat_mapping(uint32_t range, uint32_t *out) { auto m = flat_mapping(range); std::memcpy(out, m.data(), m.size() * 4); }
 }  // extern "C"

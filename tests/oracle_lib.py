@@ -49,6 +49,7 @@ _SIGS = {
     "orc_flat_info": (None, [P, P]),
     "orc_flat_evaluate": (None, [P, P, ctypes.c_size_t, P, P]),
     "orc_flat_mapping": (None, [U32, P]),
+    "orc_flat_thresholds": (None, [P, P]),
     "orc_sh1_leaf_rot": (None, [P, U32, U64, ctypes.POINTER(U64), ctypes.POINTER(U32), P]),
     "orc_sh1_rot_query_cell": (U64, [U32, U64, U64, U64, U64]),
     "orc_build_buckets": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, P, ctypes.c_size_t,
@@ -284,6 +285,13 @@ class OracleFlat:
         out = np.zeros(len(mhcs), dtype=np.uint64)
         lib().orc_flat_query(self.h, ptr(mhcs), len(mhcs), ptr(out))
         return out
+
+    def thresholds(self):
+        """Compacted threshold index of every bucket of the layers (the first layer_bases[layers] buckets)."""
+        handled = int(self.layer_bases[self.layers])
+        out = np.zeros(max(handled, 1), dtype=np.uint32)
+        lib().orc_flat_thresholds(self.h, ptr(out))
+        return out[:handled]
 
     def evaluate(self, mhcs):
         mhcs = u64(mhcs)

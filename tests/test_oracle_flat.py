@@ -59,3 +59,35 @@ def test_flat_layers_decide_first():
     assert elsewhere.sum() <= f.bumped and elsewhere.sum() >= 0.95 * f.bumped
     counts = np.bincount(bucket[bucket < f.nbuckets].astype(np.int64), minlength=f.nbuckets)
     assert (counts == k).all()
+
+
+@pytest.mark.parametrize("k,n", [(16, 16 * 500 + 7), (64, 64 * 900), (100, 100 * 400 + 33)])
+def test_flat_thresholds_by_brute_force(k, n):
+    """flushBucket (ShockHash2Flat.h:204-233) stated over every threshold index instead of by its two compares: a
+    bucket of at most k keys keeps the top index; a larger one keeps the largest index t, no larger than the index
+    its k-th smallest threshold value maps to (compact_threshold, :194-202), whose threshold value admits at most k of
+    the bucket's keys. Checked on every first-layer bucket against the indices the oracle stores; a slip in the
+    rule (the k-th or (k+1)-th key, < for <=, a missing or extra decrement) changes some of them."""
+    mh = workload.fingerprints(n, seed=400 + k)[:, 1]
+    f = O.OracleFlat(k, mh)
+    trange = 1 << (int(np.floor(np.log2(k))) - 1)
+    mapping = O.flat_mapping(trange).astype(np.uint64)
+    thr = f.thresholds()
+    b0 = f.layer_bases[1]
+    bucket = ((mh & np.uint64(0xFFFFFFFF)) * np.uint64(b0)) >> np.uint64(32)
+    value = mh >> np.uint64(32)
+    order = np.lexsort((value, bucket))
+    bs, vs = bucket[order], value[order]
+    starts = np.searchsorted(bs, np.arange(b0 + 1, dtype=np.uint64))
+    decrements = 0
+    for b in range(b0):
+        v = vs[starts[b]:starts[b + 1]]
+        if len(v) <= k:
+            want = trange - 1
+        else:
+            kept = np.searchsorted(v, mapping, side="right")  # keys admitted by each index's threshold
+            top = int(np.searchsorted(mapping, v[k - 1], side="left"))
+            want = max(t for t in range(top + 1) if kept[t] <= k)
+            decrements += want < top
+        assert int(thr[b]) == want, (b, len(v))
+    assert decrements > 0  # the decrementing branch was exercised
