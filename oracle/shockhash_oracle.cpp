@@ -1313,5 +1313,10 @@ void orc_flat_thresholds(void *fp, uint32_t *out) {
     const size_t handled = f.layer_bases.empty() ? 0 : f.layer_bases.back();
     for (size_t b = 0; b < handled; b++) out[b] = f.thr[b];
 }
+// freePositions as the construction lists them (ShockHash2Flat.h:124-147), one entry per key handed to the fallback.
+void orc_flat_free_positions(void *fp, uint64_t *out) {
+    const Flat &f = *(Flat *)fp;
+    for (size_t i = 0; i < f.free_pos.size(); i++) out[i] = f.free_pos[i];
+}
 void orc_flat_mapping(uint32_t range, uint32_t *out) { auto m = flat_mapping(range); std::memcpy(out, m.data(), m.size() * 4); }
 }  // extern "C"

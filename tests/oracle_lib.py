@@ -50,6 +50,7 @@ _SIGS = {
     "orc_flat_evaluate": (None, [P, P, ctypes.c_size_t, P, P]),
     "orc_flat_mapping": (None, [U32, P]),
     "orc_flat_thresholds": (None, [P, P]),
+    "orc_flat_free_positions": (None, [P, P]),
     "orc_sh1_leaf_rot": (None, [P, U32, U64, ctypes.POINTER(U64), ctypes.POINTER(U32), P]),
     "orc_sh1_rot_query_cell": (U64, [U32, U64, U64, U64, U64]),
     "orc_build_buckets": (P, [ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t, ctypes.c_size_t, P, ctypes.c_size_t,
@@ -292,6 +293,12 @@ class OracleFlat:
         out = np.zeros(max(handled, 1), dtype=np.uint32)
         lib().orc_flat_thresholds(self.h, ptr(out))
         return out[:handled]
+
+    def free_positions(self):
+        """freePositions: the position list the fallback's values index, one entry per bumped key."""
+        out = np.zeros(max(self.bumped, 1), dtype=np.uint64)
+        lib().orc_flat_free_positions(self.h, ptr(out))
+        return out[: self.bumped]
 
     def evaluate(self, mhcs):
         mhcs = u64(mhcs)

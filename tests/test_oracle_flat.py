@@ -91,3 +91,52 @@ def test_flat_thresholds_by_brute_force(k, n):
             decrements += want < top
         assert int(thr[b]) == want, (b, len(v))
     assert decrements > 0  # the decrementing branch was exercised
+
+
+def _flush_free(bucket, value, nb, k, mapping, thr):
+    """Per bucket of one layer (keys given by bucket and threshold value): the free positions flushBucket lists
+    (k minus the bucket's size, or the bumped keys among its k smallest values) and which keys it bumps."""
+    order = np.lexsort((value, bucket))
+    bs, vs = bucket[order], value[order]
+    starts = np.searchsorted(bs, np.arange(nb + 1, dtype=np.uint64))
+    free, bumped = [], np.zeros(len(bucket), dtype=bool)
+    for b in range(nb):
+        v = vs[starts[b]:starts[b + 1]]
+        if len(v) <= k:
+            free += [b] * (k - len(v))
+        else:
+            over = v > mapping[int(thr[b])]
+            free += [b] * int(over[:k].sum())
+            bumped[order[starts[b]:starts[b + 1]][over]] = True
+    return free, bumped
+
+
+@pytest.mark.parametrize("k,n", [(16, 16 * 500 + 7), (64, 64 * 900 + 13), (100, 100 * 400)])
+def test_flat_free_positions_by_brute_force(k, n):
+    """freePositions (ShockHash2Flat.h:124-147) rebuilt from the keys: every layer bucket's free places in bucket
+    order (layer 0, then layer 1 over the keys layer 0 bumped, rehashed by remix), then one place per missing key
+    of the buckets no layer handled (k each), then the partial bucket's. The fallback's value of a bumped key
+    indexes this list, so together with the pinned thresholds and ShockHash2 leaf search it fixes every value."""
+    mh = workload.fingerprints(n, seed=500 + k)[:, 1]
+    f = O.OracleFlat(k, mh)
+    trange = 1 << (int(np.floor(np.log2(k))) - 1)
+    mapping = O.flat_mapping(trange).astype(np.uint64)
+    thr = f.thresholds()
+    lo32 = np.uint64(0xFFFFFFFF)
+    b0 = f.layer_bases[1]
+    bucket = ((mh & lo32) * np.uint64(b0)) >> np.uint64(32)
+    want, bumped = _flush_free(bucket, mh >> np.uint64(32), b0, k, mapping, thr[:b0])
+    keys = mh[bumped]
+    if f.layers == 2:
+        b1 = f.layer_bases[2] - b0
+        m1 = workload.mix13(keys)
+        free1, bumped1 = _flush_free(((m1 & lo32) * np.uint64(b1)) >> np.uint64(32), m1 >> np.uint64(32), b1, k,
+                                     mapping, thr[b0:])
+        want += [b0 + b for b in free1]
+        keys = keys[bumped1]
+    assert len(keys) == f.bumped
+    handled, in_end = f.layer_bases[f.layers], n - f.nbuckets * k
+    additional = f.bumped - len(want)
+    want += [handled + i // k for i in range(additional - in_end)] + [f.nbuckets + i for i in range(additional - in_end, additional)]
+    got = f.free_positions()
+    assert len(got) == len(want) and np.array_equal(got, np.array(want, dtype=np.uint64))
