@@ -59,6 +59,7 @@ class Engine {
     Buf str_data_buf, str_off_buf, str_hash_buf;      // string keys: bytes, offsets and fingerprints on the device
     Buf str_data_host, str_off_host;                  // ... and the packed bytes and offsets in page-locked memory
     Buf orient_host;                                  // the orientation pass's task list, in page-locked memory
+    Buf ribbon_host;                                  // the retrieval structure as it comes back, page-locked
     int ensure(Buf &b, size_t bytes) {
         if (b.bytes >= bytes) return SHGPU_OK;
         if (b.p) shgpu_device_free(e, b.p);
@@ -84,7 +85,7 @@ class Engine {
         for (Buf *b : {&tasks_buf, &out_keys_buf, &out_bits_buf, &leaf_keys_buf, &leaf_task_buf, &leaf_code_buf, &flat_mhc_buf,
                        &str_data_buf, &str_off_buf, &str_hash_buf})
             if (b->p) shgpu_device_free(e, b->p);
-        for (Buf *b : {&str_data_host, &str_off_host, &orient_host})
+        for (Buf *b : {&str_data_host, &str_off_host, &orient_host, &ribbon_host})
             if (b->p) shgpu_host_free(e, b->p);
         shgpu_engine_destroy(e);
     }
@@ -335,15 +336,24 @@ class Engine {
             rc = shgpu_leaf_orientation(e, variant, d_keys, (const shgpu_orient_task *)d_tasks, ntasks,
                                         (uint64_t *)d_out_keys, (uint8_t *)d_out_bits);
         const auto t1 = std::chrono::steady_clock::now();
-        const uint64_t shards = shgpu_ribbon_shards(total);
-        offsets.assign(shards + 1, 0);
-        seeds.assign(shards, 0);
-        words.assign(shgpu_ribbon_max_words(total), 0);
+        // the structure comes back into page-locked memory (offsets, seed bytes, slot words) and is copied out
+        const uint64_t shards = shgpu_ribbon_shards(total), maxw = shgpu_ribbon_max_words(total);
+        const size_t off_b = ((shards + 1) * 4 + 7) / 8 * 8, seed_b = (shards + 7) / 8 * 8;  // 8-byte aligned parts
+        if (rc == SHGPU_OK) rc = ensure_host(ribbon_host, off_b + seed_b + maxw * 8 + 8);
         uint64_t used = 0;
-        if (rc == SHGPU_OK)
-            rc = shgpu_ribbon_build(e, (const uint64_t *)d_out_keys, (const uint8_t *)d_out_bits, total, offsets.data(),
-                                    seeds.data(), words.data(), words.size(), &used);
-        if (rc == SHGPU_OK) words.resize(used);
+        if (rc == SHGPU_OK) {
+            uint8_t *base = (uint8_t *)ribbon_host.p;
+            uint32_t *h_off = (uint32_t *)base;
+            uint8_t *h_seeds = base + off_b;
+            uint64_t *h_words = (uint64_t *)(base + off_b + seed_b);
+            rc = shgpu_ribbon_build(e, (const uint64_t *)d_out_keys, (const uint8_t *)d_out_bits, total, h_off, h_seeds,
+                                    h_words, maxw, &used);
+            if (rc == SHGPU_OK) {
+                offsets.assign(h_off, h_off + shards + 1);
+                seeds.assign(h_seeds, h_seeds + shards);
+                words.assign(h_words, h_words + used);
+            }
+        }
         input_keys.assign(keep ? total : 0, 0);
         input_bits.assign(keep ? total : 0, 0);
         if (rc == SHGPU_OK && keep && total) rc = shgpu_copy_to_host(e, input_keys.data(), d_out_keys, total * 8);
